@@ -1,0 +1,254 @@
+"""ctypes front end of the oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and
+by bench.py's cpu_baseline leg, always as the checker (or as the timed CPU
+baseline), never on the product path.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+_lib = None
+
+_F32P = ctypes.POINTER(ctypes.c_float)
+_U32P = ctypes.POINTER(ctypes.c_uint32)
+_I32P = ctypes.POINTER(ctypes.c_int32)
+_F64P = ctypes.POINTER(ctypes.c_double)
+
+
+class Lattice(ctypes.Structure):
+    _fields_ = [
+        ("sun", ctypes.c_int),
+        ("n", ctypes.c_int * 4),
+        ("beta", ctypes.c_double),
+        ("nhit", ctypes.c_int),
+    ]
+
+
+class Ranlux(ctypes.Structure):
+    _fields_ = [
+        ("s", ctypes.c_float * 24),
+        ("i24", ctypes.c_int),
+        ("j24", ctypes.c_int),
+        ("in24", ctypes.c_int),
+        ("carry", ctypes.c_float),
+        ("nskip", ctypes.c_int),
+    ]
+
+
+class Rng(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int),
+        ("seq_generator", ctypes.c_int),
+        ("ranlux", ctypes.POINTER(Ranlux)),
+        ("pm_seed", _I32P),
+        ("quads", _F32P),
+        ("stride", ctypes.c_ulong),
+    ]
+
+
+RNG_GID, RNG_SEQ, RNG_QUADS = 0, 1, 2
+HB, OR = 0, 1
+
+
+def build():
+    """Compile liboracle.so with gcc (the oracle is plain C)."""
+    import subprocess
+
+    subprocess.run(
+        ["gcc", "-O2", "-shared", "-fPIC", os.path.join(HERE, "oracle.c"), "-o", LIB_PATH, "-lm"],
+        check=True,
+    )
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        L.orc_xor128_step.restype = ctypes.c_uint32
+        L.orc_pm_step.restype = ctypes.c_uint32
+        L.orc_pm_step.argtypes = [ctypes.c_uint32]
+        L.orc_pm_one_cpu.restype = ctypes.c_float
+        L.orc_ranlux_next.restype = ctypes.c_float
+        L.orc_sites.restype = ctypes.c_uint
+        L.orc_neighbour.restype = ctypes.c_uint
+        L.orc_parity_site.restype = ctypes.c_uint
+        L.orc_link_action.restype = ctypes.c_double
+        _lib = L
+    return _lib
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+def lattice(sun, dims, beta=5.9, nhit=10):
+    L = Lattice()
+    L.sun = sun
+    for i in range(4):
+        L.n[i] = dims[i]
+    L.beta = beta
+    L.nhit = nhit
+    return L
+
+
+def link_shape(L):
+    V = int(np.prod(list(L.n)))
+    return (V, 4, L.sun, L.sun, 2)
+
+
+def cold(L):
+    a = np.zeros(link_shape(L))
+    lib().orc_init_cold(ctypes.byref(L), _p(a, _F64P))
+    return a
+
+
+def gid(L):
+    a = np.zeros(link_shape(L))
+    lib().orc_init_gid(ctypes.byref(L), _p(a, _F64P))
+    return a
+
+
+def rand_fill(seed, count):
+    out = np.zeros(count, dtype=np.int32)
+    lib().orc_rand_fill(seed, count, _p(out, _I32P))
+    return out
+
+
+def xor128_seed_table(randseries, n):
+    t = np.zeros((n, 4), dtype=np.uint32)
+    lib().orc_xor128_seed_table(randseries, n, _p(t, _U32P))
+    return t
+
+
+def xor128_produce(table, samples):
+    n = table.shape[0]
+    q = np.zeros((samples, n, 4), dtype=np.float32)
+    lib().orc_xor128_produce(_p(table, _U32P), n, samples, _p(q, _F32P))
+    return q
+
+
+def pm_seed_table(randseries, n):
+    t = np.zeros((n, 4), dtype=np.uint32)
+    lib().orc_pm_seed_table(randseries, n, _p(t, _U32P))
+    return t
+
+
+def pm_produce(table, samples):
+    n = table.shape[0]
+    q = np.zeros((samples, n, 4), dtype=np.float32)
+    lib().orc_pm_produce(_p(table, _U32P), n, samples, _p(q, _F32P))
+    return q
+
+
+def pm_cpu_stream(seed, count):
+    s = ctypes.c_int32(seed)
+    return np.array([lib().orc_pm_one_cpu(ctypes.byref(s)) for _ in range(count)], dtype=np.float32)
+
+
+def ranlux_nskip(lux):
+    return lib().orc_ranlux_nskip(lux)
+
+
+def ranlux_cpu_stream(randseries, nskip, count):
+    out = np.zeros(count, dtype=np.float32)
+    lib().orc_ranlux_cpu_stream(randseries, nskip, count, _p(out, _F32P))
+    return out
+
+
+def ranlux_seed_table(randseries, n):
+    t = np.zeros((7, n, 4), dtype=np.float32)
+    lib().orc_ranlux_seed_table(randseries, n, _p(t, _F32P))
+    return t
+
+
+def ranlux_produce(table, samples, nskip):
+    n = table.shape[1]
+    q = np.zeros((samples, n, 4), dtype=np.float32)
+    lib().orc_ranlux_produce(_p(table, _F32P), n, samples, nskip, _p(q, _F32P))
+    return q
+
+
+class SeqRng:
+    """Sequential scalar source (suncpp semantics) after srand(randseries)."""
+
+    def __init__(self, generator, randseries, nskip=223):
+        self.rng = Rng()
+        self.rng.kind = RNG_SEQ
+        if generator == "PM":
+            lib().orc_srand(randseries)
+            self._seed = ctypes.c_int32(lib().orc_rand() % 2147483647)
+            self.rng.seq_generator = 1
+            self.rng.pm_seed = ctypes.pointer(self._seed)
+        else:
+            self._rl = Ranlux()
+            lib().orc_srand(randseries)
+            jseed = lib().orc_rand() % 2147483647
+            lib().orc_ranlux_init_scalar(ctypes.byref(self._rl), jseed, nskip)
+            self.rng.seq_generator = 0
+            self.rng.ranlux = ctypes.pointer(self._rl)
+
+
+class QuadRng:
+    """Reference GPU buffer semantics: thread i reads quads[i + k*stride]."""
+
+    def __init__(self, quads, stride):
+        self._q = np.ascontiguousarray(quads, dtype=np.float32)
+        self.rng = Rng()
+        self.rng.kind = RNG_QUADS
+        self.rng.quads = _p(self._q, _F32P)
+        self.rng.stride = stride
+
+
+class GidRng:
+    def __init__(self):
+        self.rng = Rng()
+        self.rng.kind = RNG_GID
+
+
+def halfstep(L, links, parity, mu, mode, rng):
+    return lib().orc_halfstep(ctypes.byref(L), _p(links, _F64P), parity, mu, mode, ctypes.byref(rng.rng))
+
+
+def sweep(L, links, mode, rng):
+    return lib().orc_sweep(ctypes.byref(L), _p(links, _F64P), mode, ctypes.byref(rng.rng))
+
+
+def plaquette(L, links):
+    out = np.zeros(2)
+    lib().orc_plaquette(ctypes.byref(L), _p(links, _F64P), _p(out, _F64P))
+    return out
+
+
+def polyakov(L, links):
+    out = np.zeros(4)
+    lib().orc_polyakov(ctypes.byref(L), _p(links, _F64P), _p(out, _F64P))
+    return out
+
+
+def staple(L, links, site, mu):
+    A = np.zeros((L.sun, L.sun, 2))
+    lib().orc_staple(ctypes.byref(L), _p(links, _F64P), site, mu, _p(A, _F64P))
+    return A
+
+
+def link_action(L, links, site, mu):
+    return lib().orc_link_action(ctypes.byref(L), _p(links, _F64P), site, mu)
+
+
+def reunitarize(L, links):
+    lib().orc_reunitarize(ctypes.byref(L), _p(links, _F64P))
+
+
+def parity_site(L, i, parity):
+    return lib().orc_parity_site(ctypes.byref(L), i, parity)
+
+
+def neighbour(L, site, d, sign):
+    return lib().orc_neighbour(ctypes.byref(L), site, d, sign)
