@@ -1,0 +1,176 @@
+"""Headline benchmark: SU(3) link updates per second on one MI355X.
+
+Workload (one "step" = one sweep of the measurement cycle):
+  SU(3) Wilson action, 32^3 x 8 lattice, beta = 5.9, FP32 links stored as
+  18 reals (72 B), FP64 reunitarisation and FP64 measurement sums, cold start,
+  XOR128 per-site streams in reference stream layout, NHIT = 10.
+  One sweep = 1 Cabibbo-Marinari heatbath sweep + 4 overrelaxation sweeps +
+  plaquette + Polyakov loop, captured once as a hipGraph and replayed.
+  Link updates per sweep = 5 * 4 * V (V = 262144).
+
+Prints one JSON line.  `roofline` is for the dominant kernel (the
+overrelaxation half-step, 32 of the 40 update launches of a sweep):
+algorithmic bytes per launch = 9 * 72 B * V/2 (each updated link reads the
+three other directions' links of both parities and its own direction's
+links of both parities once, and writes itself), divided by the mean kernel
+duration measured with HIP events on the lattice stream over `--steps`
+sweeps.  `cpu_baseline` times the reference CPU path (suncpp, built from the
+reference sources into oracle/_ref/ref_harness) on one host core for a
+bounded sample of heatbath sweeps of the same lattice.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+DIMS = (32, 32, 32, 8)
+BETA = 5.9
+N_OR = 4
+LINK_BYTES = 72
+HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(REPO, "profiles", "or_halfstep_su3_32c8_fp32_traffic.json")
+
+
+def cpu_baseline(sweeps=3):
+    """The reference's own CPU path on one core (kind 'reference'), or the
+    oracle restatement when the reference build is absent (kind 'port')."""
+    V = 1
+    for d in DIMS:
+        V *= d
+    harness = os.path.join(REPO, "oracle", "_ref", "ref_harness")
+    sample = "%d cold-start Cabibbo-Marinari heatbath sweeps of SU(3) %dx%dx%dx%d beta=%.1f, FP64, RANLUX3 (suncpp)" % (
+        (sweeps,) + DIMS + (BETA,)
+    )
+    if os.path.exists(harness):
+        out = subprocess.run(
+            [harness, "bench", "3"] + [str(d) for d in DIMS] + [str(BETA), "10", "RANLUX3", "54", str(sweeps)],
+            capture_output=True,
+            text=True,
+            timeout=300,
+            check=True,
+        ).stdout.split()
+        return {"value": float(out[2]), "unit": "link updates/s", "cores": 1, "kind": "reference", "sample": sample}
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import pyoracle as O
+
+    L = O.lattice(3, DIMS, BETA, 10)
+    a = O.cold(L)
+    rng = O.SeqRng("RANLUX", 54, 223)
+    t0 = time.perf_counter()
+    O.sweep(L, a, O.HB, rng)
+    dt = time.perf_counter() - t0
+    return {
+        "value": 4.0 * V / dt,
+        "unit": "link updates/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": "1 cold-start heatbath sweep of SU(3) 32x32x32x8, FP64 oracle restatement of suncpp",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=450)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--storage", type=int, default=18, choices=(18, 12))
+    ap.add_argument("--stream", default="reference", choices=("reference", "fast"))
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile-steps", type=int, default=50)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(
+            "bench.py: this build measures one GPU; --gpus %d is not supported "
+            "(multi-GPU sharding is a design note only)" % args.gpus,
+            file=sys.stderr,
+        )
+        sys.exit(2)
+
+    import qcdgpu_amd as Q
+
+    V = 1
+    for d in DIMS:
+        V *= d
+    lat = Q.Lattice(
+        sun=3,
+        dims=DIMS,
+        beta=BETA,
+        precision="fp32",
+        storage_reals=args.storage,
+        prng="XOR128",
+        stream=args.stream,
+        randseries=54,
+        max_measurements=args.steps + args.warmup + args.profile_steps + 8,
+    )
+    lat.init("cold")
+    lat.prng_init()
+    lat.capture_cycle(n_hb=1, n_or=N_OR, plaquette=True, polyakov=True)
+    lat.launch_cycle(args.warmup)
+    lat.sync()
+    # Timed region: K replays of the captured sweep, HIP events on the
+    # lattice stream around the whole region (synchronised on both sides).
+    ms = lat.time_cycle(args.steps)
+    updates = 5.0 * 4.0 * V * args.steps
+    value = updates / (ms * 1e-3)
+
+    prof = lat.profile_cycle(max(1, min(args.profile_steps, args.steps)))
+    bytes_per_launch = 9.0 * args.storage * 4 * (V // 2)
+    achieved = bytes_per_launch / (prof["or_us"] * 1e-6) / 1e9
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE) and args.storage == 18:
+        try:
+            traffic = json.load(open(TRAFFIC_FILE)).get("hbm_bytes_per_launch")
+        except Exception:
+            traffic = None
+    meas = lat.measurements()
+    lat.close()
+
+    line = {
+        "metric": "SU(3) link updates/sec + % HBM roofline",
+        "value": value,
+        "unit": "link updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {
+            "workload": "SU(3) Wilson 32^3x8 beta=5.9: 1 CM heatbath + 4 overrelaxation sweeps + plaquette + Polyakov per step",
+            "lattice": list(DIMS),
+            "beta": BETA,
+            "link_storage": "fp32 %d-real" % args.storage,
+            "reunitarisation": "fp64, fused into every link update",
+            "prng": "XOR128 per-site streams (%s stream layout)" % args.stream,
+            "nhit": 10,
+            "start": "cold",
+            "updates_per_step": int(5 * 4 * V),
+            "kernel_us": {"heatbath_halfstep": prof["hb_us"], "overrelax_halfstep": prof["or_us"], "measure": prof["meas_us"]},
+            "sweeps_per_s": args.steps / (ms * 1e-3),
+            "plaquette_last": float(meas["plq_total"][-1] / 3.0) if "plq_total" in meas else None,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_update<float,3,%d,overrelax>" % args.storage,
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "algorithmic_bytes_per_launch": bytes_per_launch,
+        },
+        "cpu_baseline": None if args.no_cpu else cpu_baseline(),
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

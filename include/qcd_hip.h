@@ -1,0 +1,176 @@
+/*
+ * qcd_hip.h -- C ABI of libqcdhip.so, the MI355X (gfx950) implementation of
+ * QCDGPU's SU(N) Monte Carlo update and measurement path.
+ *
+ * This header replaces the device boundary the reference host program talks
+ * to: GPU_CL::GPU (SU(N)/clinterface/clinterface.h:54-415) plus the PRNG_CL::PRNG
+ * produce/initialize calls (SU(N)/random/random.h:61-80) and the kernels the
+ * driver binds by name in model::lattice_make_programs
+ * (SU(N)/suncl/suncl.cpp:3265-3648).  Instead of an OpenCL-style "kernel by
+ * name + positional arguments" registry, each reference kernel family is one
+ * entry point with plain pointers and sizes; the mapping is listed per entry.
+ *
+ * Conventions
+ *  - Every function returns 0 on success and a non-zero QCD_E_* code on error;
+ *    qcd_last_error() gives the message.  The reference treats every OpenCL
+ *    error as fatal (clinterface.cpp:225-283); callers that want that
+ *    behaviour wrap each call and exit on a non-zero return.
+ *  - Calls are asynchronous on the lattice's own HIP stream unless stated;
+ *    qcd_lattice_sync() waits (the reference's kernel_run waits after every
+ *    launch, clinterface.cpp:1251-1278).
+ *  - Host link arrays use the canonical layout
+ *        links[((site*4 + dir)*N + row)*N + col][re, im]   (float64)
+ *    with site = y + z*N2 + t*N2*N3 + x*N2*N3*N4 (sun_common.cl:62-66).
+ */
+#ifndef QCD_HIP_H
+#define QCD_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QCD_ABI_VERSION 1
+
+enum {
+    QCD_OK = 0,
+    QCD_E_ARG = 1,       /* invalid argument                                 */
+    QCD_E_HIP = 2,       /* HIP runtime error                                */
+    QCD_E_NOMEM = 3,     /* allocation failed                                */
+    QCD_E_STATE = 4,     /* call not valid in the current state              */
+    QCD_E_UNSUPPORTED = 5
+};
+
+/* Link precision (model::precision, QCDGPU.cpp:179). */
+enum { QCD_FP32 = 0, QCD_FP64 = 1 };
+
+/* Start types (model::ints: model_start_hot/cold/gid, suncl.h:67). */
+enum { QCD_START_HOT = 0, QCD_START_COLD = 1, QCD_START_GID = 2 };
+
+/* Generators (PRNG::PRNG_generators, random.h:13-27, the subset on the path). */
+enum { QCD_PRNG_XOR128 = 1, QCD_PRNG_RANLUX = 2, QCD_PRNG_PM = 3 };
+
+/* Stream semantics of the per-site generators.
+ *  QCD_STREAM_REFERENCE: stream i is reference instance i with
+ *    PRNG_instances = PRNGSTEP; every heatbath half-step advances each stream
+ *    by exactly S = 1 + (align64(3*PRNGSTEP*(NHIT+1)*NHITPar) - 1)/PRNGSTEP
+ *    quads (one PRNG::produce, random.cpp:278-284), so the numbers a link
+ *    update consumes are bit-identical to the reference's PRN buffer.
+ *  QCD_STREAM_FAST: same generators, each stream only advances by the quads
+ *    actually consumed.                                                      */
+enum { QCD_STREAM_REFERENCE = 0, QCD_STREAM_FAST = 1 };
+
+/* Update kinds. */
+enum { QCD_HEATBATH = 0, QCD_OVERRELAX = 1 };
+
+/* Measurements (bit mask). */
+enum { QCD_MEAS_PLAQUETTE = 1, QCD_MEAS_POLYAKOV = 2 };
+
+typedef struct {
+    int      sun;            /* 2 or 3 (GROUP)                                  */
+    int      n[4];           /* N1..N4 = extents along x, y, z, t (L1..L4)      */
+    int      precision;      /* QCD_FP32 / QCD_FP64                             */
+    int      storage_reals;  /* SU(3): 18 or 12; SU(2): 4                       */
+    double   beta;           /* BETA (the update uses BETA / N)                 */
+    int      nhit;           /* NHIT                                            */
+    int      nhitpar;        /* NHITPar (only 1 is implemented)                 */
+    int      prng;           /* QCD_PRNG_*                                      */
+    int      ranlux_luxury;  /* 0..4 for QCD_PRNG_RANLUX                        */
+    int      stream_mode;    /* QCD_STREAM_*                                    */
+    unsigned randseries;     /* RANDSERIES (0 = time based, as the reference)   */
+    int      max_measurements; /* capacity of the on-device measurement arrays  */
+} qcd_params;
+
+typedef struct qcd_lattice qcd_lattice;
+
+/* ---- device (GPU::device_select / device_initialize / device_finalize,
+        clinterface.cpp:347-459) ------------------------------------------ */
+int         qcd_device_open(int platform, int device);
+int         qcd_device_close(void);
+int         qcd_device_info(char *name, int name_len, int *compute_units, size_t *global_mem);
+const char *qcd_last_error(void);
+int         qcd_abi_version(void);
+
+/* ---- lattice lifetime (model::model_lattice_init, suncl.cpp:2212-2384;
+        lattice_create_buffers, :3877-3991) --------------------------------- */
+int  qcd_lattice_create(const qcd_params *params, qcd_lattice **out);
+int  qcd_lattice_destroy(qcd_lattice *lat);
+int  qcd_lattice_sync(qcd_lattice *lat);
+/* Launch-shape knobs of the update kernels (set by qcdgpu_amd.autotune, which
+   times candidates on the device; both only change scheduling, never results):
+   QCD_TUNE_XCD_MAP       1: each of the 8 XCDs gets one contiguous range of
+                          compact site indices (default), 0: hardware order.
+   QCD_TUNE_BLOCKS_PER_CU cap on resident update workgroups per CU (reserved
+                          LDS), 0 = no cap (default).                        */
+enum { QCD_TUNE_XCD_MAP = 1, QCD_TUNE_BLOCKS_PER_CU = 2 };
+int  qcd_lattice_tune(qcd_lattice *lat, int knob, int value);
+/* Bytes of device memory held by the link table. */
+int  qcd_lattice_table_bytes(const qcd_lattice *lat, size_t *bytes);
+
+/* ---- starts: kernels lattice_init_cold / lattice_init_gid (suncl.cl:164-347);
+        QCD_START_HOT draws Haar-distributed links from the per-site streams
+        (the reference's lattice_init_hot_* is not Haar, su2_update_cl.cl:110) */
+int  qcd_lattice_init(qcd_lattice *lat, int start);
+
+/* ---- PRNG (PRNG::initialize, random.cpp:216-277): seeds the per-site
+        streams through glibc srand(RANDSERIES)/rand() in the reference order
+        and performs the initial produce().                                  */
+int  qcd_prng_init(qcd_lattice *lat);
+
+/* ---- host copies (buffer_init / buffer_map, clinterface.cpp:1316-1433) */
+int  qcd_lattice_import(qcd_lattice *lat, const double *links);
+int  qcd_lattice_export(qcd_lattice *lat, double *links);   /* synchronous */
+
+/* ---- updates: update_{even,odd}_{X,Y,Z,T} (suncl.cl:399-975) with the PRNG
+        produce fused in; parity 0 = even, 1 = odd; mu = 0..3 (X,Y,Z,T).     */
+int  qcd_halfstep(qcd_lattice *lat, int parity, int mu, int kind);
+/* One sweep = odd X,Y,Z,T then even X,Y,Z,T (suncl.cpp:5392-5409). */
+int  qcd_sweep(qcd_lattice *lat, int kind);
+/* Whole-lattice reunitarisation: lattice_GramSchmidt (suncl.cl:350-396),
+   in FP64 for both link precisions.                                         */
+int  qcd_reunitarize(qcd_lattice *lat);
+
+/* ---- measurements: lattice_measurement_plq + reduce_measurement_plq_double2
+        (sun_measurements_cl.cl:1178-1323), lattice_polyakov +
+        reduce_polyakov_double2 (polyakov.cl:54-431).  Each call appends one
+        record on the device; sums are FP64 and unnormalised:
+          plaquette: (sum ReTr spatial, sum ReTr temporal)
+          polyakov : (sum ReTr P, sum ImTr P, sum |Tr P|^2, sum |Tr P|^4)   */
+int  qcd_measure(qcd_lattice *lat, int mask);
+/* Copies the records so far (synchronous).  plq: 2*count, poly: 4*count.  */
+int  qcd_read_measurements(qcd_lattice *lat, double *plq, double *poly, int *count_plq, int *count_poly);
+int  qcd_reset_measurements(qcd_lattice *lat);
+
+/* ---- graphs: one captured cycle of n_hb heatbath sweeps, n_or
+        overrelaxation sweeps and the measurements in `mask`, replayed with
+        qcd_cycle_launch.  Replaces the synchronous per-kernel loop of
+        model::lattice_simulate (suncl.cpp:5391-5483).                       */
+int  qcd_cycle_capture(qcd_lattice *lat, int n_hb, int n_or, int mask);
+/* The same cycle enqueued eagerly (no graph).  In both forms the plaquette
+   of the configuration left by the last sweep is accumulated inside that
+   sweep's even Y, Z and T half-steps (each plaquette of those planes holds
+   exactly one even-site link of the later direction), so no separate
+   plaquette kernel runs; the Polyakov loop is its own kernel.              */
+int  qcd_cycle_run(qcd_lattice *lat, int n_hb, int n_or, int mask);
+int  qcd_cycle_launch(qcd_lattice *lat, int repeats);
+/* Wall time of `repeats` graph launches, bracketed by HIP events on the
+   lattice stream (synchronous).                                             */
+int  qcd_cycle_time(qcd_lattice *lat, int repeats, double *ms);
+/* Runs `repeats` cycles eagerly with HIP events around every update kernel
+   and reports the mean kernel duration per kind (us) and the total (ms).   */
+int  qcd_cycle_profile(qcd_lattice *lat, int repeats, double *hb_us, double *or_us,
+                       double *meas_us, double *total_ms);
+
+/* ---- reference PRN buffer semantics (PRNG::produce, random.cpp:278-284;
+        kernels xor128 prngcl_xor128.cl:100, pm prngcl_pm.cl:86,
+        ranlux prngcl_ranlux.cl:534).  Host seed tables in, quads out:
+        quads[(i*n + g)*4 + c] for instance g, sample i.  Tables advance.    */
+int  qcd_prng_produce_host(int prng, int ranlux_luxury, unsigned n_instances, unsigned samples,
+                           void *table, float *quads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

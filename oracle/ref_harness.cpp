@@ -2,9 +2,9 @@
 //
 // TEST INFRASTRUCTURE ONLY.  This file is compiled together with the
 // reference's own sources, where they lie under the reference tree, by
-// oracle/build_ref.sh into oracle/_ref/ref_harness.  It is run only in the
-// build container to (re)generate the fixtures under tests/golden/; nothing
-// on the product path links it.
+// oracle/build_ref.sh into oracle/_ref/ref_harness.  It (re)generates the
+// fixtures under tests/golden/ and, through its "bench" mode, is the timed
+// CPU baseline of bench.py; nothing on the product path links it.
 //
 // What it drives (reference file:line):
 //   PRNG::initialize_CPU / produce_CPU   random/random.cpp:174-213, 301-316
@@ -22,6 +22,10 @@
 //       ints: 1 = cold, 2 = gid.  Writes <out_prefix>_s<k>.bin (links as
 //       float64, [site][dir][row][col][re,im]) after each sweep k (k=0 is the
 //       start) and <out_prefix>_plq.txt (sweep, spat, temp, total).
+//   ref_harness bench <sun> <n1> <n2> <n3> <n4> <beta> <nhit> <GEN>
+//                     <randseries> <nsweeps>
+//       Cold start, then times <nsweeps> heatbath sweeps of suncpp on one
+//       thread; prints "sweeps seconds link_updates_per_second plaquette".
 
 #include "suncpp/Update/sun_update.h"
 #include "suncpp/Measurements/Plq.h"
@@ -29,6 +33,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <ctime>
 
 static PRNG_CL::PRNG::PRNG_generators gen_from_name(const char* s) {
     using P = PRNG_CL::PRNG;
@@ -93,7 +98,48 @@ static int run_sweeps(int argc, char** argv) {
     return 0;
 }
 
+template <typename su_n>
+static int run_bench(char** argv) {
+    modelCPU<su_n> m;
+    m.lattice_ndCPU = 4;
+    m.lattice_size = (int*)calloc(4, sizeof(int));
+    for (int i = 0; i < 4; ++i) m.lattice_size[i] = atoi(argv[3 + i]);
+    m.beta = atof(argv[7]);
+    m.nhit = atoi(argv[8]);
+    m.ints = 1;
+    m.lattice_group = atoi(argv[2]);
+    m.lattice_sitesCPU = 1;
+    for (int i = 0; i < 4; ++i) m.lattice_sitesCPU *= m.lattice_size[i];
+    PRNG_CL::PRNG prng;
+    prng.PRNG_generator = gen_from_name(argv[9]);
+    prng.PRNG_randseries = atoi(argv[10]);
+    prng.PRNG_precision = PRNG_CL::PRNG::PRNG_precision_single;
+    prng.initialize_CPU();
+    int nsweeps = atoi(argv[11]);
+    m.create_latticeCPU();
+    m.lattice_initializeCPU();
+    timespec t0, t1;
+    clock_gettime(CLOCK_MONOTONIC, &t0);
+    for (int k = 0; k < nsweeps; ++k) {
+        for (int dir = 0; dir < 4; ++dir) lattice_update_odd(&m, dir, &prng);
+        for (int dir = 0; dir < 4; ++dir) lattice_update_even(&m, dir, &prng);
+    }
+    clock_gettime(CLOCK_MONOTONIC, &t1);
+    double sec = (t1.tv_sec - t0.tv_sec) + 1e-9 * (t1.tv_nsec - t0.tv_nsec);
+    hgpu_double total = 0.0;
+    plqConf(&m, &total);
+    printf("%d %.6f %.6e %.12f\n", nsweeps, sec, 4.0 * m.lattice_sitesCPU * nsweeps / sec, total);
+    m.delete_latticeCPU();
+    free(m.lattice_size);
+    return 0;
+}
+
 int main(int argc, char** argv) {
+    if (argc >= 12 && !strcmp(argv[1], "bench")) {
+        int sun = atoi(argv[2]);
+        if (sun == 2) return run_bench<su_2>(argv);
+        if (sun == 3) return run_bench<su_3>(argv);
+    }
     if (argc >= 6 && !strcmp(argv[1], "prng")) {
         PRNG_CL::PRNG prng;
         prng.PRNG_generator = gen_from_name(argv[2]);

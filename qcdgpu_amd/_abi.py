@@ -1,0 +1,117 @@
+"""ctypes binding of libqcdhip.so (declarations in include/qcd_hip.h).
+
+The library is the product: this module only loads it and declares argument
+types.  It raises if the library is missing -- there is no CPU fallback.
+"""
+import ctypes
+import os
+import re
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(PKG)
+LIB_PATH = os.path.join(PKG, "libqcdhip.so")
+HEADER = os.path.join(REPO, "include", "qcd_hip.h")
+
+QCD_OK = 0
+FP32, FP64 = 0, 1
+START_HOT, START_COLD, START_GID = 0, 1, 2
+PRNG_XOR128, PRNG_RANLUX, PRNG_PM = 1, 2, 3
+STREAM_REFERENCE, STREAM_FAST = 0, 1
+HEATBATH, OVERRELAX = 0, 1
+MEAS_PLAQUETTE, MEAS_POLYAKOV = 1, 2
+TUNE_XCD_MAP, TUNE_BLOCKS_PER_CU = 1, 2
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("sun", ctypes.c_int),
+        ("n", ctypes.c_int * 4),
+        ("precision", ctypes.c_int),
+        ("storage_reals", ctypes.c_int),
+        ("beta", ctypes.c_double),
+        ("nhit", ctypes.c_int),
+        ("nhitpar", ctypes.c_int),
+        ("prng", ctypes.c_int),
+        ("ranlux_luxury", ctypes.c_int),
+        ("stream_mode", ctypes.c_int),
+        ("randseries", ctypes.c_uint),
+        ("max_measurements", ctypes.c_int),
+    ]
+
+
+class QCDError(RuntimeError):
+    pass
+
+
+_lib = None
+_P = ctypes.c_void_p
+_D = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.POINTER(ctypes.c_int)
+
+_SIGNATURES = {
+    "qcd_device_open": [ctypes.c_int, ctypes.c_int],
+    "qcd_device_close": [],
+    "qcd_device_info": [ctypes.c_char_p, ctypes.c_int, _I, ctypes.POINTER(ctypes.c_size_t)],
+    "qcd_abi_version": [],
+    "qcd_lattice_create": [ctypes.POINTER(Params), ctypes.POINTER(_P)],
+    "qcd_lattice_destroy": [_P],
+    "qcd_lattice_sync": [_P],
+    "qcd_lattice_table_bytes": [_P, ctypes.POINTER(ctypes.c_size_t)],
+    "qcd_lattice_tune": [_P, ctypes.c_int, ctypes.c_int],
+    "qcd_lattice_init": [_P, ctypes.c_int],
+    "qcd_prng_init": [_P],
+    "qcd_lattice_import": [_P, _D],
+    "qcd_lattice_export": [_P, _D],
+    "qcd_halfstep": [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int],
+    "qcd_sweep": [_P, ctypes.c_int],
+    "qcd_reunitarize": [_P],
+    "qcd_measure": [_P, ctypes.c_int],
+    "qcd_read_measurements": [_P, _D, _D, _I, _I],
+    "qcd_reset_measurements": [_P],
+    "qcd_cycle_capture": [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int],
+    "qcd_cycle_run": [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int],
+    "qcd_cycle_launch": [_P, ctypes.c_int],
+    "qcd_cycle_time": [_P, ctypes.c_int, _D],
+    "qcd_cycle_profile": [_P, ctypes.c_int, _D, _D, _D, _D],
+    "qcd_prng_produce_host": [
+        ctypes.c_int,
+        ctypes.c_int,
+        ctypes.c_uint,
+        ctypes.c_uint,
+        ctypes.c_void_p,
+        ctypes.POINTER(ctypes.c_float),
+    ],
+}
+
+
+def header_symbols():
+    """Function names declared in include/qcd_hip.h."""
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(qcd_[a-z0-9_]+)\s*\(", text)))
+
+
+def lib():
+    """Load libqcdhip.so; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise QCDError(
+            "libqcdhip.so not found at %s: build it with __graft_entry__.build() "
+            "(there is no CPU fallback)" % LIB_PATH
+        )
+    L = ctypes.CDLL(LIB_PATH)
+    for name, args in _SIGNATURES.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = ctypes.c_int
+    L.qcd_last_error.restype = ctypes.c_char_p
+    L.qcd_last_error.argtypes = []
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != QCD_OK:
+        raise QCDError("qcd error %d: %s" % (rc, lib().qcd_last_error().decode()))
+    return rc

@@ -1,0 +1,439 @@
+// qcd_device.hpp -- device-side building blocks for the SU(N) update and
+// measurement kernels on gfx950: complex/SU(N) algebra in registers, the
+// checkerboard-compacted link layout, and the per-stream generators.
+//
+// Link layout in HBM (one table per lattice):
+//   plane-major, checkerboard-compacted structure of arrays.  For parity p,
+//   direction mu and complex component k (row-major entry of the N x N
+//   matrix; 9 for SU(3) 18-real, 6 for SU(3) 12-real, 2 for SU(2)), the
+//   values of all Vh = V/2 sites of that parity are contiguous:
+//       table[((p * 4 + mu) * NPLANE + k) * vh_pad + i]      (float2/double2)
+//   i is the compact index of site s: i = s >> 1 (y is the fastest
+//   coordinate and N2 is even, so each pair (2i, 2i+1) holds one site of each
+//   parity).  One wavefront reads 64 consecutive complex values per plane:
+//   512 B (FP32) or 1 KiB (FP64) fully coalesced.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace qcd {
+
+// ------------------------------------------------------------------ complex
+template <typename R> struct C {
+    R re, im;
+};
+template <typename R> __device__ __forceinline__ C<R> cmk(R a, R b) { return C<R>{a, b}; }
+template <typename R> __device__ __forceinline__ C<R> cadd(C<R> a, C<R> b) { return {a.re + b.re, a.im + b.im}; }
+template <typename R> __device__ __forceinline__ C<R> csub(C<R> a, C<R> b) { return {a.re - b.re, a.im - b.im}; }
+template <typename R> __device__ __forceinline__ C<R> cmul(C<R> a, C<R> b)
+{
+    return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+// a * conj(b)
+template <typename R> __device__ __forceinline__ C<R> cmulc(C<R> a, C<R> b)
+{
+    return {a.re * b.re + a.im * b.im, a.im * b.re - a.re * b.im};
+}
+// conj(a) * b
+template <typename R> __device__ __forceinline__ C<R> ccmul(C<R> a, C<R> b)
+{
+    return {a.re * b.re + a.im * b.im, a.re * b.im - a.im * b.re};
+}
+template <typename R> __device__ __forceinline__ C<R> cconj(C<R> a) { return {a.re, -a.im}; }
+
+// ------------------------------------------------------------------ matrices
+template <typename R, int N> struct M {
+    C<R> e[N][N];
+};
+
+template <typename R, int N> __device__ __forceinline__ void m_zero(M<R, N>& a)
+{
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+        for (int j = 0; j < N; ++j) a.e[i][j] = {R(0), R(0)};
+}
+
+template <typename R, int N> __device__ __forceinline__ void m_unit(M<R, N>& a)
+{
+    m_zero(a);
+#pragma unroll
+    for (int i = 0; i < N; ++i) a.e[i][i].re = R(1);
+}
+
+// Complex multiply-accumulate as four FMAs: s += a * b, s += a * conj(b),
+// s += conj(a) * b.  The matrix products below accumulate entry by entry
+// with these, so a 3x3 product is 27 x 4 FMAs and no separate adds.
+template <typename R> __device__ __forceinline__ void cfma(C<R>& s, C<R> a, C<R> b)
+{
+    s.re = fma(a.re, b.re, s.re);
+    s.re = fma(-a.im, b.im, s.re);
+    s.im = fma(a.re, b.im, s.im);
+    s.im = fma(a.im, b.re, s.im);
+}
+template <typename R> __device__ __forceinline__ void cfmac(C<R>& s, C<R> a, C<R> b)
+{
+    s.re = fma(a.re, b.re, s.re);
+    s.re = fma(a.im, b.im, s.re);
+    s.im = fma(a.im, b.re, s.im);
+    s.im = fma(-a.re, b.im, s.im);
+}
+template <typename R> __device__ __forceinline__ void ccfma(C<R>& s, C<R> a, C<R> b)
+{
+    s.re = fma(a.re, b.re, s.re);
+    s.re = fma(a.im, b.im, s.re);
+    s.im = fma(a.re, b.im, s.im);
+    s.im = fma(-a.im, b.re, s.im);
+}
+
+// c = a * b
+template <typename R, int N> __device__ __forceinline__ M<R, N> m_mul(const M<R, N>& a, const M<R, N>& b)
+{
+    M<R, N> c;
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            C<R> s = cmul(a.e[i][0], b.e[0][j]);
+#pragma unroll
+            for (int k = 1; k < N; ++k) cfma(s, a.e[i][k], b.e[k][j]);
+            c.e[i][j] = s;
+        }
+    return c;
+}
+
+// c = a * b^dagger
+template <typename R, int N> __device__ __forceinline__ M<R, N> m_mul_nd(const M<R, N>& a, const M<R, N>& b)
+{
+    M<R, N> c;
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            C<R> s = cmulc(a.e[i][0], b.e[j][0]);
+#pragma unroll
+            for (int k = 1; k < N; ++k) cfmac(s, a.e[i][k], b.e[j][k]);
+            c.e[i][j] = s;
+        }
+    return c;
+}
+
+// c = a^dagger * b
+template <typename R, int N> __device__ __forceinline__ M<R, N> m_mul_dn(const M<R, N>& a, const M<R, N>& b)
+{
+    M<R, N> c;
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            C<R> s = ccmul(a.e[0][i], b.e[0][j]);
+#pragma unroll
+            for (int k = 1; k < N; ++k) ccfma(s, a.e[k][i], b.e[k][j]);
+            c.e[i][j] = s;
+        }
+    return c;
+}
+
+// acc += a * b^dagger
+template <typename R, int N> __device__ __forceinline__ void m_acc_nd(M<R, N>& acc, const M<R, N>& a, const M<R, N>& b)
+{
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+        for (int j = 0; j < N; ++j)
+#pragma unroll
+            for (int k = 0; k < N; ++k) cfmac(acc.e[i][j], a.e[i][k], b.e[j][k]);
+}
+
+// acc += a^dagger * b
+template <typename R, int N> __device__ __forceinline__ void m_acc_dn(M<R, N>& acc, const M<R, N>& a, const M<R, N>& b)
+{
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+        for (int j = 0; j < N; ++j)
+#pragma unroll
+            for (int k = 0; k < N; ++k) ccfma(acc.e[i][j], a.e[k][i], b.e[k][j]);
+}
+
+// Re Tr(a * b) accumulated in double.
+template <typename R, int N> __device__ __forceinline__ double m_retr_mul(const M<R, N>& a, const M<R, N>& b)
+{
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            s = fma((double)a.e[i][k].re, (double)b.e[k][i].re, s);
+            s = fma(-(double)a.e[i][k].im, (double)b.e[k][i].im, s);
+        }
+    return s;
+}
+
+template <typename R, int N> __device__ __forceinline__ void m_acc(M<R, N>& a, const M<R, N>& b)
+{
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+        for (int j = 0; j < N; ++j) a.e[i][j] = cadd(a.e[i][j], b.e[i][j]);
+}
+
+// Re Tr(a * b^dagger) = sum_ij Re(a_ij conj(b_ij)), accumulated in double.
+template <typename R, int N> __device__ __forceinline__ double m_retr_nd(const M<R, N>& a, const M<R, N>& b)
+{
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+        for (int j = 0; j < N; ++j)
+            s += (double)a.e[i][j].re * (double)b.e[i][j].re + (double)a.e[i][j].im * (double)b.e[i][j].im;
+    return s;
+}
+
+template <typename Rout, typename Rin, int N> __device__ __forceinline__ M<Rout, N> m_cast(const M<Rin, N>& a)
+{
+    M<Rout, N> b;
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+        for (int j = 0; j < N; ++j) b.e[i][j] = {(Rout)a.e[i][j].re, (Rout)a.e[i][j].im};
+    return b;
+}
+
+// Row 3 of an SU(3) matrix from rows 1 and 2: conj(row1 x row2).
+template <typename R> __device__ __forceinline__ void su3_row3(M<R, 3>& a)
+{
+    a.e[2][0] = cconj(csub(cmul(a.e[0][1], a.e[1][2]), cmul(a.e[0][2], a.e[1][1])));
+    a.e[2][1] = cconj(csub(cmul(a.e[0][2], a.e[1][0]), cmul(a.e[0][0], a.e[1][2])));
+    a.e[2][2] = cconj(csub(cmul(a.e[0][0], a.e[1][1]), cmul(a.e[0][1], a.e[1][0])));
+}
+
+// Gram-Schmidt reunitarisation in FP64 (rows 1-2 orthonormalised, row 3 =
+// conj(row1 x row2)), the algorithm of suncpp/su3/algebra_su3.cpp:185-226.
+__device__ __forceinline__ void su3_gram_schmidt(M<double, 3>& a)
+{
+    double nu = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) nu += a.e[0][k].re * a.e[0][k].re + a.e[0][k].im * a.e[0][k].im;
+    double inu = 1.0 / sqrt(nu);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) a.e[0][k] = {a.e[0][k].re * inu, a.e[0][k].im * inu};
+    C<double> sp = cmulc(a.e[1][0], a.e[0][0]);
+#pragma unroll
+    for (int k = 1; k < 3; ++k) sp = cadd(sp, cmulc(a.e[1][k], a.e[0][k]));
+#pragma unroll
+    for (int k = 0; k < 3; ++k) a.e[1][k] = csub(a.e[1][k], cmul(a.e[0][k], sp));
+    double nv = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) nv += a.e[1][k].re * a.e[1][k].re + a.e[1][k].im * a.e[1][k].im;
+    double inv = 1.0 / sqrt(nv);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) a.e[1][k] = {a.e[1][k].re * inv, a.e[1][k].im * inv};
+    su3_row3(a);
+}
+
+// SU(2) normalisation of the quaternion (Re u11, Im u11, Re u12, Im u12).
+__device__ __forceinline__ void su2_normalize(M<double, 2>& a)
+{
+    double n = a.e[0][0].re * a.e[0][0].re + a.e[0][0].im * a.e[0][0].im + a.e[0][1].re * a.e[0][1].re +
+               a.e[0][1].im * a.e[0][1].im;
+    double in = 1.0 / sqrt(n);
+    a.e[0][0] = {a.e[0][0].re * in, a.e[0][0].im * in};
+    a.e[0][1] = {a.e[0][1].re * in, a.e[0][1].im * in};
+    a.e[1][0] = {-a.e[0][1].re, a.e[0][1].im};
+    a.e[1][1] = {a.e[0][0].re, -a.e[0][0].im};
+}
+
+// ------------------------------------------------------------------ layout
+template <typename R> struct Vec2;
+template <> struct Vec2<float> {
+    using T = float2;
+};
+template <> struct Vec2<double> {
+    using T = double2;
+};
+
+// Number of complex planes stored per link.
+template <int N, int NREAL> struct Planes {
+    static constexpr int value = NREAL / 2;
+};
+
+struct Geom {
+    int n1, n2, n3, n4;   // extents along x, y, z, t
+    int h;                // N2 / 2
+    uint32_t vh;          // V / 2
+    uint32_t vh_pad;      // plane stride (multiple of 64)
+};
+
+struct Site {
+    int x, y, z, t;
+};
+
+// Coordinates of the site of parity p with compact index i (y fastest).
+__device__ __forceinline__ Site site_of(const Geom& g, uint32_t i, int p)
+{
+    Site s;
+    uint32_t yh = i % (uint32_t)g.h;
+    uint32_t r = i / (uint32_t)g.h;
+    s.z = (int)(r % (uint32_t)g.n3);
+    r /= (uint32_t)g.n3;
+    s.t = (int)(r % (uint32_t)g.n4);
+    s.x = (int)(r / (uint32_t)g.n4);
+    s.y = 2 * (int)yh + ((s.x + s.z + s.t + p) & 1);
+    return s;
+}
+
+__device__ __forceinline__ uint32_t cindex(const Geom& g, int x, int y, int z, int t)
+{
+    return (uint32_t)(y >> 1) + (uint32_t)g.h * ((uint32_t)z + (uint32_t)g.n3 * ((uint32_t)t + (uint32_t)g.n4 * (uint32_t)x));
+}
+
+__device__ __forceinline__ int wrap(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
+
+// Compact index of s + sign * e_dir, and of s + s1 * e_d1 + s2 * e_d2.
+// Directions are selected with branches (no dynamically indexed arrays, so
+// nothing spills to scratch when dir is a runtime value).
+__device__ __forceinline__ void shift(const Geom& g, Site& s, int dir, int sign)
+{
+    if (dir == 0) s.x = wrap(s.x + sign, g.n1);
+    else if (dir == 1) s.y = wrap(s.y + sign, g.n2);
+    else if (dir == 2) s.z = wrap(s.z + sign, g.n3);
+    else s.t = wrap(s.t + sign, g.n4);
+}
+
+__device__ __forceinline__ uint32_t nb_index(const Geom& g, Site s, int dir, int sign)
+{
+    shift(g, s, dir, sign);
+    return cindex(g, s.x, s.y, s.z, s.t);
+}
+
+__device__ __forceinline__ uint32_t nb2_index(const Geom& g, Site s, int d1, int s1, int d2, int s2)
+{
+    shift(g, s, d1, s1);
+    shift(g, s, d2, s2);
+    return cindex(g, s.x, s.y, s.z, s.t);
+}
+
+template <typename R, int N, int NREAL> struct Table {
+    using V2 = typename Vec2<R>::T;
+    static constexpr int NP = Planes<N, NREAL>::value;
+    V2* __restrict__ base;
+    uint32_t vh_pad;
+
+    // Plane k of (parity, mu): uniform across the wavefront whenever p and mu
+    // are, so loads use a scalar base and a 32-bit per-lane byte offset.
+    __device__ __forceinline__ const V2* plane(int p, int mu, int k) const
+    {
+        return base + (size_t)((p * 4 + mu) * NP + k) * vh_pad;
+    }
+    // The plane base goes through readfirstlane so the compiler keeps it in
+    // SGPRs as an opaque value and selects the global_load saddr form
+    // (scalar base + 32-bit lane offset) instead of a 64-bit add per load.
+    using gchar = __attribute__((address_space(1))) char;
+    __device__ __forceinline__ static gchar* sbase(const V2* pl)
+    {
+        const uint64_t b = (uint64_t)pl;
+        const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)b);
+        const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(b >> 32));
+        return (gchar*)(((uint64_t)hi << 32) | lo);
+    }
+    __device__ __forceinline__ V2 ld(const V2* pl, uint32_t i) const
+    {
+        using gR = __attribute__((address_space(1))) const R;
+        const uint32_t off = i * (uint32_t)sizeof(V2);
+        const gR* q = (const gR*)(sbase(pl) + off);
+        V2 v;
+        v.x = q[0];
+        v.y = q[1];
+        return v;
+    }
+
+    // Generic load: p and mu may differ between the lanes of a wavefront.
+    __device__ __forceinline__ V2 ld_any(const V2* pl, uint32_t i) const { return pl[i]; }
+
+    template <bool UNIFORM>
+    __device__ __forceinline__ M<R, N> load_t(int p, int mu, uint32_t i) const
+    {
+        M<R, N> a;
+        if constexpr (N == 3) {
+#pragma unroll
+            for (int k = 0; k < NP; ++k) {
+                V2 v = UNIFORM ? ld(plane(p, mu, k), i) : ld_any(plane(p, mu, k), i);
+                a.e[k / 3][k % 3] = {v.x, v.y};
+            }
+            if constexpr (NP == 6) su3_row3(a);
+        } else {
+            V2 v0 = UNIFORM ? ld(plane(p, mu, 0), i) : ld_any(plane(p, mu, 0), i);
+            V2 v1 = UNIFORM ? ld(plane(p, mu, 1), i) : ld_any(plane(p, mu, 1), i);
+            a.e[0][0] = {v0.x, v0.y};
+            a.e[0][1] = {v1.x, v1.y};
+            a.e[1][0] = {-v1.x, v1.y};
+            a.e[1][1] = {v0.x, -v0.y};
+        }
+        return a;
+    }
+    // Load with wavefront-uniform parity p and direction mu (scalar plane
+    // base).  PRECONDITION: p and mu are the same in every active lane.
+    __device__ __forceinline__ M<R, N> load_u(int p, int mu, uint32_t i) const { return load_t<true>(p, mu, i); }
+    // Load where p or mu may vary across lanes.
+    __device__ __forceinline__ M<R, N> load(int p, int mu, uint32_t i) const { return load_t<false>(p, mu, i); }
+
+    // Store with wavefront-uniform p and mu (same precondition as load_u).
+    __device__ __forceinline__ void store_u(int p, int mu, uint32_t i, const M<R, N>& a) const
+    {
+        const uint32_t off = i * (uint32_t)sizeof(V2);
+#pragma unroll
+        for (int k = 0; k < NP; ++k) {
+            V2 v;
+            v.x = a.e[k / N][k % N].re;
+            v.y = a.e[k / N][k % N].im;
+            using gRw = __attribute__((address_space(1))) R;
+            gRw* q = (gRw*)(sbase(plane(p, mu, k)) + off);
+            q[0] = v.x;
+            q[1] = v.y;
+        }
+    }
+};
+
+// ------------------------------------------------------------------ generators
+// XOR128 (Marsaglia 2003), one step as prngcl_xor128.cl:62-74.
+__device__ __forceinline__ uint32_t xor128_step(uint4& s)
+{
+    uint32_t t = s.x ^ (s.x << 11);
+    s.x = s.y;
+    s.y = s.z;
+    s.z = s.w;
+    s.w = (s.w ^ (s.w >> 19)) ^ (t ^ (t >> 8));
+    return s.w;
+}
+
+__device__ __forceinline__ float4 xor128_quad(uint4& s)
+{
+    float4 q;
+    q.x = (float)xor128_step(s) * 2.3283064365386962890625e-10f;   // / 2^32, exact scaling
+    q.y = (float)xor128_step(s) * 2.3283064365386962890625e-10f;
+    q.z = (float)xor128_step(s) * 2.3283064365386962890625e-10f;
+    q.w = (float)xor128_step(s) * 2.3283064365386962890625e-10f;
+    return q;
+}
+
+// Park-Miller on four independent lanes (prngcl_pm.cl:68-84); output is an
+// exact scaling by 2^-31 (the reference divides by (float)2147483647.0).
+__device__ __forceinline__ uint32_t pm_step(uint32_t s)
+{
+    uint32_t t1 = 16807u * (s % 127773u);
+    uint32_t t2 = 2836u * (s / 127773u);
+    return (t1 > t2) ? t1 - t2 : t1 - t2 + 2147483647u;
+}
+
+__device__ __forceinline__ float4 pm_quad(uint4& s)
+{
+    s.x = pm_step(s.x);
+    s.y = pm_step(s.y);
+    s.z = pm_step(s.z);
+    s.w = pm_step(s.w);
+    const float k = 4.656612873077392578125e-10f;   // 2^-31
+    return make_float4((float)s.x * k, (float)s.y * k, (float)s.z * k, (float)s.w * k);
+}
+
+}  // namespace qcd

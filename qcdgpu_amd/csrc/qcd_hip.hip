@@ -1,0 +1,1514 @@
+// qcd_hip.hip -- gfx950 kernels and the C ABI (include/qcd_hip.h) for the
+// SU(N) heatbath / overrelaxation update and the plaquette / Polyakov
+// measurements of QCDGPU.
+//
+// Reference path replaced (paths relative to the reference's SU(N)/ tree):
+//   update_{even,odd}_{X,Y,Z,T}   suncl/suncl.cl:399-975
+//     lattice_staple_3/_2         suncl/su3_update_cl.cl:341-565, su2_update_cl.cl:206-431
+//     lattice_heatbath2/3         suncl/su3_update_cl.cl:568-779, su2_update_cl.cl:434-550
+//   PRNG produce (xor128, pm)     random/prngcl_xor128.cl:100, prngcl_pm.cl:86
+//   lattice_init_cold / _gid      suncl/suncl.cl:164-347
+//   lattice_GramSchmidt           suncl/suncl.cl:350-396
+//   lattice_measurement_plq       suncl/sun_measurements_cl.cl:1178-1323
+//   lattice_polyakov (+ reduce)   suncl/polyakov.cl:54-141, 405-431
+//
+// What is different by design:
+//   * one thread per link of the (parity, mu) half-step reads a
+//     checkerboard-compacted plane-major table (qcd_device.hpp), so every
+//     neighbour load of a wavefront is a contiguous run;
+//   * the PRN buffer is gone: each thread owns one generator stream and draws
+//     in registers; in reference stream mode it then skips to the next
+//     multiple of S quads, which reproduces the buffer the reference kernel
+//     would have read (instance i = thread i, PRNG_instances = PRNGSTEP);
+//   * reunitarisation is fused into every link update and done in FP64;
+//   * measurements reduce in FP64 with a deterministic two-level tree (wave
+//     shuffles, LDS, then the last workgroup sums the per-workgroup partials in
+//     index order), appending one record per call on the device;
+//   * a whole measurement cycle is captured once into a hipGraph.
+#include <hip/hip_ext.h>
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <ctime>
+#include <vector>
+
+#include "../../include/qcd_hip.h"
+#include "qcd_device.hpp"
+
+using namespace qcd;
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr double kPi2 = 6.2831853071795864769252867665590;
+
+enum RngKind { RK_NONE = 0, RK_GID = 1, RK_XOR128 = 2, RK_PM = 3 };
+
+struct UpdArgs {
+    void* table;
+    Geom g;
+    int parity;
+    int mu;
+    double beta;          // BETA / N
+    int nhit;
+    uint32_t sites;       // V, for GID mode
+    uint4* rng;           // per-stream state (stream = compact index)
+    uint32_t s_quads;     // quads per heatbath half-step per stream (0 = fast)
+    // fused plaquette measurement (last sweep of a measurement cycle):
+    // partials[(meas_mode - 1) * gridDim.x + block] = sum over the block of
+    // ReTr(U' * sum_{nu<mu} S_nu); modes 1, 2, 3 = even Y, Z, T half-steps.
+    double* partials;
+    int meas_mode;
+    int xcd_map;          // 1: contiguous block range per XCD, 0: hardware order
+};
+
+// Hardware dispatches workgroup b to XCD (b mod 8).  Remap so that each XCD
+// works on one contiguous range of compact indices (an x-slab of the
+// lattice): z/t/x neighbours then mostly hit the same XCD's L2.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb)
+{
+    if (nb % 8u) return b;
+    return (b % 8u) * (nb / 8u) + (b / 8u);
+}
+
+// ---------------------------------------------------------------- RNG sources
+struct SrcGid {
+    double gid, fs;
+    int nhit;
+    __device__ float4 try_quad()
+    {
+        // suncpp/su2/update_su2.cpp:71-76: the four numbers of a try, rounded
+        // to float as suncpp does (the OpenCL FP64 build keeps them double).
+        double c = 0.005 * (1.0 + nhit);
+        return make_float4(fabsf((float)sin((c + 270.0 / fs) * gid)), fabsf((float)cos((c + 60.0 / fs) * gid)),
+                           fabsf((float)sin((c - 150.0 / fs) * gid)), fabsf((float)cos((c - 380.0 / fs) * gid)));
+    }
+    __device__ float4 accept_quad()
+    {
+        return make_float4(fabsf((float)cos((0.08 - 270.0 / fs) * gid)), fabsf((float)sin((0.08 - 60.0 / fs) * gid)),
+                           0.f, 0.f);
+    }
+    __device__ void skip() {}
+};
+
+struct SrcXor {
+    uint4 s;
+    uint32_t used;
+    __device__ float4 try_quad()
+    {
+        used++;
+        return xor128_quad(s);
+    }
+    __device__ float4 accept_quad() { return try_quad(); }
+    __device__ void skip()
+    {
+        used++;
+        (void)xor128_quad(s);
+    }
+    __device__ void finish(uint32_t s_quads)
+    {
+        for (uint32_t k = used; k < s_quads; ++k) {
+            (void)xor128_step(s);
+            (void)xor128_step(s);
+            (void)xor128_step(s);
+            (void)xor128_step(s);
+        }
+    }
+};
+
+struct SrcPm {
+    uint4 s;
+    uint32_t used;
+    __device__ float4 try_quad()
+    {
+        used++;
+        return pm_quad(s);
+    }
+    __device__ float4 accept_quad() { return try_quad(); }
+    __device__ void skip()
+    {
+        used++;
+        (void)pm_quad(s);
+    }
+    __device__ void finish(uint32_t s_quads)
+    {
+        for (uint32_t k = used; k < s_quads; ++k) (void)pm_quad(s);
+    }
+};
+
+// ---------------------------------------------------------------- KP / OR
+// b: 2x2 complex block (b00, b01, b10, b11) as 8 values of the compute type
+// T (float for FP32 links, double for FP64 links).  out = c * aH with
+// aH = (quaternion part of b / its norm)^dagger and c drawn from the
+// Kennedy-Pendleton density (su3_update_cl.cl:568-669 with the argument
+// order of suncpp/su2/update_su2.cpp:40-126), or c = aH for the
+// overrelaxation reflection.  Returns false if all NHIT tries were rejected.
+template <typename T> __device__ __forceinline__ void sincos_t(T x, T* s, T* c)
+{
+    if constexpr (sizeof(T) == 4) sincosf(x, s, c);
+    else sincos(x, s, c);
+}
+
+template <typename T, bool OVER, int RK, typename Src>
+__device__ __forceinline__ bool kp_update(const T* b, T beta, int nhit, Src& src, T* out)
+{
+    const T Mx = b[0] + b[6];
+    const T My = b[1] - b[7];
+    const T Mz = b[2] - b[4];
+    const T Mw = b[3] + b[5];
+    const T det = sqrt(Mx * Mx + My * My + Mz * Mz + Mw * Mw);
+    const T id = T(1) / det;
+    const T a00r = Mx * id, a00i = -My * id, a01r = -Mz * id, a01i = -Mw * id;
+    // aH = [[a00, a01], [-conj(a01), conj(a00)]]
+    T c00r, c00i, c01r, c01i;
+    if constexpr (OVER) {
+        c00r = a00r;
+        c00i = a00i;
+        c01r = a01r;
+        c01i = a01i;
+    } else {
+        const T pi2 = (T)kPi2;
+        const T bdet = beta * det;
+        bool flag = false;
+        T delta = T(0);
+        for (int i = 0; i < nhit && !flag; ++i) {
+            float4 r = src.try_quad();
+            T cosrnd = cos(pi2 * (T)r.y);
+            if constexpr (RK == RK_GID) {
+                delta = cosrnd * cosrnd / bdet;
+                flag = true;
+            } else {
+                delta = -(log(T(1) - (T)r.x) + cosrnd * cosrnd * log(T(1) - (T)r.z)) / bdet;
+                float w2 = r.w * r.w;
+                flag = ((T)w2 <= (T(1) - T(0.5) * delta));
+            }
+        }
+        if (!flag) {
+            src.skip();   // su3_update_cl.cl:665-668: an all-rejected update skips one quad
+            return false;
+        }
+        float4 r = src.accept_quad();
+        // sqrt(1 - c^2) near |c| = 1 loses the digits a float cannot carry:
+        // these four are formed in double for both link precisions (the
+        // reference's FP32 kernel does the same through its double literals,
+        // su3_update_cl.cl:650-654).
+        const double cosal = 1.0 - (double)delta;
+        const double costh = 2.0 * (double)r.x - 1.0;
+        const double sinth = sqrt(1.0 - costh * costh);
+        const double sinal = sqrt(1.0 - cosal * cosal);
+        T sinphi, cosphi;
+        sincos_t(pi2 * (T)r.y, &sinphi, &cosphi);
+        c00r = (T)cosal;
+        c00i = (T)(sinal * costh);
+        c01r = (T)(sinal * sinth) * sinphi;
+        c01i = (T)(sinal * sinth) * cosphi;
+    }
+    // out = c * aH with c = [[c00, c01], [-conj(c01), conj(c00)]]
+    const T a10r = -a01r, a10i = a01i, a11r = a00r, a11i = -a00i;
+    const T c10r = -c01r, c10i = c01i, c11r = c00r, c11i = -c00i;
+    out[0] = c00r * a00r - c00i * a00i + c01r * a10r - c01i * a10i;
+    out[1] = c00r * a00i + c00i * a00r + c01r * a10i + c01i * a10r;
+    out[2] = c00r * a01r - c00i * a01i + c01r * a11r - c01i * a11i;
+    out[3] = c00r * a01i + c00i * a01r + c01r * a11i + c01i * a11r;
+    out[4] = c10r * a00r - c10i * a00i + c11r * a10r - c11i * a10i;
+    out[5] = c10r * a00i + c10i * a00r + c11r * a10i + c11i * a10r;
+    out[6] = c10r * a01r - c10i * a01i + c11r * a11r - c11i * a11i;
+    out[7] = c10r * a01i + c10i * a01r + c11r * a11i + c11i * a11r;
+    return true;
+}
+
+// Left-multiply rows a, b of X by the 2x2 matrix r.
+template <typename R>
+__device__ __forceinline__ void apply_rows(M<R, 3>& X, int a, int b, const R* r)
+{
+    const C<R> r00 = {r[0], r[1]}, r01 = {r[2], r[3]};
+    const C<R> r10 = {r[4], r[5]}, r11 = {r[6], r[7]};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const C<R> xa = X.e[a][k], xb = X.e[b][k];
+        C<R> na = cmul(r00, xa), nb = cmul(r10, xa);
+        cfma(na, r01, xb);
+        cfma(nb, r11, xb);
+        X.e[a][k] = na;
+        X.e[b][k] = nb;
+    }
+}
+
+// Entry (r, c) of U * A.
+template <typename R>
+__device__ __forceinline__ C<R> ua_entry(const M<R, 3>& U, const M<R, 3>& A, int r, int c)
+{
+    C<R> s = cmul(U.e[r][0], A.e[0][c]);
+    cfma(s, U.e[r][1], A.e[1][c]);
+    cfma(s, U.e[r][2], A.e[2][c]);
+    return s;
+}
+
+// ---------------------------------------------------------------- staple
+// One direction's share of the staple sum,
+//   S_nu = U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+ + U_nu(x+mu-nu)^+ U_mu(x-nu)^+ U_nu(x-nu)
+// (suncpp/Update/sun_update.h:44-77; su3_update_cl.cl:341-565).  nu, mu and
+// the parity p are wavefront-uniform.
+template <typename R, int N, int NREAL>
+__device__ __forceinline__ M<R, N> staple_nu(const Table<R, N, NREAL>& T, const Geom& g, Site s, uint32_t i, int p,
+                                             int mu, int nu)
+{
+    const int q = p ^ 1;
+    M<R, N> S;
+    m_zero(S);
+    {
+        const uint32_t i_pmu = nb_index(g, s, mu, +1);
+        const uint32_t i_pnu = nb_index(g, s, nu, +1);
+        const M<R, N> f1 = T.load_u(q, nu, i_pmu);
+        const M<R, N> f2 = T.load_u(q, mu, i_pnu);
+        const M<R, N> tf = m_mul_nd(f1, f2);
+        const M<R, N> f3 = T.load_u(p, nu, i);
+        m_acc_nd(S, tf, f3);
+    }
+    {
+        const uint32_t i_mnu = nb_index(g, s, nu, -1);
+        const uint32_t i_pmu_mnu = nb2_index(g, s, mu, +1, nu, -1);
+        const M<R, N> b1 = T.load_u(q, mu, i_mnu);
+        const M<R, N> b2 = T.load_u(p, nu, i_pmu_mnu);
+        const M<R, N> tb = m_mul(b1, b2);
+        const M<R, N> b3 = T.load_u(q, nu, i_mnu);
+        m_acc_dn(S, tb, b3);
+    }
+    return S;
+}
+
+// ---------------------------------------------------------------- fused measurement
+// Reductions avoid device-scope atomics (on MI355X they serialise across the
+// eight XCDs): every workgroup writes its partial sum to its own slot and one
+// single-workgroup kernel (k_finalize) sums the slots in index order.
+__device__ __forceinline__ double wave_sum(double v)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+// ---------------------------------------------------------------- update kernel
+// Workgroup = 3 wavefronts for 64 links of one (parity, mu) half-step.
+// Wavefront w computes the staple share of the w-th direction nu != mu
+// (uniform per wavefront, so no divergence); shares 1 and 2 pass through LDS
+// to wavefront 0, which sums them, updates the link (heatbath or
+// overrelaxation in the three SU(2) subgroups, FP64 Gram-Schmidt), stores it
+// and, for the last even Y/Z/T half-steps of a measurement cycle, adds
+// ReTr(U' * sum_{nu<mu} S_nu) to the plaquette record.
+constexpr int kLinks = 64;
+constexpr int kUpdThreads = 3 * kLinks;
+
+template <typename R, int N, int NREAL, bool OVER, int RK, bool MEAS>
+__global__ __launch_bounds__(kUpdThreads) void k_update(UpdArgs a)
+{
+    using T = R;   // compute type of the SU(2) subgroup arithmetic
+    constexpr int NC = 2 * N * N;   // reals per matrix
+    __shared__ R lds[2][NC][kLinks];
+    const int w = threadIdx.x >> 6;          // wavefront index: uniform
+    const int lane = threadIdx.x & 63;
+    const uint32_t blk = a.xcd_map ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+    const uint32_t i = blk * kLinks + lane;
+    const bool active = i < a.g.vh;
+    Table<R, N, NREAL> Tb{(typename Vec2<R>::T*)a.table, a.g.vh_pad};
+    const int p = a.parity, mu = a.mu;
+    const int nu = w + (w >= mu ? 1 : 0);
+    const Site s = site_of(a.g, active ? i : 0u, p);
+
+    M<R, N> S;
+    if (active) S = staple_nu<R, N, NREAL>(Tb, a.g, s, i, p, mu, nu);
+    else m_zero(S);
+    if (w > 0) {
+#pragma unroll
+        for (int r = 0; r < N; ++r)
+#pragma unroll
+            for (int c = 0; c < N; ++c) {
+                lds[w - 1][2 * (r * N + c)][lane] = S.e[r][c].re;
+                lds[w - 1][2 * (r * N + c) + 1][lane] = S.e[r][c].im;
+            }
+    }
+    __syncthreads();
+    if (w > 0) return;
+
+    M<R, N> S1, S2;
+#pragma unroll
+    for (int r = 0; r < N; ++r)
+#pragma unroll
+        for (int c = 0; c < N; ++c) {
+            S1.e[r][c] = {lds[0][2 * (r * N + c)][lane], lds[0][2 * (r * N + c) + 1][lane]};
+            S2.e[r][c] = {lds[1][2 * (r * N + c)][lane], lds[1][2 * (r * N + c) + 1][lane]};
+        }
+    // Directions below mu are shares 0 .. mu-1 (nu = w for w < mu).
+    M<R, N> Alow;
+    if constexpr (MEAS) {
+        m_zero(Alow);
+        if (mu > 0) m_acc(Alow, S);
+        if (mu > 1) m_acc(Alow, S1);
+        if (mu > 2) m_acc(Alow, S2);
+    }
+    M<R, N> A = S;
+    m_acc(A, S1);
+    m_acc(A, S2);
+
+    double trace = 0.0;
+    if (active) {
+        M<R, N> U = Tb.load_u(p, mu, i);
+        SrcGid sg{};
+        SrcXor sx{};
+        SrcPm sp{};
+        if constexpr (RK == RK_GID) {
+            sg.gid = (double)((uint32_t)s.y + (uint32_t)a.g.n2 * ((uint32_t)s.z + (uint32_t)a.g.n3 *
+                                                                     ((uint32_t)s.t + (uint32_t)a.g.n4 * (uint32_t)s.x)));
+            sg.fs = (double)a.sites;
+            sg.nhit = a.nhit;
+        } else if constexpr (RK == RK_XOR128) {
+            sx.s = a.rng[i];
+            sx.used = 0;
+        } else if constexpr (RK == RK_PM) {
+            sp.s = a.rng[i];
+            sp.used = 0;
+        }
+        const T beta = (T)a.beta;
+
+        T blk8[8], r[8];
+        if constexpr (N == 3) {
+#pragma unroll
+            for (int sgi = 0; sgi < 3; ++sgi) {
+                const int ra = (sgi == 2) ? 1 : 0;
+                const int rb = (sgi == 0) ? 1 : 2;
+                const C<R> waa = ua_entry(U, A, ra, ra), wab = ua_entry(U, A, ra, rb);
+                const C<R> wba = ua_entry(U, A, rb, ra), wbb = ua_entry(U, A, rb, rb);
+                blk8[0] = waa.re; blk8[1] = waa.im; blk8[2] = wab.re; blk8[3] = wab.im;
+                blk8[4] = wba.re; blk8[5] = wba.im; blk8[6] = wbb.re; blk8[7] = wbb.im;
+                bool ok;
+                if constexpr (RK == RK_GID) ok = kp_update<T, OVER, RK>(blk8, beta, a.nhit, sg, r);
+                else if constexpr (RK == RK_XOR128) ok = kp_update<T, OVER, RK>(blk8, beta, a.nhit, sx, r);
+                else if constexpr (RK == RK_PM) ok = kp_update<T, OVER, RK>(blk8, beta, a.nhit, sp, r);
+                else ok = kp_update<T, OVER, RK>(blk8, beta, a.nhit, sg, r);
+                if (!ok) {
+                    // lattice_unity_2 (su3_update_cl.cl:665): identity in this subgroup
+                    r[0] = T(1); r[1] = T(0); r[2] = T(0); r[3] = T(0);
+                    r[4] = T(0); r[5] = T(0); r[6] = T(1); r[7] = T(0);
+                }
+                apply_rows(U, ra, rb, r);
+            }
+            M<double, 3> Ud = m_cast<double>(U);
+            su3_gram_schmidt(Ud);
+            U = m_cast<R>(Ud);
+        } else {
+            if constexpr (OVER) {
+                const M<R, 2> W = m_mul(U, A);
+                blk8[0] = W.e[0][0].re; blk8[1] = W.e[0][0].im; blk8[2] = W.e[0][1].re; blk8[3] = W.e[0][1].im;
+                blk8[4] = W.e[1][0].re; blk8[5] = W.e[1][0].im; blk8[6] = W.e[1][1].re; blk8[7] = W.e[1][1].im;
+                kp_update<T, true, RK>(blk8, beta, a.nhit, sg, r);
+                M<R, 2> Rm;
+                Rm.e[0][0] = {r[0], r[1]};
+                Rm.e[0][1] = {r[2], r[3]};
+                Rm.e[1][0] = {r[4], r[5]};
+                Rm.e[1][1] = {r[6], r[7]};
+                U = m_mul(Rm, U);
+            } else {
+                blk8[0] = A.e[0][0].re; blk8[1] = A.e[0][0].im; blk8[2] = A.e[0][1].re; blk8[3] = A.e[0][1].im;
+                blk8[4] = A.e[1][0].re; blk8[5] = A.e[1][0].im; blk8[6] = A.e[1][1].re; blk8[7] = A.e[1][1].im;
+                bool ok;
+                if constexpr (RK == RK_GID) ok = kp_update<T, false, RK>(blk8, beta, a.nhit, sg, r);
+                else if constexpr (RK == RK_XOR128) ok = kp_update<T, false, RK>(blk8, beta, a.nhit, sx, r);
+                else ok = kp_update<T, false, RK>(blk8, beta, a.nhit, sp, r);
+                if (ok) {   // a rejected SU(2) update keeps the old link (su2_update_cl.cl:542-547)
+                    U.e[0][0] = {r[0], r[1]};
+                    U.e[0][1] = {r[2], r[3]};
+                    U.e[1][0] = {r[4], r[5]};
+                    U.e[1][1] = {r[6], r[7]};
+                }
+            }
+            M<double, 2> Ud = m_cast<double>(U);
+            su2_normalize(Ud);
+            U = m_cast<R>(Ud);
+        }
+        Tb.store_u(p, mu, i, U);
+        if constexpr (MEAS) trace = m_retr_mul(U, Alow);
+
+        if constexpr (RK == RK_XOR128) {
+            sx.finish(a.s_quads);
+            a.rng[i] = sx.s;
+        } else if constexpr (RK == RK_PM) {
+            sp.finish(a.s_quads);
+            a.rng[i] = sp.s;
+        }
+    }
+    if constexpr (MEAS) {
+        trace = wave_sum(trace);
+        if (lane == 0) a.partials[(size_t)(a.meas_mode - 1) * gridDim.x + blockIdx.x] = trace;
+    }
+}
+
+// ---------------------------------------------------------------- PRNG kernels
+// The reference performs one produce() at initialisation (random.cpp:276):
+// advance every stream by `quads` quads.
+__global__ __launch_bounds__(kBlock) void k_xor128_skip(uint4* st, uint32_t n, uint32_t quads)
+{
+    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    uint4 s = st[i];
+    for (uint32_t k = 0; k < 4 * quads; ++k) (void)xor128_step(s);
+    st[i] = s;
+}
+
+__global__ __launch_bounds__(kBlock) void k_pm_skip(uint4* st, uint32_t n, uint32_t quads)
+{
+    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    uint4 s = st[i];
+    for (uint32_t k = 0; k < quads; ++k) (void)pm_quad(s);
+    st[i] = s;
+}
+
+// Reference produce kernels (buffer semantics): quads[i * n + g].
+__global__ __launch_bounds__(kBlock) void k_xor128_produce(uint4* st, uint32_t n, uint32_t samples, float4* quads)
+{
+    uint32_t g = blockIdx.x * kBlock + threadIdx.x;
+    if (g >= n) return;
+    uint4 s = st[g];
+    for (uint32_t i = 0; i < samples; ++i) quads[(size_t)i * n + g] = xor128_quad(s);
+    st[g] = s;
+}
+
+__global__ __launch_bounds__(kBlock) void k_pm_produce(uint4* st, uint32_t n, uint32_t samples, float4* quads)
+{
+    uint32_t g = blockIdx.x * kBlock + threadIdx.x;
+    if (g >= n) return;
+    uint4 s = st[g];
+    for (uint32_t i = 0; i < samples; ++i) quads[(size_t)i * n + g] = pm_quad(s);
+    st[g] = s;
+}
+
+// ---------------------------------------------------------------- init kernels
+template <typename R, int N, int NREAL>
+__global__ __launch_bounds__(kBlock) void k_init_cold(void* table, Geom g)
+{
+    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= g.vh) return;
+    Table<R, N, NREAL> T{(typename Vec2<R>::T*)table, g.vh_pad};
+    M<R, N> u;
+    m_unit(u);
+    for (int p = 0; p < 2; ++p)
+        for (int mu = 0; mu < 4; ++mu) T.store_u(p, mu, i, u);
+}
+
+// lattice_init_gid (suncl.cl:190-347) / lattice_matrixGID
+// (suncpp/su3/algebra_su3.cpp:64-80, su2/algebra_su2.cpp:54-64).
+template <typename R, int N, int NREAL>
+__global__ __launch_bounds__(kBlock) void k_init_gid(void* table, Geom g, uint32_t sites)
+{
+    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= g.vh) return;
+    Table<R, N, NREAL> T{(typename Vec2<R>::T*)table, g.vh_pad};
+    for (int p = 0; p < 2; ++p) {
+        Site s = site_of(g, i, p);
+        double gid = (double)((uint32_t)s.y + (uint32_t)g.n2 * ((uint32_t)s.z + (uint32_t)g.n3 *
+                                                                  ((uint32_t)s.t + (uint32_t)g.n4 * (uint32_t)s.x)));
+        double fs = (double)sites;
+        for (int d = 0; d < 4; ++d) {
+            double f = (d + 1) / 1000.0;
+            if constexpr (N == 3) {
+                M<double, 3> a;
+                a.e[0][0] = {sin((f + 10.0 / fs) * gid) / 2.0, cos((f - 10.0 / fs) * gid) / 3.0};
+                a.e[0][1] = {sin((f - 18.0 / fs) * gid) / 4.0, cos((f + 12.0 / fs) * gid) / 4.0};
+                a.e[0][2] = {sin((f - 12.0 / fs) * gid) / 3.0, cos((f - 15.0 / fs) * gid) / 2.0};
+                a.e[1][0] = {sin((f + 15.0 / fs) * gid) / 3.0, cos((f - 11.0 / fs) * gid) / 4.0};
+                a.e[1][1] = {sin((f - 11.0 / fs) * gid) / 3.0, cos((f + 16.0 / fs) * gid) / 2.0};
+                a.e[1][2] = {sin((f - 16.0 / fs) * gid) / 3.0, cos((f - 18.0 / fs) * gid) / 2.0};
+                su3_gram_schmidt(a);
+                T.store_u(p, d, i, m_cast<R>(a));
+            } else {
+                M<double, 2> a;
+                double u1r = sin((f + 10.0 / fs) * gid) / 2.0;
+                double u2i = cos((f - 10.0 / fs) * gid) / 3.0;
+                double u2r = sin((f - 18.0 / fs) * gid) / 4.0;
+                double u1i = sqrt(1.0 - u1r * u1r - u2i * u2i - u2r * u2r);
+                a.e[0][0] = {u1r, u1i};
+                a.e[0][1] = {u2r, u2i};
+                a.e[1][0] = {-u2r, u2i};
+                a.e[1][1] = {u1r, -u1i};
+                T.store_u(p, d, i, m_cast<R>(a));
+            }
+        }
+    }
+}
+
+template <typename R, int N, int NREAL>
+__global__ __launch_bounds__(kBlock) void k_reunitarize(void* table, Geom g)
+{
+    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= g.vh) return;
+    Table<R, N, NREAL> T{(typename Vec2<R>::T*)table, g.vh_pad};
+    for (int p = 0; p < 2; ++p)
+        for (int mu = 0; mu < 4; ++mu) {
+            M<double, N> a = m_cast<double>(T.load_u(p, mu, i));
+            if constexpr (N == 3) su3_gram_schmidt(a);
+            else su2_normalize(a);
+            T.store_u(p, mu, i, m_cast<R>(a));
+        }
+}
+
+// ---------------------------------------------------------------- reductions
+// Workgroup partial of K doubles per thread: wave shuffles, then LDS, written
+// to partials[block * K + k].  Deterministic; no atomics.
+template <int K>
+__device__ __forceinline__ void block_partial(double (&v)[K], double* partials)
+{
+    __shared__ double sm[K][kBlock / 64];
+    const uint32_t bid = blockIdx.y * gridDim.x + blockIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const double x = wave_sum(v[k]);
+        if (lane == 0) sm[k][wave] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < K) {
+        double x = sm[threadIdx.x][0];
+        for (int w = 1; w < kBlock / 64; ++w) x += sm[threadIdx.x][w];
+        partials[(size_t)bid * K + threadIdx.x] = x;
+    }
+}
+
+// Final sums of one measurement cycle, one workgroup: thread t adds slots
+// t, t + 256, ... in order, then a fixed tree.  Writes one record and
+// advances the record index, so replays from a graph need no memory clears.
+struct FinArgs {
+    // plaquette from the fused even Y/Z/T half-steps: 3 arrays of n_fused
+    const double* fused;
+    uint32_t n_fused;
+    // plaquette from k_plaquette: n_plq slots of 2
+    const double* plq;
+    uint32_t n_plq;
+    // Polyakov loop: n_poly slots of 4
+    const double* poly;
+    uint32_t n_poly;
+    double* plq_rec;
+    double* poly_rec;
+    int* rec_index;   // [0] plaquette, [1] polyakov
+    int capacity;
+};
+
+__device__ __forceinline__ double block_sum_1(double v)
+{
+    __shared__ double sm[kBlock / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    v = wave_sum(v);
+    __syncthreads();
+    if (lane == 0) sm[wave] = v;
+    __syncthreads();
+    double x = sm[0];
+    for (int w = 1; w < kBlock / 64; ++w) x += sm[w];
+    return x;
+}
+
+__global__ __launch_bounds__(kBlock) void k_finalize(FinArgs f)
+{
+    const uint32_t t = threadIdx.x;
+    if (f.fused || f.plq) {
+        double sp = 0.0, tm = 0.0;
+        if (f.fused) {
+            for (uint32_t j = t; j < f.n_fused; j += kBlock) {
+                sp += f.fused[j];
+                sp += f.fused[f.n_fused + j];
+                tm += f.fused[2 * f.n_fused + j];
+            }
+        } else {
+            for (uint32_t j = t; j < f.n_plq; j += kBlock) {
+                sp += f.plq[2 * j];
+                tm += f.plq[2 * j + 1];
+            }
+        }
+        sp = block_sum_1(sp);
+        tm = block_sum_1(tm);
+        if (t == 0) {
+            const int rec = f.rec_index[0];
+            if (rec < f.capacity) {
+                f.plq_rec[2 * rec] = sp;
+                f.plq_rec[2 * rec + 1] = tm;
+            }
+            f.rec_index[0] = rec + 1;
+        }
+    }
+    if (f.poly) {
+        double v[4] = {0.0, 0.0, 0.0, 0.0};
+        for (uint32_t j = t; j < f.n_poly; j += kBlock)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) v[k] += f.poly[4 * j + k];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) v[k] = block_sum_1(v[k]);
+        if (t == 0) {
+            const int rec = f.rec_index[1];
+            if (rec < f.capacity)
+#pragma unroll
+                for (int k = 0; k < 4; ++k) f.poly_rec[4 * rec + k] = v[k];
+            f.rec_index[1] = rec + 1;
+        }
+    }
+}
+
+// Plaquette: per site sum of ReTr U_p over the spatial planes (x,y), (x,z),
+// (y,z) and the temporal planes (x,t), (y,t), (z,t), products in FP64
+// (lattice_retrace_plaquette3, su3_measurements_cl.cl:527-549).
+template <typename R, int N, int NREAL>
+__global__ __launch_bounds__(kBlock) void k_plaquette(void* table, Geom g, double* partials)
+{
+    const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    const int p = blockIdx.y, q = p ^ 1;
+    double v[2] = {0.0, 0.0};
+    if (i < g.vh) {
+        Table<R, N, NREAL> T{(typename Vec2<R>::T*)table, g.vh_pad};
+        Site s = site_of(g, i, p);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            M<double, N> Ua = m_cast<double>(T.load_u(p, a, i));
+            const uint32_t i_pa = nb_index(g, s, a, +1);
+#pragma unroll
+            for (int b = a + 1; b < 4; ++b) {
+                const uint32_t i_pb = nb_index(g, s, b, +1);
+                M<double, N> Ub = m_cast<double>(T.load_u(p, b, i));
+                M<double, N> Ubxa = m_cast<double>(T.load_u(q, b, i_pa));
+                M<double, N> Uaxb = m_cast<double>(T.load_u(q, a, i_pb));
+                // ReTr[Ua(x) Ub(x+a) Ua(x+b)^+ Ub(x)^+] = ReTr[(Ua Ubxa)(Ub Uaxb)^+]
+                double tr = m_retr_nd(m_mul(Ua, Ubxa), m_mul(Ub, Uaxb));
+                if (b == 3) v[1] += tr;
+                else v[0] += tr;
+            }
+        }
+    }
+    block_partial<2>(v, partials);
+}
+
+// Polyakov loop per spatial site, product of the N4 time links in link
+// precision and traces in FP64 (polyakov.cl:54-141).  Eight lanes share one
+// spatial site: lane l multiplies a contiguous chunk of ceil(N4/8) time links
+// and the eight partial products are combined in order by a three-level
+// shuffle tree, so a wavefront holds 8 independent loads per site in flight.
+template <typename R, int N>
+__device__ __forceinline__ M<R, N> shfl_down_m(const M<R, N>& a, int d)
+{
+    M<R, N> b;
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            b.e[i][j].re = __shfl_down(a.e[i][j].re, d, 64);
+            b.e[i][j].im = __shfl_down(a.e[i][j].im, d, 64);
+        }
+    return b;
+}
+
+template <typename R, int N, int NREAL>
+__global__ __launch_bounds__(kBlock) void k_polyakov(void* table, Geom g, double* partials)
+{
+    const uint32_t tid = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t j = tid >> 3;
+    const int l8 = (int)(tid & 7u);
+    const uint32_t vs = (uint32_t)g.n1 * g.n2 * g.n3;
+    Table<R, N, NREAL> T{(typename Vec2<R>::T*)table, g.vh_pad};
+    M<R, N> P;
+    m_unit(P);
+    int x = 0, y = 0, z = 0;
+    if (j < vs) {
+        y = (int)(j % (uint32_t)g.n2);
+        z = (int)((j / (uint32_t)g.n2) % (uint32_t)g.n3);
+        x = (int)(j / ((uint32_t)g.n2 * g.n3));
+        const int chunk = (g.n4 + 7) / 8;
+        const int t0 = l8 * chunk;
+        const int t1 = min(t0 + chunk, g.n4);
+        for (int t = t0; t < t1; ++t) {
+            const M<R, N> L = T.load((x + y + z + t) & 1, 3, cindex(g, x, y, z, t));
+            P = (t == t0) ? L : m_mul(P, L);
+        }
+    }
+#pragma unroll
+    for (int d = 1; d < 8; d <<= 1) {
+        const M<R, N> Q = shfl_down_m(P, d);
+        if ((l8 & (2 * d - 1)) == 0) P = m_mul(P, Q);
+    }
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    if (j < vs && l8 == 0) {
+        double re = 0.0, im = 0.0;
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            re += (double)P.e[k][k].re;
+            im += (double)P.e[k][k].im;
+        }
+        const double a2 = re * re + im * im;
+        v[0] = re;
+        v[1] = im;
+        v[2] = a2;
+        v[3] = a2 * a2;
+    }
+    block_partial<4>(v, partials);
+}
+
+}  // namespace
+
+// ============================================================================
+// Host side
+// ============================================================================
+struct qcd_lattice {
+    qcd_params p;
+    Geom g;
+    int np;                // complex planes per link
+    size_t elem;           // bytes per complex value
+    void* table = nullptr;
+    size_t table_bytes = 0;
+    uint4* rng = nullptr;
+    uint32_t n_streams = 0;
+    uint32_t s_quads = 0;   // S
+    double* partials = nullptr;     // one allocation, three regions:
+    double* part_fused = nullptr;   //   3 x nb_fused  (fused plaquette)
+    double* part_plq = nullptr;     //   2 x plq blocks (k_plaquette)
+    double* part_poly = nullptr;    //   4 x poly_blocks (k_polyakov)
+    uint32_t nb_fused = 0;
+    int* rec_index = nullptr;       // [0] plaquette, [1] polyakov
+    double* plq_rec = nullptr;
+    double* poly_rec = nullptr;
+    uint32_t upd_blocks = 0, poly_blocks = 0;
+    hipStream_t stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int cyc_hb = 0, cyc_or = 0, cyc_mask = 0;
+    bool rng_ready = false;
+    bool gid_updates = false;   // INTS = 2: GID known-answer numbers in the heatbath
+    int tune_xcd_map = 1;       // QCD_TUNE_XCD_MAP
+    int tune_blocks_per_cu = 0; // QCD_TUNE_BLOCKS_PER_CU (0 = no cap)
+};
+
+namespace {
+
+thread_local char g_err[512] = "";
+int g_device = -1;
+
+int fail(int code, const char* fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define HIPCHK(x)                                                                              \
+    do {                                                                                       \
+        hipError_t e_ = (x);                                                                   \
+        if (e_ != hipSuccess) return fail(QCD_E_HIP, "%s: %s", #x, hipGetErrorString(e_)); \
+    } while (0)
+
+uint32_t align_up(uint32_t x, uint32_t a) { return (1u + (x - 1u) / a) * a; }
+
+// Optional per-launch extras: fused plaquette measurement and a pair of HIP
+// events recorded at the start and end of the kernel itself
+// (hipExtLaunchKernelGGL), used to time kernels without launch overhead.
+struct LaunchOpt {
+    int meas_mode = 0;
+    hipEvent_t e0 = nullptr;
+    hipEvent_t e1 = nullptr;
+    uint32_t lds = 0;   // extra dynamic LDS per workgroup (caps workgroups per CU)
+};
+
+template <typename Kern, typename... Args>
+void launch(Kern k, dim3 grid, dim3 block, hipStream_t s, const LaunchOpt& o, Args... args)
+{
+    if (o.e0) hipExtLaunchKernelGGL(k, grid, block, o.lds, s, o.e0, o.e1, 0, args...);
+    else hipLaunchKernelGGL(k, grid, block, o.lds, s, args...);
+}
+
+template <typename R, int N, int NREAL, bool OVER, int RK>
+void launch_update_t(qcd_lattice* L, const UpdArgs& a, const LaunchOpt& o)
+{
+    const uint32_t nb = align_up(L->g.vh, kLinks) / kLinks;
+    if (o.meas_mode) launch(k_update<R, N, NREAL, OVER, RK, true>, dim3(nb), dim3(kUpdThreads), L->stream, o, a);
+    else launch(k_update<R, N, NREAL, OVER, RK, false>, dim3(nb), dim3(kUpdThreads), L->stream, o, a);
+}
+
+template <typename R, int N, int NREAL>
+int launch_update_rn(qcd_lattice* L, int parity, int mu, int kind, int start_gid, const LaunchOpt& o)
+{
+    UpdArgs a;
+    a.table = L->table;
+    a.g = L->g;
+    a.parity = parity;
+    a.mu = mu;
+    a.beta = L->p.beta / L->p.sun;
+    a.nhit = L->p.nhit;
+    a.sites = 2u * L->g.vh;
+    a.rng = L->rng;
+    a.s_quads = (L->p.stream_mode == QCD_STREAM_REFERENCE) ? L->s_quads : 0u;
+    a.partials = L->part_fused;
+    a.meas_mode = o.meas_mode;
+    a.xcd_map = L->tune_xcd_map;
+    LaunchOpt ol = o;
+    if (L->tune_blocks_per_cu > 0) {
+        // 160 KiB of LDS per CU on gfx950; reserve so at most k workgroups fit.
+        const uint32_t per = (160u * 1024u) / (uint32_t)L->tune_blocks_per_cu;
+        const uint32_t stat = (uint32_t)(2 * 2 * N * N * kLinks * sizeof(R));
+        ol.lds = per > stat + 256 ? per - stat - 256 : 0;
+    }
+    if (kind == QCD_OVERRELAX) {
+        launch_update_t<R, N, NREAL, true, RK_NONE>(L, a, ol);
+    } else if (start_gid) {
+        launch_update_t<R, N, NREAL, false, RK_GID>(L, a, ol);
+    } else {
+        if (!L->rng_ready) return fail(QCD_E_STATE, "heatbath needs qcd_prng_init first");
+        if (L->p.prng == QCD_PRNG_XOR128) launch_update_t<R, N, NREAL, false, RK_XOR128>(L, a, ol);
+        else if (L->p.prng == QCD_PRNG_PM) launch_update_t<R, N, NREAL, false, RK_PM>(L, a, ol);
+        else return fail(QCD_E_UNSUPPORTED, "generator %d not available for fused updates", L->p.prng);
+    }
+    HIPCHK(hipGetLastError());
+    return QCD_OK;
+}
+
+#define DISPATCH(L, FN, ...)                                                                          \
+    ((L)->p.precision == QCD_FP32                                                                     \
+         ? ((L)->p.sun == 3 ? ((L)->p.storage_reals == 12 ? FN<float, 3, 12>(__VA_ARGS__)            \
+                                                          : FN<float, 3, 18>(__VA_ARGS__))           \
+                            : FN<float, 2, 4>(__VA_ARGS__))                                           \
+         : ((L)->p.sun == 3 ? ((L)->p.storage_reals == 12 ? FN<double, 3, 12>(__VA_ARGS__)           \
+                                                          : FN<double, 3, 18>(__VA_ARGS__))          \
+                            : FN<double, 2, 4>(__VA_ARGS__)))
+
+template <typename R, int N, int NREAL>
+int launch_simple(qcd_lattice* L, int which, const LaunchOpt& o)
+{
+    uint32_t nb = align_up(L->g.vh, kBlock) / kBlock;
+    switch (which) {
+        case 0:
+            hipLaunchKernelGGL((k_init_cold<R, N, NREAL>), dim3(nb), dim3(kBlock), 0, L->stream, L->table, L->g);
+            break;
+        case 1:
+            hipLaunchKernelGGL((k_init_gid<R, N, NREAL>), dim3(nb), dim3(kBlock), 0, L->stream, L->table, L->g,
+                               2u * L->g.vh);
+            break;
+        case 2:
+            hipLaunchKernelGGL((k_reunitarize<R, N, NREAL>), dim3(nb), dim3(kBlock), 0, L->stream, L->table, L->g);
+            break;
+        case 3:
+            launch(k_plaquette<R, N, NREAL>, dim3(L->upd_blocks, 2), dim3(kBlock), L->stream, o, L->table, L->g,
+                   L->part_plq);
+            break;
+        case 4:
+            launch(k_polyakov<R, N, NREAL>, dim3(L->poly_blocks), dim3(kBlock), L->stream, o, L->table, L->g,
+                   L->part_poly);
+            break;
+        default:
+            return fail(QCD_E_ARG, "internal: bad launch %d", which);
+    }
+    HIPCHK(hipGetLastError());
+    return QCD_OK;
+}
+
+// flags: 1 = fused plaquette partials, 2 = k_plaquette partials, 4 = Polyakov.
+int launch_finalize(qcd_lattice* L, int flags, const LaunchOpt& o)
+{
+    FinArgs f;
+    f.fused = (flags & 1) ? L->part_fused : nullptr;
+    f.n_fused = L->nb_fused;
+    f.plq = (flags & 2) ? L->part_plq : nullptr;
+    f.n_plq = 2u * L->upd_blocks;
+    f.poly = (flags & 4) ? L->part_poly : nullptr;
+    f.n_poly = L->poly_blocks;
+    f.plq_rec = L->plq_rec;
+    f.poly_rec = L->poly_rec;
+    f.rec_index = L->rec_index;
+    f.capacity = L->p.max_measurements;
+    launch(k_finalize, dim3(1), dim3(kBlock), L->stream, o, f);
+    HIPCHK(hipGetLastError());
+    return QCD_OK;
+}
+
+int ensure_device()
+{
+    if (g_device < 0) {
+        int rc = qcd_device_open(0, 0);
+        if (rc) return rc;
+    }
+    return QCD_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- C ABI
+extern "C" {
+
+const char* qcd_last_error(void) { return g_err; }
+int qcd_abi_version(void) { return QCD_ABI_VERSION; }
+
+int qcd_device_open(int platform, int device)
+{
+    if (platform != 0) return fail(QCD_E_ARG, "platform %d: only platform 0 (HIP) exists", platform);
+    int count = 0;
+    HIPCHK(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count) return fail(QCD_E_ARG, "device %d not present (%d devices)", device, count);
+    HIPCHK(hipSetDevice(device));
+    g_device = device;
+    return QCD_OK;
+}
+
+int qcd_device_close(void)
+{
+    if (g_device >= 0) HIPCHK(hipDeviceSynchronize());
+    g_device = -1;
+    return QCD_OK;
+}
+
+int qcd_device_info(char* name, int name_len, int* compute_units, size_t* global_mem)
+{
+    int rc = ensure_device();
+    if (rc) return rc;
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, g_device));
+    if (name && name_len > 0) snprintf(name, (size_t)name_len, "%s (%s)", prop.name, prop.gcnArchName);
+    if (compute_units) *compute_units = prop.multiProcessorCount;
+    if (global_mem) *global_mem = prop.totalGlobalMem;
+    return QCD_OK;
+}
+
+int qcd_lattice_create(const qcd_params* pp, qcd_lattice** out)
+{
+    if (!pp || !out) return fail(QCD_E_ARG, "null argument");
+    const qcd_params& p = *pp;
+    if (p.sun != 2 && p.sun != 3) return fail(QCD_E_ARG, "sun must be 2 or 3");
+    for (int d = 0; d < 4; ++d)
+        if (p.n[d] < 2 || (p.n[d] & 1)) return fail(QCD_E_ARG, "extent N%d = %d must be even and >= 2", d + 1, p.n[d]);
+    if (p.precision != QCD_FP32 && p.precision != QCD_FP64) return fail(QCD_E_ARG, "bad precision");
+    if (p.sun == 3 && p.storage_reals != 18 && p.storage_reals != 12)
+        return fail(QCD_E_ARG, "SU(3) storage must be 18 or 12 reals");
+    if (p.sun == 2 && p.storage_reals != 4) return fail(QCD_E_ARG, "SU(2) storage must be 4 reals");
+    if (p.nhit < 1) return fail(QCD_E_ARG, "nhit must be >= 1");
+    if (p.nhitpar != 1) return fail(QCD_E_UNSUPPORTED, "NHITPar other than 1 is not implemented");
+    if (p.max_measurements < 1) return fail(QCD_E_ARG, "max_measurements must be >= 1");
+    uint64_t V = (uint64_t)p.n[0] * p.n[1] * p.n[2] * p.n[3];
+    if (V / 2 > 0x7fffffffull) return fail(QCD_E_ARG, "lattice too large");
+    int rc = ensure_device();
+    if (rc) return rc;
+
+    qcd_lattice* L = new qcd_lattice();
+    L->p = p;
+    L->g.n1 = p.n[0];
+    L->g.n2 = p.n[1];
+    L->g.n3 = p.n[2];
+    L->g.n4 = p.n[3];
+    L->g.h = p.n[1] / 2;
+    L->g.vh = (uint32_t)(V / 2);
+    // Plane stride: a multiple of 64 sites (512 B / 1 KiB, whole cache lines
+    // per wavefront) plus one extra 64-site unit, so the 72 planes of one
+    // link do not share an L2 set/channel when V/2 is a power of two (for
+    // 32^3 x 8 a bare stride would be exactly 1 MiB).
+    L->g.vh_pad = align_up(L->g.vh, 64) + 64;
+    L->np = p.storage_reals / 2;
+    L->elem = (p.precision == QCD_FP32) ? 8 : 16;
+    L->table_bytes = (size_t)2 * 4 * L->np * L->g.vh_pad * L->elem;
+    L->upd_blocks = align_up(L->g.vh, kBlock) / kBlock;
+    uint32_t vs = (uint32_t)p.n[0] * p.n[1] * p.n[2];
+    L->poly_blocks = align_up(8u * vs, kBlock) / kBlock;   // 8 lanes per spatial site
+    // Reference PRNG sizing (suncl.cpp:2302-2310, random.cpp:219-223) with
+    // PRNG_instances = PRNGSTEP = align64(V/2).
+    uint32_t prngstep = align_up(L->g.vh, 64);
+    uint64_t total = (uint64_t)align_up((uint32_t)((uint64_t)p.nhitpar * 3u * prngstep * (p.nhit + 1)), 64);
+    L->n_streams = prngstep;
+    L->s_quads = (uint32_t)(1 + (total - 1) / prngstep);
+
+    hipError_t e;
+    if ((e = hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking)) != hipSuccess) {
+        delete L;
+        return fail(QCD_E_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+    }
+    L->nb_fused = align_up(L->g.vh, kLinks) / kLinks;
+    size_t part_doubles = (size_t)3 * L->nb_fused + (size_t)2 * 2 * L->upd_blocks + (size_t)4 * L->poly_blocks;
+    if (hipMalloc(&L->table, L->table_bytes) != hipSuccess ||
+        hipMalloc((void**)&L->rng, sizeof(uint4) * L->n_streams) != hipSuccess ||
+        hipMalloc((void**)&L->partials, sizeof(double) * part_doubles) != hipSuccess ||
+        hipMalloc((void**)&L->rec_index, sizeof(int) * 4) != hipSuccess ||
+        hipMalloc((void**)&L->plq_rec, sizeof(double) * 2 * p.max_measurements) != hipSuccess ||
+        hipMalloc((void**)&L->poly_rec, sizeof(double) * 4 * p.max_measurements) != hipSuccess) {
+        qcd_lattice_destroy(L);
+        return fail(QCD_E_NOMEM, "device allocation failed (table %zu bytes)", L->table_bytes);
+    }
+    L->part_fused = L->partials;
+    L->part_plq = L->part_fused + (size_t)3 * L->nb_fused;
+    L->part_poly = L->part_plq + (size_t)2 * 2 * L->upd_blocks;
+    (void)hipMemset(L->rec_index, 0, sizeof(int) * 4);
+    (void)hipMemset(L->table, 0, L->table_bytes);
+    if (hipDeviceSynchronize() != hipSuccess) {
+        qcd_lattice_destroy(L);
+        return fail(QCD_E_HIP, "initial clear failed");
+    }
+    *out = L;
+    return QCD_OK;
+}
+
+int qcd_lattice_destroy(qcd_lattice* L)
+{
+    if (!L) return QCD_OK;
+    if (L->stream) (void)hipStreamSynchronize(L->stream);
+    if (L->exec) (void)hipGraphExecDestroy(L->exec);
+    if (L->graph) (void)hipGraphDestroy(L->graph);
+    (void)hipFree(L->table);
+    (void)hipFree(L->rng);
+    (void)hipFree(L->partials);
+    (void)hipFree(L->rec_index);
+    (void)hipFree(L->plq_rec);
+    (void)hipFree(L->poly_rec);
+    if (L->stream) (void)hipStreamDestroy(L->stream);
+    delete L;
+    return QCD_OK;
+}
+
+int qcd_lattice_sync(qcd_lattice* L)
+{
+    if (!L) return fail(QCD_E_ARG, "null lattice");
+    HIPCHK(hipStreamSynchronize(L->stream));
+    return QCD_OK;
+}
+
+int qcd_lattice_tune(qcd_lattice* L, int knob, int value)
+{
+    if (!L) return fail(QCD_E_ARG, "null lattice");
+    switch (knob) {
+        case QCD_TUNE_XCD_MAP:
+            if (value != 0 && value != 1) return fail(QCD_E_ARG, "xcd map must be 0 or 1");
+            L->tune_xcd_map = value;
+            break;
+        case QCD_TUNE_BLOCKS_PER_CU:
+            if (value < 0 || value > 32) return fail(QCD_E_ARG, "blocks per CU must be 0..32");
+            L->tune_blocks_per_cu = value;
+            break;
+        default:
+            return fail(QCD_E_ARG, "unknown tuning knob %d", knob);
+    }
+    return QCD_OK;
+}
+
+int qcd_lattice_table_bytes(const qcd_lattice* L, size_t* bytes)
+{
+    if (!L || !bytes) return fail(QCD_E_ARG, "null argument");
+    *bytes = L->table_bytes;
+    return QCD_OK;
+}
+
+int qcd_lattice_init(qcd_lattice* L, int start)
+{
+    if (!L) return fail(QCD_E_ARG, "null lattice");
+    // The GID known-answer start (INTS = 2) also switches the heatbath to the
+    // GID numbers, as -D GID_UPD does in the reference (suncl.cpp:3278-3279).
+    if (start == QCD_START_COLD) {
+        L->gid_updates = false;
+        return DISPATCH(L, launch_simple, L, 0, LaunchOpt{});
+    }
+    if (start == QCD_START_GID) {
+        L->gid_updates = true;
+        return DISPATCH(L, launch_simple, L, 1, LaunchOpt{});
+    }
+    if (start == QCD_START_HOT) return fail(QCD_E_UNSUPPORTED, "hot start is not implemented yet");
+    return fail(QCD_E_ARG, "bad start type %d", start);
+}
+
+int qcd_prng_init(qcd_lattice* L)
+{
+    if (!L) return fail(QCD_E_ARG, "null lattice");
+    if (L->p.prng != QCD_PRNG_XOR128 && L->p.prng != QCD_PRNG_PM)
+        return fail(QCD_E_UNSUPPORTED, "generator %d is not available for fused streams yet", L->p.prng);
+    unsigned seed = L->p.randseries ? L->p.randseries : (unsigned)time(nullptr);
+    std::vector<uint32_t> tab((size_t)4 * L->n_streams);
+    srand(seed);
+    // XOR128: instance 0 = XOR128_initialize_CPU (4 x rand(), random.cpp:675-681),
+    // others 4 x rand() (random.cpp:653-659).  PM: every entry rand() % (2^31-1)
+    // (random.cpp:867-876,892-894).
+    for (size_t k = 0; k < tab.size(); ++k)
+        tab[k] = (L->p.prng == QCD_PRNG_XOR128) ? (uint32_t)rand() : (uint32_t)(rand() % 2147483647);
+    HIPCHK(hipMemcpyAsync(L->rng, tab.data(), sizeof(uint32_t) * tab.size(), hipMemcpyHostToDevice, L->stream));
+    // The initial produce() of PRNG::initialize (random.cpp:276) consumes S
+    // quads of every instance before the first half-step.
+    uint32_t nb = align_up(L->n_streams, kBlock) / kBlock;
+    if (L->p.stream_mode == QCD_STREAM_REFERENCE) {
+        if (L->p.prng == QCD_PRNG_XOR128)
+            hipLaunchKernelGGL(k_xor128_skip, dim3(nb), dim3(kBlock), 0, L->stream, L->rng, L->n_streams, L->s_quads);
+        else
+            hipLaunchKernelGGL(k_pm_skip, dim3(nb), dim3(kBlock), 0, L->stream, L->rng, L->n_streams, L->s_quads);
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipStreamSynchronize(L->stream));
+    L->rng_ready = true;
+    return QCD_OK;
+}
+
+int qcd_lattice_import(qcd_lattice* L, const double* links)
+{
+    if (!L || !links) return fail(QCD_E_ARG, "null argument");
+    const int N = L->p.sun;
+    std::vector<unsigned char> host(L->table_bytes, 0);
+    const uint32_t vh = L->g.vh, vp = L->g.vh_pad;
+    const uint64_t V = 2ull * vh;
+    for (uint64_t s = 0; s < V; ++s) {
+        uint32_t y = (uint32_t)(s % L->g.n2);
+        uint64_t r = s / L->g.n2;
+        uint32_t z = (uint32_t)(r % L->g.n3);
+        r /= L->g.n3;
+        uint32_t t = (uint32_t)(r % L->g.n4);
+        uint32_t x = (uint32_t)(r / L->g.n4);
+        int par = (int)((x + y + z + t) & 1);
+        uint64_t i = s >> 1;
+        for (int d = 0; d < 4; ++d)
+            for (int k = 0; k < L->np; ++k) {
+                const double* src = links + 2 * (((size_t)s * 4 + d) * N * N + k);
+                size_t off = ((size_t)((par * 4 + d) * L->np + k) * vp + i);
+                if (L->elem == 8) {
+                    float* dst = (float*)host.data() + 2 * off;
+                    dst[0] = (float)src[0];
+                    dst[1] = (float)src[1];
+                } else {
+                    double* dst = (double*)host.data() + 2 * off;
+                    dst[0] = src[0];
+                    dst[1] = src[1];
+                }
+            }
+    }
+    HIPCHK(hipMemcpyAsync(L->table, host.data(), L->table_bytes, hipMemcpyHostToDevice, L->stream));
+    HIPCHK(hipStreamSynchronize(L->stream));
+    return QCD_OK;
+}
+
+int qcd_lattice_export(qcd_lattice* L, double* links)
+{
+    if (!L || !links) return fail(QCD_E_ARG, "null argument");
+    const int N = L->p.sun;
+    std::vector<unsigned char> host(L->table_bytes);
+    HIPCHK(hipMemcpyAsync(host.data(), L->table, L->table_bytes, hipMemcpyDeviceToHost, L->stream));
+    HIPCHK(hipStreamSynchronize(L->stream));
+    const uint32_t vh = L->g.vh, vp = L->g.vh_pad;
+    const uint64_t V = 2ull * vh;
+    for (uint64_t s = 0; s < V; ++s) {
+        uint32_t y = (uint32_t)(s % L->g.n2);
+        uint64_t r = s / L->g.n2;
+        uint32_t z = (uint32_t)(r % L->g.n3);
+        r /= L->g.n3;
+        uint32_t t = (uint32_t)(r % L->g.n4);
+        uint32_t x = (uint32_t)(r / L->g.n4);
+        int par = (int)((x + y + z + t) & 1);
+        uint64_t i = s >> 1;
+        for (int d = 0; d < 4; ++d) {
+            double* m = links + 2 * (((size_t)s * 4 + d) * N * N);
+            for (int k = 0; k < L->np; ++k) {
+                size_t off = ((size_t)((par * 4 + d) * L->np + k) * vp + i);
+                if (L->elem == 8) {
+                    const float* src = (const float*)host.data() + 2 * off;
+                    m[2 * k] = src[0];
+                    m[2 * k + 1] = src[1];
+                } else {
+                    const double* src = (const double*)host.data() + 2 * off;
+                    m[2 * k] = src[0];
+                    m[2 * k + 1] = src[1];
+                }
+            }
+            if (N == 3 && L->np == 6) {
+                // row 3 = conj(row1 x row2)
+                double* u = m;
+                double* v = m + 6;
+                double* w = m + 12;
+                auto cm = [](double ar, double ai, double br, double bi, double& cr, double& ci) {
+                    cr = ar * br - ai * bi;
+                    ci = ar * bi + ai * br;
+                };
+                double p1r, p1i, p2r, p2i;
+                cm(u[2], u[3], v[4], v[5], p1r, p1i);
+                cm(u[4], u[5], v[2], v[3], p2r, p2i);
+                w[0] = p1r - p2r;
+                w[1] = -(p1i - p2i);
+                cm(u[4], u[5], v[0], v[1], p1r, p1i);
+                cm(u[0], u[1], v[4], v[5], p2r, p2i);
+                w[2] = p1r - p2r;
+                w[3] = -(p1i - p2i);
+                cm(u[0], u[1], v[2], v[3], p1r, p1i);
+                cm(u[2], u[3], v[0], v[1], p2r, p2i);
+                w[4] = p1r - p2r;
+                w[5] = -(p1i - p2i);
+            }
+            if (N == 2) {
+                m[4] = -m[2];
+                m[5] = m[3];
+                m[6] = m[0];
+                m[7] = -m[1];
+            }
+        }
+    }
+    return QCD_OK;
+}
+
+static int halfstep_impl(qcd_lattice* L, int parity, int mu, int kind, const LaunchOpt& o)
+{
+    if (!L) return fail(QCD_E_ARG, "null lattice");
+    if ((parity != 0 && parity != 1) || mu < 0 || mu > 3) return fail(QCD_E_ARG, "bad parity/mu");
+    if (kind != QCD_HEATBATH && kind != QCD_OVERRELAX) return fail(QCD_E_ARG, "bad update kind");
+    return DISPATCH(L, launch_update_rn, L, parity, mu, kind, L->gid_updates ? 1 : 0, o);
+}
+
+int qcd_halfstep(qcd_lattice* L, int parity, int mu, int kind)
+{
+    return halfstep_impl(L, parity, mu, kind, LaunchOpt{});
+}
+
+// One sweep; with fuse_plaquette the even Y, Z and T half-steps also produce
+// the plaquette record of the configuration the sweep leaves behind.
+static int sweep_impl(qcd_lattice* L, int kind, bool fuse_plaquette, std::vector<hipEvent_t>* ev, int* ev_used)
+{
+    for (int parity = 1; parity >= 0; --parity)
+        for (int mu = 0; mu < 4; ++mu) {
+            LaunchOpt o;
+            if (fuse_plaquette && parity == 0 && mu >= 1) o.meas_mode = mu;   // 1, 2, 3
+            if (ev) {
+                o.e0 = (*ev)[2 * (*ev_used)];
+                o.e1 = (*ev)[2 * (*ev_used) + 1];
+                ++*ev_used;
+            }
+            int rc = halfstep_impl(L, parity, mu, kind, o);
+            if (rc) return rc;
+        }
+    return QCD_OK;
+}
+
+int qcd_sweep(qcd_lattice* L, int kind)
+{
+    if (!L) return fail(QCD_E_ARG, "null lattice");
+    return sweep_impl(L, kind, false, nullptr, nullptr);
+}
+
+int qcd_reunitarize(qcd_lattice* L)
+{
+    if (!L) return fail(QCD_E_ARG, "null lattice");
+    return DISPATCH(L, launch_simple, L, 2, LaunchOpt{});
+}
+
+static int measure_impl(qcd_lattice* L, int mask, bool fused_plq, const LaunchOpt& o_plq, const LaunchOpt& o_poly,
+                        const LaunchOpt& o_fin)
+{
+    int rc = QCD_OK;
+    int flags = 0;
+    if (fused_plq) flags |= 1;
+    if ((mask & QCD_MEAS_PLAQUETTE) && !fused_plq) {
+        rc = DISPATCH(L, launch_simple, L, 3, o_plq);
+        flags |= 2;
+    }
+    if (rc) return rc;
+    if (mask & QCD_MEAS_POLYAKOV) {
+        rc = DISPATCH(L, launch_simple, L, 4, o_poly);
+        flags |= 4;
+    }
+    if (rc) return rc;
+    return flags ? launch_finalize(L, flags, o_fin) : QCD_OK;
+}
+
+int qcd_measure(qcd_lattice* L, int mask)
+{
+    if (!L) return fail(QCD_E_ARG, "null lattice");
+    return measure_impl(L, mask, false, LaunchOpt{}, LaunchOpt{}, LaunchOpt{});
+}
+
+int qcd_read_measurements(qcd_lattice* L, double* plq, double* poly, int* count_plq, int* count_poly)
+{
+    if (!L) return fail(QCD_E_ARG, "null lattice");
+    int idx[4];
+    HIPCHK(hipMemcpyAsync(idx, L->rec_index, sizeof(idx), hipMemcpyDeviceToHost, L->stream));
+    HIPCHK(hipStreamSynchronize(L->stream));
+    int np = idx[0] < L->p.max_measurements ? idx[0] : L->p.max_measurements;
+    int nq = idx[1] < L->p.max_measurements ? idx[1] : L->p.max_measurements;
+    if (plq && np) HIPCHK(hipMemcpy(plq, L->plq_rec, sizeof(double) * 2 * np, hipMemcpyDeviceToHost));
+    if (poly && nq) HIPCHK(hipMemcpy(poly, L->poly_rec, sizeof(double) * 4 * nq, hipMemcpyDeviceToHost));
+    if (count_plq) *count_plq = idx[0];
+    if (count_poly) *count_poly = idx[1];
+    return QCD_OK;
+}
+
+int qcd_reset_measurements(qcd_lattice* L)
+{
+    if (!L) return fail(QCD_E_ARG, "null lattice");
+    HIPCHK(hipStreamSynchronize(L->stream));
+    HIPCHK(hipMemset(L->rec_index, 0, sizeof(int) * 4));
+    HIPCHK(hipDeviceSynchronize());
+    return QCD_OK;
+}
+
+// A measurement cycle: n_hb heatbath sweeps, n_or overrelaxation sweeps,
+// then the measurements.  The plaquette rides on the last sweep's even
+// Y/Z/T half-steps when there is a sweep; the Polyakov loop is its own kernel.
+static int enqueue_cycle(qcd_lattice* L, int n_hb, int n_or, int mask, std::vector<hipEvent_t>* ev, int* ev_used)
+{
+    const int total = n_hb + n_or;
+    const bool fuse = (mask & QCD_MEAS_PLAQUETTE) && total > 0;
+    for (int k = 0; k < total; ++k) {
+        int rc = sweep_impl(L, k < n_hb ? QCD_HEATBATH : QCD_OVERRELAX, fuse && k == total - 1, ev, ev_used);
+        if (rc) return rc;
+    }
+    if (!mask) return QCD_OK;
+    LaunchOpt op, oq, of;
+    if (ev) {
+        LaunchOpt* slots[3] = {&op, &oq, &of};
+        const bool want[3] = {(mask & QCD_MEAS_PLAQUETTE) && !fuse, (mask & QCD_MEAS_POLYAKOV) != 0, true};
+        for (int k = 0; k < 3; ++k)
+            if (want[k]) {
+                slots[k]->e0 = (*ev)[2 * (*ev_used)];
+                slots[k]->e1 = (*ev)[2 * (*ev_used) + 1];
+                ++*ev_used;
+            }
+    }
+    return measure_impl(L, mask, fuse, op, oq, of);
+}
+
+int qcd_cycle_capture(qcd_lattice* L, int n_hb, int n_or, int mask)
+{
+    if (!L) return fail(QCD_E_ARG, "null lattice");
+    if (n_hb < 0 || n_or < 0) return fail(QCD_E_ARG, "negative sweep count");
+    if (L->exec) {
+        (void)hipGraphExecDestroy(L->exec);
+        L->exec = nullptr;
+    }
+    if (L->graph) {
+        (void)hipGraphDestroy(L->graph);
+        L->graph = nullptr;
+    }
+    HIPCHK(hipStreamSynchronize(L->stream));
+    HIPCHK(hipStreamBeginCapture(L->stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_cycle(L, n_hb, n_or, mask, nullptr, nullptr);
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(L->stream, &graph);
+    if (rc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(QCD_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    L->graph = graph;
+    HIPCHK(hipGraphInstantiate(&L->exec, graph, nullptr, nullptr, 0));
+    L->cyc_hb = n_hb;
+    L->cyc_or = n_or;
+    L->cyc_mask = mask;
+    return QCD_OK;
+}
+
+int qcd_cycle_run(qcd_lattice* L, int n_hb, int n_or, int mask)
+{
+    if (!L) return fail(QCD_E_ARG, "null lattice");
+    if (n_hb < 0 || n_or < 0) return fail(QCD_E_ARG, "negative sweep count");
+    return enqueue_cycle(L, n_hb, n_or, mask, nullptr, nullptr);
+}
+
+int qcd_cycle_launch(qcd_lattice* L, int repeats)
+{
+    if (!L || !L->exec) return fail(QCD_E_STATE, "no captured cycle");
+    for (int r = 0; r < repeats; ++r) HIPCHK(hipGraphLaunch(L->exec, L->stream));
+    return QCD_OK;
+}
+
+int qcd_cycle_time(qcd_lattice* L, int repeats, double* ms)
+{
+    if (!L || !L->exec) return fail(QCD_E_STATE, "no captured cycle");
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    HIPCHK(hipStreamSynchronize(L->stream));
+    HIPCHK(hipEventRecord(a, L->stream));
+    for (int r = 0; r < repeats; ++r) HIPCHK(hipGraphLaunch(L->exec, L->stream));
+    HIPCHK(hipEventRecord(b, L->stream));
+    HIPCHK(hipEventSynchronize(b));
+    float f = 0.f;
+    HIPCHK(hipEventElapsedTime(&f, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (ms) *ms = f;
+    return QCD_OK;
+}
+
+// Eager replay of the captured cycle with every kernel launched through
+// hipExtLaunchKernelGGL, which records a HIP event pair at the start and end
+// of the kernel itself on the lattice stream: per-kernel durations without
+// launch gaps.  Reports means per update kind and per cycle for the
+// measurement kernels.
+int qcd_cycle_profile(qcd_lattice* L, int repeats, double* hb_us, double* or_us, double* meas_us, double* total_ms)
+{
+    if (!L) return fail(QCD_E_ARG, "null lattice");
+    const int n_hb = L->cyc_hb, n_or = L->cyc_or, mask = L->cyc_mask;
+    const int per = 8 * (n_hb + n_or) + 3;
+    std::vector<hipEvent_t> ev((size_t)2 * per);
+    for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+    hipEvent_t t0, t1;
+    HIPCHK(hipEventCreate(&t0));
+    HIPCHK(hipEventCreate(&t1));
+    double t_hb = 0.0, t_or = 0.0, t_meas = 0.0, t_tot = 0.0;
+    int n_fused_seen = 0;
+    HIPCHK(hipStreamSynchronize(L->stream));
+    for (int r = 0; r < repeats; ++r) {
+        int used = 0;
+        HIPCHK(hipEventRecord(t0, L->stream));
+        int rc = enqueue_cycle(L, n_hb, n_or, mask, &ev, &used);
+        if (rc) return rc;
+        HIPCHK(hipEventRecord(t1, L->stream));
+        HIPCHK(hipEventSynchronize(t1));
+        const int total = n_hb + n_or;
+        const bool fuse = (mask & QCD_MEAS_PLAQUETTE) && total > 0;
+        for (int j = 0; j < used; ++j) {
+            float f = 0.f;
+            HIPCHK(hipEventElapsedTime(&f, ev[2 * j], ev[2 * j + 1]));
+            const int sw = j / 8, k = j % 8;
+            // the last sweep's even Y/Z/T half-steps carry the plaquette
+            const bool fused_launch = fuse && sw == total - 1 && k >= 5;
+            if (j >= 8 * total || fused_launch) t_meas += f;
+            else if (sw < n_hb) t_hb += f;
+            else t_or += f;
+            if (fused_launch) ++n_fused_seen;
+        }
+        float f = 0.f;
+        HIPCHK(hipEventElapsedTime(&f, t0, t1));
+        t_tot += f;
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    // Means over the plain update launches; the three half-steps that carry
+    // the fused plaquette are counted as measurement time.
+    const int per_rep_fused = repeats ? n_fused_seen / repeats : 0;
+    const int hb_launches = 8 * n_hb - (n_or == 0 ? per_rep_fused : 0);
+    const int or_launches = 8 * n_or - (n_or > 0 ? per_rep_fused : 0);
+    if (hb_us) *hb_us = hb_launches > 0 ? 1000.0 * t_hb / ((double)repeats * hb_launches) : 0.0;
+    if (or_us) *or_us = or_launches > 0 ? 1000.0 * t_or / ((double)repeats * or_launches) : 0.0;
+    if (meas_us) *meas_us = mask ? 1000.0 * t_meas / repeats : 0.0;
+    if (total_ms) *total_ms = t_tot;
+    return QCD_OK;
+}
+
+int qcd_prng_produce_host(int prng, int ranlux_luxury, unsigned n, unsigned samples, void* table, float* quads)
+{
+    (void)ranlux_luxury;
+    if (!table || !quads || n == 0) return fail(QCD_E_ARG, "null argument");
+    if (prng != QCD_PRNG_XOR128 && prng != QCD_PRNG_PM)
+        return fail(QCD_E_UNSUPPORTED, "generator %d not available", prng);
+    int rc = ensure_device();
+    if (rc) return rc;
+    uint4* dt = nullptr;
+    float4* dq = nullptr;
+    size_t qbytes = sizeof(float4) * (size_t)n * samples;
+    HIPCHK(hipMalloc((void**)&dt, sizeof(uint4) * n));
+    if (hipMalloc((void**)&dq, qbytes) != hipSuccess) {
+        (void)hipFree(dt);
+        return fail(QCD_E_NOMEM, "quads buffer");
+    }
+    HIPCHK(hipMemcpy(dt, table, sizeof(uint4) * n, hipMemcpyHostToDevice));
+    uint32_t nb = align_up(n, kBlock) / kBlock;
+    if (prng == QCD_PRNG_XOR128) hipLaunchKernelGGL(k_xor128_produce, dim3(nb), dim3(kBlock), 0, 0, dt, n, samples, dq);
+    else hipLaunchKernelGGL(k_pm_produce, dim3(nb), dim3(kBlock), 0, 0, dt, n, samples, dq);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(quads, dq, qbytes, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(table, dt, sizeof(uint4) * n, hipMemcpyDeviceToHost));
+    (void)hipFree(dt);
+    (void)hipFree(dq);
+    return QCD_OK;
+}
+
+}  // extern "C"

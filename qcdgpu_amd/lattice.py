@@ -1,0 +1,235 @@
+"""Host-side mirror of QCDGPU's model driver for the update/measurement path.
+
+`Lattice` plays the role of model_CL::model (SU(N)/suncl/suncl.h:61-467) for
+this path: it holds the parameters the reference reads from init.dat and its
+hardcoded defaults (SU(N)/QCDGPU.cpp:98-235), owns one device lattice in
+libqcdhip.so, and runs the reference's loop (suncl.cpp:5212-5505): NAV
+thermalisation sweeps, then ITER measurements each after NITER sweeps, the
+first measurement on the start configuration.  Normalisations follow
+model::lattice_analysis (suncl.cpp:1328-1612).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _abi
+from ._abi import check, lib
+
+
+class Lattice:
+    def __init__(
+        self,
+        sun=3,
+        dims=(32, 32, 32, 8),
+        beta=5.9,
+        precision="fp32",
+        storage_reals=None,
+        nhit=10,
+        prng="XOR128",
+        ranlux_luxury=3,
+        stream="reference",
+        randseries=54,
+        max_measurements=1024,
+    ):
+        p = _abi.Params()
+        p.sun = sun
+        for i in range(4):
+            p.n[i] = int(dims[i])
+        p.precision = _abi.FP32 if precision == "fp32" else _abi.FP64
+        p.storage_reals = storage_reals or (18 if sun == 3 else 4)
+        p.beta = float(beta)
+        p.nhit = int(nhit)
+        p.nhitpar = 1
+        p.prng = {"XOR128": _abi.PRNG_XOR128, "RANLUX": _abi.PRNG_RANLUX, "PM": _abi.PRNG_PM}[prng]
+        p.ranlux_luxury = int(ranlux_luxury)
+        p.stream_mode = _abi.STREAM_REFERENCE if stream == "reference" else _abi.STREAM_FAST
+        p.randseries = int(randseries)
+        p.max_measurements = int(max_measurements)
+        self.params = p
+        self.sun = sun
+        self.dims = tuple(int(d) for d in dims)
+        self.volume = int(np.prod(self.dims))
+        self._h = ctypes.c_void_p()
+        check(lib().qcd_lattice_create(ctypes.byref(p), ctypes.byref(self._h)))
+
+    # -- lifetime -----------------------------------------------------------
+    def close(self):
+        if self._h:
+            check(lib().qcd_lattice_destroy(self._h))
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    # -- starts, PRNG, host copies -----------------------------------------
+    def init(self, start="cold"):
+        code = {"hot": _abi.START_HOT, "cold": _abi.START_COLD, "gid": _abi.START_GID}[start]
+        check(lib().qcd_lattice_init(self._h, code))
+
+    def prng_init(self):
+        check(lib().qcd_prng_init(self._h))
+
+    def link_shape(self):
+        return (self.volume, 4, self.sun, self.sun, 2)
+
+    def export_links(self):
+        a = np.zeros(self.link_shape())
+        check(lib().qcd_lattice_export(self._h, a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return a
+
+    def import_links(self, links):
+        a = np.ascontiguousarray(links, dtype=np.float64).reshape(self.link_shape())
+        check(lib().qcd_lattice_import(self._h, a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+
+    # -- updates --------------------------------------------------------------
+    def halfstep(self, parity, mu, kind="hb"):
+        check(lib().qcd_halfstep(self._h, parity, mu, _abi.HEATBATH if kind == "hb" else _abi.OVERRELAX))
+
+    def sweep(self, kind="hb"):
+        check(lib().qcd_sweep(self._h, _abi.HEATBATH if kind == "hb" else _abi.OVERRELAX))
+
+    def reunitarize(self):
+        check(lib().qcd_reunitarize(self._h))
+
+    def sync(self):
+        check(lib().qcd_lattice_sync(self._h))
+
+    # -- measurements ---------------------------------------------------------
+    def measure(self, plaquette=True, polyakov=True):
+        mask = (_abi.MEAS_PLAQUETTE if plaquette else 0) | (_abi.MEAS_POLYAKOV if polyakov else 0)
+        check(lib().qcd_measure(self._h, mask))
+
+    def reset_measurements(self):
+        check(lib().qcd_reset_measurements(self._h))
+
+    def raw_measurements(self):
+        cap = self.params.max_measurements
+        plq = np.zeros((cap, 2))
+        poly = np.zeros((cap, 4))
+        n1, n2 = ctypes.c_int(), ctypes.c_int()
+        D = ctypes.POINTER(ctypes.c_double)
+        check(
+            lib().qcd_read_measurements(
+                self._h, plq.ctypes.data_as(D), poly.ctypes.data_as(D), ctypes.byref(n1), ctypes.byref(n2)
+            )
+        )
+        return plq[: min(n1.value, cap)], poly[: min(n2.value, cap)]
+
+    def measurements(self):
+        """Normalised like model::lattice_analysis (suncl.cpp:1356-1373, 1483-1494):
+        Plq_spat/temp = sum / (3 V) (mean ReTr U_p, not divided by N),
+        Plq_total = (spat + temp) / 2, Polyakov = sum / (V_s N),
+        |P|^2 site average / N^2, |P|^4 site average / N^4."""
+        plq, poly = self.raw_measurements()
+        V = float(self.volume)
+        Vs = V / self.dims[3]
+        N = float(self.sun)
+        out = {}
+        if len(plq):
+            out["plq_spat"] = plq[:, 0] / (3 * V)
+            out["plq_temp"] = plq[:, 1] / (3 * V)
+            out["plq_total"] = 0.5 * (out["plq_spat"] + out["plq_temp"])
+        if len(poly):
+            out["polyakov_re"] = poly[:, 0] / (Vs * N)
+            out["polyakov_im"] = poly[:, 1] / (Vs * N)
+            out["polyakov_p2"] = poly[:, 2] / (Vs * N * N)
+            out["polyakov_p4"] = poly[:, 3] / (Vs * N ** 4)
+        return out
+
+    # -- graphs ---------------------------------------------------------------
+    def capture_cycle(self, n_hb=1, n_or=0, plaquette=True, polyakov=True):
+        mask = (_abi.MEAS_PLAQUETTE if plaquette else 0) | (_abi.MEAS_POLYAKOV if polyakov else 0)
+        check(lib().qcd_cycle_capture(self._h, n_hb, n_or, mask))
+
+    def run_cycle(self, n_hb=1, n_or=0, plaquette=True, polyakov=True):
+        """The measurement cycle enqueued eagerly (same kernels as the graph)."""
+        mask = (_abi.MEAS_PLAQUETTE if plaquette else 0) | (_abi.MEAS_POLYAKOV if polyakov else 0)
+        check(lib().qcd_cycle_run(self._h, n_hb, n_or, mask))
+
+    def launch_cycle(self, repeats=1):
+        check(lib().qcd_cycle_launch(self._h, repeats))
+
+    def time_cycle(self, repeats):
+        ms = ctypes.c_double()
+        check(lib().qcd_cycle_time(self._h, repeats, ctypes.byref(ms)))
+        return ms.value
+
+    def profile_cycle(self, repeats):
+        hb, orr, meas, tot = (ctypes.c_double() for _ in range(4))
+        check(
+            lib().qcd_cycle_profile(
+                self._h, repeats, ctypes.byref(hb), ctypes.byref(orr), ctypes.byref(meas), ctypes.byref(tot)
+            )
+        )
+        return {"hb_us": hb.value, "or_us": orr.value, "meas_us": meas.value, "total_ms": tot.value}
+
+    def tune(self, xcd_map=None, blocks_per_cu=None):
+        if xcd_map is not None:
+            check(lib().qcd_lattice_tune(self._h, _abi.TUNE_XCD_MAP, int(xcd_map)))
+        if blocks_per_cu is not None:
+            check(lib().qcd_lattice_tune(self._h, _abi.TUNE_BLOCKS_PER_CU, int(blocks_per_cu)))
+
+    def table_bytes(self):
+        b = ctypes.c_size_t()
+        check(lib().qcd_lattice_table_bytes(self._h, ctypes.byref(b)))
+        return b.value
+
+    # -- the reference run loop -------------------------------------------------
+    def simulate(self, nav=0, iters=2, niter=1, n_or=0, start="cold"):
+        """model::lattice_simulate: start, measure, NAV sweeps, then ITER-1
+        measurements each after NITER sweeps (suncl.cpp:5236-5483).  Each sweep
+        is one heatbath sweep plus `n_or` overrelaxation sweeps."""
+        self.init(start)
+        if start != "gid":
+            self.prng_init()
+        self.reset_measurements()
+        self.measure()
+        for _ in range(nav):
+            self.sweep("hb")
+            for _ in range(n_or):
+                self.sweep("or")
+        for _ in range(iters - 1):
+            for _ in range(niter):
+                self.sweep("hb")
+                for _ in range(n_or):
+                    self.sweep("or")
+            self.measure()
+        self.sync()
+        return self.measurements()
+
+
+def device_info():
+    name = ctypes.create_string_buffer(256)
+    cus = ctypes.c_int()
+    mem = ctypes.c_size_t()
+    check(lib().qcd_device_info(name, 256, ctypes.byref(cus), ctypes.byref(mem)))
+    return {"name": name.value.decode(), "compute_units": cus.value, "global_mem": mem.value}
+
+
+def prng_produce(prng, table, samples):
+    """Reference PRNG::produce buffer semantics on the device: returns
+    quads[samples, n, 4]; `table` (uint32 [n, 4]) is advanced in place."""
+    code = {"XOR128": _abi.PRNG_XOR128, "PM": _abi.PRNG_PM}[prng]
+    t = np.ascontiguousarray(table, dtype=np.uint32)
+    n = t.shape[0]
+    q = np.zeros((samples, n, 4), dtype=np.float32)
+    check(
+        lib().qcd_prng_produce_host(
+            code, 0, n, samples, t.ctypes.data_as(ctypes.c_void_p), q.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+        )
+    )
+    table[...] = t
+    return q

@@ -1,0 +1,334 @@
+"""GPU parity: libqcdhip.so (through the C ABI) against the oracle and the
+reference's own golden vectors.
+
+Bars (all per link, links are O(1) unitary matrices, so the max abs
+difference of any real component is the relative error):
+  FP64 links : <= 1e-12   FP32 links : <= 1e-5
+  PRNG streams and produce buffers : bit-exact
+  FP64 measurement sums : <= 1e-12 relative to the oracle on the same links.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import qcdgpu_amd as Q
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"fp64": 1e-12, "fp32": 1e-5}
+
+
+def _case(msg):
+    print("[case]", msg, flush=True)
+
+
+def _lat(sun, dims, prec, storage=None, beta=5.9, nhit=10, prng="XOR128", stream="reference", seed=54):
+    return Q.Lattice(
+        sun=sun,
+        dims=dims,
+        beta=beta,
+        precision=prec,
+        storage_reals=storage,
+        nhit=nhit,
+        prng=prng,
+        stream=stream,
+        randseries=seed,
+        max_measurements=64,
+    )
+
+
+# ---------------------------------------------------------------- PRNG buffers
+@pytest.mark.parametrize("prng", ["XOR128", "PM"])
+def test_prng_produce_bit_exact(O, prng):
+    _case("produce %s" % prng)
+    n, samples = 1000, 37
+    if prng == "XOR128":
+        t = O.xor128_seed_table(54, n)
+        ref = O.xor128_produce(t.copy(), samples)
+    else:
+        t = O.pm_seed_table(54, n)
+        ref = O.pm_produce(t.copy(), samples)
+    t_gpu = t.copy()
+    got = Q.prng_produce(prng, t_gpu, samples)
+    assert (got.view(np.uint32) == ref.view(np.uint32)).all()
+    # tables advanced identically: a second call continues the streams
+    ref2 = O.xor128_produce(t.copy(), 2 * samples)[samples:] if prng == "XOR128" else O.pm_produce(t.copy(), 2 * samples)[samples:]
+    got2 = Q.prng_produce(prng, t_gpu, samples)
+    assert (got2.view(np.uint32) == ref2.view(np.uint32)).all()
+
+
+# ---------------------------------------------------------------- GID known answer vs reference suncpp
+GID_CASES = [
+    ("su3_gid_4444", 3, (4, 4, 4, 4), 5.9, (1, 2)),
+    ("su3_gid_6424", 3, (6, 4, 2, 4), 5.9, (1,)),
+    ("su2_gid_4444", 2, (4, 4, 4, 4), 2.3, (1, 2)),
+]
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("storage", [None, 12])
+@pytest.mark.parametrize("case", GID_CASES, ids=[c[0] for c in GID_CASES])
+def test_gid_sweeps_vs_reference_suncpp(golden, case, prec, storage):
+    tag, sun, dims, beta, keep = case
+    if sun == 2 and storage == 12:
+        pytest.skip("12-real storage is SU(3) only")
+    _case("gid %s %s storage=%s" % (tag, prec, storage))
+    with _lat(sun, dims, prec, storage, beta=beta) as lat:
+        lat.init("gid")
+        V = int(np.prod(dims))
+        plq_ref = np.loadtxt(os.path.join(golden, "sweep_%s_plq.txt" % tag))
+        lat.measure(polyakov=False)
+        for k in range(1, max(keep) + 1):
+            lat.sweep("hb")
+            lat.measure(polyakov=False)
+            if k in keep:
+                ref = np.load(os.path.join(golden, "sweep_%s_s%d.npy" % (tag, k)))
+                got = lat.export_links()
+                err = np.abs(got - ref).max()
+                print("  sweep %d max|dU| = %.3e" % (k, err))
+                assert err <= TOL[prec] * (1 if k == 1 else 2)
+        m = lat.measurements()
+        for k in range(len(m["plq_spat"])):
+            rt = 1e-12 if prec == "fp64" else 1e-5
+            assert abs(m["plq_spat"][k] - plq_ref[k, 1]) <= rt * abs(plq_ref[k, 1])
+            assert abs(m["plq_temp"][k] - plq_ref[k, 2]) <= rt * abs(plq_ref[k, 2])
+
+
+# ---------------------------------------------------------------- reference stream layout vs oracle buffer semantics
+def _oracle_reference_stream_sweeps(O, L, links, prng, seed, nsweeps):
+    """The reference's GPU path restated: seed table for PRNG_instances =
+    PRNGSTEP, one produce() at initialisation, then one produce() of S quads
+    before every half-step; thread i reads quads[i + k*PRNGSTEP]."""
+    V = int(np.prod(list(L.n)))
+    prngstep = -(-(V // 2) // 64) * 64
+    total = -(-(3 * prngstep * (L.nhit + 1)) // 64) * 64
+    S = 1 + (total - 1) // prngstep
+    if prng == "XOR128":
+        table = O.xor128_seed_table(seed, prngstep)
+        produce = O.xor128_produce
+    else:
+        table = O.pm_seed_table(seed, prngstep)
+        produce = O.pm_produce
+    produce(table, S)
+    rejected = 0
+    for _ in range(nsweeps):
+        for parity in (1, 0):
+            for mu in range(4):
+                q = produce(table, S)
+                rejected += O.halfstep(L, links, parity, mu, O.HB, O.QuadRng(q, prngstep))
+    return rejected
+
+
+STREAM_CASES = [
+    (3, (6, 4, 2, 4), 5.9, "XOR128", 54),
+    (3, (4, 4, 4, 4), 6.0, "PM", 7),
+    (2, (6, 4, 2, 4), 2.3, "XOR128", 1),
+    (3, (8, 4, 4, 8), 5.9, "XOR128", 3),
+]
+
+
+@pytest.mark.parametrize("case", STREAM_CASES, ids=["%s-%s-%s" % (c[0], "x".join(map(str, c[1])), c[3]) for c in STREAM_CASES])
+def test_reference_stream_sweeps_fp64(O, case):
+    sun, dims, beta, prng, seed = case
+    _case("reference-stream fp64 %s" % (case,))
+    L = O.lattice(sun, dims, beta, 10)
+    ref = O.cold(L)
+    _oracle_reference_stream_sweeps(O, L, ref, prng, seed, 2)
+    with _lat(sun, dims, "fp64", beta=beta, prng=prng, seed=seed) as lat:
+        lat.init("cold")
+        lat.prng_init()
+        lat.sweep("hb")
+        lat.sweep("hb")
+        got = lat.export_links()
+    err = np.abs(got - ref).max()
+    print("  max|dU| = %.3e" % err)
+    assert err <= 1e-12
+
+
+def test_reference_stream_fp32_with_flip_count(O):
+    """FP32 links: links whose accept/reject decision matches the FP64
+    oracle agree to 1e-5; a decision that flips (a KP test within rounding of
+    its threshold) shows up as an O(1) difference and is counted -- none are
+    expected on this lattice."""
+    _case("reference-stream fp32 SU(3) 8x4x4x8")
+    dims = (8, 4, 4, 8)
+    L = O.lattice(3, dims, 5.9, 10)
+    ref = O.cold(L)
+    _oracle_reference_stream_sweeps(O, L, ref, "XOR128", 3, 1)
+    with _lat(3, dims, "fp32", seed=3) as lat:
+        lat.init("cold")
+        lat.prng_init()
+        lat.sweep("hb")
+        got = lat.export_links()
+    per_link = np.abs(got - ref).reshape(got.shape[0], 4, -1).max(axis=-1)
+    flips = int((per_link > 1e-3).sum())
+    print("  flips=%d max|dU| over agreeing links = %.3e" % (flips, per_link[per_link <= 1e-3].max()))
+    assert flips == 0
+    assert per_link.max() <= 1e-5
+
+
+def _thermal_config(O, sun, dims, beta, sweeps=3, seed=54):
+    """Cold start + a few sequential-RANLUX heatbath sweeps in the oracle."""
+    L = O.lattice(sun, dims, beta, 10)
+    a = O.cold(L)
+    rng = O.SeqRng("RANLUX", seed, 223)
+    for _ in range(sweeps):
+        O.sweep(L, a, O.HB, rng)
+    return L, a
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("sun", [3, 2])
+def test_overrelaxation_vs_oracle(O, sun, prec):
+    """Two overrelaxation sweeps of a heatbath-thermalised configuration.
+    (Overrelaxation is a deterministic map whose error amplification depends
+    on the configuration: on the rough GID start an SU(3) sweep amplifies
+    rounding by ~1e4, on thermalised links by < 1e2.)"""
+    _case("overrelax sun=%d %s" % (sun, prec))
+    dims = (4, 6, 2, 4)
+    beta = 5.9 if sun == 3 else 2.3
+    L, a = _thermal_config(O, sun, dims, beta)
+    if prec == "fp32":
+        a = a.astype(np.float32).astype(np.float64)
+    with _lat(sun, dims, prec, beta=beta) as lat:
+        lat.import_links(a)
+        lat.sweep("or")
+        lat.sweep("or")
+        got = lat.export_links()
+    O.sweep(L, a, O.OR, O.GidRng())
+    O.sweep(L, a, O.OR, O.GidRng())
+    err = np.abs(got - a).max()
+    print("  max|dU| = %.3e" % err)
+    assert err <= TOL[prec] * 2
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("sun", [3, 2])
+def test_measurements_vs_oracle(O, sun, prec):
+    """Plaquette and Polyakov sums on the GPU's own links vs the oracle on the
+    exported copy of the same links."""
+    _case("measure sun=%d %s" % (sun, prec))
+    dims = (6, 4, 4, 6)
+    beta = 5.9 if sun == 3 else 2.3
+    with _lat(sun, dims, prec, beta=beta, seed=11) as lat:
+        lat.init("cold")
+        lat.prng_init()
+        for _ in range(3):
+            lat.sweep("hb")
+            lat.sweep("or")
+        lat.measure()
+        links = lat.export_links()
+        plq, poly = lat.raw_measurements()
+    L = O.lattice(sun, dims, beta, 10)
+    p_ref = O.plaquette(L, links)
+    y_ref = O.polyakov(L, links)
+    assert np.allclose(plq[-1], p_ref, rtol=1e-12, atol=0)
+    scale = np.array([np.prod(dims[:3]) * sun, np.prod(dims[:3]) * sun, np.prod(dims[:3]) * sun ** 2, np.prod(dims[:3]) * sun ** 4])
+    tol = 1e-12 if prec == "fp64" else 1e-6
+    assert (np.abs(poly[-1] - y_ref) <= tol * scale).all()
+
+
+def test_graph_replay_equals_eager(O):
+    """A captured cycle (1 HB + 2 OR + measurements) replayed three times
+    gives bit-identical links and records to three eager runs of the same
+    cycle."""
+    _case("graph replay")
+    dims = (8, 4, 4, 8)
+    outs = []
+    for use_graph in (False, True):
+        with _lat(3, dims, "fp32", seed=5) as lat:
+            lat.init("cold")
+            lat.prng_init()
+            if use_graph:
+                lat.capture_cycle(1, 2)
+                lat.launch_cycle(1)
+                lat.sync()
+                lat.launch_cycle(2)
+            else:
+                for _ in range(3):
+                    lat.run_cycle(1, 2)
+            lat.sync()
+            outs.append((lat.export_links(), lat.raw_measurements()))
+    (a, (pa, ya)), (b, (pb, yb)) = outs
+    assert (a == b).all()
+    assert (pa == pb).all() and (ya == yb).all()
+    assert len(pa) == 3 and len(ya) == 3
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("sun", [3, 2])
+def test_fused_plaquette_equals_standalone(O, sun, prec):
+    """The plaquette accumulated inside the last sweep's even Y/Z/T
+    half-steps equals the standalone plaquette kernel and the oracle on the
+    same links (FP64 links: 1e-12 relative; FP32 links: the fused path
+    multiplies in FP32 and sums in FP64, 1e-6 relative)."""
+    _case("fused plaquette sun=%d %s" % (sun, prec))
+    dims = (6, 4, 4, 6)
+    beta = 5.9 if sun == 3 else 2.3
+    with _lat(sun, dims, prec, beta=beta, seed=17) as lat:
+        lat.init("cold")
+        lat.prng_init()
+        lat.run_cycle(2, 1, plaquette=True, polyakov=False)
+        lat.measure(polyakov=False)
+        lat.sync()
+        plq, _ = lat.raw_measurements()
+        links = lat.export_links()
+    L = O.lattice(sun, dims, beta, 10)
+    ref = O.plaquette(L, links)
+    rt = 1e-12 if prec == "fp64" else 1e-6
+    assert len(plq) == 2
+    assert np.allclose(plq[1], ref, rtol=1e-12)
+    assert np.allclose(plq[0], ref, rtol=rt)
+
+
+def test_fast_stream_mode_statistics():
+    """Fast stream mode (no skip to S) is a different but equally valid
+    stream: plaquette after thermalisation agrees with reference mode within
+    statistical errors, and links stay unitary."""
+    _case("fast stream")
+    dims = (8, 8, 8, 4)
+    res = {}
+    for mode in ("reference", "fast"):
+        with _lat(3, dims, "fp32", beta=5.7, stream=mode, seed=9) as lat:
+            lat.init("cold")
+            lat.prng_init()
+            for _ in range(40):
+                lat.sweep("hb")
+                lat.sweep("or")
+            for _ in range(20):
+                lat.sweep("hb")
+                lat.sweep("or")
+                lat.measure(polyakov=False)
+            lat.sync()
+            res[mode] = lat.measurements()["plq_total"] / 3.0
+            U = lat.export_links()
+            Uc = U[..., 0] + 1j * U[..., 1]
+            dev = np.abs(np.einsum("...ji,...jk->...ik", Uc.conj(), Uc) - np.eye(3)).max()
+            assert dev < 1e-5
+    a, b = res["reference"], res["fast"]
+    err = np.hypot(a.std(), b.std()) / np.sqrt(5)
+    assert abs(a.mean() - b.mean()) < 4 * err + 1e-4
+
+
+def test_c1_size_invariants():
+    """The headline shape (SU(3) 32^3 x 8, beta 5.9, FP32): links stay in SU(3)
+    to FP32 precision, the plaquette drops from 3 into the physical range, and
+    the measurement records count up."""
+    _case("C1 shape invariants")
+    with _lat(3, (32, 32, 32, 8), "fp32", beta=5.9, seed=54) as lat:
+        lat.init("cold")
+        lat.prng_init()
+        lat.capture_cycle(1, 4)
+        lat.launch_cycle(10)
+        lat.sync()
+        m = lat.measurements()
+        assert len(m["plq_total"]) == 10
+        p = m["plq_total"] / 3.0
+        assert (np.diff(p[:3]) < 0).all()
+        assert 0.5 < p[-1] < 0.75
+        assert abs(m["polyakov_re"][-1]) < 0.5
+        U = lat.export_links()
+        Uc = U[..., 0] + 1j * U[..., 1]
+        dev = np.abs(np.einsum("...ji,...jk->...ik", Uc[::97].conj(), Uc[::97]) - np.eye(3)).max()
+        assert dev < 2e-6
