@@ -36,7 +36,7 @@ HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(REPO, "profiles", "or_halfstep_su3_32c8_fp32_traffic.json")
 
 
-def cpu_baseline(sweeps=3):
+def cpu_baseline(sweeps=6):
     """The reference's own CPU path on one core (kind 'reference'), or the
     oracle restatement when the reference build is absent (kind 'port')."""
     V = 1
@@ -73,6 +73,21 @@ def cpu_baseline(sweeps=3):
     }
 
 
+def copy_calibration(mib=256, repeats=20):
+    """A plain copy of known size: each launch reads and writes exactly
+    `mib` MiB, so its FETCH_SIZE / WRITE_SIZE per dispatch give the factors
+    that turn those counters into HBM bytes."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(repeats):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "launches": repeats}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -82,7 +97,16 @@ def main():
     ap.add_argument("--stream", default="reference", choices=("reference", "fast"))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=50)
+    ap.add_argument(
+        "--copy-calibration",
+        action="store_true",
+        help="only run a plain 256 MiB device-to-device copy 20 times (the known-size "
+        "kernel that scales the FETCH_SIZE / WRITE_SIZE counters)",
+    )
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(
             "bench.py: this build measures one GPU; --gpus %d is not supported "

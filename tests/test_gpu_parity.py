@@ -66,13 +66,14 @@ GID_CASES = [
 ]
 
 
+# 12-real storage exists for SU(3) only.
+GID_STORAGE = [(c, st) for c in GID_CASES for st in ((None, 12) if c[1] == 3 else (None,))]
+
+
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
-@pytest.mark.parametrize("storage", [None, 12])
-@pytest.mark.parametrize("case", GID_CASES, ids=[c[0] for c in GID_CASES])
-def test_gid_sweeps_vs_reference_suncpp(golden, case, prec, storage):
+@pytest.mark.parametrize("case,storage", GID_STORAGE, ids=["%s-%s" % (c[0], st or "full") for c, st in GID_STORAGE])
+def test_gid_sweeps_vs_reference_suncpp(golden, case, storage, prec):
     tag, sun, dims, beta, keep = case
-    if sun == 2 and storage == 12:
-        pytest.skip("12-real storage is SU(3) only")
     _case("gid %s %s storage=%s" % (tag, prec, storage))
     with _lat(sun, dims, prec, storage, beta=beta) as lat:
         lat.init("gid")
