@@ -101,7 +101,9 @@ int  qcd_lattice_sync(qcd_lattice *lat);
 /* Launch-shape knobs of the update kernels (set by qcdgpu_amd.autotune, which
    times candidates on the device; both only change scheduling, never results):
    QCD_TUNE_XCD_MAP       1: each of the 8 XCDs gets one contiguous range of
-                          compact site indices (default), 0: hardware order.
+                          compact site indices, an x-slab (default);
+                          2: the same slabs, dispatched cross-section first
+                          (x, then t, then z) for L2 reuse; 0: hardware order.
    QCD_TUNE_BLOCKS_PER_CU cap on resident update workgroups per CU (reserved
                           LDS), 0 = no cap (default).                        */
 enum { QCD_TUNE_XCD_MAP = 1, QCD_TUNE_BLOCKS_PER_CU = 2 };

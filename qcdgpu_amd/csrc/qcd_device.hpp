@@ -210,12 +210,58 @@ template <typename R> __device__ __forceinline__ void su3_row3(M<R, 3>& a)
 
 // Gram-Schmidt reunitarisation in FP64 (rows 1-2 orthonormalised, row 3 =
 // conj(row1 x row2)), the algorithm of suncpp/su3/algebra_su3.cpp:185-226.
+// ------------------------------------------------------------------ fast math
+// 1/sqrt(x), x > 0, in double: the hardware estimate refined by two Newton
+// steps y <- y + (y/2)(1 - x y^2) (quadratic convergence; about one ulp).
+__device__ __forceinline__ double rsqrt_d(double x)
+{
+    double y = __builtin_amdgcn_rsq(x);
+    y = fma(0.5 * y, fma(-x * y, y, 1.0), y);
+    y = fma(0.5 * y, fma(-x * y, y, 1.0), y);
+    return y;
+}
+
+// sin and cos of 2*pi*x for |x| <= 1 (x in revolutions).  Quadrant
+// n = rint(4x); f = x - n/4 lies in [-1/8, 1/8] and is exact; theta = 2*pi*f;
+// Taylor polynomials through theta^9 (sin) and theta^8 (cos) have truncation
+// error below 3e-8 on |theta| <= pi/4; the result is rotated by n quarter
+// turns.
+__host__ __device__ __forceinline__ void sincos2pi_f(float x, float* sn, float* cs)
+{
+    const float n = rintf(4.0f * x);
+    const float f = fmaf(n, -0.25f, x);
+    const float t = f * 6.28318530717958647692f;
+    const float t2 = t * t;
+    float s = fmaf(t2, 2.75573192e-6f, -1.98412698e-4f);
+    s = fmaf(t2, s, 8.33333333e-3f);
+    s = fmaf(t2, s, -1.66666667e-1f);
+    s = fmaf(t2 * t, s, t);
+    float c = fmaf(t2, 2.48015873e-5f, -1.38888889e-3f);
+    c = fmaf(t2, c, 4.16666667e-2f);
+    c = fmaf(t2, c, -0.5f);
+    c = fmaf(t2, c, 1.0f);
+    const int q = (int)n & 3;
+    float so = s, co = c;
+    if (q == 1) {
+        so = c;
+        co = -s;
+    } else if (q == 2) {
+        so = -s;
+        co = -c;
+    } else if (q == 3) {
+        so = -c;
+        co = s;
+    }
+    *sn = so;
+    *cs = co;
+}
+
 __device__ __forceinline__ void su3_gram_schmidt(M<double, 3>& a)
 {
     double nu = 0.0;
 #pragma unroll
     for (int k = 0; k < 3; ++k) nu += a.e[0][k].re * a.e[0][k].re + a.e[0][k].im * a.e[0][k].im;
-    double inu = 1.0 / sqrt(nu);
+    double inu = rsqrt_d(nu);
 #pragma unroll
     for (int k = 0; k < 3; ++k) a.e[0][k] = {a.e[0][k].re * inu, a.e[0][k].im * inu};
     C<double> sp = cmulc(a.e[1][0], a.e[0][0]);
@@ -226,7 +272,7 @@ __device__ __forceinline__ void su3_gram_schmidt(M<double, 3>& a)
     double nv = 0.0;
 #pragma unroll
     for (int k = 0; k < 3; ++k) nv += a.e[1][k].re * a.e[1][k].re + a.e[1][k].im * a.e[1][k].im;
-    double inv = 1.0 / sqrt(nv);
+    double inv = rsqrt_d(nv);
 #pragma unroll
     for (int k = 0; k < 3; ++k) a.e[1][k] = {a.e[1][k].re * inv, a.e[1][k].im * inv};
     su3_row3(a);
@@ -237,7 +283,7 @@ __device__ __forceinline__ void su2_normalize(M<double, 2>& a)
 {
     double n = a.e[0][0].re * a.e[0][0].re + a.e[0][0].im * a.e[0][0].im + a.e[0][1].re * a.e[0][1].re +
                a.e[0][1].im * a.e[0][1].im;
-    double in = 1.0 / sqrt(n);
+    double in = rsqrt_d(n);
     a.e[0][0] = {a.e[0][0].re * in, a.e[0][0].im * in};
     a.e[0][1] = {a.e[0][1].re * in, a.e[0][1].im * in};
     a.e[1][0] = {-a.e[0][1].re, a.e[0][1].im};
@@ -371,6 +417,27 @@ template <typename R, int N, int NREAL> struct Table {
             a.e[1][1] = {v0.x, -v0.y};
         }
         return a;
+    }
+    // Stored planes only (12-real SU(3): rows 1 and 2; call finish() for
+    // row 3), so that a batch of gathers is not split by arithmetic.
+    // PRECONDITION: p and mu are the same in every active lane.
+    __device__ __forceinline__ M<R, N> load_raw_u(int p, int mu, uint32_t i) const
+    {
+        M<R, N> a;
+        if constexpr (N == 3) {
+#pragma unroll
+            for (int k = 0; k < NP; ++k) {
+                V2 v = ld(plane(p, mu, k), i);
+                a.e[k / 3][k % 3] = {v.x, v.y};
+            }
+        } else {
+            a = load_t<true>(p, mu, i);
+        }
+        return a;
+    }
+    __device__ __forceinline__ static void finish(M<R, N>& a)
+    {
+        if constexpr (N == 3 && NP == 6) su3_row3(a);
     }
     // Load with wavefront-uniform parity p and direction mu (scalar plane
     // base).  PRECONDITION: p and mu are the same in every active lane.

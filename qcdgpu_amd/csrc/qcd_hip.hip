@@ -56,14 +56,16 @@ struct UpdArgs {
     double beta;          // BETA / N
     int nhit;
     uint32_t sites;       // V, for GID mode
-    uint4* rng;           // per-stream state (stream = compact index)
+    uint4* rng;           // per-stream state at the start of the half-step (stream = compact index)
+    uint4* rng_out;       // state after the half-step (the two buffers alternate)
     uint32_t s_quads;     // quads per heatbath half-step per stream (0 = fast)
     // fused plaquette measurement (last sweep of a measurement cycle):
     // partials[(meas_mode - 1) * gridDim.x + block] = sum over the block of
     // ReTr(U' * sum_{nu<mu} S_nu); modes 1, 2, 3 = even Y, Z, T half-steps.
     double* partials;
     int meas_mode;
-    int xcd_map;          // 1: contiguous block range per XCD, 0: hardware order
+    int xcd_map;          // 0: hardware order, 1: one x-slab per XCD, 2: 1 + cross-section-first order
+    uint32_t ord_ng, ord_nt, ord_xs;   // xcd_map 2: z-groups, N4, x-values per XCD slab
 };
 
 // Hardware dispatches workgroup b to XCD (b mod 8).  Remap so that each XCD
@@ -109,15 +111,6 @@ struct SrcXor {
         used++;
         (void)xor128_quad(s);
     }
-    __device__ void finish(uint32_t s_quads)
-    {
-        for (uint32_t k = used; k < s_quads; ++k) {
-            (void)xor128_step(s);
-            (void)xor128_step(s);
-            (void)xor128_step(s);
-            (void)xor128_step(s);
-        }
-    }
 };
 
 struct SrcPm {
@@ -133,10 +126,6 @@ struct SrcPm {
     {
         used++;
         (void)pm_quad(s);
-    }
-    __device__ void finish(uint32_t s_quads)
-    {
-        for (uint32_t k = used; k < s_quads; ++k) (void)pm_quad(s);
     }
 };
 
@@ -160,8 +149,11 @@ __device__ __forceinline__ bool kp_update(const T* b, T beta, int nhit, Src& src
     const T My = b[1] - b[7];
     const T Mz = b[2] - b[4];
     const T Mw = b[3] + b[5];
-    const T det = sqrt(Mx * Mx + My * My + Mz * Mz + Mw * Mw);
-    const T id = T(1) / det;
+    const T n2 = Mx * Mx + My * My + Mz * Mz + Mw * Mw;
+    T id;
+    if constexpr (sizeof(T) == 4) id = __builtin_amdgcn_rsqf(n2);   // one ulp
+    else id = rsqrt_d(n2);
+    const T det = n2 * id;
     const T a00r = Mx * id, a00i = -My * id, a01r = -Mz * id, a01i = -Mw * id;
     // aH = [[a00, a01], [-conj(a01), conj(a00)]]
     T c00r, c00i, c01r, c01i;
@@ -173,16 +165,23 @@ __device__ __forceinline__ bool kp_update(const T* b, T beta, int nhit, Src& src
     } else {
         const T pi2 = (T)kPi2;
         const T bdet = beta * det;
+        const T ibdet = T(1) / bdet;
         bool flag = false;
         T delta = T(0);
         for (int i = 0; i < nhit && !flag; ++i) {
             float4 r = src.try_quad();
-            T cosrnd = cos(pi2 * (T)r.y);
+            T cosrnd;
+            if constexpr (sizeof(T) == 4) {
+                float sn;
+                sincos2pi_f(r.y, &sn, &cosrnd);
+            } else {
+                cosrnd = cos(pi2 * (T)r.y);
+            }
             if constexpr (RK == RK_GID) {
-                delta = cosrnd * cosrnd / bdet;
+                delta = cosrnd * cosrnd * ibdet;
                 flag = true;
             } else {
-                delta = -(log(T(1) - (T)r.x) + cosrnd * cosrnd * log(T(1) - (T)r.z)) / bdet;
+                delta = -(log(T(1) - (T)r.x) + cosrnd * cosrnd * log(T(1) - (T)r.z)) * ibdet;
                 float w2 = r.w * r.w;
                 flag = ((T)w2 <= (T(1) - T(0.5) * delta));
             }
@@ -192,20 +191,33 @@ __device__ __forceinline__ bool kp_update(const T* b, T beta, int nhit, Src& src
             return false;
         }
         float4 r = src.accept_quad();
-        // sqrt(1 - c^2) near |c| = 1 loses the digits a float cannot carry:
-        // these four are formed in double for both link precisions (the
-        // reference's FP32 kernel does the same through its double literals,
-        // su3_update_cl.cl:650-654).
-        const double cosal = 1.0 - (double)delta;
-        const double costh = 2.0 * (double)r.x - 1.0;
-        const double sinth = sqrt(1.0 - costh * costh);
-        const double sinal = sqrt(1.0 - cosal * cosal);
-        T sinphi, cosphi;
-        sincos_t(pi2 * (T)r.y, &sinphi, &cosphi);
-        c00r = (T)cosal;
-        c00i = (T)(sinal * costh);
-        c01r = (T)(sinal * sinth) * sinphi;
-        c01i = (T)(sinal * sinth) * cosphi;
+        // a0 = cos(alpha) = 1 - delta, cos(theta) = 2 r.x - 1, phi = 2 pi r.y.
+        // The sines are formed without the cancellation of sqrt(1 - c^2)
+        // near |c| = 1: sin(alpha) = sqrt(delta (2 - delta)) and
+        // sin(theta) = 2 sqrt(r.x (1 - r.x)), so FP32 keeps full relative
+        // precision; FP64 links form them in double as sqrt(1 - c^2).
+        if constexpr (sizeof(T) == 4) {
+            const float costh = fmaf(2.0f, r.x, -1.0f);
+            const float sinth = 2.0f * __builtin_amdgcn_sqrtf(r.x * (1.0f - r.x));
+            const float sinal = __builtin_amdgcn_sqrtf(delta * (2.0f - delta));
+            float sinphi, cosphi;
+            sincos2pi_f(r.y, &sinphi, &cosphi);
+            c00r = 1.0f - delta;
+            c00i = sinal * costh;
+            c01r = (sinal * sinth) * sinphi;
+            c01i = (sinal * sinth) * cosphi;
+        } else {
+            const double cosal = 1.0 - (double)delta;
+            const double costh = 2.0 * (double)r.x - 1.0;
+            const double sinth = sqrt(1.0 - costh * costh);
+            const double sinal = sqrt(1.0 - cosal * cosal);
+            T sinphi, cosphi;
+            sincos_t(pi2 * (T)r.y, &sinphi, &cosphi);
+            c00r = (T)cosal;
+            c00i = (T)(sinal * costh);
+            c01r = (T)(sinal * sinth) * sinphi;
+            c01i = (T)(sinal * sinth) * cosphi;
+        }
     }
     // out = c * aH with c = [[c00, c01], [-conj(c01), conj(c00)]]
     const T a10r = -a01r, a10i = a01i, a11r = a00r, a11i = -a00i;
@@ -253,31 +265,43 @@ __device__ __forceinline__ C<R> ua_entry(const M<R, 3>& U, const M<R, 3>& A, int
 //   S_nu = U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+ + U_nu(x+mu-nu)^+ U_mu(x-nu)^+ U_nu(x-nu)
 // (suncpp/Update/sun_update.h:44-77; su3_update_cl.cl:341-565).  nu, mu and
 // the parity p are wavefront-uniform.
-template <typename R, int N, int NREAL>
+template <typename R, int N, int NREAL, typename Hook>
 __device__ __forceinline__ M<R, N> staple_nu(const Table<R, N, NREAL>& T, const Geom& g, Site s, uint32_t i, int p,
-                                             int mu, int nu)
+                                             int mu, int nu, Hook&& while_loading)
 {
+    using Tab = Table<R, N, NREAL>;
     const int q = p ^ 1;
+    // All six gathers are issued as one batch, grouped by plane, before any
+    // arithmetic (the scheduling barrier keeps the compiler from splitting
+    // the batch to save registers): wavefronts that share a neighbour line
+    // then request it at the same point of their programs, while it is
+    // still in L2.
+    const uint32_t i_pmu = nb_index(g, s, mu, +1);
+    const uint32_t i_pnu = nb_index(g, s, nu, +1);
+    const uint32_t i_mnu = nb_index(g, s, nu, -1);
+    const uint32_t i_pmu_mnu = nb2_index(g, s, mu, +1, nu, -1);
+    M<R, N> f2 = T.load_raw_u(q, mu, i_pnu);
+    M<R, N> b1 = T.load_raw_u(q, mu, i_mnu);
+    M<R, N> f1 = T.load_raw_u(q, nu, i_pmu);
+    M<R, N> b3 = T.load_raw_u(q, nu, i_mnu);
+    M<R, N> f3 = T.load_raw_u(p, nu, i);
+    M<R, N> b2 = T.load_raw_u(p, nu, i_pmu_mnu);
+    __builtin_amdgcn_sched_barrier(0);
+    // Independent work placed here runs while the gathers are in flight.
+    while_loading();
+    __builtin_amdgcn_sched_barrier(0);
+    Tab::finish(f1);
+    Tab::finish(f2);
+    Tab::finish(f3);
+    Tab::finish(b1);
+    Tab::finish(b2);
+    Tab::finish(b3);
     M<R, N> S;
     m_zero(S);
-    {
-        const uint32_t i_pmu = nb_index(g, s, mu, +1);
-        const uint32_t i_pnu = nb_index(g, s, nu, +1);
-        const M<R, N> f1 = T.load_u(q, nu, i_pmu);
-        const M<R, N> f2 = T.load_u(q, mu, i_pnu);
-        const M<R, N> tf = m_mul_nd(f1, f2);
-        const M<R, N> f3 = T.load_u(p, nu, i);
-        m_acc_nd(S, tf, f3);
-    }
-    {
-        const uint32_t i_mnu = nb_index(g, s, nu, -1);
-        const uint32_t i_pmu_mnu = nb2_index(g, s, mu, +1, nu, -1);
-        const M<R, N> b1 = T.load_u(q, mu, i_mnu);
-        const M<R, N> b2 = T.load_u(p, nu, i_pmu_mnu);
-        const M<R, N> tb = m_mul(b1, b2);
-        const M<R, N> b3 = T.load_u(q, nu, i_mnu);
-        m_acc_dn(S, tb, b3);
-    }
+    const M<R, N> tf = m_mul_nd(f1, f2);
+    m_acc_nd(S, tf, f3);
+    const M<R, N> tb = m_mul(b1, b2);
+    m_acc_dn(S, tb, b3);
     return S;
 }
 
@@ -302,6 +326,26 @@ __device__ __forceinline__ double wave_sum(double v)
 constexpr int kLinks = 64;
 constexpr int kUpdThreads = 3 * kLinks;
 
+// Workgroup -> 64-link block.  Mode 1: XCD k (hardware block b goes to XCD
+// b mod 8) gets the k-th contiguous eighth of the compact index range, an
+// x-slab of the lattice.  Mode 2: the same slabs, but within a slab blocks
+// are dispatched cross-section first (x offset fastest, then t, then the
+// group of z rows a block covers), so a block's x, t and z neighbours were
+// dispatched at most one cross-section (N4 * x-values blocks) earlier and
+// their links are still in the XCD's L2.
+__device__ __forceinline__ uint32_t upd_block(const UpdArgs& a)
+{
+    const uint32_t b = blockIdx.x, nb = gridDim.x;
+    if (a.xcd_map == 0 || (nb % 8u)) return b;
+    const uint32_t per = nb / 8u, k = b % 8u, l = b / 8u;
+    if (a.xcd_map == 2) {
+        const uint32_t c = l % a.ord_xs, r = l / a.ord_xs;
+        const uint32_t t = r % a.ord_nt, zg = r / a.ord_nt;
+        return k * per + zg + a.ord_ng * (t + a.ord_nt * c);
+    }
+    return k * per + l;
+}
+
 template <typename R, int N, int NREAL, bool OVER, int RK, bool MEAS>
 __global__ __launch_bounds__(kUpdThreads) void k_update(UpdArgs a)
 {
@@ -310,7 +354,7 @@ __global__ __launch_bounds__(kUpdThreads) void k_update(UpdArgs a)
     __shared__ R lds[2][NC][kLinks];
     const int w = threadIdx.x >> 6;          // wavefront index: uniform
     const int lane = threadIdx.x & 63;
-    const uint32_t blk = a.xcd_map ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+    const uint32_t blk = upd_block(a);
     const uint32_t i = blk * kLinks + lane;
     const bool active = i < a.g.vh;
     Table<R, N, NREAL> Tb{(typename Vec2<R>::T*)a.table, a.g.vh_pad};
@@ -318,9 +362,35 @@ __global__ __launch_bounds__(kUpdThreads) void k_update(UpdArgs a)
     const int nu = w + (w >= mu ? 1 : 0);
     const Site s = site_of(a.g, active ? i : 0u, p);
 
+    // Generator state at the start of the half-step: wavefront 0 consumes
+    // it; in the reference stream layout wavefront 1 advances every stream
+    // by exactly s_quads quads (the reference's produce() of the whole
+    // buffer), hidden under its gather latency, and writes the result to
+    // the other state buffer.  In the fast layout wavefront 0 writes the
+    // state it ends with.
+    constexpr bool kStream = (RK == RK_XOR128 || RK == RK_PM);
+    uint4 rs = make_uint4(0, 0, 0, 0);
+    if constexpr (kStream) {
+        if ((w == 0 && active) || (w == 1 && a.s_quads)) rs = a.rng[i];
+    }
+    auto skip_streams = [&]() {
+        if constexpr (kStream) {
+            if (w == 1 && a.s_quads) {
+                if constexpr (RK == RK_XOR128) {
+                    for (uint32_t k = 0; k < a.s_quads; ++k) (void)xor128_quad(rs);
+                } else {
+                    for (uint32_t k = 0; k < a.s_quads; ++k) (void)pm_quad(rs);
+                }
+                a.rng_out[i] = rs;
+            }
+        }
+    };
     M<R, N> S;
-    if (active) S = staple_nu<R, N, NREAL>(Tb, a.g, s, i, p, mu, nu);
-    else m_zero(S);
+    if (active) S = staple_nu<R, N, NREAL>(Tb, a.g, s, i, p, mu, nu, skip_streams);
+    else {
+        skip_streams();
+        m_zero(S);
+    }
     if (w > 0) {
 #pragma unroll
         for (int r = 0; r < N; ++r)
@@ -365,10 +435,10 @@ __global__ __launch_bounds__(kUpdThreads) void k_update(UpdArgs a)
             sg.fs = (double)a.sites;
             sg.nhit = a.nhit;
         } else if constexpr (RK == RK_XOR128) {
-            sx.s = a.rng[i];
+            sx.s = rs;
             sx.used = 0;
         } else if constexpr (RK == RK_PM) {
-            sp.s = a.rng[i];
+            sp.s = rs;
             sp.used = 0;
         }
         const T beta = (T)a.beta;
@@ -431,12 +501,9 @@ __global__ __launch_bounds__(kUpdThreads) void k_update(UpdArgs a)
         Tb.store_u(p, mu, i, U);
         if constexpr (MEAS) trace = m_retr_mul(U, Alow);
 
-        if constexpr (RK == RK_XOR128) {
-            sx.finish(a.s_quads);
-            a.rng[i] = sx.s;
-        } else if constexpr (RK == RK_PM) {
-            sp.finish(a.s_quads);
-            a.rng[i] = sp.s;
+        if (a.s_quads == 0) {
+            if constexpr (RK == RK_XOR128) a.rng_out[i] = sx.s;
+            else if constexpr (RK == RK_PM) a.rng_out[i] = sp.s;
         }
     }
     if constexpr (MEAS) {
@@ -761,7 +828,8 @@ struct qcd_lattice {
     size_t elem;           // bytes per complex value
     void* table = nullptr;
     size_t table_bytes = 0;
-    uint4* rng = nullptr;
+    uint4* rng_buf[2] = {nullptr, nullptr};   // generator states; rng_buf[rng_cur] is current
+    int rng_cur = 0;
     uint32_t n_streams = 0;
     uint32_t s_quads = 0;   // S
     double* partials = nullptr;     // one allocation, three regions:
@@ -779,7 +847,7 @@ struct qcd_lattice {
     int cyc_hb = 0, cyc_or = 0, cyc_mask = 0;
     bool rng_ready = false;
     bool gid_updates = false;   // INTS = 2: GID known-answer numbers in the heatbath
-    int tune_xcd_map = 1;       // QCD_TUNE_XCD_MAP
+    int tune_xcd_map = 2;       // QCD_TUNE_XCD_MAP
     int tune_blocks_per_cu = 0; // QCD_TUNE_BLOCKS_PER_CU (0 = no cap)
 };
 
@@ -841,11 +909,30 @@ int launch_update_rn(qcd_lattice* L, int parity, int mu, int kind, int start_gid
     a.beta = L->p.beta / L->p.sun;
     a.nhit = L->p.nhit;
     a.sites = 2u * L->g.vh;
-    a.rng = L->rng;
+    a.rng = L->rng_buf[L->rng_cur];
+    a.rng_out = L->rng_buf[L->rng_cur ^ 1];
     a.s_quads = (L->p.stream_mode == QCD_STREAM_REFERENCE) ? L->s_quads : 0u;
     a.partials = L->part_fused;
     a.meas_mode = o.meas_mode;
     a.xcd_map = L->tune_xcd_map;
+    a.ord_ng = a.ord_nt = a.ord_xs = 1;
+    if (a.xcd_map == 2) {
+        // Cross-section order needs whole y-rows per block, whole z-groups
+        // and whole x-slabs per XCD; otherwise fall back to mode 1.
+        const uint32_t h = (uint32_t)L->g.h, nb = L->g.vh / kLinks;
+        const uint32_t rb = (h <= kLinks && kLinks % h == 0) ? kLinks / h : 0;
+        bool ok = rb && (L->g.vh % kLinks == 0) && (nb % 8 == 0) && ((uint32_t)L->g.n3 % rb == 0);
+        if (ok) {
+            const uint32_t ng = (uint32_t)L->g.n3 / rb, nt = (uint32_t)L->g.n4, per = nb / 8;
+            ok = per % (ng * nt) == 0;
+            if (ok) {
+                a.ord_ng = ng;
+                a.ord_nt = nt;
+                a.ord_xs = per / (ng * nt);
+            }
+        }
+        if (!ok) a.xcd_map = 1;
+    }
     LaunchOpt ol = o;
     if (L->tune_blocks_per_cu > 0) {
         // 160 KiB of LDS per CU on gfx950; reserve so at most k workgroups fit.
@@ -862,6 +949,7 @@ int launch_update_rn(qcd_lattice* L, int parity, int mu, int kind, int start_gid
         if (L->p.prng == QCD_PRNG_XOR128) launch_update_t<R, N, NREAL, false, RK_XOR128>(L, a, ol);
         else if (L->p.prng == QCD_PRNG_PM) launch_update_t<R, N, NREAL, false, RK_PM>(L, a, ol);
         else return fail(QCD_E_UNSUPPORTED, "generator %d not available for fused updates", L->p.prng);
+        L->rng_cur ^= 1;   // the kernel wrote the states into the other buffer
     }
     HIPCHK(hipGetLastError());
     return QCD_OK;
@@ -1025,7 +1113,8 @@ int qcd_lattice_create(const qcd_params* pp, qcd_lattice** out)
     L->nb_fused = align_up(L->g.vh, kLinks) / kLinks;
     size_t part_doubles = (size_t)3 * L->nb_fused + (size_t)2 * 2 * L->upd_blocks + (size_t)4 * L->poly_blocks;
     if (hipMalloc(&L->table, L->table_bytes) != hipSuccess ||
-        hipMalloc((void**)&L->rng, sizeof(uint4) * L->n_streams) != hipSuccess ||
+        hipMalloc((void**)&L->rng_buf[0], sizeof(uint4) * L->n_streams) != hipSuccess ||
+        hipMalloc((void**)&L->rng_buf[1], sizeof(uint4) * L->n_streams) != hipSuccess ||
         hipMalloc((void**)&L->partials, sizeof(double) * part_doubles) != hipSuccess ||
         hipMalloc((void**)&L->rec_index, sizeof(int) * 4) != hipSuccess ||
         hipMalloc((void**)&L->plq_rec, sizeof(double) * 2 * p.max_measurements) != hipSuccess ||
@@ -1053,7 +1142,8 @@ int qcd_lattice_destroy(qcd_lattice* L)
     if (L->exec) (void)hipGraphExecDestroy(L->exec);
     if (L->graph) (void)hipGraphDestroy(L->graph);
     (void)hipFree(L->table);
-    (void)hipFree(L->rng);
+    (void)hipFree(L->rng_buf[0]);
+    (void)hipFree(L->rng_buf[1]);
     (void)hipFree(L->partials);
     (void)hipFree(L->rec_index);
     (void)hipFree(L->plq_rec);
@@ -1075,7 +1165,7 @@ int qcd_lattice_tune(qcd_lattice* L, int knob, int value)
     if (!L) return fail(QCD_E_ARG, "null lattice");
     switch (knob) {
         case QCD_TUNE_XCD_MAP:
-            if (value != 0 && value != 1) return fail(QCD_E_ARG, "xcd map must be 0 or 1");
+            if (value < 0 || value > 2) return fail(QCD_E_ARG, "xcd map must be 0, 1 or 2");
             L->tune_xcd_map = value;
             break;
         case QCD_TUNE_BLOCKS_PER_CU:
@@ -1125,15 +1215,16 @@ int qcd_prng_init(qcd_lattice* L)
     // (random.cpp:867-876,892-894).
     for (size_t k = 0; k < tab.size(); ++k)
         tab[k] = (L->p.prng == QCD_PRNG_XOR128) ? (uint32_t)rand() : (uint32_t)(rand() % 2147483647);
-    HIPCHK(hipMemcpyAsync(L->rng, tab.data(), sizeof(uint32_t) * tab.size(), hipMemcpyHostToDevice, L->stream));
+    L->rng_cur = 0;
+    HIPCHK(hipMemcpyAsync(L->rng_buf[0], tab.data(), sizeof(uint32_t) * tab.size(), hipMemcpyHostToDevice, L->stream));
     // The initial produce() of PRNG::initialize (random.cpp:276) consumes S
     // quads of every instance before the first half-step.
     uint32_t nb = align_up(L->n_streams, kBlock) / kBlock;
     if (L->p.stream_mode == QCD_STREAM_REFERENCE) {
         if (L->p.prng == QCD_PRNG_XOR128)
-            hipLaunchKernelGGL(k_xor128_skip, dim3(nb), dim3(kBlock), 0, L->stream, L->rng, L->n_streams, L->s_quads);
+            hipLaunchKernelGGL(k_xor128_skip, dim3(nb), dim3(kBlock), 0, L->stream, L->rng_buf[0], L->n_streams, L->s_quads);
         else
-            hipLaunchKernelGGL(k_pm_skip, dim3(nb), dim3(kBlock), 0, L->stream, L->rng, L->n_streams, L->s_quads);
+            hipLaunchKernelGGL(k_pm_skip, dim3(nb), dim3(kBlock), 0, L->stream, L->rng_buf[0], L->n_streams, L->s_quads);
         HIPCHK(hipGetLastError());
     }
     HIPCHK(hipStreamSynchronize(L->stream));
