@@ -304,11 +304,44 @@ template <int N, int NREAL> struct Planes {
     static constexpr int value = NREAL / 2;
 };
 
+// Unsigned division by a run-time constant d >= 1 as multiply-high, shift
+// and add (Granlund and Montgomery, PLDI 1994, round-up method with the
+// 33-bit magic number folded into the add): with l = ceil(log2 d),
+// m = floor(2^32 (2^l - d) / d) + 1 and t = mulhi(m, n),
+// n / d = (t + ((n - t) >> s1)) >> s2, s1 = 1, s2 = l - 1 (d = 1: m = 0,
+// s1 = s2 = 0).  Exact for every 32-bit n.
+struct FastDiv {
+    uint32_t d, m, s1, s2;
+    __host__ __device__ __forceinline__ uint32_t div(uint32_t n) const
+    {
+#if defined(__HIP_DEVICE_COMPILE__)
+        const uint32_t t = __umulhi(m, n);
+#else
+        const uint32_t t = (uint32_t)(((uint64_t)m * n) >> 32);
+#endif
+        return (t + ((n - t) >> s1)) >> s2;
+    }
+};
+
+inline FastDiv make_fastdiv(uint32_t d)
+{
+    FastDiv f{d, 0u, 0u, 0u};
+    if (d > 1) {
+        uint32_t l = 0;
+        while ((1ull << l) < d) ++l;
+        f.m = (uint32_t)((((1ull << l) - d) << 32) / d + 1);
+        f.s1 = 1;
+        f.s2 = l - 1;
+    }
+    return f;
+}
+
 struct Geom {
     int n1, n2, n3, n4;   // extents along x, y, z, t
     int h;                // N2 / 2
     uint32_t vh;          // V / 2
     uint32_t vh_pad;      // plane stride (multiple of 64)
+    FastDiv dh, d3, d4;   // division by h, N3, N4
 };
 
 struct Site {
@@ -316,30 +349,67 @@ struct Site {
 };
 
 // Coordinates of the site of parity p with compact index i (y fastest).
-__device__ __forceinline__ Site site_of(const Geom& g, uint32_t i, int p)
+__host__ __device__ __forceinline__ Site site_of(const Geom& g, uint32_t i, int p)
 {
     Site s;
-    uint32_t yh = i % (uint32_t)g.h;
-    uint32_t r = i / (uint32_t)g.h;
-    s.z = (int)(r % (uint32_t)g.n3);
-    r /= (uint32_t)g.n3;
-    s.t = (int)(r % (uint32_t)g.n4);
-    s.x = (int)(r / (uint32_t)g.n4);
+    const uint32_t r = g.dh.div(i);
+    const uint32_t yh = i - r * (uint32_t)g.h;
+    const uint32_t r2 = g.d3.div(r);
+    s.z = (int)(r - r2 * (uint32_t)g.n3);
+    const uint32_t r3 = g.d4.div(r2);
+    s.t = (int)(r2 - r3 * (uint32_t)g.n4);
+    s.x = (int)r3;
     s.y = 2 * (int)yh + ((s.x + s.z + s.t + p) & 1);
     return s;
 }
 
-__device__ __forceinline__ uint32_t cindex(const Geom& g, int x, int y, int z, int t)
+// Neighbour steps along a wavefront-uniform direction d (so every select on
+// d is scalar).  c = the site's coordinate along d, o = y & 1 of the site,
+// i = its compact index.  x, z and t steps move the compact index by the
+// direction's stride (wrapping at the boundary); a y step keeps or moves the
+// half-row index depending on o.  A step along one direction leaves the
+// coordinate along any other direction unchanged, and changes o only when
+// it is a y step, so two-step neighbours chain two calls with the original
+// c and o (the second direction is never y when the first one is).
+__host__ __device__ __forceinline__ int coord(const Site& s, int d)
+{
+    return d == 0 ? s.x : (d == 1 ? s.y : (d == 2 ? s.z : s.t));
+}
+__host__ __device__ __forceinline__ uint32_t cstride(const Geom& g, int d)
+{
+    return d == 0 ? (uint32_t)g.h * (uint32_t)g.n3 * (uint32_t)g.n4
+                  : (d == 2 ? (uint32_t)g.h : (uint32_t)g.h * (uint32_t)g.n3);
+}
+__host__ __device__ __forceinline__ int extent(const Geom& g, int d)
+{
+    return d == 0 ? g.n1 : (d == 1 ? g.n2 : (d == 2 ? g.n3 : g.n4));
+}
+__host__ __device__ __forceinline__ uint32_t step_fwd(const Geom& g, uint32_t i, int d, int c, uint32_t o)
+{
+    const int n = extent(g, d);
+    if (d == 1) return (c < n - 1) ? i + o : i - (uint32_t)(g.h - 1);
+    const uint32_t st = cstride(g, d);
+    return (c < n - 1) ? i + st : i - st * (uint32_t)(n - 1);
+}
+__host__ __device__ __forceinline__ uint32_t step_bwd(const Geom& g, uint32_t i, int d, int c, uint32_t o)
+{
+    const int n = extent(g, d);
+    if (d == 1) return (c > 0) ? i - (1u - o) : i + (uint32_t)(g.h - 1);
+    const uint32_t st = cstride(g, d);
+    return (c > 0) ? i - st : i + st * (uint32_t)(n - 1);
+}
+
+__host__ __device__ __forceinline__ uint32_t cindex(const Geom& g, int x, int y, int z, int t)
 {
     return (uint32_t)(y >> 1) + (uint32_t)g.h * ((uint32_t)z + (uint32_t)g.n3 * ((uint32_t)t + (uint32_t)g.n4 * (uint32_t)x));
 }
 
-__device__ __forceinline__ int wrap(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
+__host__ __device__ __forceinline__ int wrap(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
 
 // Compact index of s + sign * e_dir, and of s + s1 * e_d1 + s2 * e_d2.
 // Directions are selected with branches (no dynamically indexed arrays, so
 // nothing spills to scratch when dir is a runtime value).
-__device__ __forceinline__ void shift(const Geom& g, Site& s, int dir, int sign)
+__host__ __device__ __forceinline__ void shift(const Geom& g, Site& s, int dir, int sign)
 {
     if (dir == 0) s.x = wrap(s.x + sign, g.n1);
     else if (dir == 1) s.y = wrap(s.y + sign, g.n2);
@@ -347,13 +417,13 @@ __device__ __forceinline__ void shift(const Geom& g, Site& s, int dir, int sign)
     else s.t = wrap(s.t + sign, g.n4);
 }
 
-__device__ __forceinline__ uint32_t nb_index(const Geom& g, Site s, int dir, int sign)
+__host__ __device__ __forceinline__ uint32_t nb_index(const Geom& g, Site s, int dir, int sign)
 {
     shift(g, s, dir, sign);
     return cindex(g, s.x, s.y, s.z, s.t);
 }
 
-__device__ __forceinline__ uint32_t nb2_index(const Geom& g, Site s, int d1, int s1, int d2, int s2)
+__host__ __device__ __forceinline__ uint32_t nb2_index(const Geom& g, Site s, int d1, int s1, int d2, int s2)
 {
     shift(g, s, d1, s1);
     shift(g, s, d2, s2);

@@ -276,10 +276,12 @@ __device__ __forceinline__ M<R, N> staple_nu(const Table<R, N, NREAL>& T, const 
     // the batch to save registers): wavefronts that share a neighbour line
     // then request it at the same point of their programs, while it is
     // still in L2.
-    const uint32_t i_pmu = nb_index(g, s, mu, +1);
-    const uint32_t i_pnu = nb_index(g, s, nu, +1);
-    const uint32_t i_mnu = nb_index(g, s, nu, -1);
-    const uint32_t i_pmu_mnu = nb2_index(g, s, mu, +1, nu, -1);
+    const int cmu = coord(s, mu), cnu = coord(s, nu);
+    const uint32_t o = (uint32_t)s.y & 1u;
+    const uint32_t i_pmu = step_fwd(g, i, mu, cmu, o);
+    const uint32_t i_pnu = step_fwd(g, i, nu, cnu, o);
+    const uint32_t i_mnu = step_bwd(g, i, nu, cnu, o);
+    const uint32_t i_pmu_mnu = step_bwd(g, i_pmu, nu, cnu, o);
     M<R, N> f2 = T.load_raw_u(q, mu, i_pnu);
     M<R, N> b1 = T.load_raw_u(q, mu, i_mnu);
     M<R, N> f1 = T.load_raw_u(q, nu, i_pmu);
@@ -352,7 +354,9 @@ __global__ __launch_bounds__(kUpdThreads) void k_update(UpdArgs a)
     using T = R;   // compute type of the SU(2) subgroup arithmetic
     constexpr int NC = 2 * N * N;   // reals per matrix
     __shared__ R lds[2][NC][kLinks];
-    const int w = threadIdx.x >> 6;          // wavefront index: uniform
+    // Wavefront index, made visibly uniform so that nu and every plane base
+    // derived from it live in scalar registers.
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int lane = threadIdx.x & 63;
     const uint32_t blk = upd_block(a);
     const uint32_t i = blk * kLinks + lane;
@@ -1086,6 +1090,9 @@ int qcd_lattice_create(const qcd_params* pp, qcd_lattice** out)
     L->g.n3 = p.n[2];
     L->g.n4 = p.n[3];
     L->g.h = p.n[1] / 2;
+    L->g.dh = make_fastdiv((uint32_t)L->g.h);
+    L->g.d3 = make_fastdiv((uint32_t)p.n[2]);
+    L->g.d4 = make_fastdiv((uint32_t)p.n[3]);
     L->g.vh = (uint32_t)(V / 2);
     // Plane stride: a multiple of 64 sites (512 B / 1 KiB, whole cache lines
     // per wavefront) plus one extra 64-site unit, so the 72 planes of one
