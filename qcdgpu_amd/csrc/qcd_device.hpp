@@ -156,6 +156,61 @@ template <typename R, int N> __device__ __forceinline__ void m_acc_dn(M<R, N>& a
             for (int k = 0; k < N; ++k) ccfma(acc.e[i][j], a.e[k][i], b.e[k][j]);
 }
 
+// Staple terms of SU(3) links with two computed rows: a product of SU(3)
+// matrices is in SU(3), so its third row is conj(row 1 x row 2).
+// F = f1 f2^+ f3^+: rows 0, 1 of f1 f2^+, then rows 0, 1 of that times f3^+.
+template <typename R>
+__device__ __forceinline__ M<R, 3> su3_fwd_term(const M<R, 3>& f1, const M<R, 3>& f2, const M<R, 3>& f3)
+{
+    M<R, 3> t, F;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            C<R> s = cmulc(f1.e[i][0], f2.e[j][0]);
+            cfmac(s, f1.e[i][1], f2.e[j][1]);
+            cfmac(s, f1.e[i][2], f2.e[j][2]);
+            t.e[i][j] = s;
+        }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            C<R> s = cmulc(t.e[i][0], f3.e[j][0]);
+            cfmac(s, t.e[i][1], f3.e[j][1]);
+            cfmac(s, t.e[i][2], f3.e[j][2]);
+            F.e[i][j] = s;
+        }
+    su3_row3(F);
+    return F;
+}
+// B = (b1 b2)^+ b3: columns 0, 1 of b1 b2 are rows 0, 1 of (b1 b2)^+.
+template <typename R>
+__device__ __forceinline__ M<R, 3> su3_bwd_term(const M<R, 3>& b1, const M<R, 3>& b2, const M<R, 3>& b3)
+{
+    M<R, 3> t, B;
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            C<R> s = cmul(b1.e[k][0], b2.e[0][c]);
+            cfma(s, b1.e[k][1], b2.e[1][c]);
+            cfma(s, b1.e[k][2], b2.e[2][c]);
+            t.e[k][c] = s;
+        }
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            C<R> s = ccmul(t.e[0][r], b3.e[0][c]);
+            ccfma(s, t.e[1][r], b3.e[1][c]);
+            ccfma(s, t.e[2][r], b3.e[2][c]);
+            B.e[r][c] = s;
+        }
+    su3_row3(B);
+    return B;
+}
+
 // Re Tr(a * b) accumulated in double.
 template <typename R, int N> __device__ __forceinline__ double m_retr_mul(const M<R, N>& a, const M<R, N>& b)
 {
@@ -534,7 +589,7 @@ template <typename R, int N, int NREAL> struct Table {
 
 // ------------------------------------------------------------------ generators
 // XOR128 (Marsaglia 2003), one step as prngcl_xor128.cl:62-74.
-__device__ __forceinline__ uint32_t xor128_step(uint4& s)
+__host__ __device__ __forceinline__ uint32_t xor128_step(uint4& s)
 {
     uint32_t t = s.x ^ (s.x << 11);
     s.x = s.y;
@@ -556,7 +611,57 @@ __device__ __forceinline__ float4 xor128_quad(uint4& s)
 
 // Park-Miller on four independent lanes (prngcl_pm.cl:68-84); output is an
 // exact scaling by 2^-31 (the reference divides by (float)2147483647.0).
-__device__ __forceinline__ uint32_t pm_step(uint32_t s)
+// Jump-ahead by a fixed number of steps.  The xorshift128 step is linear
+// over GF(2) in the 128 state bits (shifts and XORs only), so n steps are a
+// fixed 128x128 bit matrix; cols[b] = the state reached from the state with
+// only bit b set (bit b = bit b%32 of word b/32 of (x, y, z, w)), and the
+// jumped state is the XOR of the columns of the set bits.  cols is
+// wavefront-uniform (scalar loads).
+__device__ __forceinline__ uint4 xor128_jump(uint4 s, const uint4* cols_g)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    // Read through the constant address space so the uniform column loads
+    // are scalar; o ^= (col & m) is one v_bitop3_b32 per word (truth table
+    // 0x6c for operands (col, o, m)).
+    using cu32 = __attribute__((address_space(4))) const uint32_t;
+    cu32* cols = (cu32*)cols_g;
+    uint32_t o0 = 0u, o1 = 0u, o2 = 0u, o3 = 0u;
+    const uint32_t w[4] = {s.x, s.y, s.z, s.w};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+#pragma unroll 8
+        for (int b = 0; b < 32; ++b) {
+            const uint32_t m = (uint32_t)__builtin_amdgcn_sbfe((int)w[q], b, 1);   // 0 or ~0
+            cu32* c = cols + 4 * (q * 32 + b);
+            o0 = __builtin_amdgcn_bitop3_b32(c[0], o0, m, 0x6c);
+            o1 = __builtin_amdgcn_bitop3_b32(c[1], o1, m, 0x6c);
+            o2 = __builtin_amdgcn_bitop3_b32(c[2], o2, m, 0x6c);
+            o3 = __builtin_amdgcn_bitop3_b32(c[3], o3, m, 0x6c);
+        }
+    }
+    return make_uint4(o0, o1, o2, o3);
+#else
+    (void)cols_g;
+    return s;   // device-only function; the host pass never calls it
+#endif
+}
+
+// Park-Miller (Schrage form) for 0 < s < m is s -> 16807 s mod m, so k steps
+// are one multiplication by a_k = 16807^k mod m; the Schrage form maps 0 to m
+// and keeps m, which the jump reproduces.
+__host__ __device__ __forceinline__ uint32_t pm_mulmod(uint32_t s, uint32_t a)
+{
+    const uint64_t p = (uint64_t)s * a;                 // < 2^62
+    uint64_t r = (p & 2147483647ull) + (p >> 31);        // < 2^32, == p mod m
+    r = (r & 2147483647ull) + (r >> 31);
+    return (r >= 2147483647ull) ? (uint32_t)(r - 2147483647ull) : (uint32_t)r;
+}
+__host__ __device__ __forceinline__ uint32_t pm_jump(uint32_t s, uint32_t ak)
+{
+    return (s == 0u || s == 2147483647u) ? 2147483647u : pm_mulmod(s, ak);
+}
+
+__host__ __device__ __forceinline__ uint32_t pm_step(uint32_t s)
 {
     uint32_t t1 = 16807u * (s % 127773u);
     uint32_t t2 = 2836u * (s / 127773u);

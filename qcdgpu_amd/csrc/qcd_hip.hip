@@ -58,6 +58,8 @@ struct UpdArgs {
     uint32_t sites;       // V, for GID mode
     uint4* rng;           // per-stream state at the start of the half-step (stream = compact index)
     uint4* rng_out;       // state after the half-step (the two buffers alternate)
+    const uint4* jump;    // XOR128: jump matrix for 4 * s_quads steps (128 columns)
+    uint32_t pm_ak;       // Park-Miller: 16807^s_quads mod (2^31 - 1)
     uint32_t s_quads;     // quads per heatbath half-step per stream (0 = fast)
     // fused plaquette measurement (last sweep of a measurement cycle):
     // partials[(meas_mode - 1) * gridDim.x + block] = sum over the block of
@@ -166,6 +168,7 @@ __device__ __forceinline__ bool kp_update(const T* b, T beta, int nhit, Src& src
         const T pi2 = (T)kPi2;
         const T bdet = beta * det;
         const T ibdet = T(1) / bdet;
+        const T ibdet_ln2 = ibdet * (T)0.69314718055994530942;
         bool flag = false;
         T delta = T(0);
         for (int i = 0; i < nhit && !flag; ++i) {
@@ -181,7 +184,16 @@ __device__ __forceinline__ bool kp_update(const T* b, T beta, int nhit, Src& src
                 delta = cosrnd * cosrnd * ibdet;
                 flag = true;
             } else {
-                delta = -(log(T(1) - (T)r.x) + cosrnd * cosrnd * log(T(1) - (T)r.z)) * ibdet;
+                if constexpr (sizeof(T) == 4) {
+                    // ln(u) = log2(u) ln 2 with the hardware log2 (one ulp; 1 - r
+                    // is a normal float or 0, which gives -inf and a rejection
+                    // exactly as ln(0) does).
+                    const float l1 = __builtin_amdgcn_logf(1.0f - r.x);
+                    const float l2 = __builtin_amdgcn_logf(1.0f - r.z);
+                    delta = -fmaf(cosrnd * cosrnd, l2, l1) * ibdet_ln2;
+                } else {
+                    delta = -(log(T(1) - (T)r.x) + cosrnd * cosrnd * log(T(1) - (T)r.z)) * ibdet;
+                }
                 float w2 = r.w * r.w;
                 flag = ((T)w2 <= (T(1) - T(0.5) * delta));
             }
@@ -299,11 +311,20 @@ __device__ __forceinline__ M<R, N> staple_nu(const Table<R, N, NREAL>& T, const 
     Tab::finish(b2);
     Tab::finish(b3);
     M<R, N> S;
-    m_zero(S);
-    const M<R, N> tf = m_mul_nd(f1, f2);
-    m_acc_nd(S, tf, f3);
-    const M<R, N> tb = m_mul(b1, b2);
-    m_acc_dn(S, tb, b3);
+    if constexpr (N == 3) {
+        const M<R, 3> F = su3_fwd_term(f1, f2, f3);
+        const M<R, 3> B = su3_bwd_term(b1, b2, b3);
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) S.e[r][c] = cadd(F.e[r][c], B.e[r][c]);
+    } else {
+        m_zero(S);
+        const M<R, N> tf = m_mul_nd(f1, f2);
+        m_acc_nd(S, tf, f3);
+        const M<R, N> tb = m_mul(b1, b2);
+        m_acc_dn(S, tb, b3);
+    }
     return S;
 }
 
@@ -381,9 +402,10 @@ __global__ __launch_bounds__(kUpdThreads) void k_update(UpdArgs a)
         if constexpr (kStream) {
             if (w == 1 && a.s_quads) {
                 if constexpr (RK == RK_XOR128) {
-                    for (uint32_t k = 0; k < a.s_quads; ++k) (void)xor128_quad(rs);
+                    rs = xor128_jump(rs, a.jump);
                 } else {
-                    for (uint32_t k = 0; k < a.s_quads; ++k) (void)pm_quad(rs);
+                    rs = make_uint4(pm_jump(rs.x, a.pm_ak), pm_jump(rs.y, a.pm_ak), pm_jump(rs.z, a.pm_ak),
+                                    pm_jump(rs.w, a.pm_ak));
                 }
                 a.rng_out[i] = rs;
             }
@@ -833,6 +855,8 @@ struct qcd_lattice {
     void* table = nullptr;
     size_t table_bytes = 0;
     uint4* rng_buf[2] = {nullptr, nullptr};   // generator states; rng_buf[rng_cur] is current
+    uint4* jump_cols = nullptr;               // XOR128 jump-ahead matrix (128 columns)
+    uint32_t pm_ak = 1;                       // Park-Miller jump multiplier
     int rng_cur = 0;
     uint32_t n_streams = 0;
     uint32_t s_quads = 0;   // S
@@ -915,6 +939,8 @@ int launch_update_rn(qcd_lattice* L, int parity, int mu, int kind, int start_gid
     a.sites = 2u * L->g.vh;
     a.rng = L->rng_buf[L->rng_cur];
     a.rng_out = L->rng_buf[L->rng_cur ^ 1];
+    a.jump = L->jump_cols;
+    a.pm_ak = L->pm_ak;
     a.s_quads = (L->p.stream_mode == QCD_STREAM_REFERENCE) ? L->s_quads : 0u;
     a.partials = L->part_fused;
     a.meas_mode = o.meas_mode;
@@ -1122,6 +1148,7 @@ int qcd_lattice_create(const qcd_params* pp, qcd_lattice** out)
     if (hipMalloc(&L->table, L->table_bytes) != hipSuccess ||
         hipMalloc((void**)&L->rng_buf[0], sizeof(uint4) * L->n_streams) != hipSuccess ||
         hipMalloc((void**)&L->rng_buf[1], sizeof(uint4) * L->n_streams) != hipSuccess ||
+        hipMalloc((void**)&L->jump_cols, sizeof(uint4) * 128) != hipSuccess ||
         hipMalloc((void**)&L->partials, sizeof(double) * part_doubles) != hipSuccess ||
         hipMalloc((void**)&L->rec_index, sizeof(int) * 4) != hipSuccess ||
         hipMalloc((void**)&L->plq_rec, sizeof(double) * 2 * p.max_measurements) != hipSuccess ||
@@ -1134,6 +1161,27 @@ int qcd_lattice_create(const qcd_params* pp, qcd_lattice** out)
     L->part_poly = L->part_plq + (size_t)2 * 2 * L->upd_blocks;
     (void)hipMemset(L->rec_index, 0, sizeof(int) * 4);
     (void)hipMemset(L->table, 0, L->table_bytes);
+    {
+        // Reference-stream skip of s_quads quads per heatbath half-step:
+        // XOR128 jump matrix for 4 * s_quads steps (four steps per quad),
+        // Park-Miller multiplier for s_quads steps (pm_quad steps each of its
+        // four lanes once).
+        uint4 cols[128];
+        for (int b = 0; b < 128; ++b) {
+            uint4 st = make_uint4(0u, 0u, 0u, 0u);
+            uint32_t* w = (uint32_t*)&st;
+            w[b / 32] = 1u << (b % 32);
+            for (uint32_t k = 0; k < 4u * L->s_quads; ++k) (void)xor128_step(st);
+            cols[b] = st;
+        }
+        if (hipMemcpy(L->jump_cols, cols, sizeof(cols), hipMemcpyHostToDevice) != hipSuccess) {
+            qcd_lattice_destroy(L);
+            return fail(QCD_E_HIP, "jump matrix upload failed");
+        }
+        uint32_t ak = 1;
+        for (uint32_t k = 0; k < L->s_quads; ++k) ak = pm_mulmod(ak, 16807u);
+        L->pm_ak = ak;
+    }
     if (hipDeviceSynchronize() != hipSuccess) {
         qcd_lattice_destroy(L);
         return fail(QCD_E_HIP, "initial clear failed");
@@ -1151,6 +1199,7 @@ int qcd_lattice_destroy(qcd_lattice* L)
     (void)hipFree(L->table);
     (void)hipFree(L->rng_buf[0]);
     (void)hipFree(L->rng_buf[1]);
+    (void)hipFree(L->jump_cols);
     (void)hipFree(L->partials);
     (void)hipFree(L->rec_index);
     (void)hipFree(L->plq_rec);
