@@ -105,15 +105,20 @@ int  qcd_lattice_sync(qcd_lattice *lat);
                           2: the same slabs, dispatched cross-section first
                           (x, then t, then z) for L2 reuse; 0: hardware order.
    QCD_TUNE_BLOCKS_PER_CU cap on resident update workgroups per CU (reserved
-                          LDS), 0 = no cap (default).                        */
-enum { QCD_TUNE_XCD_MAP = 1, QCD_TUNE_BLOCKS_PER_CU = 2 };
+                          LDS), 0 = no cap (default).
+   QCD_TUNE_UPDATE_SHAPE  3: three wavefronts share 64 links, one staple
+                          direction each (default); 1: one wavefront per 64
+                          links, workgroups of four.                         */
+enum { QCD_TUNE_XCD_MAP = 1, QCD_TUNE_BLOCKS_PER_CU = 2, QCD_TUNE_UPDATE_SHAPE = 3 };
 int  qcd_lattice_tune(qcd_lattice *lat, int knob, int value);
 /* Bytes of device memory held by the link table. */
 int  qcd_lattice_table_bytes(const qcd_lattice *lat, size_t *bytes);
 
 /* ---- starts: kernels lattice_init_cold / lattice_init_gid (suncl.cl:164-347);
         QCD_START_HOT draws Haar-distributed links from the per-site streams
-        (the reference's lattice_init_hot_* is not Haar, su2_update_cl.cl:110) */
+        (the reference's lattice_init_hot_* is not Haar, su2_update_cl.cl:110),
+        so it needs qcd_prng_init first; stream i fills the links of the two
+        sites with compact index i (parity 0 then 1, directions 0..3).       */
 int  qcd_lattice_init(qcd_lattice *lat, int start);
 
 /* ---- PRNG (PRNG::initialize, random.cpp:216-277): seeds the per-site
@@ -168,7 +173,13 @@ int  qcd_cycle_profile(qcd_lattice *lat, int repeats, double *hb_us, double *or_
 /* ---- reference PRN buffer semantics (PRNG::produce, random.cpp:278-284;
         kernels xor128 prngcl_xor128.cl:100, pm prngcl_pm.cl:86,
         ranlux prngcl_ranlux.cl:534).  Host seed tables in, quads out:
-        quads[(i*n + g)*4 + c] for instance g, sample i.  Tables advance.    */
+        quads[(i*n + g)*4 + c] for instance g, sample i.  Tables advance.
+        XOR128 / PM tables: uint32 [n][4].  RANLUX: float [7][n][4] (six
+        float4 of lagged values, then (23, 9, block quad counter, carry));
+        ranlux_luxury 0..4 selects the skip (24, 48, 97, 223, 389).  RANLUX
+        lattice streams always follow the reference layout: one produce()
+        of S quads per heatbath half-step (qcd_params.stream_mode is not
+        used for RANLUX).                                                     */
 int  qcd_prng_produce_host(int prng, int ranlux_luxury, unsigned n_instances, unsigned samples,
                            void *table, float *quads);
 

@@ -514,6 +514,62 @@ void orc_init_gid(const orc_lattice *L, double *links)
         }
 }
 
+/* Haar-random start from per-stream uniforms (new work: the reference's
+   lattice_init_hot_* is not Haar, su2_update_cl.cl:110-146,
+   su3_update_cl.cl:165-225).  Stream i (= compact index) fills the links of
+   its two sites in the order parity 0, 1 and direction 0..3; each link takes
+   Q quads (Q = 1 for SU(2), 3 for SU(3)): quad k of stream i is
+   quads[(k * stride + i) * 4 + c].  Gaussians by Box-Muller in double,
+   g = sqrt(-2 ln t) (cos 2 pi u2, sin 2 pi u2), t = 1 - u1 (t = 2^-33 if
+   u1 = 1), from the pairs (x, y) and (z, w) of each quad.
+   SU(2): (Re u11, Im u11, Re u12, Im u12) = g / |g|, uniform on S^3.
+   SU(3): rows 1, 2 = (g0 + i g1, g2 + i g3, g4 + i g5), (g6 + i g7, ...),
+   Gram-Schmidt, row 3 = conj(row 1 x row 2): a uniform orthonormal 2-frame
+   completed to SU(3), which is Haar (SU(3) acts simply transitively on
+   orthonormal 2-frames of C^3).                                            */
+static void orc_box_muller(float u1, float u2, double *g0, double *g1)
+{
+    double t = 1.0 - (double)u1;
+    if (t <= 0.0) t = 1.1641532182693481e-10;   /* 2^-33 */
+    double r = sqrt(-2.0 * log(t));
+    double th = 6.2831853071795864769 * (double)u2;
+    *g0 = r * cos(th);
+    *g1 = r * sin(th);
+}
+
+void orc_init_hot(const orc_lattice *L, double *links, const float *quads, unsigned stride)
+{
+    unsigned vh = orc_sites(L) / 2;
+    int Q = (L->sun == 3) ? 3 : 1;
+    for (unsigned i = 0; i < vh; ++i)
+        for (int p = 0; p < 2; ++p) {
+            unsigned s = orc_parity_site(L, i, p);
+            for (int d = 0; d < 4; ++d) {
+                double g[12];
+                for (int q = 0; q < Q; ++q) {
+                    const float *u = quads + 4 * ((size_t)((p * 4 + d) * Q + q) * stride + i);
+                    orc_box_muller(u[0], u[1], &g[4 * q], &g[4 * q + 1]);
+                    orc_box_muller(u[2], u[3], &g[4 * q + 2], &g[4 * q + 3]);
+                }
+                double *a = LINK(L, links, s, d);
+                if (L->sun == 3) {
+                    for (int k = 0; k < 12; ++k) a[k] = g[k];
+                    orc_gram_schmidt3(a);
+                } else {
+                    double n = 1.0 / sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2] + g[3] * g[3]);
+                    a[0] = g[0] * n;
+                    a[1] = g[1] * n;
+                    a[2] = g[2] * n;
+                    a[3] = g[3] * n;
+                    a[4] = -a[2];
+                    a[5] = a[3];
+                    a[6] = a[0];
+                    a[7] = -a[1];
+                }
+            }
+        }
+}
+
 /* ======================================================================== */
 /* Staple -- suncpp/Update/sun_update.h:44-77 (term and summation order)     */
 /* ======================================================================== */

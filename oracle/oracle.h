@@ -80,6 +80,10 @@ unsigned orc_parity_site(const orc_lattice *L, unsigned i, int parity);
 
 void     orc_init_cold(const orc_lattice *L, double *links);
 void     orc_init_gid(const orc_lattice *L, double *links);
+/* Haar start from per-stream quads: quads[(k * stride + i) * 4 + c] is quad
+   k of stream i (compact index); k runs over parity, direction and the
+   Q = 1 (SU(2)) or 3 (SU(3)) quads of a link. */
+void     orc_init_hot(const orc_lattice *L, double *links, const float *quads, unsigned stride);
 
 /* Random-number sources for the update. */
 enum { ORC_RNG_GID = 0, ORC_RNG_SEQ = 1, ORC_RNG_QUADS = 2 };

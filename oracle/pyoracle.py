@@ -115,6 +115,15 @@ def gid(L):
     return a
 
 
+def hot(L, quads, stride):
+    """Haar start from per-stream quads (quads[k, i, :] = quad k of stream i,
+    as returned by the produce functions with n = stride instances)."""
+    q = np.ascontiguousarray(quads, dtype=np.float32)
+    a = np.zeros(link_shape(L))
+    lib().orc_init_hot(ctypes.byref(L), _p(a, _F64P), _p(q, _F32P), stride)
+    return a
+
+
 def rand_fill(seed, count):
     out = np.zeros(count, dtype=np.int32)
     lib().orc_rand_fill(seed, count, _p(out, _I32P))

@@ -19,7 +19,7 @@ PRNG_XOR128, PRNG_RANLUX, PRNG_PM = 1, 2, 3
 STREAM_REFERENCE, STREAM_FAST = 0, 1
 HEATBATH, OVERRELAX = 0, 1
 MEAS_PLAQUETTE, MEAS_POLYAKOV = 1, 2
-TUNE_XCD_MAP, TUNE_BLOCKS_PER_CU = 1, 2
+TUNE_XCD_MAP, TUNE_BLOCKS_PER_CU, TUNE_UPDATE_SHAPE = 1, 2, 3
 
 
 class Params(ctypes.Structure):

@@ -617,15 +617,17 @@ __device__ __forceinline__ float4 xor128_quad(uint4& s)
 // only bit b set (bit b = bit b%32 of word b/32 of (x, y, z, w)), and the
 // jumped state is the XOR of the columns of the set bits.  cols is
 // wavefront-uniform (scalar loads).
-__device__ __forceinline__ uint4 xor128_jump(uint4 s, const uint4* cols_g)
+// Words 2*half and 2*half+1 of the jumped state (half is wavefront-uniform,
+// so two wavefronts can share one jump).
+__device__ __forceinline__ uint2 xor128_jump_half(uint4 s, const uint4* cols_g, int half)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
     // Read through the constant address space so the uniform column loads
     // are scalar; o ^= (col & m) is one v_bitop3_b32 per word (truth table
     // 0x6c for operands (col, o, m)).
     using cu32 = __attribute__((address_space(4))) const uint32_t;
-    cu32* cols = (cu32*)cols_g;
-    uint32_t o0 = 0u, o1 = 0u, o2 = 0u, o3 = 0u;
+    cu32* cols = (cu32*)cols_g + 2 * half;
+    uint32_t oa = 0u, ob = 0u;
     const uint32_t w[4] = {s.x, s.y, s.z, s.w};
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -633,16 +635,14 @@ __device__ __forceinline__ uint4 xor128_jump(uint4 s, const uint4* cols_g)
         for (int b = 0; b < 32; ++b) {
             const uint32_t m = (uint32_t)__builtin_amdgcn_sbfe((int)w[q], b, 1);   // 0 or ~0
             cu32* c = cols + 4 * (q * 32 + b);
-            o0 = __builtin_amdgcn_bitop3_b32(c[0], o0, m, 0x6c);
-            o1 = __builtin_amdgcn_bitop3_b32(c[1], o1, m, 0x6c);
-            o2 = __builtin_amdgcn_bitop3_b32(c[2], o2, m, 0x6c);
-            o3 = __builtin_amdgcn_bitop3_b32(c[3], o3, m, 0x6c);
+            oa = __builtin_amdgcn_bitop3_b32(c[0], oa, m, 0x6c);
+            ob = __builtin_amdgcn_bitop3_b32(c[1], ob, m, 0x6c);
         }
     }
-    return make_uint4(o0, o1, o2, o3);
+    return make_uint2(oa, ob);
 #else
     (void)cols_g;
-    return s;   // device-only function; the host pass never calls it
+    return half ? make_uint2(s.z, s.w) : make_uint2(s.x, s.y);   // device-only; never called on the host
 #endif
 }
 
@@ -677,5 +677,90 @@ __device__ __forceinline__ float4 pm_quad(uint4& s)
     const float k = 4.656612873077392578125e-10f;   // 2^-31
     return make_float4((float)s.x * k, (float)s.y * k, (float)s.z * k, (float)s.w * k);
 }
+
+// ------------------------------------------------------------------ RANLUX
+// The reference's float subtract-with-borrow generator (prngcl_ranlux.cl)
+// with the 24 lagged values in registers.  All streams of a lattice advance
+// in lockstep (every produce call draws the same number of quads from every
+// stream), so the position inside a 24-block is uniform and every ring index
+// below is a compile-time constant after unrolling: quad k of a block uses
+// ring positions i = 23 - 4k .. 20 - 4k with j = (i + 10) mod 24; a block
+// ends at i = 23 again; the skip of P numbers is P / 24 full cycles, P mod 24
+// single steps, then a renaming of the ring by P mod 24 positions so the
+// next block starts at i = 23 (the RL_skip tails of prngcl_ranlux.cl:229-466).
+constexpr float kRlTwom24 = 5.9604644775390625e-08f;
+constexpr float kRlTwom12 = 2.44140625e-04f;
+
+template <int I, int J>
+__host__ __device__ __forceinline__ float rl_swb(float (&s)[24], float& carry)
+{
+    float uni = s[J] - s[I] - carry;
+    if (uni < 0.0f) {
+        uni += 1.0f;
+        carry = kRlTwom24;
+    } else {
+        carry = 0.0f;
+    }
+    s[I] = uni;
+    return uni;
+}
+
+// One output: SWB at (I, J) then the small-value fix with the value at the
+// decremented j pointer (prngcl_ranlux.cl:178-179).
+template <int I>
+__host__ __device__ __forceinline__ float rl_out(float (&s)[24], float& carry)
+{
+    constexpr int J = (I + 10) % 24;
+    float uni = rl_swb<I, J>(s, carry);
+    if (uni < kRlTwom12) {
+        uni = uni + kRlTwom24 * s[(J + 23) % 24];
+        if (uni == 0.0f) uni = kRlTwom24 * kRlTwom24;
+    }
+    return uni;
+}
+
+template <int K>
+__host__ __device__ __forceinline__ float4 rl_quad(float (&s)[24], float& carry)
+{
+    float4 q;
+    q.x = rl_out<23 - 4 * K>(s, carry);
+    q.y = rl_out<22 - 4 * K>(s, carry);
+    q.z = rl_out<21 - 4 * K>(s, carry);
+    q.w = rl_out<20 - 4 * K>(s, carry);
+    return q;
+}
+
+template <int T>
+__host__ __device__ __forceinline__ void rl_steps(float (&s)[24], float& carry)
+{
+    if constexpr (T > 0) {
+        constexpr int I = (23 - (T - 1) + 24) % 24;   // step T-1 of a cycle
+        rl_steps<T - 1>(s, carry);
+        (void)rl_swb<I, (I + 10) % 24>(s, carry);
+    }
+}
+
+// Skip P = 24 * FULL + REM numbers from the start of a block, then rename.
+template <int FULL, int REM>
+__host__ __device__ __forceinline__ void rl_skip(float (&s)[24], float& carry)
+{
+    for (int f = 0; f < FULL; ++f) rl_steps<24>(s, carry);
+    rl_steps<REM>(s, carry);
+    if constexpr (REM != 0) {
+        float t[24];
+#pragma unroll
+        for (int i = 0; i < 24; ++i) t[(i + REM) % 24] = s[i];
+#pragma unroll
+        for (int i = 0; i < 24; ++i) s[i] = t[i];
+    }
+}
+
+// Luxury level -> skip: 24, 48, 97, 223, 389 (random.cpp:190-194).
+template <int LUX> struct RlSkip;
+template <> struct RlSkip<0> { static constexpr int full = 1, rem = 0; };
+template <> struct RlSkip<1> { static constexpr int full = 2, rem = 0; };
+template <> struct RlSkip<2> { static constexpr int full = 4, rem = 1; };
+template <> struct RlSkip<3> { static constexpr int full = 9, rem = 7; };
+template <> struct RlSkip<4> { static constexpr int full = 16, rem = 5; };
 
 }  // namespace qcd

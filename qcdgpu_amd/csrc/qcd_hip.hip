@@ -46,7 +46,9 @@ namespace {
 constexpr int kBlock = 256;
 constexpr double kPi2 = 6.2831853071795864769252867665590;
 
-enum RngKind { RK_NONE = 0, RK_GID = 1, RK_XOR128 = 2, RK_PM = 3 };
+// RK_BUF: quads read from a buffer filled by a produce kernel (RANLUX),
+// quad k of stream i at buf[k * n + i] as the reference's PRN buffer.
+enum RngKind { RK_NONE = 0, RK_GID = 1, RK_XOR128 = 2, RK_PM = 3, RK_BUF = 4 };
 
 struct UpdArgs {
     void* table;
@@ -59,6 +61,8 @@ struct UpdArgs {
     uint4* rng;           // per-stream state at the start of the half-step (stream = compact index)
     uint4* rng_out;       // state after the half-step (the two buffers alternate)
     const uint4* jump;    // XOR128: jump matrix for 4 * s_quads steps (128 columns)
+    const float4* prn;    // RK_BUF: quad k of stream i at prn[k * prn_stride + i]
+    uint32_t prn_stride;
     uint32_t pm_ak;       // Park-Miller: 16807^s_quads mod (2^31 - 1)
     uint32_t s_quads;     // quads per heatbath half-step per stream (0 = fast)
     // fused plaquette measurement (last sweep of a measurement cycle):
@@ -129,6 +133,13 @@ struct SrcPm {
         used++;
         (void)pm_quad(s);
     }
+};
+struct SrcBuf {
+    const float4* buf;
+    uint32_t n, i, k;
+    __device__ float4 try_quad() { return buf[(size_t)(k++) * n + i]; }
+    __device__ float4 accept_quad() { return try_quad(); }
+    __device__ void skip() { ++k; }
 };
 
 // ---------------------------------------------------------------- KP / OR
@@ -277,9 +288,9 @@ __device__ __forceinline__ C<R> ua_entry(const M<R, 3>& U, const M<R, 3>& A, int
 //   S_nu = U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+ + U_nu(x+mu-nu)^+ U_mu(x-nu)^+ U_nu(x-nu)
 // (suncpp/Update/sun_update.h:44-77; su3_update_cl.cl:341-565).  nu, mu and
 // the parity p are wavefront-uniform.
-template <typename R, int N, int NREAL, typename Hook>
+template <typename R, int N, int NREAL>
 __device__ __forceinline__ M<R, N> staple_nu(const Table<R, N, NREAL>& T, const Geom& g, Site s, uint32_t i, int p,
-                                             int mu, int nu, Hook&& while_loading)
+                                             int mu, int nu)
 {
     using Tab = Table<R, N, NREAL>;
     const int q = p ^ 1;
@@ -300,9 +311,6 @@ __device__ __forceinline__ M<R, N> staple_nu(const Table<R, N, NREAL>& T, const 
     M<R, N> b3 = T.load_raw_u(q, nu, i_mnu);
     M<R, N> f3 = T.load_raw_u(p, nu, i);
     M<R, N> b2 = T.load_raw_u(p, nu, i_pmu_mnu);
-    __builtin_amdgcn_sched_barrier(0);
-    // Independent work placed here runs while the gathers are in flight.
-    while_loading();
     __builtin_amdgcn_sched_barrier(0);
     Tab::finish(f1);
     Tab::finish(f2);
@@ -349,6 +357,105 @@ __device__ __forceinline__ double wave_sum(double v)
 constexpr int kLinks = 64;
 constexpr int kUpdThreads = 3 * kLinks;
 
+// The link update of one active lane: load U, the three SU(2) subgroups
+// (heatbath or overrelaxation; SU(2): one), FP64 reunitarisation, store.
+// Returns ReTr(U' * Alow) for the fused plaquette (0 unless MEAS).  `rs` is
+// the generator state at the start of the half-step; in the fast stream
+// layout the state after consumption is written to rng_out.
+template <typename R, int N, int NREAL, bool OVER, int RK, bool MEAS>
+__device__ __forceinline__ double update_link(const UpdArgs& a, const Table<R, N, NREAL>& Tb, uint32_t i, const Site& s,
+                                              const M<R, N>& A, const M<R, N>& Alow, uint4 rs)
+{
+    using T = R;
+    const int p = a.parity, mu = a.mu;
+    double trace = 0.0;
+    M<R, N> U = Tb.load_u(p, mu, i);
+    SrcGid sg{};
+    SrcXor sx{};
+    SrcPm sp{};
+    SrcBuf sb{};
+    if constexpr (RK == RK_BUF) sb = SrcBuf{a.prn, a.prn_stride, i, 0u};
+    if constexpr (RK == RK_GID) {
+        sg.gid = (double)((uint32_t)s.y + (uint32_t)a.g.n2 * ((uint32_t)s.z + (uint32_t)a.g.n3 *
+                                                                 ((uint32_t)s.t + (uint32_t)a.g.n4 * (uint32_t)s.x)));
+        sg.fs = (double)a.sites;
+        sg.nhit = a.nhit;
+    } else if constexpr (RK == RK_XOR128) {
+        sx.s = rs;
+        sx.used = 0;
+    } else if constexpr (RK == RK_PM) {
+        sp.s = rs;
+        sp.used = 0;
+    }
+    const T beta = (T)a.beta;
+
+    T blk8[8], r[8];
+    if constexpr (N == 3) {
+#pragma unroll
+        for (int sgi = 0; sgi < 3; ++sgi) {
+            const int ra = (sgi == 2) ? 1 : 0;
+            const int rb = (sgi == 0) ? 1 : 2;
+            const C<R> waa = ua_entry(U, A, ra, ra), wab = ua_entry(U, A, ra, rb);
+            const C<R> wba = ua_entry(U, A, rb, ra), wbb = ua_entry(U, A, rb, rb);
+            blk8[0] = waa.re; blk8[1] = waa.im; blk8[2] = wab.re; blk8[3] = wab.im;
+            blk8[4] = wba.re; blk8[5] = wba.im; blk8[6] = wbb.re; blk8[7] = wbb.im;
+            bool ok;
+            if constexpr (RK == RK_GID) ok = kp_update<T, OVER, RK>(blk8, beta, a.nhit, sg, r);
+            else if constexpr (RK == RK_XOR128) ok = kp_update<T, OVER, RK>(blk8, beta, a.nhit, sx, r);
+            else if constexpr (RK == RK_PM) ok = kp_update<T, OVER, RK>(blk8, beta, a.nhit, sp, r);
+            else if constexpr (RK == RK_BUF) ok = kp_update<T, OVER, RK>(blk8, beta, a.nhit, sb, r);
+            else ok = kp_update<T, OVER, RK>(blk8, beta, a.nhit, sg, r);
+            if (!ok) {
+                // lattice_unity_2 (su3_update_cl.cl:665): identity in this subgroup
+                r[0] = T(1); r[1] = T(0); r[2] = T(0); r[3] = T(0);
+                r[4] = T(0); r[5] = T(0); r[6] = T(1); r[7] = T(0);
+            }
+            apply_rows(U, ra, rb, r);
+        }
+        M<double, 3> Ud = m_cast<double>(U);
+        su3_gram_schmidt(Ud);
+        U = m_cast<R>(Ud);
+    } else {
+        if constexpr (OVER) {
+            const M<R, 2> W = m_mul(U, A);
+            blk8[0] = W.e[0][0].re; blk8[1] = W.e[0][0].im; blk8[2] = W.e[0][1].re; blk8[3] = W.e[0][1].im;
+            blk8[4] = W.e[1][0].re; blk8[5] = W.e[1][0].im; blk8[6] = W.e[1][1].re; blk8[7] = W.e[1][1].im;
+            kp_update<T, true, RK>(blk8, beta, a.nhit, sg, r);
+            M<R, 2> Rm;
+            Rm.e[0][0] = {r[0], r[1]};
+            Rm.e[0][1] = {r[2], r[3]};
+            Rm.e[1][0] = {r[4], r[5]};
+            Rm.e[1][1] = {r[6], r[7]};
+            U = m_mul(Rm, U);
+        } else {
+            blk8[0] = A.e[0][0].re; blk8[1] = A.e[0][0].im; blk8[2] = A.e[0][1].re; blk8[3] = A.e[0][1].im;
+            blk8[4] = A.e[1][0].re; blk8[5] = A.e[1][0].im; blk8[6] = A.e[1][1].re; blk8[7] = A.e[1][1].im;
+            bool ok;
+            if constexpr (RK == RK_GID) ok = kp_update<T, false, RK>(blk8, beta, a.nhit, sg, r);
+            else if constexpr (RK == RK_XOR128) ok = kp_update<T, false, RK>(blk8, beta, a.nhit, sx, r);
+            else if constexpr (RK == RK_BUF) ok = kp_update<T, false, RK>(blk8, beta, a.nhit, sb, r);
+            else ok = kp_update<T, false, RK>(blk8, beta, a.nhit, sp, r);
+            if (ok) {   // a rejected SU(2) update keeps the old link (su2_update_cl.cl:542-547)
+                U.e[0][0] = {r[0], r[1]};
+                U.e[0][1] = {r[2], r[3]};
+                U.e[1][0] = {r[4], r[5]};
+                U.e[1][1] = {r[6], r[7]};
+            }
+        }
+        M<double, 2> Ud = m_cast<double>(U);
+        su2_normalize(Ud);
+        U = m_cast<R>(Ud);
+    }
+    Tb.store_u(p, mu, i, U);
+    if constexpr (MEAS) trace = m_retr_mul(U, Alow);
+
+    if (a.s_quads == 0) {
+        if constexpr (RK == RK_XOR128) a.rng_out[i] = sx.s;
+        else if constexpr (RK == RK_PM) a.rng_out[i] = sp.s;
+    }
+    return trace;
+}
+
 // Workgroup -> 64-link block.  Mode 1: XCD k (hardware block b goes to XCD
 // b mod 8) gets the k-th contiguous eighth of the compact index range, an
 // x-slab of the lattice.  Mode 2: the same slabs, but within a slab blocks
@@ -388,35 +495,20 @@ __global__ __launch_bounds__(kUpdThreads) void k_update(UpdArgs a)
     const Site s = site_of(a.g, active ? i : 0u, p);
 
     // Generator state at the start of the half-step: wavefront 0 consumes
-    // it; in the reference stream layout wavefront 1 advances every stream
-    // by exactly s_quads quads (the reference's produce() of the whole
-    // buffer), hidden under its gather latency, and writes the result to
-    // the other state buffer.  In the fast layout wavefront 0 writes the
-    // state it ends with.
+    // it.  In the reference stream layout every stream then advances by
+    // exactly s_quads quads (the reference's produce() of the whole buffer):
+    // after the barrier, while wavefront 0 updates the links, wavefronts 1
+    // and 2 jump the start state ahead, each computing two of its four
+    // words, into the other state buffer.  In the fast layout wavefront 0
+    // writes the state it ends with.
     constexpr bool kStream = (RK == RK_XOR128 || RK == RK_PM);
     uint4 rs = make_uint4(0, 0, 0, 0);
     if constexpr (kStream) {
-        if ((w == 0 && active) || (w == 1 && a.s_quads)) rs = a.rng[i];
+        if ((w == 0 && active) || (w > 0 && a.s_quads)) rs = a.rng[i];
     }
-    auto skip_streams = [&]() {
-        if constexpr (kStream) {
-            if (w == 1 && a.s_quads) {
-                if constexpr (RK == RK_XOR128) {
-                    rs = xor128_jump(rs, a.jump);
-                } else {
-                    rs = make_uint4(pm_jump(rs.x, a.pm_ak), pm_jump(rs.y, a.pm_ak), pm_jump(rs.z, a.pm_ak),
-                                    pm_jump(rs.w, a.pm_ak));
-                }
-                a.rng_out[i] = rs;
-            }
-        }
-    };
     M<R, N> S;
-    if (active) S = staple_nu<R, N, NREAL>(Tb, a.g, s, i, p, mu, nu, skip_streams);
-    else {
-        skip_streams();
-        m_zero(S);
-    }
+    if (active) S = staple_nu<R, N, NREAL>(Tb, a.g, s, i, p, mu, nu);
+    else m_zero(S);
     if (w > 0) {
 #pragma unroll
         for (int r = 0; r < N; ++r)
@@ -427,7 +519,22 @@ __global__ __launch_bounds__(kUpdThreads) void k_update(UpdArgs a)
             }
     }
     __syncthreads();
-    if (w > 0) return;
+    if (w > 0) {
+        if constexpr (kStream) {
+            if (a.s_quads) {
+                const int half = w - 1;
+                uint2 o;
+                if constexpr (RK == RK_XOR128) {
+                    o = xor128_jump_half(rs, a.jump, half);
+                } else {
+                    o = half ? make_uint2(pm_jump(rs.z, a.pm_ak), pm_jump(rs.w, a.pm_ak))
+                             : make_uint2(pm_jump(rs.x, a.pm_ak), pm_jump(rs.y, a.pm_ak));
+                }
+                reinterpret_cast<uint2*>(a.rng_out + i)[half] = o;
+            }
+        }
+        return;
+    }
 
     M<R, N> S1, S2;
 #pragma unroll
@@ -450,91 +557,85 @@ __global__ __launch_bounds__(kUpdThreads) void k_update(UpdArgs a)
     m_acc(A, S2);
 
     double trace = 0.0;
+    if (active) trace = update_link<R, N, NREAL, OVER, RK, MEAS>(a, Tb, i, s, A, Alow, rs);
+    if constexpr (MEAS) {
+        trace = wave_sum(trace);
+        // slot = 64-link unit, so the final sum runs in lattice order
+        // whatever the launch order or kernel shape
+        if (lane == 0) a.partials[(size_t)(a.meas_mode - 1) * gridDim.x + blk] = trace;
+    }
+}
+
+// One wavefront per 64 links: each wavefront gathers and sums the three
+// staple shares of its own links (one batch of gathers per direction) and
+// updates them; a workgroup is 4 independent wavefronts.  More links are in
+// flight per CU than with kUpdThreads workgroups, at the cost of a longer
+// path per wavefront.  The 64-link unit map is the same as upd_block's.
+constexpr int kWaves1 = 4;
+__device__ __forceinline__ uint32_t upd_unit1(const UpdArgs& a, int w)
+{
+    const uint32_t b = blockIdx.x, nb = gridDim.x;
+    if (a.xcd_map == 0 || (nb % 8u)) return b * kWaves1 + (uint32_t)w;
+    const uint32_t per = nb * kWaves1 / 8u, k = b % 8u, l = (b / 8u) * kWaves1 + (uint32_t)w;
+    if (a.xcd_map == 2) {
+        const uint32_t c = l % a.ord_xs, r = l / a.ord_xs;
+        const uint32_t t = r % a.ord_nt, zg = r / a.ord_nt;
+        return k * per + zg + a.ord_ng * (t + a.ord_nt * c);
+    }
+    return k * per + l;
+}
+
+template <typename R, int N, int NREAL, bool OVER, int RK, bool MEAS>
+__global__ __launch_bounds__(kWaves1 * kLinks) void k_update1(UpdArgs a, uint32_t units)
+{
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int lane = threadIdx.x & 63;
+    const uint32_t unit = upd_unit1(a, w);
+    if (unit >= units) return;   // whole wavefront: beyond the last 64-link unit
+    const uint32_t i = unit * kLinks + lane;   // < units * 64 = number of streams
+    const bool active = i < a.g.vh;
+    Table<R, N, NREAL> Tb{(typename Vec2<R>::T*)a.table, a.g.vh_pad};
+    const int p = a.parity, mu = a.mu;
+    const Site s = site_of(a.g, active ? i : 0u, p);
+
+    constexpr bool kStream = (RK == RK_XOR128 || RK == RK_PM);
+    uint4 rs = make_uint4(0, 0, 0, 0);
+    if constexpr (kStream) {
+        if (active || a.s_quads) rs = a.rng[i];
+    }
+    M<R, N> A, Alow;
+    m_zero(A);
+    m_zero(Alow);
     if (active) {
-        M<R, N> U = Tb.load_u(p, mu, i);
-        SrcGid sg{};
-        SrcXor sx{};
-        SrcPm sp{};
-        if constexpr (RK == RK_GID) {
-            sg.gid = (double)((uint32_t)s.y + (uint32_t)a.g.n2 * ((uint32_t)s.z + (uint32_t)a.g.n3 *
-                                                                     ((uint32_t)s.t + (uint32_t)a.g.n4 * (uint32_t)s.x)));
-            sg.fs = (double)a.sites;
-            sg.nhit = a.nhit;
-        } else if constexpr (RK == RK_XOR128) {
-            sx.s = rs;
-            sx.used = 0;
-        } else if constexpr (RK == RK_PM) {
-            sp.s = rs;
-            sp.used = 0;
-        }
-        const T beta = (T)a.beta;
-
-        T blk8[8], r[8];
-        if constexpr (N == 3) {
-#pragma unroll
-            for (int sgi = 0; sgi < 3; ++sgi) {
-                const int ra = (sgi == 2) ? 1 : 0;
-                const int rb = (sgi == 0) ? 1 : 2;
-                const C<R> waa = ua_entry(U, A, ra, ra), wab = ua_entry(U, A, ra, rb);
-                const C<R> wba = ua_entry(U, A, rb, ra), wbb = ua_entry(U, A, rb, rb);
-                blk8[0] = waa.re; blk8[1] = waa.im; blk8[2] = wab.re; blk8[3] = wab.im;
-                blk8[4] = wba.re; blk8[5] = wba.im; blk8[6] = wbb.re; blk8[7] = wbb.im;
-                bool ok;
-                if constexpr (RK == RK_GID) ok = kp_update<T, OVER, RK>(blk8, beta, a.nhit, sg, r);
-                else if constexpr (RK == RK_XOR128) ok = kp_update<T, OVER, RK>(blk8, beta, a.nhit, sx, r);
-                else if constexpr (RK == RK_PM) ok = kp_update<T, OVER, RK>(blk8, beta, a.nhit, sp, r);
-                else ok = kp_update<T, OVER, RK>(blk8, beta, a.nhit, sg, r);
-                if (!ok) {
-                    // lattice_unity_2 (su3_update_cl.cl:665): identity in this subgroup
-                    r[0] = T(1); r[1] = T(0); r[2] = T(0); r[3] = T(0);
-                    r[4] = T(0); r[5] = T(0); r[6] = T(1); r[7] = T(0);
-                }
-                apply_rows(U, ra, rb, r);
+        for (int k = 0; k < 3; ++k) {
+            const int nu = k + (k >= mu ? 1 : 0);
+            const M<R, N> S = staple_nu<R, N, NREAL>(Tb, a.g, s, i, p, mu, nu);
+            m_acc(A, S);
+            if constexpr (MEAS) {
+                if (nu < mu) m_acc(Alow, S);
             }
-            M<double, 3> Ud = m_cast<double>(U);
-            su3_gram_schmidt(Ud);
-            U = m_cast<R>(Ud);
-        } else {
-            if constexpr (OVER) {
-                const M<R, 2> W = m_mul(U, A);
-                blk8[0] = W.e[0][0].re; blk8[1] = W.e[0][0].im; blk8[2] = W.e[0][1].re; blk8[3] = W.e[0][1].im;
-                blk8[4] = W.e[1][0].re; blk8[5] = W.e[1][0].im; blk8[6] = W.e[1][1].re; blk8[7] = W.e[1][1].im;
-                kp_update<T, true, RK>(blk8, beta, a.nhit, sg, r);
-                M<R, 2> Rm;
-                Rm.e[0][0] = {r[0], r[1]};
-                Rm.e[0][1] = {r[2], r[3]};
-                Rm.e[1][0] = {r[4], r[5]};
-                Rm.e[1][1] = {r[6], r[7]};
-                U = m_mul(Rm, U);
+        }
+    }
+    double trace = 0.0;
+    if (active) trace = update_link<R, N, NREAL, OVER, RK, MEAS>(a, Tb, i, s, A, Alow, rs);
+    if constexpr (kStream) {
+        // Reference stream layout: every stream advances by exactly s_quads
+        // quads from its start state.
+        if (a.s_quads) {
+            uint4 o;
+            if constexpr (RK == RK_XOR128) {
+                const uint2 lo = xor128_jump_half(rs, a.jump, 0), hi = xor128_jump_half(rs, a.jump, 1);
+                o = make_uint4(lo.x, lo.y, hi.x, hi.y);
             } else {
-                blk8[0] = A.e[0][0].re; blk8[1] = A.e[0][0].im; blk8[2] = A.e[0][1].re; blk8[3] = A.e[0][1].im;
-                blk8[4] = A.e[1][0].re; blk8[5] = A.e[1][0].im; blk8[6] = A.e[1][1].re; blk8[7] = A.e[1][1].im;
-                bool ok;
-                if constexpr (RK == RK_GID) ok = kp_update<T, false, RK>(blk8, beta, a.nhit, sg, r);
-                else if constexpr (RK == RK_XOR128) ok = kp_update<T, false, RK>(blk8, beta, a.nhit, sx, r);
-                else ok = kp_update<T, false, RK>(blk8, beta, a.nhit, sp, r);
-                if (ok) {   // a rejected SU(2) update keeps the old link (su2_update_cl.cl:542-547)
-                    U.e[0][0] = {r[0], r[1]};
-                    U.e[0][1] = {r[2], r[3]};
-                    U.e[1][0] = {r[4], r[5]};
-                    U.e[1][1] = {r[6], r[7]};
-                }
+                o = make_uint4(pm_jump(rs.x, a.pm_ak), pm_jump(rs.y, a.pm_ak), pm_jump(rs.z, a.pm_ak),
+                               pm_jump(rs.w, a.pm_ak));
             }
-            M<double, 2> Ud = m_cast<double>(U);
-            su2_normalize(Ud);
-            U = m_cast<R>(Ud);
-        }
-        Tb.store_u(p, mu, i, U);
-        if constexpr (MEAS) trace = m_retr_mul(U, Alow);
-
-        if (a.s_quads == 0) {
-            if constexpr (RK == RK_XOR128) a.rng_out[i] = sx.s;
-            else if constexpr (RK == RK_PM) a.rng_out[i] = sp.s;
+            a.rng_out[i] = o;
         }
     }
     if constexpr (MEAS) {
         trace = wave_sum(trace);
-        if (lane == 0) a.partials[(size_t)(a.meas_mode - 1) * gridDim.x + blockIdx.x] = trace;
+        if (lane == 0) a.partials[(size_t)(a.meas_mode - 1) * units + unit] = trace;
     }
 }
 
@@ -578,6 +679,89 @@ __global__ __launch_bounds__(kBlock) void k_pm_produce(uint4* st, uint32_t n, ui
     st[g] = s;
 }
 
+// Reference RANLUX produce() (prngcl_ranlux.cl:534-563): every stream draws
+// `samples` quads, written to quads[k * n + g] unless quads is null (the
+// initial produce of PRNG::initialize only advances the streams).  state:
+// 7 float4 per stream at state[q * n + g]; q = 0..5 hold ring values
+// s[4q .. 4q+3], q = 6 = (23, 9, quad counter of the block, carry).  All
+// streams advance together, so the quad counter is uniform.
+template <int LUX>
+__global__ __launch_bounds__(kBlock) void k_ranlux_produce(float4* state, uint32_t n, uint32_t samples, float4* quads)
+{
+    const uint32_t g = blockIdx.x * kBlock + threadIdx.x;
+    if (g >= n) return;
+    float s[24];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        const float4 v = state[(size_t)q * n + g];
+        s[4 * q] = v.x;
+        s[4 * q + 1] = v.y;
+        s[4 * q + 2] = v.z;
+        s[4 * q + 3] = v.w;
+    }
+    const float4 c6 = state[(size_t)6 * n + g];
+    int k = __builtin_amdgcn_readfirstlane((int)c6.z);
+    float carry = c6.w;
+    for (uint32_t m = 0; m < samples; ++m) {
+        if (k >= 6) {
+            rl_skip<RlSkip<LUX>::full, RlSkip<LUX>::rem>(s, carry);
+            k = 0;
+        }
+        float4 q;
+        switch (k) {
+            case 0: q = rl_quad<0>(s, carry); break;
+            case 1: q = rl_quad<1>(s, carry); break;
+            case 2: q = rl_quad<2>(s, carry); break;
+            case 3: q = rl_quad<3>(s, carry); break;
+            case 4: q = rl_quad<4>(s, carry); break;
+            default: q = rl_quad<5>(s, carry); break;
+        }
+        if (quads) quads[(size_t)m * n + g] = q;
+        ++k;
+    }
+#pragma unroll
+    for (int q = 0; q < 6; ++q) state[(size_t)q * n + g] = make_float4(s[4 * q], s[4 * q + 1], s[4 * q + 2], s[4 * q + 3]);
+    state[(size_t)6 * n + g] = make_float4(23.0f, 9.0f, (float)k, carry);
+}
+
+void launch_ranlux_produce(int lux, float4* state, uint32_t n, uint32_t samples, float4* quads, hipStream_t st)
+{
+    const uint32_t nb = (n + kBlock - 1) / kBlock;
+    switch (lux) {
+        case 0: hipLaunchKernelGGL(k_ranlux_produce<0>, dim3(nb), dim3(kBlock), 0, st, state, n, samples, quads); break;
+        case 1: hipLaunchKernelGGL(k_ranlux_produce<1>, dim3(nb), dim3(kBlock), 0, st, state, n, samples, quads); break;
+        case 2: hipLaunchKernelGGL(k_ranlux_produce<2>, dim3(nb), dim3(kBlock), 0, st, state, n, samples, quads); break;
+        case 3: hipLaunchKernelGGL(k_ranlux_produce<3>, dim3(nb), dim3(kBlock), 0, st, state, n, samples, quads); break;
+        default: hipLaunchKernelGGL(k_ranlux_produce<4>, dim3(nb), dim3(kBlock), 0, st, state, n, samples, quads); break;
+    }
+}
+
+// Host seeding of the RANLUX table as PRNG::initialize / RL_initialize
+// (random.cpp:425-520): instance 0 from rand() % (2^31 - 1), the others from
+// rand(); 24 values from the L'Ecuyer MLCG 40014/53668/12211
+// (prngcl_ranlux.cl:98-135), carry = 2^-24 if the last value is 0; the block
+// counter starts at 6 so the first quad is preceded by a skip.
+void ranlux_seed_table(unsigned n, std::vector<float>& tab)
+{
+    tab.assign((size_t)28 * n, 0.0f);
+    for (unsigned g = 0; g < n; ++g) {
+        int j = (g == 0) ? rand() % 2147483647 : rand();
+        float v[24];
+        for (int q = 0; q < 24; ++q) {
+            const int kk = j / 53668;
+            j = 40014 * (j - kk * 53668) - kk * 12211;
+            if (j < 0) j += 2147483563;
+            v[q] = (float)(j % 16777216) * 5.9604644775390625e-08f;
+        }
+        for (int q = 0; q < 6; ++q)
+            for (int c = 0; c < 4; ++c) tab[4 * ((size_t)q * n + g) + c] = v[4 * q + c];
+        float* car = &tab[4 * ((size_t)6 * n + g)];
+        car[0] = 23.0f;
+        car[1] = 9.0f;
+        car[2] = 6.0f;
+        car[3] = (v[23] == 0.0f) ? 5.9604644775390625e-08f : 0.0f;
+    }
+}
 // ---------------------------------------------------------------- init kernels
 template <typename R, int N, int NREAL>
 __global__ __launch_bounds__(kBlock) void k_init_cold(void* table, Geom g)
@@ -589,6 +773,64 @@ __global__ __launch_bounds__(kBlock) void k_init_cold(void* table, Geom g)
     m_unit(u);
     for (int p = 0; p < 2; ++p)
         for (int mu = 0; mu < 4; ++mu) T.store_u(p, mu, i, u);
+}
+
+// Haar-random start (the reference's lattice_init_hot_* is not Haar,
+// su2_update_cl.cl:110-146).  Thread i consumes stream i for the links of its
+// two sites in the order parity 0, 1, direction 0..3: one quad per SU(2)
+// link, three per SU(3) link.  Box-Muller Gaussians in double
+// (t = 1 - u1, 2^-33 if u1 = 1): SU(2) links are a normalised 4-vector
+// (uniform on S^3), SU(3) links two complex Gaussian rows through
+// Gram-Schmidt with row 3 = conj(row 1 x row 2), a uniform orthonormal
+// 2-frame completed to SU(3), which is Haar.
+__device__ __forceinline__ void box_muller(float u1, float u2, double& g0, double& g1)
+{
+    double t = 1.0 - (double)u1;
+    if (t <= 0.0) t = 1.1641532182693481e-10;   // 2^-33
+    const double r = sqrt(-2.0 * log(t));
+    double sn, cs;
+    sincos(6.2831853071795864769 * (double)u2, &sn, &cs);
+    g0 = r * cs;
+    g1 = r * sn;
+}
+
+template <typename R, int N, int NREAL, int RK>
+__global__ __launch_bounds__(kBlock) void k_init_hot(void* table, Geom g, uint4* rng, const float4* buf, uint32_t bstride)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= g.vh) return;
+    Table<R, N, NREAL> T{(typename Vec2<R>::T*)table, g.vh_pad};
+    constexpr int Q = (N == 3) ? 3 : 1;
+    uint4 st = make_uint4(0u, 0u, 0u, 0u);
+    if constexpr (RK != RK_BUF) st = rng[i];
+    uint32_t kq = 0;   // RK_BUF: next quad of stream i
+    for (int p = 0; p < 2; ++p)
+        for (int d = 0; d < 4; ++d) {
+            double gs[4 * Q];
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                float4 u;
+                if constexpr (RK == RK_XOR128) u = xor128_quad(st);
+                else if constexpr (RK == RK_PM) u = pm_quad(st);
+                else u = buf[(size_t)(kq++) * bstride + i];
+                box_muller(u.x, u.y, gs[4 * q], gs[4 * q + 1]);
+                box_muller(u.z, u.w, gs[4 * q + 2], gs[4 * q + 3]);
+            }
+            M<double, N> a;
+            if constexpr (N == 3) {
+#pragma unroll
+                for (int r = 0; r < 2; ++r)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) a.e[r][c] = {gs[6 * r + 2 * c], gs[6 * r + 2 * c + 1]};
+                su3_gram_schmidt(a);
+            } else {
+                a.e[0][0] = {gs[0], gs[1]};
+                a.e[0][1] = {gs[2], gs[3]};
+                su2_normalize(a);
+            }
+            T.store_u(p, d, i, m_cast<R>(a));
+        }
+    if constexpr (RK != RK_BUF) rng[i] = st;
 }
 
 // lattice_init_gid (suncl.cl:190-347) / lattice_matrixGID
@@ -856,6 +1098,9 @@ struct qcd_lattice {
     size_t table_bytes = 0;
     uint4* rng_buf[2] = {nullptr, nullptr};   // generator states; rng_buf[rng_cur] is current
     uint4* jump_cols = nullptr;               // XOR128 jump-ahead matrix (128 columns)
+    float4* rl_state = nullptr;               // RANLUX: 7 float4 per stream
+    float4* prn_buf = nullptr;                // RANLUX: produced quads, prn_quads per stream
+    uint32_t prn_quads = 0;
     uint32_t pm_ak = 1;                       // Park-Miller jump multiplier
     int rng_cur = 0;
     uint32_t n_streams = 0;
@@ -877,6 +1122,7 @@ struct qcd_lattice {
     bool gid_updates = false;   // INTS = 2: GID known-answer numbers in the heatbath
     int tune_xcd_map = 2;       // QCD_TUNE_XCD_MAP
     int tune_blocks_per_cu = 0; // QCD_TUNE_BLOCKS_PER_CU (0 = no cap)
+    int tune_update_shape = 3;  // QCD_TUNE_UPDATE_SHAPE: wavefronts per 64 links (3 or 1)
 };
 
 namespace {
@@ -921,7 +1167,15 @@ void launch(Kern k, dim3 grid, dim3 block, hipStream_t s, const LaunchOpt& o, Ar
 template <typename R, int N, int NREAL, bool OVER, int RK>
 void launch_update_t(qcd_lattice* L, const UpdArgs& a, const LaunchOpt& o)
 {
-    const uint32_t nb = align_up(L->g.vh, kLinks) / kLinks;
+    const uint32_t nb = align_up(L->g.vh, kLinks) / kLinks;   // 64-link units
+    if (L->tune_update_shape == 1) {
+        const uint32_t grid = align_up(nb, kWaves1) / kWaves1;
+        if (o.meas_mode)
+            launch(k_update1<R, N, NREAL, OVER, RK, true>, dim3(grid), dim3(kWaves1 * kLinks), L->stream, o, a, nb);
+        else
+            launch(k_update1<R, N, NREAL, OVER, RK, false>, dim3(grid), dim3(kWaves1 * kLinks), L->stream, o, a, nb);
+        return;
+    }
     if (o.meas_mode) launch(k_update<R, N, NREAL, OVER, RK, true>, dim3(nb), dim3(kUpdThreads), L->stream, o, a);
     else launch(k_update<R, N, NREAL, OVER, RK, false>, dim3(nb), dim3(kUpdThreads), L->stream, o, a);
 }
@@ -940,6 +1194,8 @@ int launch_update_rn(qcd_lattice* L, int parity, int mu, int kind, int start_gid
     a.rng = L->rng_buf[L->rng_cur];
     a.rng_out = L->rng_buf[L->rng_cur ^ 1];
     a.jump = L->jump_cols;
+    a.prn = nullptr;
+    a.prn_stride = 0;
     a.pm_ak = L->pm_ak;
     a.s_quads = (L->p.stream_mode == QCD_STREAM_REFERENCE) ? L->s_quads : 0u;
     a.partials = L->part_fused;
@@ -963,6 +1219,11 @@ int launch_update_rn(qcd_lattice* L, int parity, int mu, int kind, int start_gid
         }
         if (!ok) a.xcd_map = 1;
     }
+    if (L->tune_update_shape == 1 && a.xcd_map != 0) {
+        // four 64-link units per workgroup: the XCD slabs need whole workgroups
+        const uint32_t nb = align_up(L->g.vh, kLinks) / kLinks;
+        if (nb % (8u * kWaves1)) a.xcd_map = 0;
+    }
     LaunchOpt ol = o;
     if (L->tune_blocks_per_cu > 0) {
         // 160 KiB of LDS per CU on gfx950; reserve so at most k workgroups fit.
@@ -976,10 +1237,17 @@ int launch_update_rn(qcd_lattice* L, int parity, int mu, int kind, int start_gid
         launch_update_t<R, N, NREAL, false, RK_GID>(L, a, ol);
     } else {
         if (!L->rng_ready) return fail(QCD_E_STATE, "heatbath needs qcd_prng_init first");
-        if (L->p.prng == QCD_PRNG_XOR128) launch_update_t<R, N, NREAL, false, RK_XOR128>(L, a, ol);
-        else if (L->p.prng == QCD_PRNG_PM) launch_update_t<R, N, NREAL, false, RK_PM>(L, a, ol);
-        else return fail(QCD_E_UNSUPPORTED, "generator %d not available for fused updates", L->p.prng);
-        L->rng_cur ^= 1;   // the kernel wrote the states into the other buffer
+        if (L->p.prng == QCD_PRNG_RANLUX) {
+            launch_ranlux_produce(L->p.ranlux_luxury, L->rl_state, L->n_streams, L->s_quads, L->prn_buf, L->stream);
+            a.prn = L->prn_buf;
+            a.prn_stride = L->n_streams;
+            launch_update_t<R, N, NREAL, false, RK_BUF>(L, a, ol);
+        } else {
+            if (L->p.prng == QCD_PRNG_XOR128) launch_update_t<R, N, NREAL, false, RK_XOR128>(L, a, ol);
+            else if (L->p.prng == QCD_PRNG_PM) launch_update_t<R, N, NREAL, false, RK_PM>(L, a, ol);
+            else return fail(QCD_E_UNSUPPORTED, "generator %d not available for fused updates", L->p.prng);
+            L->rng_cur ^= 1;   // the kernel wrote the states into the other buffer
+        }
     }
     HIPCHK(hipGetLastError());
     return QCD_OK;
@@ -1016,6 +1284,22 @@ int launch_simple(qcd_lattice* L, int which, const LaunchOpt& o)
         case 4:
             launch(k_polyakov<R, N, NREAL>, dim3(L->poly_blocks), dim3(kBlock), L->stream, o, L->table, L->g,
                    L->part_poly);
+            break;
+        case 5:
+            if (L->p.prng == QCD_PRNG_XOR128) {
+                hipLaunchKernelGGL((k_init_hot<R, N, NREAL, RK_XOR128>), dim3(nb), dim3(kBlock), 0, L->stream, L->table,
+                                   L->g, L->rng_buf[L->rng_cur], (const float4*)nullptr, 0u);
+            } else if (L->p.prng == QCD_PRNG_PM) {
+                hipLaunchKernelGGL((k_init_hot<R, N, NREAL, RK_PM>), dim3(nb), dim3(kBlock), 0, L->stream, L->table,
+                                   L->g, L->rng_buf[L->rng_cur], (const float4*)nullptr, 0u);
+            } else if (L->p.prng == QCD_PRNG_RANLUX) {
+                const uint32_t q = (N == 3) ? 24u : 8u;   // 8 links x 3 or 1 quads
+                launch_ranlux_produce(L->p.ranlux_luxury, L->rl_state, L->n_streams, q, L->prn_buf, L->stream);
+                hipLaunchKernelGGL((k_init_hot<R, N, NREAL, RK_BUF>), dim3(nb), dim3(kBlock), 0, L->stream, L->table,
+                                   L->g, (uint4*)nullptr, (const float4*)L->prn_buf, L->n_streams);
+            } else {
+                return fail(QCD_E_UNSUPPORTED, "generator %d not available for the hot start", L->p.prng);
+            }
             break;
         default:
             return fail(QCD_E_ARG, "internal: bad launch %d", which);
@@ -1156,6 +1440,16 @@ int qcd_lattice_create(const qcd_params* pp, qcd_lattice** out)
         qcd_lattice_destroy(L);
         return fail(QCD_E_NOMEM, "device allocation failed (table %zu bytes)", L->table_bytes);
     }
+    if (p.prng == QCD_PRNG_RANLUX) {
+        // One produce() per heatbath half-step (s_quads per stream) or per hot
+        // start (8 links x 3 quads for SU(3)).
+        L->prn_quads = L->s_quads > 24u ? L->s_quads : 24u;
+        if (hipMalloc((void**)&L->rl_state, sizeof(float4) * 7 * L->n_streams) != hipSuccess ||
+            hipMalloc((void**)&L->prn_buf, sizeof(float4) * (size_t)L->prn_quads * L->n_streams) != hipSuccess) {
+            qcd_lattice_destroy(L);
+            return fail(QCD_E_NOMEM, "RANLUX buffers");
+        }
+    }
     L->part_fused = L->partials;
     L->part_plq = L->part_fused + (size_t)3 * L->nb_fused;
     L->part_poly = L->part_plq + (size_t)2 * 2 * L->upd_blocks;
@@ -1200,6 +1494,8 @@ int qcd_lattice_destroy(qcd_lattice* L)
     (void)hipFree(L->rng_buf[0]);
     (void)hipFree(L->rng_buf[1]);
     (void)hipFree(L->jump_cols);
+    (void)hipFree(L->rl_state);
+    (void)hipFree(L->prn_buf);
     (void)hipFree(L->partials);
     (void)hipFree(L->rec_index);
     (void)hipFree(L->plq_rec);
@@ -1228,6 +1524,10 @@ int qcd_lattice_tune(qcd_lattice* L, int knob, int value)
             if (value < 0 || value > 32) return fail(QCD_E_ARG, "blocks per CU must be 0..32");
             L->tune_blocks_per_cu = value;
             break;
+        case QCD_TUNE_UPDATE_SHAPE:
+            if (value != 1 && value != 3) return fail(QCD_E_ARG, "update shape must be 1 or 3");
+            L->tune_update_shape = value;
+            break;
         default:
             return fail(QCD_E_ARG, "unknown tuning knob %d", knob);
     }
@@ -1254,16 +1554,33 @@ int qcd_lattice_init(qcd_lattice* L, int start)
         L->gid_updates = true;
         return DISPATCH(L, launch_simple, L, 1, LaunchOpt{});
     }
-    if (start == QCD_START_HOT) return fail(QCD_E_UNSUPPORTED, "hot start is not implemented yet");
+    if (start == QCD_START_HOT) {
+        if (!L->rng_ready) return fail(QCD_E_STATE, "the hot start draws from the streams: call qcd_prng_init first");
+        L->gid_updates = false;
+        return DISPATCH(L, launch_simple, L, 5, LaunchOpt{});
+    }
     return fail(QCD_E_ARG, "bad start type %d", start);
 }
 
 int qcd_prng_init(qcd_lattice* L)
 {
     if (!L) return fail(QCD_E_ARG, "null lattice");
-    if (L->p.prng != QCD_PRNG_XOR128 && L->p.prng != QCD_PRNG_PM)
-        return fail(QCD_E_UNSUPPORTED, "generator %d is not available for fused streams yet", L->p.prng);
     unsigned seed = L->p.randseries ? L->p.randseries : (unsigned)time(nullptr);
+    if (L->p.prng == QCD_PRNG_RANLUX) {
+        // RANLUX streams are always in the reference layout: a produce()
+        // fills the quad buffer before every heatbath half-step.
+        std::vector<float> tab;
+        srand(seed);
+        ranlux_seed_table(L->n_streams, tab);
+        HIPCHK(hipMemcpyAsync(L->rl_state, tab.data(), sizeof(float) * tab.size(), hipMemcpyHostToDevice, L->stream));
+        launch_ranlux_produce(L->p.ranlux_luxury, L->rl_state, L->n_streams, L->s_quads, nullptr, L->stream);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(L->stream));
+        L->rng_ready = true;
+        return QCD_OK;
+    }
+    if (L->p.prng != QCD_PRNG_XOR128 && L->p.prng != QCD_PRNG_PM)
+        return fail(QCD_E_UNSUPPORTED, "generator %d is not available", L->p.prng);
     std::vector<uint32_t> tab((size_t)4 * L->n_streams);
     srand(seed);
     // XOR128: instance 0 = XOR128_initialize_CPU (4 x rand(), random.cpp:675-681),
@@ -1631,12 +1948,36 @@ int qcd_cycle_profile(qcd_lattice* L, int repeats, double* hb_us, double* or_us,
 
 int qcd_prng_produce_host(int prng, int ranlux_luxury, unsigned n, unsigned samples, void* table, float* quads)
 {
-    (void)ranlux_luxury;
     if (!table || !quads || n == 0) return fail(QCD_E_ARG, "null argument");
-    if (prng != QCD_PRNG_XOR128 && prng != QCD_PRNG_PM)
+    if (prng != QCD_PRNG_XOR128 && prng != QCD_PRNG_PM && prng != QCD_PRNG_RANLUX)
         return fail(QCD_E_UNSUPPORTED, "generator %d not available", prng);
+    if (prng == QCD_PRNG_RANLUX && (ranlux_luxury < 0 || ranlux_luxury > 4))
+        return fail(QCD_E_ARG, "luxury level must be 0..4");
     int rc = ensure_device();
     if (rc) return rc;
+    if (prng == QCD_PRNG_RANLUX) {
+        // table: 7 float4 per instance, table[(q * n + g) * 4 + c]
+        float4* ds = nullptr;
+        float4* dq = nullptr;
+        const size_t sbytes = sizeof(float4) * 7 * (size_t)n, qbytes = sizeof(float4) * (size_t)n * samples;
+        HIPCHK(hipMalloc((void**)&ds, sbytes));
+        if (hipMalloc((void**)&dq, qbytes ? qbytes : 16) != hipSuccess) {
+            (void)hipFree(ds);
+            return fail(QCD_E_NOMEM, "quads buffer");
+        }
+        hipError_t e = hipMemcpy(ds, table, sbytes, hipMemcpyHostToDevice);
+        if (e == hipSuccess) {
+            launch_ranlux_produce(ranlux_luxury, ds, n, samples, dq, 0);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipDeviceSynchronize();
+        if (e == hipSuccess && qbytes) e = hipMemcpy(quads, dq, qbytes, hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(table, ds, sbytes, hipMemcpyDeviceToHost);
+        (void)hipFree(ds);
+        (void)hipFree(dq);
+        if (e != hipSuccess) return fail(QCD_E_HIP, "ranlux produce: %s", hipGetErrorString(e));
+        return QCD_OK;
+    }
     uint4* dt = nullptr;
     float4* dq = nullptr;
     size_t qbytes = sizeof(float4) * (size_t)n * samples;

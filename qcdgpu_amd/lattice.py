@@ -176,11 +176,16 @@ class Lattice:
         )
         return {"hb_us": hb.value, "or_us": orr.value, "meas_us": meas.value, "total_ms": tot.value}
 
-    def tune(self, xcd_map=None, blocks_per_cu=None):
+    def tune(self, xcd_map=None, blocks_per_cu=None, update_shape=None):
+        """Launch-shape knobs of the update kernels (scheduling only; results
+        are identical): xcd_map 0/1/2, blocks_per_cu cap (0 = none),
+        update_shape 3 (three wavefronts per 64 links) or 1."""
         if xcd_map is not None:
             check(lib().qcd_lattice_tune(self._h, _abi.TUNE_XCD_MAP, int(xcd_map)))
         if blocks_per_cu is not None:
             check(lib().qcd_lattice_tune(self._h, _abi.TUNE_BLOCKS_PER_CU, int(blocks_per_cu)))
+        if update_shape is not None:
+            check(lib().qcd_lattice_tune(self._h, _abi.TUNE_UPDATE_SHAPE, int(update_shape)))
 
     def table_bytes(self):
         b = ctypes.c_size_t()
@@ -192,9 +197,13 @@ class Lattice:
         """model::lattice_simulate: start, measure, NAV sweeps, then ITER-1
         measurements each after NITER sweeps (suncl.cpp:5236-5483).  Each sweep
         is one heatbath sweep plus `n_or` overrelaxation sweeps."""
-        self.init(start)
-        if start != "gid":
-            self.prng_init()
+        if start == "hot":
+            self.prng_init()   # the Haar start draws from the streams
+            self.init(start)
+        else:
+            self.init(start)
+            if start != "gid":
+                self.prng_init()
         self.reset_measurements()
         self.measure()
         for _ in range(nav):
@@ -219,16 +228,23 @@ def device_info():
     return {"name": name.value.decode(), "compute_units": cus.value, "global_mem": mem.value}
 
 
-def prng_produce(prng, table, samples):
+def prng_produce(prng, table, samples, luxury=3):
     """Reference PRNG::produce buffer semantics on the device: returns
-    quads[samples, n, 4]; `table` (uint32 [n, 4]) is advanced in place."""
-    code = {"XOR128": _abi.PRNG_XOR128, "PM": _abi.PRNG_PM}[prng]
-    t = np.ascontiguousarray(table, dtype=np.uint32)
-    n = t.shape[0]
+    quads[samples, n, 4]; `table` is advanced in place.  XOR128 / PM: uint32
+    [n, 4]; RANLUX: float32 [7, n, 4] (six float4 of lagged values and
+    (23, 9, block counter, carry) per instance)."""
+    code = {"XOR128": _abi.PRNG_XOR128, "PM": _abi.PRNG_PM, "RANLUX": _abi.PRNG_RANLUX}[prng]
+    if prng == "RANLUX":
+        t = np.ascontiguousarray(table, dtype=np.float32)
+        n = t.shape[1]
+    else:
+        t = np.ascontiguousarray(table, dtype=np.uint32)
+        n = t.shape[0]
     q = np.zeros((samples, n, 4), dtype=np.float32)
     check(
         lib().qcd_prng_produce_host(
-            code, 0, n, samples, t.ctypes.data_as(ctypes.c_void_p), q.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+            code, int(luxury), n, samples, t.ctypes.data_as(ctypes.c_void_p),
+            q.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
         )
     )
     table[...] = t

@@ -23,7 +23,13 @@ def _case(msg):
     print("[case]", msg, flush=True)
 
 
-def _lat(sun, dims, prec, storage=None, beta=5.9, nhit=10, prng="XOR128", stream="reference", seed=54):
+def _lat(sun, dims, prec, storage=None, beta=5.9, nhit=10, prng="XOR128", stream="reference", seed=54,
+         max_measurements=64):
+    """prng: "XOR128", "PM" or "RANLUX<luxury>" (e.g. "RANLUX3")."""
+    lux = 3
+    if prng.startswith("RANLUX"):
+        lux = int(prng[6:] or 3)
+        prng = "RANLUX"
     return Q.Lattice(
         sun=sun,
         dims=dims,
@@ -32,9 +38,10 @@ def _lat(sun, dims, prec, storage=None, beta=5.9, nhit=10, prng="XOR128", stream
         storage_reals=storage,
         nhit=nhit,
         prng=prng,
+        ranlux_luxury=lux,
         stream=stream,
         randseries=seed,
-        max_measurements=64,
+        max_measurements=max_measurements,
     )
 
 
@@ -97,6 +104,23 @@ def test_gid_sweeps_vs_reference_suncpp(golden, case, storage, prec):
 
 
 # ---------------------------------------------------------------- reference stream layout vs oracle buffer semantics
+@pytest.mark.parametrize("lux", [0, 1, 2, 3, 4])
+def test_ranlux_produce_bit_exact(O, lux):
+    """Device RANLUX produce() vs the oracle restatement (itself pinned to the
+    reference's scalar generator): several calls of odd sizes, quads and the
+    advanced tables bit for bit."""
+    _case("ranlux produce lux=%d" % lux)
+    n = 200
+    ta = O.ranlux_seed_table(54, n)
+    tb = ta.copy().reshape(7, n, 4)
+    nskip = [24, 48, 97, 223, 389][lux]
+    for samples in (33, 7, 1, 50):
+        ref = O.ranlux_produce(ta, samples, nskip)
+        got = Q.prng_produce("RANLUX", tb, samples, luxury=lux)
+        assert (got.view(np.uint32) == ref.reshape(got.shape).view(np.uint32)).all()
+        assert (tb.reshape(-1).view(np.uint32) == ta.reshape(-1).view(np.uint32)).all()
+
+
 def _oracle_reference_stream_sweeps(O, L, links, prng, seed, nsweeps):
     """The reference's GPU path restated: seed table for PRNG_instances =
     PRNGSTEP, one produce() at initialisation, then one produce() of S quads
@@ -108,9 +132,13 @@ def _oracle_reference_stream_sweeps(O, L, links, prng, seed, nsweeps):
     if prng == "XOR128":
         table = O.xor128_seed_table(seed, prngstep)
         produce = O.xor128_produce
-    else:
+    elif prng == "PM":
         table = O.pm_seed_table(seed, prngstep)
         produce = O.pm_produce
+    else:   # "RANLUX<lux>"
+        table = O.ranlux_seed_table(seed, prngstep)
+        nskip = [24, 48, 97, 223, 389][int(prng[-1])]
+        produce = lambda t, k: O.ranlux_produce(t, k, nskip)   # noqa: E731
     produce(table, S)
     rejected = 0
     for _ in range(nsweeps):
@@ -126,6 +154,8 @@ STREAM_CASES = [
     (3, (4, 4, 4, 4), 6.0, "PM", 7),
     (2, (6, 4, 2, 4), 2.3, "XOR128", 1),
     (3, (8, 4, 4, 8), 5.9, "XOR128", 3),
+    (2, (4, 4, 4, 4), 2.3, "RANLUX3", 54),
+    (3, (6, 4, 2, 4), 5.9, "RANLUX0", 5),
 ]
 
 
@@ -333,3 +363,115 @@ def test_c1_size_invariants():
         Uc = U[..., 0] + 1j * U[..., 1]
         dev = np.abs(np.einsum("...ji,...jk->...ik", Uc[::97].conj(), Uc[::97]) - np.eye(3)).max()
         assert dev < 2e-6
+
+
+@pytest.mark.parametrize("prec,storage", [("fp32", None), ("fp32", 12), ("fp64", None)])
+def test_update_shapes_identical(prec, storage):
+    """The two update-kernel shapes (three wavefronts per 64 links, one
+    wavefront per 64 links) run the same arithmetic in the same order:
+    heatbath + overrelaxation sweeps and the fused plaquette must agree
+    bit for bit, in both stream layouts."""
+    _case("update shapes %s storage=%s" % (prec, storage))
+    dims = (16, 16, 16, 8)   # large enough for the XCD slab orders of both shapes
+    res = {}
+    for shape in (3, 1):
+        for stream in ("reference", "fast"):
+            with _lat(3, dims, prec, storage, seed=5, stream=stream) as lat:
+                lat.tune(update_shape=shape)
+                lat.init("cold")
+                lat.prng_init()
+                lat.run_cycle(n_hb=2, n_or=2)
+                lat.sync()
+                res[(shape, stream)] = (lat.export_links(), lat.raw_measurements())
+    for stream in ("reference", "fast"):
+        a, b = res[(3, stream)], res[(1, stream)]
+        assert np.array_equal(a[0], b[0])
+        assert np.array_equal(a[1][0], b[1][0]) and np.array_equal(a[1][1], b[1][1])
+
+
+@pytest.mark.parametrize("sun,prec,storage,prng", [(3, "fp64", None, "XOR128"), (3, "fp32", 12, "XOR128"),
+                                                   (2, "fp64", None, "PM"), (3, "fp32", None, "PM"),
+                                                   (2, "fp64", None, "RANLUX3"), (3, "fp64", None, "RANLUX2")])
+def test_hot_start_vs_oracle(O, sun, prec, storage, prng):
+    """Haar hot start on the device vs the oracle on the same streams
+    (reference layout: the initial skip of S quads, then the start), then one
+    reference-stream heatbath sweep from it."""
+    _case("hot start sun=%d %s storage=%s %s" % (sun, prec, storage, prng))
+    dims = (6, 4, 4, 4)
+    beta = 5.9 if sun == 3 else 2.3
+    L = O.lattice(sun, dims, beta, 10)
+    V = int(np.prod(dims))
+    prngstep = -(-(V // 2) // 64) * 64
+    S = 1 + (-(-(3 * prngstep * 11) // 64) * 64 - 1) // prngstep
+    seed = 19
+    if prng == "XOR128":
+        table, produce = O.xor128_seed_table(seed, prngstep), O.xor128_produce
+    elif prng == "PM":
+        table, produce = O.pm_seed_table(seed, prngstep), O.pm_produce
+    else:
+        nskip = [24, 48, 97, 223, 389][int(prng[-1])]
+        table = O.ranlux_seed_table(seed, prngstep)
+        produce = lambda t, k: O.ranlux_produce(t, k, nskip)   # noqa: E731
+    produce(table, S)
+    Q = 3 if sun == 3 else 1
+    ref = O.hot(L, produce(table, 8 * Q), prngstep)
+    with _lat(sun, dims, prec, storage, beta=beta, prng=prng, seed=seed) as lat:
+        lat.prng_init()
+        lat.init("hot")
+        got0 = lat.export_links()
+        lat.sweep("hb")
+        got1 = lat.export_links()
+    tol0 = 1e-13 if prec == "fp64" else 2e-7
+    assert np.abs(got0 - ref).max() <= tol0
+    if prec == "fp64":
+        for parity in (1, 0):
+            for mu in range(4):
+                O.halfstep(L, ref, parity, mu, O.HB, O.QuadRng(produce(table, S), prngstep))
+        err = np.abs(got1 - ref).max()
+        print("  sweep after hot start max|dU| = %.3e" % err)
+        assert err <= 1e-12
+
+
+def _binned(x, nbins=10):
+    x = np.asarray(x)
+    n = len(x) // nbins * nbins
+    b = x[:n].reshape(nbins, -1).mean(axis=1)
+    return b.mean(), b.std(ddof=1) / np.sqrt(nbins)
+
+
+def test_c0_su2_ensemble_vs_oracle(O):
+    """C0: SU(2) 8^4, beta 2.3, FP64 links, Haar hot start, RANLUX3 streams,
+    200 x (1 Kennedy-Pendleton heatbath + 3 overrelaxation) sweeps with the
+    plaquette every sweep.  Against an independent chain of the oracle with
+    the reference CPU path's semantics (cold start, sequential RANLUX3,
+    heatbath only): the mean plaquette over sweeps 50..199 agrees within
+    3 combined binned standard errors."""
+    _case("C0 SU(2) 8^4 beta 2.3 ensemble")
+    dims, beta = (8, 8, 8, 8), 2.3
+    with _lat(2, dims, "fp64", beta=beta, prng="RANLUX3", seed=54, max_measurements=256) as lat:
+        lat.prng_init()
+        lat.init("hot")
+        lat.reset_measurements()
+        lat.capture_cycle(n_hb=1, n_or=3, plaquette=True, polyakov=True)
+        lat.launch_cycle(200)
+        lat.sync()
+        m = lat.measurements()
+    plq_gpu = m["plq_total"][50:]
+    assert len(m["plq_total"]) == 200
+    L = O.lattice(2, dims, beta, 10)
+    a = O.cold(L)
+    rng = O.SeqRng("RANLUX", 7, 223)
+    V = int(np.prod(dims))
+    plq_orc = []
+    for k in range(200):
+        O.sweep(L, a, O.HB, rng)
+        if k >= 50:
+            plq_orc.append(O.plaquette(L, a).sum() / (6 * V))
+    g_mean, g_err = _binned(plq_gpu)
+    o_mean, o_err = _binned(plq_orc)
+    z = (g_mean - o_mean) / np.hypot(g_err, o_err)
+    print("  <ReTr U_p>: GPU %.5f +- %.5f, oracle %.5f +- %.5f, z = %.2f (per colour: %.5f)"
+          % (g_mean, g_err, o_mean, o_err, z, g_mean / 2))
+    assert abs(z) < 3.0
+    # Polyakov loop of a confined SU(2) ensemble at this beta averages to ~0
+    assert abs(np.mean(m["polyakov_re"][50:])) < 0.1

@@ -304,3 +304,45 @@ def test_quads_first_reject_then_accept(O):
     rej = O.halfstep(L, a, 0, 3, O.HB, O.QuadRng(q, 8))
     assert rej == 0
     assert np.isfinite(a).all()
+
+
+# ---------------------------------------------------------------- Haar hot start
+def _moment_ok(x, exact, nsig=5.0):
+    se = np.std(x) / np.sqrt(len(x))
+    return abs(np.mean(x) - exact) <= nsig * max(se, 1e-12), (np.mean(x), exact, se)
+
+
+@pytest.mark.parametrize("sun", [3, 2])
+def test_hot_start_is_haar(O, sun):
+    """Haar moments of the oracle's hot start (independent of its formula):
+    SU(N): E Tr U = 0, E |Tr U|^2 = 1, E |Tr U|^4 = 2 (Diaconis-Shahshahani;
+    for SU(2) Tr U = 2 a0 with a0 semicircular), and for SU(3)
+    E (Tr U)^3 = 1 (the baryon-like invariant that vanishes for U(3))."""
+    dims = (12, 12, 12, 12)
+    L = O.lattice(sun, dims, 5.9, 10)
+    n = int(np.prod(dims)) // 2
+    Q = 3 if sun == 3 else 1
+    q = O.xor128_produce(O.xor128_seed_table(7, n), 8 * Q)
+    a = O.hot(L, q, n)
+    M = (a[..., 0] + 1j * a[..., 1]).reshape(-1, sun, sun)
+    uu = np.einsum("nij,nkj->nik", M, M.conj())
+    assert np.abs(uu - np.eye(sun)).max() < 1e-13
+    assert np.abs(np.linalg.det(M) - 1).max() < 1e-13
+    tr = np.trace(M, axis1=1, axis2=2)
+    for x, exact in [(tr.real, 0.0), (tr.imag, 0.0), (np.abs(tr) ** 2, 1.0), (np.abs(tr) ** 4, 2.0)]:
+        ok, info = _moment_ok(x, exact)
+        assert ok, info
+    if sun == 3:
+        ok, info = _moment_ok((tr ** 3).real, 1.0)
+        assert ok, info
+        ok, info = _moment_ok((tr ** 3).imag, 0.0)
+        assert ok, info
+
+
+def test_hot_start_box_muller_edge(O):
+    """u1 = 1 (XOR128 output 2^32 after rounding) must not give inf/nan."""
+    L = O.lattice(2, (2, 2, 2, 2), 2.3, 10)
+    n = 8
+    q = np.ones((8, n, 4), dtype=np.float32)
+    a = O.hot(L, q, n)
+    assert np.isfinite(a).all()
