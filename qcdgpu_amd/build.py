@@ -20,6 +20,26 @@ HIPCC_FLAGS = [
 ]
 
 
+HOST_SRC = os.path.join(PKG, "host", "qcdgpu_main.cpp")
+HOST_BIN = os.path.join(PKG, "qcdgpu_hip")
+
+
+def build_host(verbose=False):
+    """The command-line host program (C++ over the C ABI), next to the
+    library it loads through an $ORIGIN run path."""
+    cxx = os.environ.get("CXX", "g++")
+    tmp = HOST_BIN + ".tmp"
+    cmd = [cxx, "-O2", "-std=c++17", "-Wall", "-o", tmp, HOST_SRC, "-L" + PKG, "-lqcdhip",
+           "-Wl,-rpath,$ORIGIN", "-L/opt/rocm/lib", "-Wl,-rpath-link,/opt/rocm/lib"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError("host build failed:\n" + res.stdout + res.stderr)
+    os.replace(tmp, HOST_BIN)
+    if verbose:
+        print("built", HOST_BIN)
+    return HOST_BIN
+
+
 def build(verbose=False):
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     src = os.path.join(CSRC, "qcd_hip.hip")
@@ -31,6 +51,7 @@ def build(verbose=False):
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    build_host(verbose)
     return LIB
 
 

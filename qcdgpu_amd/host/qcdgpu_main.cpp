@@ -1,0 +1,377 @@
+// qcdgpu_main.cpp -- command-line host program with the interface of the
+// reference's QCDGPU.cpp for the SU(N) update/measurement path, calling the
+// device only through the C ABI of libqcdhip.so (include/qcd_hip.h).
+//
+//   qcdgpu_hip [init.dat]            run the simulation the file describes
+//   qcdgpu_hip --print-params [file] parse only, print the resolved parameters
+//
+// Parameters, defaults and the init.dat syntax follow SU(N)/QCDGPU.cpp:90-235
+// (defaults), model::lattice_get_init_file (SU(N)/suncl/suncl.cpp:289-460)
+// and GPU::get_init_file (SU(N)/clinterface/clinterface.cpp:2060-2140):
+// one "KEY = VALUE" per line, blanks around both, text after '#' ignored.
+// The run loop is model::lattice_simulate (suncl.cpp:5212-5505): the first
+// measurement is taken on the start configuration, then NAV thermalisation
+// sweeps, then ITER-1 measurements each after NITER sweeps.  Normalisations,
+// means and variances are those of model::lattice_analysis and
+// analysis::lattice_data_analysis (suncl.cpp:1328-1612,
+// data_analysis.cpp:107-171): means and variances leave out measurement 0.
+// Output follows model::lattice_print_measurements (suncl.cpp:867-903) and
+// model::lattice_write_results (suncl.cpp:1679-1854).
+//
+// Keys added by this program (absent from the reference): PRECISION
+// (SINGLE/DOUBLE, the reference sets it in QCDGPU.cpp), NOR (overrelaxation
+// sweeps after every heatbath sweep, default 0), STORAGE (18 or 12 reals per
+// SU(3) link), STREAMS (REFERENCE or FAST generator layout), LUXURY is
+// carried in the PRNG name (RANLUX0..RANLUX4) as in the reference.
+// Keys of features outside this path (Wilson loops, F_munu, twisted boundary
+// conditions, BIGLAT) are rejected when they ask for the feature.
+
+#include <cerrno>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <ctime>
+#include <string>
+#include <vector>
+
+#include "../../include/qcd_hip.h"
+
+namespace {
+
+struct Settings {
+    int platform = 0, device = 0;
+    int group = 3, nd = 4;
+    int n[4] = {32, 32, 32, 32};
+    int iter = 2, niter = 1, nav = 0, nhit = 10, nhitpar = 1;
+    double beta = 6.5, phi = 0.0, omega = 0.0;
+    int ints = 1;                    // 0 hot, 1 cold, 2 gid (model::convert_uint_to_start)
+    std::string prng = "RANLUX3";
+    unsigned randseries = 54;
+    int pl_level = 3;
+    bool wilson = false, fmunu = false;
+    std::string outputpath = "./", finishpath = "./";
+    bool double_precision = true;    // QCDGPU.cpp: model_precision_double
+    int nor = 0, storage = 0;        // storage 0: 18 reals (SU(3)) / 4 (SU(2))
+    bool fast_streams = false;
+};
+
+std::string trim(const std::string& s)
+{
+    size_t a = s.find_first_not_of(" \t\r\n");
+    if (a == std::string::npos) return "";
+    size_t b = s.find_last_not_of(" \t\r\n");
+    return s.substr(a, b - a + 1);
+}
+
+std::string upper(std::string s)
+{
+    for (auto& c : s) c = (char)toupper((unsigned char)c);
+    return s;
+}
+
+// Returns false (and prints why) on an unusable value.
+bool apply(Settings& st, const std::string& key, const std::string& val)
+{
+    const long iv = strtol(val.c_str(), nullptr, 10);
+    const double fv = strtod(val.c_str(), nullptr);
+    if (key == "PLATFORM") st.platform = (int)iv;
+    else if (key == "DEVICE") st.device = (int)iv;
+    else if (key == "GROUP") st.group = (int)iv;
+    else if (key == "ND") st.nd = (int)iv;
+    else if (key == "L1") st.n[0] = (int)iv;
+    else if (key == "L2") st.n[1] = (int)iv;
+    else if (key == "L3") st.n[2] = (int)iv;
+    else if (key == "L4") st.n[3] = (int)iv;
+    else if (key == "LS") st.n[0] = st.n[1] = st.n[2] = (int)iv;
+    else if (key == "LT") st.n[3] = (int)iv;
+    else if (key == "ITER") st.iter = (int)iv;
+    else if (key == "NITER") st.niter = (int)iv;
+    else if (key == "NHIT") st.nhit = (int)iv;
+    else if (key == "NHITPAR") st.nhitpar = (int)iv;
+    else if (key == "BETA") st.beta = fv;
+    else if (key == "PHI") st.phi = fv;
+    else if (key == "OMEGA") st.omega = fv;
+    else if (key == "NAV") st.nav = (int)iv;
+    else if (key == "INTS") st.ints = (iv == 1 || iv == 2) ? (int)iv : 0;   // anything else: hot
+    else if (key == "PRNG") st.prng = upper(val);
+    else if (key == "RANDSERIES") st.randseries = (unsigned)iv;
+    else if (key == "PL_LEVEL") st.pl_level = (int)iv;
+    else if (key == "OUTPUTPATH") st.outputpath = val;
+    else if (key == "FINISHPATH") st.finishpath = val;
+    else if (key == "GETWILSON") st.wilson = true;
+    else if (key == "FMUNU" || key == "F0MU") st.fmunu = true;
+    else if (key == "TURNOFFFMUNU") st.fmunu = false;
+    else if (key == "PRECISION") {
+        const std::string u = upper(val);
+        if (u == "SINGLE" || u == "1") st.double_precision = false;
+        else if (u == "DOUBLE" || u == "2") st.double_precision = true;
+        else {
+            fprintf(stderr, "PRECISION must be SINGLE or DOUBLE (got '%s')\n", val.c_str());
+            return false;
+        }
+    } else if (key == "NOR") st.nor = (int)iv;
+    else if (key == "STORAGE") st.storage = (int)iv;
+    else if (key == "STREAMS") {
+        const std::string u = upper(val);
+        if (u == "FAST") st.fast_streams = true;
+        else if (u == "REFERENCE") st.fast_streams = false;
+        else {
+            fprintf(stderr, "STREAMS must be REFERENCE or FAST (got '%s')\n", val.c_str());
+            return false;
+        }
+    }
+    // other keys (WILSONR, WILSONT, Fst, N1, TURNOFFWAITING, REBUILDBINARY,
+    // GETRETRACE ...) are accepted and have no effect, as in the reference
+    return true;
+}
+
+bool read_init_file(const char* path, Settings& st)
+{
+    FILE* f = fopen(path, "r");
+    if (!f) {
+        fprintf(stderr, "cannot open %s: %s\n", path, strerror(errno));
+        return false;
+    }
+    char line[1024];
+    bool ok = true;
+    while (fgets(line, sizeof(line), f)) {
+        std::string s(line);
+        const size_t hash = s.find('#');
+        if (hash != std::string::npos) s = s.substr(0, hash);
+        const size_t eq = s.find('=');
+        if (eq == std::string::npos) continue;
+        const std::string key = upper(trim(s.substr(0, eq)));
+        const std::string val = trim(s.substr(eq + 1));
+        if (!key.empty()) ok = apply(st, key, val) && ok;
+    }
+    fclose(f);
+    return ok;
+}
+
+// Translate to the C ABI, rejecting what this path does not provide.
+bool to_params(const Settings& st, qcd_params& p, std::string& why)
+{
+    if (st.nd != 4) return why = "ND must be 4", false;
+    if (st.group != 2 && st.group != 3) return why = "GROUP must be 2 or 3", false;
+    if (st.phi != 0.0 || st.omega != 0.0) return why = "PHI/OMEGA (twisted boundary conditions) are not supported", false;
+    if (st.wilson) return why = "GETWILSON (Wilson loops) is not supported", false;
+    if (st.fmunu) return why = "FMUNU/F0MU (field-strength measurements) are not supported", false;
+    if (st.iter < 1 || st.niter < 0 || st.nav < 0 || st.nor < 0) return why = "ITER >= 1, NITER, NAV, NOR >= 0", false;
+    memset(&p, 0, sizeof(p));
+    p.sun = st.group;
+    for (int k = 0; k < 4; ++k) p.n[k] = st.n[k];
+    p.precision = st.double_precision ? QCD_FP64 : QCD_FP32;
+    p.storage_reals = st.storage ? st.storage : (st.group == 3 ? 18 : 4);
+    p.beta = st.beta;
+    p.nhit = st.nhit;
+    p.nhitpar = st.nhitpar;
+    p.ranlux_luxury = 3;
+    if (st.prng == "XOR128") p.prng = QCD_PRNG_XOR128;
+    else if (st.prng == "PM") p.prng = QCD_PRNG_PM;
+    else if (st.prng == "RANLUX") p.prng = QCD_PRNG_RANLUX;
+    else if (st.prng.size() == 7 && st.prng.compare(0, 6, "RANLUX") == 0 && st.prng[6] >= '0' && st.prng[6] <= '4') {
+        p.prng = QCD_PRNG_RANLUX;
+        p.ranlux_luxury = st.prng[6] - '0';
+    } else {
+        return why = "PRNG " + st.prng + " is not available (XOR128, PM, RANLUX0..RANLUX4)", false;
+    }
+    p.stream_mode = st.fast_streams ? QCD_STREAM_FAST : QCD_STREAM_REFERENCE;
+    p.randseries = st.randseries;
+    p.max_measurements = st.iter;
+    return true;
+}
+
+void print_params(const Settings& st)
+{
+    printf("GROUP=%d\nND=%d\nL1=%d\nL2=%d\nL3=%d\nL4=%d\n", st.group, st.nd, st.n[0], st.n[1], st.n[2], st.n[3]);
+    printf("ITER=%d\nNITER=%d\nNAV=%d\nNHIT=%d\nNHITPAR=%d\n", st.iter, st.niter, st.nav, st.nhit, st.nhitpar);
+    printf("BETA=%.13e\nPHI=%.13e\nOMEGA=%.13e\nINTS=%d\n", st.beta, st.phi, st.omega, st.ints);
+    printf("PRNG=%s\nRANDSERIES=%u\nPL_LEVEL=%d\n", st.prng.c_str(), st.randseries, st.pl_level);
+    printf("PRECISION=%s\nNOR=%d\nSTORAGE=%d\nSTREAMS=%s\n", st.double_precision ? "DOUBLE" : "SINGLE", st.nor,
+           st.storage ? st.storage : (st.group == 3 ? 18 : 4), st.fast_streams ? "FAST" : "REFERENCE");
+    printf("OUTPUTPATH=%s\nFINISHPATH=%s\nPLATFORM=%d\nDEVICE=%d\n", st.outputpath.c_str(), st.finishpath.c_str(),
+           st.platform, st.device);
+}
+
+#define CHECK(call)                                                                   \
+    do {                                                                              \
+        const int rc_ = (call);                                                       \
+        if (rc_ != QCD_OK) {                                                          \
+            fprintf(stderr, "%s failed (%d): %s\n", #call, rc_, qcd_last_error());   \
+            exit(-1); /* every device error is fatal, as in the reference */          \
+        }                                                                             \
+    } while (0)
+
+struct Series {
+    const char* name;
+    std::vector<double> v;
+    double mean = 0.0, variance = 0.0;
+    void analyse()
+    {   // analysis::lattice_data_analysis: measurement 0 is left out
+        const size_t last = v.size() - 1;
+        mean = variance = 0.0;
+        for (size_t i = 1; i < v.size(); ++i) mean += v[i];
+        if (last > 0) mean /= (double)last;
+        for (size_t i = 1; i < v.size(); ++i) variance += (v[i] - mean) * (v[i] - mean);
+        if (last > 0) variance /= (double)last;
+    }
+};
+
+}  // namespace
+
+int main(int argc, char** argv)
+{
+    Settings st;
+    bool print_only = false;
+    const char* file = nullptr;
+    for (int k = 1; k < argc; ++k) {
+        if (!strcmp(argv[k], "--print-params")) print_only = true;
+        else file = argv[k];
+    }
+    if (file && !read_init_file(file, st)) return 1;
+    qcd_params p;
+    std::string why;
+    const bool ok = to_params(st, p, why);
+    if (print_only) {
+        print_params(st);
+        if (!ok) printf("UNSUPPORTED=%s\n", why.c_str());
+        return ok ? 0 : 2;
+    }
+    if (!ok) {
+        fprintf(stderr, "%s\n", why.c_str());
+        return 2;
+    }
+
+    const time_t t0 = time(nullptr);
+    char timestart[32];
+    strncpy(timestart, ctime(&t0), sizeof(timestart) - 1);
+    timestart[sizeof(timestart) - 1] = 0;
+    CHECK(qcd_device_open(st.platform, st.device));
+    char devname[256];
+    int cus = 0;
+    size_t mem = 0;
+    CHECK(qcd_device_info(devname, sizeof(devname), &cus, &mem));
+    printf("device: %s (%d CUs, %.1f GiB)\n", devname, cus, mem / 1073741824.0);
+
+    qcd_lattice* lat = nullptr;
+    CHECK(qcd_lattice_create(&p, &lat));
+    const int start = st.ints == 1 ? QCD_START_COLD : (st.ints == 2 ? QCD_START_GID : QCD_START_HOT);
+    if (start == QCD_START_HOT) {   // the Haar start draws from the streams
+        CHECK(qcd_prng_init(lat));
+        CHECK(qcd_lattice_init(lat, start));
+    } else {
+        CHECK(qcd_lattice_init(lat, start));
+        if (start != QCD_START_GID) CHECK(qcd_prng_init(lat));
+    }
+    const int mask = QCD_MEAS_PLAQUETTE | (st.pl_level > 0 ? QCD_MEAS_POLYAKOV : 0);
+    auto sweep = [&]() {
+        CHECK(qcd_sweep(lat, QCD_HEATBATH));
+        for (int k = 0; k < st.nor; ++k) CHECK(qcd_sweep(lat, QCD_OVERRELAX));
+    };
+    const auto w0 = std::chrono::steady_clock::now();
+    CHECK(qcd_measure(lat, mask));   // measurement 0: the start configuration
+    for (int i = 0; i < st.nav; ++i) sweep();
+    for (int i = 1; i < st.iter; ++i) {
+        for (int j = 0; j < st.niter; ++j) sweep();
+        CHECK(qcd_measure(lat, mask));
+    }
+    CHECK(qcd_lattice_sync(lat));
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - w0).count();
+
+    std::vector<double> plq(2 * (size_t)st.iter), poly(4 * (size_t)st.iter);
+    int np = 0, nq = 0;
+    CHECK(qcd_read_measurements(lat, plq.data(), poly.data(), &np, &nq));
+    CHECK(qcd_lattice_destroy(lat));
+    CHECK(qcd_device_close());
+
+    const double V = (double)st.n[0] * st.n[1] * st.n[2] * st.n[3];
+    const double Vs = V / st.n[3], N = st.group, bet = st.beta / st.group;
+    Series s_plq_s{"Plq_spat"}, s_plq_t{"Plq_temp"}, s_plq{"Plq_total"};
+    Series s_s_s{"S_spat"}, s_s_t{"S_temp"}, s_s{"S_total"};
+    Series s_pl{"Polyakov_loop"}, s_pl_im{"Polyakov_loop_im"}, s_p2{"Polyakov_loop_P2"}, s_p4{"Polyakov_loop_P4"};
+    for (int i = 0; i < np; ++i) {
+        // lattice_measurement_plq / lattice_measurement: sums over 3V
+        // plaquettes; the action kernel's bet * (3N - sum ReTr) per site
+        // gives S = (beta/N) (N - Plq) per plane.
+        const double ps = plq[2 * i] / (3 * V), pt = plq[2 * i + 1] / (3 * V);
+        s_plq_s.v.push_back(ps);
+        s_plq_t.v.push_back(pt);
+        s_plq.v.push_back(0.5 * (ps + pt));
+        s_s_s.v.push_back(bet * (N - ps));
+        s_s_t.v.push_back(bet * (N - pt));
+        s_s.v.push_back(0.5 * (bet * (N - ps) + bet * (N - pt)));
+    }
+    for (int i = 0; i < nq; ++i) {
+        s_pl.v.push_back(poly[4 * i] / (Vs * N));
+        s_pl_im.v.push_back(poly[4 * i + 1] / (Vs * N));
+        s_p2.v.push_back(poly[4 * i + 2] / (Vs * N * N));
+        s_p4.v.push_back(poly[4 * i + 3] / (Vs * N * N * N * N));
+    }
+    std::vector<Series*> cols = {&s_plq_s, &s_plq_t, &s_plq, &s_s_s, &s_s_t, &s_s};
+    if (st.pl_level > 0) {
+        cols.push_back(&s_pl);
+        cols.push_back(&s_pl_im);
+    }
+    if (st.pl_level > 1) {
+        cols.push_back(&s_p2);
+        cols.push_back(&s_p4);
+    }
+    for (auto* c : cols) c->analyse();
+
+    // lattice_print_measurements: first five and the last measurement
+    printf("\n #        ");
+    for (auto* c : cols) printf(" %-11s", c->name);
+    printf("\n");
+    for (int i = 0; i < st.iter; ++i)
+        if (i < 5 || i == st.iter - 1) {
+            printf("[%5d]: ", i);
+            for (auto* c : cols) printf(" % 10.8f", c->v[i]);
+            printf("\n");
+        }
+    printf("\n %-18s %-22s %-22s\n", "quantity", "mean", "variance");
+    for (auto* c : cols) printf(" %-18s % 16.13e % 16.13e\n", c->name, c->mean, c->variance);
+    const double sweeps = (double)st.nav + (double)(st.iter - 1) * st.niter;
+    printf("\n sweeps %.0f (%d overrelaxation sweeps each), %.3f s, %.1f sweeps/s, %.3e link updates/s\n", sweeps,
+           st.nor, secs, sweeps / secs, sweeps * (1 + st.nor) * 4 * V / secs);
+
+    // lattice_write_results: <OUTPUTPATH><su3-|su2-><yy>-<Mon>-<dd>-<hh>-<mm>-<ss>.txt
+    const time_t t1 = time(nullptr);
+    char timeend[32];
+    strncpy(timeend, ctime(&t1), sizeof(timeend) - 1);
+    timeend[sizeof(timeend) - 1] = 0;
+    char fname[1024];
+    snprintf(fname, sizeof(fname), "%s%s%.2s-%.3s-%.2s-%.2s-%.2s-%.2s.txt", st.outputpath.c_str(),
+             st.group == 3 ? "su3-" : "su2-", timeend + 22, timeend + 4, timeend + 8, timeend + 11, timeend + 14,
+             timeend + 17);
+    FILE* out = fopen(fname, "w");
+    if (!out) {
+        fprintf(stderr, "cannot write %s: %s\n", fname, strerror(errno));
+        return 1;
+    }
+    fprintf(out, " SU(%d) lattice %dx%dx%dx%d, beta = %.13e\n", st.group, st.n[0], st.n[1], st.n[2], st.n[3], st.beta);
+    fprintf(out, " device                      : %s\n", devname);
+    fprintf(out, " precision                   : %s (%d reals per link)\n", st.double_precision ? "double" : "single",
+            p.storage_reals);
+    fprintf(out, " PRN generator               : %s (%s streams), random series %u\n", st.prng.c_str(),
+            st.fast_streams ? "fast" : "reference", st.randseries);
+    fprintf(out, " start                       : %s\n", st.ints == 1 ? "cold" : (st.ints == 2 ? "gid" : "hot (Haar)"));
+    fprintf(out, " NAV %d, ITER %d, NITER %d, NHIT %d, NOR %d\n", st.nav, st.iter, st.niter, st.nhit, st.nor);
+    fprintf(out, " started %s finished %s", timestart, timeend);
+    fprintf(out, " run time %.6f s\n", secs);
+    fprintf(out, " ***************************************************\n");
+    for (auto* c : cols) fprintf(out, " %-21s mean % 16.13e variance % 16.13e\n", c->name, c->mean, c->variance);
+    fprintf(out, " ***************************************************\n");
+    fprintf(out, " Data fields:\n    #,  ");
+    for (auto* c : cols) fprintf(out, "%-21s ", c->name);
+    fprintf(out, "\n ***************************************************\n");
+    for (int i = 0; i < st.iter; ++i) {
+        fprintf(out, "%5i", i);
+        for (auto* c : cols) fprintf(out, " % 16.13e    ", c->v[i]);
+        fprintf(out, "\n");
+    }
+    fclose(out);
+    printf(" results: %s\n", fname);
+    return 0;
+}

@@ -1,0 +1,98 @@
+"""The command-line host program (qcdgpu_amd/qcdgpu_hip): init.dat parsing
+and defaults on the CPU; a run through the C ABI on the GPU, checked against
+the Python driver on the same parameters."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(REPO, "qcdgpu_amd", "qcdgpu_hip")
+
+
+def _params(text, tmp_path):
+    f = tmp_path / "init.dat"
+    f.write_text(text)
+    r = subprocess.run([BIN, "--print-params", str(f)], capture_output=True, text=True)
+    kv = dict(line.split("=", 1) for line in r.stdout.splitlines() if "=" in line)
+    return r.returncode, kv
+
+
+def test_host_built():
+    assert os.access(BIN, os.X_OK), "build with __graft_entry__.build()"
+
+
+def test_defaults_are_the_reference_defaults():
+    """SU(N)/QCDGPU.cpp:98-198: SU(3) 32^4, beta 6.5, cold start, RANLUX3,
+    random series 54, NHIT 10, ITER 2, NITER 1, NAV 0, double precision."""
+    r = subprocess.run([BIN, "--print-params"], capture_output=True, text=True)
+    assert r.returncode == 0
+    kv = dict(line.split("=", 1) for line in r.stdout.splitlines())
+    assert kv["GROUP"] == "3" and [kv["L%d" % k] for k in range(1, 5)] == ["32"] * 4
+    assert float(kv["BETA"]) == 6.5 and kv["INTS"] == "1" and kv["PRNG"] == "RANLUX3"
+    assert kv["RANDSERIES"] == "54" and kv["NHIT"] == "10" and kv["ITER"] == "2"
+    assert kv["NITER"] == "1" and kv["NAV"] == "0" and kv["PRECISION"] == "DOUBLE"
+
+
+def test_reference_init_syntax(tmp_path):
+    """Tab-indented KEY = VALUE lines, exponent floats, '#' comments, LS/LT
+    then L1 (a later key overrides), and the INTS start codes (0 hot)."""
+    rc, kv = _params(
+        "\tGROUP = 2\n\tND = 4\n\tLT = 8\n\tLS = 12\n\tL1 = 16   # x\n\tITER = 50\n\tNITER = 10\n"
+        "\tBETA = 2.3000000000000e+00\n\tNAV = 5\n\tINTS = 0\n\tPRNG = RANLUX2\n\tRANDSERIES = 7\n"
+        "\tWILSONR = 10\n\tPRECISION = SINGLE\n\tNOR = 3\n",
+        tmp_path,
+    )
+    assert rc == 0
+    assert (kv["L1"], kv["L2"], kv["L3"], kv["L4"]) == ("16", "12", "12", "8")
+    assert kv["GROUP"] == "2" and float(kv["BETA"]) == 2.3 and kv["INTS"] == "0"
+    assert kv["PRNG"] == "RANLUX2" and kv["RANDSERIES"] == "7" and kv["ITER"] == "50"
+    assert kv["PRECISION"] == "SINGLE" and kv["NOR"] == "3" and kv["STORAGE"] == "4"
+
+
+@pytest.mark.parametrize(
+    "line", ["PHI = 3.0", "OMEGA = 0.5", "GETWILSON = 1", "FMUNU = 1", "PRNG = RANMAR", "ND = 3", "GROUP = 4"]
+)
+def test_unsupported_features_rejected(tmp_path, line):
+    rc, kv = _params(line + "\n", tmp_path)
+    assert rc == 2 and "UNSUPPORTED" in kv
+
+
+def test_bad_precision_value(tmp_path):
+    rc, _ = _params("PRECISION = HALF\n", tmp_path)
+    assert rc == 1
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prng,ints", [("XOR128", 1), ("RANLUX3", 0)])
+def test_host_run_matches_python_driver(tmp_path, prng, ints):
+    """qcdgpu_hip on a small lattice writes the same plaquette and Polyakov
+    series as qcdgpu_amd.Lattice.simulate with the same parameters."""
+    import qcdgpu_amd as Q
+
+    print("[case] host run %s ints=%d" % (prng, ints), flush=True)
+    f = tmp_path / "init.dat"
+    f.write_text(
+        "GROUP = 3\nLS = 8\nLT = 4\nITER = 4\nNITER = 2\nNAV = 1\nBETA = 5.9\nINTS = %d\nPRNG = %s\n"
+        "RANDSERIES = 11\nPRECISION = DOUBLE\nNOR = 2\nOUTPUTPATH = %s/\n" % (ints, prng, tmp_path)
+    )
+    r = subprocess.run([BIN, str(f)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    out = [p for p in os.listdir(tmp_path) if p.startswith("su3-") and p.endswith(".txt")]
+    assert len(out) == 1
+    rows = []
+    for line in open(tmp_path / out[0]):
+        parts = line.split()
+        if len(parts) == 11 and parts[0].isdigit():
+            rows.append([float(x) for x in parts[1:]])
+    rows = np.array(rows)
+    assert rows.shape == (4, 10)
+    lux = int(prng[-1]) if prng.startswith("RANLUX") else 3
+    name = "RANLUX" if prng.startswith("RANLUX") else prng
+    with Q.Lattice(sun=3, dims=(8, 8, 8, 4), beta=5.9, precision="fp64", prng=name, ranlux_luxury=lux,
+                   randseries=11, max_measurements=8) as lat:
+        m = lat.simulate(nav=1, iters=4, niter=2, n_or=2, start={0: "hot", 1: "cold"}[ints])
+    assert np.allclose(rows[:, 0], m["plq_spat"], rtol=1e-13, atol=0)
+    assert np.allclose(rows[:, 1], m["plq_temp"], rtol=1e-13, atol=0)
+    assert np.allclose(rows[:, 6], m["polyakov_re"], rtol=1e-12, atol=1e-15)
