@@ -66,7 +66,7 @@ enum { QCD_STREAM_REFERENCE = 0, QCD_STREAM_FAST = 1 };
 enum { QCD_HEATBATH = 0, QCD_OVERRELAX = 1 };
 
 /* Measurements (bit mask). */
-enum { QCD_MEAS_PLAQUETTE = 1, QCD_MEAS_POLYAKOV = 2 };
+enum { QCD_MEAS_PLAQUETTE = 1, QCD_MEAS_POLYAKOV = 2, QCD_MEAS_WILSON = 4 };
 
 typedef struct {
     int      sun;            /* 2 or 3 (GROUP)                                  */
@@ -148,6 +148,13 @@ int  qcd_reunitarize(qcd_lattice *lat);
 int  qcd_measure(qcd_lattice *lat, int mask);
 /* Copies the records so far (synchronous).  plq: 2*count, poly: 4*count.  */
 int  qcd_read_measurements(qcd_lattice *lat, double *plq, double *poly, int *count_plq, int *count_poly);
+/* ---- Wilson loop: lattice_measurement_wilson + reduce (wilson_loop.cl:55-465;
+        CPU twin su3cpu.cpp:1435-1522), R links along x, y and z, T along t
+        (wilson_R / wilson_T, QCDGPU.cpp:202-203, defaults 7 and 2).  Each
+        QCD_MEAS_WILSON measurement appends sum ReTr W over sites and the
+        three spatial directions (the reference divides by 3V).            */
+int  qcd_lattice_set_wilson(qcd_lattice *lat, int R, int T);
+int  qcd_read_wilson(qcd_lattice *lat, double *w, int *count);
 int  qcd_reset_measurements(qcd_lattice *lat);
 
 /* ---- graphs: one captured cycle of n_hb heatbath sweeps, n_or

@@ -833,6 +833,50 @@ void orc_plaquette(const orc_lattice *L, const double *links, double plq[2])
     plq[1] = tm;
 }
 
+/* R x T Wilson loop summed over sites and the three spatial directions d
+   (su3cpu.cpp:1435-1522, wilson_loop.cl:55-465): W = B R T^+ L^+ with
+   B = U_t(x) ... U_t(x+(T-1)t), R = U_d(x+Tt) ... U_d(x+Tt+(R-1)d),
+   T = U_t(x+Rd) ... U_t(x+Rd+(T-1)t), L = U_d(x) ... U_d(x+(R-1)d);
+   returns sum ReTr W (the reference divides by 3V).                        */
+static void line_product(const orc_lattice *L, const double *links, unsigned site, int dir, int len, double *P)
+{
+    int N = L->sun;
+    double T[18];
+    memcpy(P, CLINK(L, links, site, dir), sizeof(double) * 2 * N * N);
+    for (int k = 1; k < len; ++k) {
+        site = orc_neighbour(L, site, dir, +1);
+        mat_mul(N, P, CLINK(L, links, site, dir), T);
+        memcpy(P, T, sizeof(double) * 2 * N * N);
+    }
+}
+
+double orc_wilson(const orc_lattice *L, const double *links, int R, int T)
+{
+    int N = L->sun;
+    unsigned V = orc_sites(L);
+    double sum = 0.0;
+    double B[18], Rm[18], Tm[18], Lm[18], m1[18], m2[18], m3[18];
+    for (unsigned x = 0; x < V; ++x) {
+        line_product(L, links, x, 3, T, B);
+        unsigned xt = x;
+        for (int k = 0; k < T; ++k) xt = orc_neighbour(L, xt, 3, +1);
+        for (int d = 0; d < 3; ++d) {
+            unsigned xr = x;
+            for (int k = 0; k < R; ++k) xr = orc_neighbour(L, xr, d, +1);
+            line_product(L, links, xt, d, R, Rm);
+            line_product(L, links, xr, 3, T, Tm);
+            line_product(L, links, x, d, R, Lm);
+            mat_mul(N, B, Rm, m1);
+            mat_dagger(N, Tm, m2);
+            mat_mul(N, m1, m2, m3);
+            mat_dagger(N, Lm, m2);
+            mat_mul(N, m3, m2, m1);
+            sum += mat_retr(N, m1);
+        }
+    }
+    return sum;
+}
+
 /* Polyakov loop: P(x) = U_T(x,0) U_T(x,1) ... U_T(x,N4-1), traces in FP64
    (polyakov.cl:54-141; su3cpu.cpp:1383-1433).                               */
 void orc_polyakov(const orc_lattice *L, const double *links, double poly[4])

@@ -118,6 +118,9 @@ void     orc_reunitarize(const orc_lattice *L, double *links);
    plq[0] = sum of ReTr over the 3 spatial planes, plq[1] over the 3 temporal. */
 void     orc_plaquette(const orc_lattice *L, const double *links, double plq[2]);
 /* poly[0..3] = sum over spatial sites of ReTr P, ImTr P, |Tr P|^2, |Tr P|^4. */
+/* Sum over sites and spatial directions of ReTr of the R x T Wilson loop
+   (spatial extent R along x, y, z; temporal extent T along t).             */
+double   orc_wilson(const orc_lattice *L, const double *links, int R, int T);
 void     orc_polyakov(const orc_lattice *L, const double *links, double poly[4]);
 /* Sum over links of ReTr(U A) (A = staple), the local action density. */
 double   orc_link_action(const orc_lattice *L, const double *links, unsigned site, int mu);

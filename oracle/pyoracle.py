@@ -80,6 +80,7 @@ def lib():
         L.orc_neighbour.restype = ctypes.c_uint
         L.orc_parity_site.restype = ctypes.c_uint
         L.orc_link_action.restype = ctypes.c_double
+        L.orc_wilson.restype = ctypes.c_double
         _lib = L
     return _lib
 
@@ -227,6 +228,11 @@ def halfstep(L, links, parity, mu, mode, rng):
 
 def sweep(L, links, mode, rng):
     return lib().orc_sweep(ctypes.byref(L), _p(links, _F64P), mode, ctypes.byref(rng.rng))
+
+
+def wilson(L, links, R, T):
+    """Sum of ReTr W(R x T) over sites and the three spatial directions."""
+    return lib().orc_wilson(ctypes.byref(L), _p(np.ascontiguousarray(links), _F64P), R, T)
 
 
 def plaquette(L, links):

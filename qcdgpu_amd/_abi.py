@@ -18,7 +18,7 @@ START_HOT, START_COLD, START_GID = 0, 1, 2
 PRNG_XOR128, PRNG_RANLUX, PRNG_PM = 1, 2, 3
 STREAM_REFERENCE, STREAM_FAST = 0, 1
 HEATBATH, OVERRELAX = 0, 1
-MEAS_PLAQUETTE, MEAS_POLYAKOV = 1, 2
+MEAS_PLAQUETTE, MEAS_POLYAKOV, MEAS_WILSON = 1, 2, 4
 TUNE_XCD_MAP, TUNE_BLOCKS_PER_CU, TUNE_UPDATE_SHAPE = 1, 2, 3
 
 
@@ -68,6 +68,8 @@ _SIGNATURES = {
     "qcd_measure": [_P, ctypes.c_int],
     "qcd_read_measurements": [_P, _D, _D, _I, _I],
     "qcd_reset_measurements": [_P],
+    "qcd_lattice_set_wilson": [_P, ctypes.c_int, ctypes.c_int],
+    "qcd_read_wilson": [_P, _D, _I],
     "qcd_cycle_capture": [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int],
     "qcd_cycle_run": [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int],
     "qcd_cycle_launch": [_P, ctypes.c_int],

@@ -924,9 +924,13 @@ struct FinArgs {
     // Polyakov loop: n_poly slots of 4
     const double* poly;
     uint32_t n_poly;
+    // Wilson loop: n_wil slots of 1
+    const double* wil;
+    uint32_t n_wil;
     double* plq_rec;
     double* poly_rec;
-    int* rec_index;   // [0] plaquette, [1] polyakov
+    double* wil_rec;
+    int* rec_index;   // [0] plaquette, [1] polyakov, [2] Wilson loop
     int capacity;
 };
 
@@ -986,6 +990,16 @@ __global__ __launch_bounds__(kBlock) void k_finalize(FinArgs f)
             f.rec_index[1] = rec + 1;
         }
     }
+    if (f.wil) {
+        double w = 0.0;
+        for (uint32_t j = t; j < f.n_wil; j += kBlock) w += f.wil[j];
+        w = block_sum_1(w);
+        if (t == 0) {
+            const int rec = f.rec_index[2];
+            if (rec < f.capacity) f.wil_rec[rec] = w;
+            f.rec_index[2] = rec + 1;
+        }
+    }
 }
 
 // Plaquette: per site sum of ReTr U_p over the spatial planes (x,y), (x,z),
@@ -1037,6 +1051,54 @@ __device__ __forceinline__ M<R, N> shfl_down_m(const M<R, N>& a, int d)
             b.e[i][j].im = __shfl_down(a.e[i][j].im, d, 64);
         }
     return b;
+}
+
+// R x T Wilson loop (wilson_loop.cl:55-465; su3cpu.cpp:1435-1522), one
+// thread per site x, summed over the three spatial directions d:
+// ReTr(B Rm Tm^+ Lm^+) with B = U_t(x) .. U_t(x+(T-1)t),
+// Rm = U_d(x+Tt) .. U_d(x+Tt+(R-1)d), Tm = U_t(x+Rd) .. U_t(x+Rd+(T-1)t),
+// Lm = U_d(x) .. U_d(x+(R-1)d).  Products in link precision, traces and
+// sums in FP64; one partial per workgroup.
+template <typename R, int N, int NREAL>
+__device__ __forceinline__ M<R, N> line_product(const Table<R, N, NREAL>& T, const Geom& g, Site s, int dir, int len)
+{
+    M<R, N> P = T.load((s.x + s.y + s.z + s.t) & 1, dir, cindex(g, s.x, s.y, s.z, s.t));
+    for (int k = 1; k < len; ++k) {
+        shift(g, s, dir, +1);
+        P = m_mul(P, T.load((s.x + s.y + s.z + s.t) & 1, dir, cindex(g, s.x, s.y, s.z, s.t)));
+    }
+    return P;
+}
+
+template <typename R, int N, int NREAL>
+__global__ __launch_bounds__(kBlock) void k_wilson(void* table, Geom g, int wr, int wt, double* partials)
+{
+    const uint32_t site = blockIdx.x * kBlock + threadIdx.x;
+    double acc = 0.0;
+    if (site < 2u * g.vh) {
+        Table<R, N, NREAL> T{(typename Vec2<R>::T*)table, g.vh_pad};
+        Site x;
+        uint32_t r = site;
+        x.y = (int)(r % (uint32_t)g.n2);
+        r /= (uint32_t)g.n2;
+        x.z = (int)(r % (uint32_t)g.n3);
+        r /= (uint32_t)g.n3;
+        x.t = (int)(r % (uint32_t)g.n4);
+        x.x = (int)(r / (uint32_t)g.n4);
+        const M<R, N> B = line_product(T, g, x, 3, wt);
+        Site xt = x;
+        for (int k = 0; k < wt; ++k) shift(g, xt, 3, +1);
+        for (int d = 0; d < 3; ++d) {
+            Site xr = x;
+            for (int k = 0; k < wr; ++k) shift(g, xr, d, +1);
+            const M<R, N> Rm = line_product(T, g, xt, d, wr);
+            const M<R, N> Tm = line_product(T, g, xr, 3, wt);
+            const M<R, N> Lm = line_product(T, g, x, d, wr);
+            acc += m_retr_nd(m_mul_nd(m_mul(B, Rm), Tm), Lm);
+        }
+    }
+    double v[1] = {acc};
+    block_partial<1>(v, partials);
 }
 
 template <typename R, int N, int NREAL>
@@ -1113,6 +1175,10 @@ struct qcd_lattice {
     int* rec_index = nullptr;       // [0] plaquette, [1] polyakov
     double* plq_rec = nullptr;
     double* poly_rec = nullptr;
+    double* wil_rec = nullptr;          // Wilson loop records
+    double* part_wil = nullptr;         //   wil_blocks (k_wilson)
+    uint32_t wil_blocks = 0;
+    int wil_R = 7, wil_T = 2;           // wilson_R, wilson_T defaults of QCDGPU.cpp
     uint32_t upd_blocks = 0, poly_blocks = 0;
     hipStream_t stream = nullptr;
     hipGraph_t graph = nullptr;
@@ -1285,6 +1351,10 @@ int launch_simple(qcd_lattice* L, int which, const LaunchOpt& o)
             launch(k_polyakov<R, N, NREAL>, dim3(L->poly_blocks), dim3(kBlock), L->stream, o, L->table, L->g,
                    L->part_poly);
             break;
+        case 6:
+            hipLaunchKernelGGL((k_wilson<R, N, NREAL>), dim3(L->wil_blocks), dim3(kBlock), 0, L->stream, L->table,
+                               L->g, L->wil_R, L->wil_T, L->part_wil);
+            break;
         case 5:
             if (L->p.prng == QCD_PRNG_XOR128) {
                 hipLaunchKernelGGL((k_init_hot<R, N, NREAL, RK_XOR128>), dim3(nb), dim3(kBlock), 0, L->stream, L->table,
@@ -1318,8 +1388,11 @@ int launch_finalize(qcd_lattice* L, int flags, const LaunchOpt& o)
     f.n_plq = 2u * L->upd_blocks;
     f.poly = (flags & 4) ? L->part_poly : nullptr;
     f.n_poly = L->poly_blocks;
+    f.wil = (flags & 8) ? L->part_wil : nullptr;
+    f.n_wil = L->wil_blocks;
     f.plq_rec = L->plq_rec;
     f.poly_rec = L->poly_rec;
+    f.wil_rec = L->wil_rec;
     f.rec_index = L->rec_index;
     f.capacity = L->p.max_measurements;
     launch(k_finalize, dim3(1), dim3(kBlock), L->stream, o, f);
@@ -1428,7 +1501,9 @@ int qcd_lattice_create(const qcd_params* pp, qcd_lattice** out)
         return fail(QCD_E_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
     }
     L->nb_fused = align_up(L->g.vh, kLinks) / kLinks;
-    size_t part_doubles = (size_t)3 * L->nb_fused + (size_t)2 * 2 * L->upd_blocks + (size_t)4 * L->poly_blocks;
+    L->wil_blocks = align_up(2u * L->g.vh, kBlock) / kBlock;
+    size_t part_doubles = (size_t)3 * L->nb_fused + (size_t)2 * 2 * L->upd_blocks + (size_t)4 * L->poly_blocks +
+                          (size_t)L->wil_blocks;
     if (hipMalloc(&L->table, L->table_bytes) != hipSuccess ||
         hipMalloc((void**)&L->rng_buf[0], sizeof(uint4) * L->n_streams) != hipSuccess ||
         hipMalloc((void**)&L->rng_buf[1], sizeof(uint4) * L->n_streams) != hipSuccess ||
@@ -1436,7 +1511,8 @@ int qcd_lattice_create(const qcd_params* pp, qcd_lattice** out)
         hipMalloc((void**)&L->partials, sizeof(double) * part_doubles) != hipSuccess ||
         hipMalloc((void**)&L->rec_index, sizeof(int) * 4) != hipSuccess ||
         hipMalloc((void**)&L->plq_rec, sizeof(double) * 2 * p.max_measurements) != hipSuccess ||
-        hipMalloc((void**)&L->poly_rec, sizeof(double) * 4 * p.max_measurements) != hipSuccess) {
+        hipMalloc((void**)&L->poly_rec, sizeof(double) * 4 * p.max_measurements) != hipSuccess ||
+        hipMalloc((void**)&L->wil_rec, sizeof(double) * p.max_measurements) != hipSuccess) {
         qcd_lattice_destroy(L);
         return fail(QCD_E_NOMEM, "device allocation failed (table %zu bytes)", L->table_bytes);
     }
@@ -1453,6 +1529,7 @@ int qcd_lattice_create(const qcd_params* pp, qcd_lattice** out)
     L->part_fused = L->partials;
     L->part_plq = L->part_fused + (size_t)3 * L->nb_fused;
     L->part_poly = L->part_plq + (size_t)2 * 2 * L->upd_blocks;
+    L->part_wil = L->part_poly + (size_t)4 * L->poly_blocks;
     (void)hipMemset(L->rec_index, 0, sizeof(int) * 4);
     (void)hipMemset(L->table, 0, L->table_bytes);
     {
@@ -1500,6 +1577,7 @@ int qcd_lattice_destroy(qcd_lattice* L)
     (void)hipFree(L->rec_index);
     (void)hipFree(L->plq_rec);
     (void)hipFree(L->poly_rec);
+    (void)hipFree(L->wil_rec);
     if (L->stream) (void)hipStreamDestroy(L->stream);
     delete L;
     return QCD_OK;
@@ -1767,6 +1845,11 @@ static int measure_impl(qcd_lattice* L, int mask, bool fused_plq, const LaunchOp
         flags |= 4;
     }
     if (rc) return rc;
+    if (mask & QCD_MEAS_WILSON) {
+        rc = DISPATCH(L, launch_simple, L, 6, LaunchOpt{});
+        flags |= 8;
+    }
+    if (rc) return rc;
     return flags ? launch_finalize(L, flags, o_fin) : QCD_OK;
 }
 
@@ -1788,6 +1871,27 @@ int qcd_read_measurements(qcd_lattice* L, double* plq, double* poly, int* count_
     if (poly && nq) HIPCHK(hipMemcpy(poly, L->poly_rec, sizeof(double) * 4 * nq, hipMemcpyDeviceToHost));
     if (count_plq) *count_plq = idx[0];
     if (count_poly) *count_poly = idx[1];
+    return QCD_OK;
+}
+
+int qcd_read_wilson(qcd_lattice* L, double* w, int* count)
+{
+    if (!L) return fail(QCD_E_ARG, "null lattice");
+    int idx[4];
+    HIPCHK(hipMemcpyAsync(idx, L->rec_index, sizeof(idx), hipMemcpyDeviceToHost, L->stream));
+    HIPCHK(hipStreamSynchronize(L->stream));
+    const int nw = idx[2] < L->p.max_measurements ? idx[2] : L->p.max_measurements;
+    if (w && nw) HIPCHK(hipMemcpy(w, L->wil_rec, sizeof(double) * nw, hipMemcpyDeviceToHost));
+    if (count) *count = idx[2];
+    return QCD_OK;
+}
+
+int qcd_lattice_set_wilson(qcd_lattice* L, int R, int T)
+{
+    if (!L) return fail(QCD_E_ARG, "null lattice");
+    if (R < 1 || T < 1) return fail(QCD_E_ARG, "Wilson loop extents must be >= 1 (got %d x %d)", R, T);
+    L->wil_R = R;
+    L->wil_T = T;
     return QCD_OK;
 }
 

@@ -23,8 +23,9 @@
 // sweeps after every heatbath sweep, default 0), STORAGE (18 or 12 reals per
 // SU(3) link), STREAMS (REFERENCE or FAST generator layout), LUXURY is
 // carried in the PRNG name (RANLUX0..RANLUX4) as in the reference.
-// Keys of features outside this path (Wilson loops, F_munu, twisted boundary
-// conditions, BIGLAT) are rejected when they ask for the feature.
+// GETWILSON / WILSONR / WILSONT measure the R x T Wilson loop.  Keys of
+// features outside this path (F_munu, twisted boundary conditions, BIGLAT)
+// are rejected when they ask for the feature.
 
 #include <cerrno>
 #include <chrono>
@@ -51,6 +52,7 @@ struct Settings {
     unsigned randseries = 54;
     int pl_level = 3;
     bool wilson = false, fmunu = false;
+    int wilson_r = 7, wilson_t = 2;  // QCDGPU.cpp:202-203
     std::string outputpath = "./", finishpath = "./";
     bool double_precision = true;    // QCDGPU.cpp: model_precision_double
     int nor = 0, storage = 0;        // storage 0: 18 reals (SU(3)) / 4 (SU(2))
@@ -101,6 +103,8 @@ bool apply(Settings& st, const std::string& key, const std::string& val)
     else if (key == "OUTPUTPATH") st.outputpath = val;
     else if (key == "FINISHPATH") st.finishpath = val;
     else if (key == "GETWILSON") st.wilson = true;
+    else if (key == "WILSONR") st.wilson_r = (int)iv;
+    else if (key == "WILSONT") st.wilson_t = (int)iv;
     else if (key == "FMUNU" || key == "F0MU") st.fmunu = true;
     else if (key == "TURNOFFFMUNU") st.fmunu = false;
     else if (key == "PRECISION") {
@@ -122,7 +126,7 @@ bool apply(Settings& st, const std::string& key, const std::string& val)
             return false;
         }
     }
-    // other keys (WILSONR, WILSONT, Fst, N1, TURNOFFWAITING, REBUILDBINARY,
+    // other keys (Fst, N1, TURNOFFWAITING, REBUILDBINARY,
     // GETRETRACE ...) are accepted and have no effect, as in the reference
     return true;
 }
@@ -156,7 +160,7 @@ bool to_params(const Settings& st, qcd_params& p, std::string& why)
     if (st.nd != 4) return why = "ND must be 4", false;
     if (st.group != 2 && st.group != 3) return why = "GROUP must be 2 or 3", false;
     if (st.phi != 0.0 || st.omega != 0.0) return why = "PHI/OMEGA (twisted boundary conditions) are not supported", false;
-    if (st.wilson) return why = "GETWILSON (Wilson loops) is not supported", false;
+    if (st.wilson && (st.wilson_r < 1 || st.wilson_t < 1)) return why = "WILSONR and WILSONT must be >= 1", false;
     if (st.fmunu) return why = "FMUNU/F0MU (field-strength measurements) are not supported", false;
     if (st.iter < 1 || st.niter < 0 || st.nav < 0 || st.nor < 0) return why = "ITER >= 1, NITER, NAV, NOR >= 0", false;
     memset(&p, 0, sizeof(p));
@@ -189,6 +193,7 @@ void print_params(const Settings& st)
     printf("ITER=%d\nNITER=%d\nNAV=%d\nNHIT=%d\nNHITPAR=%d\n", st.iter, st.niter, st.nav, st.nhit, st.nhitpar);
     printf("BETA=%.13e\nPHI=%.13e\nOMEGA=%.13e\nINTS=%d\n", st.beta, st.phi, st.omega, st.ints);
     printf("PRNG=%s\nRANDSERIES=%u\nPL_LEVEL=%d\n", st.prng.c_str(), st.randseries, st.pl_level);
+    printf("GETWILSON=%d\nWILSONR=%d\nWILSONT=%d\n", st.wilson ? 1 : 0, st.wilson_r, st.wilson_t);
     printf("PRECISION=%s\nNOR=%d\nSTORAGE=%d\nSTREAMS=%s\n", st.double_precision ? "DOUBLE" : "SINGLE", st.nor,
            st.storage ? st.storage : (st.group == 3 ? 18 : 4), st.fast_streams ? "FAST" : "REFERENCE");
     printf("OUTPUTPATH=%s\nFINISHPATH=%s\nPLATFORM=%d\nDEVICE=%d\n", st.outputpath.c_str(), st.finishpath.c_str(),
@@ -265,7 +270,8 @@ int main(int argc, char** argv)
         CHECK(qcd_lattice_init(lat, start));
         if (start != QCD_START_GID) CHECK(qcd_prng_init(lat));
     }
-    const int mask = QCD_MEAS_PLAQUETTE | (st.pl_level > 0 ? QCD_MEAS_POLYAKOV : 0);
+    const int mask = QCD_MEAS_PLAQUETTE | (st.pl_level > 0 ? QCD_MEAS_POLYAKOV : 0) | (st.wilson ? QCD_MEAS_WILSON : 0);
+    if (st.wilson) CHECK(qcd_lattice_set_wilson(lat, st.wilson_r, st.wilson_t));
     auto sweep = [&]() {
         CHECK(qcd_sweep(lat, QCD_HEATBATH));
         for (int k = 0; k < st.nor; ++k) CHECK(qcd_sweep(lat, QCD_OVERRELAX));
@@ -280,9 +286,10 @@ int main(int argc, char** argv)
     CHECK(qcd_lattice_sync(lat));
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - w0).count();
 
-    std::vector<double> plq(2 * (size_t)st.iter), poly(4 * (size_t)st.iter);
-    int np = 0, nq = 0;
+    std::vector<double> plq(2 * (size_t)st.iter), poly(4 * (size_t)st.iter), wil((size_t)st.iter);
+    int np = 0, nq = 0, nw = 0;
     CHECK(qcd_read_measurements(lat, plq.data(), poly.data(), &np, &nq));
+    if (st.wilson) CHECK(qcd_read_wilson(lat, wil.data(), &nw));
     CHECK(qcd_lattice_destroy(lat));
     CHECK(qcd_device_close());
 
@@ -290,6 +297,8 @@ int main(int argc, char** argv)
     const double Vs = V / st.n[3], N = st.group, bet = st.beta / st.group;
     Series s_plq_s{"Plq_spat"}, s_plq_t{"Plq_temp"}, s_plq{"Plq_total"};
     Series s_s_s{"S_spat"}, s_s_t{"S_temp"}, s_s{"S_total"};
+    Series s_wil{"Wilson_loop"};
+    for (int i = 0; i < nw; ++i) s_wil.v.push_back(wil[i] / (3 * V));   // suncl.cpp:1606-1611
     Series s_pl{"Polyakov_loop"}, s_pl_im{"Polyakov_loop_im"}, s_p2{"Polyakov_loop_P2"}, s_p4{"Polyakov_loop_P4"};
     for (int i = 0; i < np; ++i) {
         // lattice_measurement_plq / lattice_measurement: sums over 3V
@@ -309,7 +318,11 @@ int main(int argc, char** argv)
         s_p2.v.push_back(poly[4 * i + 2] / (Vs * N * N));
         s_p4.v.push_back(poly[4 * i + 3] / (Vs * N * N * N * N));
     }
-    std::vector<Series*> cols = {&s_plq_s, &s_plq_t, &s_plq, &s_s_s, &s_s_t, &s_s};
+    std::vector<Series*> cols = {&s_plq_s, &s_plq_t, &s_plq};
+    if (st.wilson) cols.push_back(&s_wil);
+    cols.push_back(&s_s_s);
+    cols.push_back(&s_s_t);
+    cols.push_back(&s_s);
     if (st.pl_level > 0) {
         cols.push_back(&s_pl);
         cols.push_back(&s_pl_im);

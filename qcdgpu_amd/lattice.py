@@ -108,9 +108,29 @@ class Lattice:
         check(lib().qcd_lattice_sync(self._h))
 
     # -- measurements ---------------------------------------------------------
-    def measure(self, plaquette=True, polyakov=True):
-        mask = (_abi.MEAS_PLAQUETTE if plaquette else 0) | (_abi.MEAS_POLYAKOV if polyakov else 0)
+    def measure(self, plaquette=True, polyakov=True, wilson=False):
+        mask = (
+            (_abi.MEAS_PLAQUETTE if plaquette else 0)
+            | (_abi.MEAS_POLYAKOV if polyakov else 0)
+            | (_abi.MEAS_WILSON if wilson else 0)
+        )
         check(lib().qcd_measure(self._h, mask))
+
+    def set_wilson(self, R, T):
+        """Extents of the Wilson loop measured with measure(wilson=True):
+        R along x, y, z and T along t (reference defaults 7 and 2)."""
+        check(lib().qcd_lattice_set_wilson(self._h, int(R), int(T)))
+
+    def wilson_measurements(self, raw=False):
+        """Wilson loop records: sums of ReTr W over sites and spatial
+        directions, or (default) divided by 3V as the reference's analysis
+        (suncl.cpp:1606-1611)."""
+        cap = self.params.max_measurements
+        w = np.zeros(cap)
+        n = ctypes.c_int()
+        check(lib().qcd_read_wilson(self._h, w.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.byref(n)))
+        w = w[: min(n.value, cap)]
+        return w if raw else w / (3.0 * self.volume)
 
     def reset_measurements(self):
         check(lib().qcd_reset_measurements(self._h))
@@ -193,10 +213,11 @@ class Lattice:
         return b.value
 
     # -- the reference run loop -------------------------------------------------
-    def simulate(self, nav=0, iters=2, niter=1, n_or=0, start="cold"):
+    def simulate(self, nav=0, iters=2, niter=1, n_or=0, start="cold", wilson=False):
         """model::lattice_simulate: start, measure, NAV sweeps, then ITER-1
         measurements each after NITER sweeps (suncl.cpp:5236-5483).  Each sweep
-        is one heatbath sweep plus `n_or` overrelaxation sweeps."""
+        is one heatbath sweep plus `n_or` overrelaxation sweeps; wilson=True
+        adds the Wilson loop (set_wilson) to every measurement."""
         if start == "hot":
             self.prng_init()   # the Haar start draws from the streams
             self.init(start)
@@ -205,7 +226,7 @@ class Lattice:
             if start != "gid":
                 self.prng_init()
         self.reset_measurements()
-        self.measure()
+        self.measure(wilson=wilson)
         for _ in range(nav):
             self.sweep("hb")
             for _ in range(n_or):
@@ -215,7 +236,7 @@ class Lattice:
                 self.sweep("hb")
                 for _ in range(n_or):
                     self.sweep("or")
-            self.measure()
+            self.measure(wilson=wilson)
         self.sync()
         return self.measurements()
 

@@ -475,3 +475,29 @@ def test_c0_su2_ensemble_vs_oracle(O):
     assert abs(z) < 3.0
     # Polyakov loop of a confined SU(2) ensemble at this beta averages to ~0
     assert abs(np.mean(m["polyakov_re"][50:])) < 0.1
+
+
+@pytest.mark.parametrize("sun,prec", [(3, "fp64"), (3, "fp32"), (2, "fp64")])
+@pytest.mark.parametrize("R,T", [(1, 1), (2, 3), (3, 2)])
+def test_wilson_loop_vs_oracle(O, sun, prec, R, T):
+    """R x T Wilson loop on the device vs the oracle on the exported links of
+    a few heatbath sweeps; W(1,1) equals the temporal plaquette sum."""
+    _case("wilson sun=%d %s %dx%d" % (sun, prec, R, T))
+    dims = (6, 4, 4, 6)
+    beta = 5.9 if sun == 3 else 2.3
+    with _lat(sun, dims, prec, beta=beta, seed=23) as lat:
+        lat.init("cold")
+        lat.prng_init()
+        for _ in range(3):
+            lat.sweep("hb")
+        lat.set_wilson(R, T)
+        lat.measure(wilson=True)
+        links = lat.export_links()
+        w = lat.wilson_measurements(raw=True)
+        plq, _ = lat.raw_measurements()
+    L = O.lattice(sun, dims, beta, 10)
+    ref = O.wilson(L, links, R, T)
+    rtol = 1e-12 if prec == "fp64" else 2e-6
+    assert len(w) == 1 and abs(w[0] - ref) <= rtol * abs(ref), (w, ref)
+    if (R, T) == (1, 1):
+        assert abs(w[0] - plq[-1][1]) <= rtol * abs(plq[-1][1])

@@ -52,7 +52,8 @@ def test_reference_init_syntax(tmp_path):
 
 
 @pytest.mark.parametrize(
-    "line", ["PHI = 3.0", "OMEGA = 0.5", "GETWILSON = 1", "FMUNU = 1", "PRNG = RANMAR", "ND = 3", "GROUP = 4"]
+    "line", ["PHI = 3.0", "OMEGA = 0.5", "FMUNU = 1", "PRNG = RANMAR", "ND = 3", "GROUP = 4",
+             "GETWILSON = 1\nWILSONR = 0"]
 )
 def test_unsupported_features_rejected(tmp_path, line):
     rc, kv = _params(line + "\n", tmp_path)
@@ -75,7 +76,8 @@ def test_host_run_matches_python_driver(tmp_path, prng, ints):
     f = tmp_path / "init.dat"
     f.write_text(
         "GROUP = 3\nLS = 8\nLT = 4\nITER = 4\nNITER = 2\nNAV = 1\nBETA = 5.9\nINTS = %d\nPRNG = %s\n"
-        "RANDSERIES = 11\nPRECISION = DOUBLE\nNOR = 2\nOUTPUTPATH = %s/\n" % (ints, prng, tmp_path)
+        "RANDSERIES = 11\nPRECISION = DOUBLE\nNOR = 2\nGETWILSON = 1\nWILSONR = 2\nWILSONT = 2\n"
+        "OUTPUTPATH = %s/\n" % (ints, prng, tmp_path)
     )
     r = subprocess.run([BIN, str(f)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
@@ -84,15 +86,18 @@ def test_host_run_matches_python_driver(tmp_path, prng, ints):
     rows = []
     for line in open(tmp_path / out[0]):
         parts = line.split()
-        if len(parts) == 11 and parts[0].isdigit():
+        if len(parts) == 12 and parts[0].isdigit():
             rows.append([float(x) for x in parts[1:]])
     rows = np.array(rows)
-    assert rows.shape == (4, 10)
+    assert rows.shape == (4, 11)   # Plq x3, Wilson_loop, S x3, Polyakov x4
     lux = int(prng[-1]) if prng.startswith("RANLUX") else 3
     name = "RANLUX" if prng.startswith("RANLUX") else prng
     with Q.Lattice(sun=3, dims=(8, 8, 8, 4), beta=5.9, precision="fp64", prng=name, ranlux_luxury=lux,
                    randseries=11, max_measurements=8) as lat:
-        m = lat.simulate(nav=1, iters=4, niter=2, n_or=2, start={0: "hot", 1: "cold"}[ints])
+        lat.set_wilson(2, 2)
+        m = lat.simulate(nav=1, iters=4, niter=2, n_or=2, start={0: "hot", 1: "cold"}[ints], wilson=True)
+        wl = lat.wilson_measurements()
     assert np.allclose(rows[:, 0], m["plq_spat"], rtol=1e-13, atol=0)
     assert np.allclose(rows[:, 1], m["plq_temp"], rtol=1e-13, atol=0)
-    assert np.allclose(rows[:, 6], m["polyakov_re"], rtol=1e-12, atol=1e-15)
+    assert np.allclose(rows[:, 3], wl, rtol=1e-12, atol=1e-15)
+    assert np.allclose(rows[:, 7], m["polyakov_re"], rtol=1e-12, atol=1e-15)

@@ -346,3 +346,50 @@ def test_hot_start_box_muller_edge(O):
     q = np.ones((8, n, 4), dtype=np.float32)
     a = O.hot(L, q, n)
     assert np.isfinite(a).all()
+
+
+# ---------------------------------------------------------------- Wilson loop
+@pytest.mark.parametrize("sun", [3, 2])
+def test_wilson_loop_pins(O, sun):
+    """W(1,1) summed over sites and spatial directions is the temporal
+    plaquette sum (each term is a temporal plaquette traversed the other way,
+    same real trace); a cold lattice gives 3 V N for any R, T; gauge
+    invariance under random g(x); W(R,T) by explicit matrix products."""
+    dims = (4, 4, 4, 4)
+    L = O.lattice(sun, dims, 5.9, 10)
+    a = O.gid(L)
+    V = int(np.prod(dims))
+    assert np.isclose(O.wilson(L, a, 1, 1), O.plaquette(L, a)[1], rtol=1e-12)
+    c = O.cold(L)
+    for R, T in [(1, 1), (2, 3), (3, 2)]:
+        assert np.isclose(O.wilson(L, c, R, T), 3 * V * sun, rtol=1e-14)
+    # gauge invariance
+    rng = np.random.default_rng(3)
+    M = (a[..., 0] + 1j * a[..., 1]).reshape(V, 4, sun, sun)
+    g = np.linalg.qr(rng.normal(size=(V, sun, sun)) + 1j * rng.normal(size=(V, sun, sun)))[0]
+    Mg = M.copy()
+    for s in range(V):
+        for d in range(4):
+            Mg[s, d] = g[s] @ M[s, d] @ g[O.neighbour(L, s, d, +1)].conj().T
+    ag = np.stack([Mg.real, Mg.imag], axis=-1).reshape(a.shape)
+    assert np.isclose(O.wilson(L, ag, 2, 2), O.wilson(L, a, 2, 2), rtol=1e-11)
+    # explicit loop at one site and direction, summed by brute force
+    def U(site, d):
+        return M[site, d]
+
+    def step(site, d, k=1):
+        for _ in range(k):
+            site = O.neighbour(L, site, d, +1)
+        return site
+
+    R, T = 2, 2
+    tot = 0.0
+    for x in range(V):
+        B = U(x, 3) @ U(step(x, 3), 3)
+        for d in range(3):
+            xt, xr = step(x, 3, T), step(x, d, R)
+            Rm = U(xt, d) @ U(step(xt, d), d)
+            Tm = U(xr, 3) @ U(step(xr, 3), 3)
+            Lm = U(x, d) @ U(step(x, d), d)
+            tot += np.trace(B @ Rm @ Tm.conj().T @ Lm.conj().T).real
+    assert np.isclose(O.wilson(L, a, 2, 2), tot, rtol=1e-12)

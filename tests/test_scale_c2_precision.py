@@ -1,0 +1,69 @@
+"""C2: one large lattice, SU(3) 64^4, beta 6.0, cold start, heatbath +
+overrelaxation as the headline workload, run in FP32 and in FP64 link storage
+from the identical start and the identical reference-layout XOR128 streams.
+Reports the per-link deviation of the FP32 run from the FP64 run after 1 and
+10 sweeps (max, median, 99th percentile of max |dU| over a link's entries)
+and the number of links whose Kennedy-Pendleton accept decision flipped
+(|dU| = O(1)), plus plaquette and Polyakov loop of both runs.  This is a
+report: the assertions only bound what rounding alone must satisfy."""
+import numpy as np
+import pytest
+
+import qcdgpu_amd as Q
+
+DIMS = (64, 64, 64, 64)
+
+
+def _lattice(prec):
+    lat = Q.Lattice(sun=3, dims=DIMS, beta=6.0, precision=prec, prng="XOR128", randseries=54, max_measurements=4)
+    lat.init("cold")
+    lat.prng_init()
+    return lat
+
+
+def _deviation(a, b, chunk=1 << 20):
+    """Per-link max |dU| over the 18 reals, in chunks of sites (each export is
+    9.7 GB at 64^4)."""
+    a = a.reshape(-1, 4 * 18)
+    b = b.reshape(-1, 4 * 18)
+    out = np.empty(a.shape[0] * 4)
+    for s0 in range(0, a.shape[0], chunk):
+        d = np.abs(a[s0:s0 + chunk] - b[s0:s0 + chunk]).reshape(-1, 18).max(axis=1)
+        out[4 * s0:4 * s0 + d.size] = d
+    return out
+
+
+@pytest.mark.gpu
+@pytest.mark.timeout(300)
+def test_c2_fp32_vs_fp64_deviation():
+    print("[case] C2 64^4 FP32 vs FP64", flush=True)
+    lat32, lat64 = _lattice("fp32"), _lattice("fp64")
+    done = 0
+    try:
+        for k, target in enumerate((1, 10)):
+            while done < target:
+                for lat in (lat32, lat64):
+                    lat.sweep("hb")
+                    for _ in range(4):
+                        lat.sweep("or")
+                done += 1
+            for lat in (lat32, lat64):
+                lat.measure()
+            d = _deviation(lat32.export_links(), lat64.export_links())
+            flips = int((d > 1e-3).sum())
+            q50, q99 = np.percentile(d, [50, 99])
+            m32, m64 = lat32.measurements(), lat64.measurements()
+            print("  after %2d sweep(s): max %.3e  median %.3e  p99 %.3e  flipped links %d of %d"
+                  % (target, d.max(), q50, q99, flips, d.size))
+            print("    plaquette fp32 %.10f fp64 %.10f | Polyakov fp32 %+.3e fp64 %+.3e"
+                  % (m32["plq_total"][k] / 3, m64["plq_total"][k] / 3, m32["polyakov_re"][k], m64["polyakov_re"][k]),
+                  flush=True)
+            if target == 1:
+                # one sweep: FP32 rounding of O(1e-7) per link, flips are rare
+                assert q50 < 1e-5 and q99 < 1e-4
+                assert flips < 1e-4 * d.size
+            assert abs(m32["plq_total"][k] - m64["plq_total"][k]) < 1e-3
+            del d
+    finally:
+        lat32.close()
+        lat64.close()
