@@ -145,6 +145,10 @@ def main():
     prof = lat.profile_cycle(max(1, min(args.profile_steps, args.steps)))
     bytes_per_launch = 9.0 * args.storage * 4 * (V // 2)
     achieved = bytes_per_launch / (prof["or_us"] * 1e-6) / 1e9
+    # heatbath half-step: the same links plus a 16-byte stream state read and
+    # written per updated link
+    hb_bytes = bytes_per_launch + 2.0 * 16 * (V // 2)
+    hb_achieved = hb_bytes / (prof["hb_us"] * 1e-6) / 1e9
     traffic = None
     if os.path.exists(TRAFFIC_FILE) and args.storage == 18:
         try:
@@ -179,6 +183,12 @@ def main():
             "updates_per_step": int(5 * 4 * V),
             "kernel_us": {"heatbath_halfstep": prof["hb_us"], "overrelax_halfstep": prof["or_us"], "measure": prof["meas_us"]},
             "sweeps_per_s": args.steps / (ms * 1e-3),
+            "heatbath_roofline": {
+                "achieved": hb_achieved,
+                "unit": "GB/s",
+                "frac": hb_achieved / HBM_PEAK_GBS,
+                "algorithmic_bytes_per_launch": hb_bytes,
+            },
             "plaquette_last": float(meas["plq_total"][-1] / 3.0) if "plq_total" in meas else None,
         },
         "roofline": {
