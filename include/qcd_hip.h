@@ -177,6 +177,43 @@ int  qcd_cycle_time(qcd_lattice *lat, int repeats, double *ms);
 int  qcd_cycle_profile(qcd_lattice *lat, int repeats, double *hb_us, double *or_us,
                        double *meas_us, double *total_ms);
 
+/* ---- batched ensembles (new; the reference runs one lattice per process and
+        parallelises ensembles trivially, paper p.7 sec. 5).  Many independent
+        chains with their own extents, beta and random series are updated by
+        one launch per half-step and bucket (chains of equal spatial extent);
+        each chain is a full lattice with its own records.  `common` gives
+        everything else (group, precision, storage, NHIT, generator, stream
+        layout, record capacity); its n, beta and randseries are ignored.     */
+typedef struct {
+    int      n[4];        /* N1..N4 of this chain                             */
+    double   beta;        /* BETA of this chain                               */
+    unsigned randseries;  /* RANDSERIES of this chain's streams               */
+} qcd_chain_desc;
+typedef struct qcd_batch qcd_batch;
+int  qcd_batch_create(const qcd_params *common, int nchains, const qcd_chain_desc *chains, qcd_batch **out);
+int  qcd_batch_destroy(qcd_batch *batch);
+int  qcd_batch_info(qcd_batch *batch, int *nchains, int *nbuckets);
+/* Starts every chain (QCD_START_HOT or QCD_START_COLD) and seeds its streams. */
+int  qcd_batch_init(qcd_batch *batch, int start);
+int  qcd_batch_sweep(qcd_batch *batch, int kind);
+/* n_hb heatbath + n_or overrelaxation sweeps of every chain, then the
+   measurements in `mask` (plaquette fused into the last sweep, Polyakov),
+   appended to each chain's records; eagerly, or captured once and replayed. */
+int  qcd_batch_cycle_run(qcd_batch *batch, int n_hb, int n_or, int mask);
+int  qcd_batch_cycle_capture(qcd_batch *batch, int n_hb, int n_or, int mask);
+int  qcd_batch_cycle_launch(qcd_batch *batch, int repeats);
+int  qcd_batch_cycle_time(qcd_batch *batch, int repeats, double *ms);
+int  qcd_batch_sync(qcd_batch *batch);
+/* Autotuner: times every bucket's half-steps with each update-kernel shape
+   (three wavefronts per 64 links or one) and keeps the faster one per bucket
+   and update kind; chains are saved and restored around the timing, so
+   results do not depend on it.  shapes_out (optional): 2 ints per bucket
+   (heatbath, overrelaxation).  Capture the cycle after tuning.            */
+int  qcd_batch_autotune(qcd_batch *batch, int repeats, int *shapes_out);
+/* Chain `index` as a lattice handle (export, records, measure); the batch's
+   work is complete when this returns.  Do not destroy it.                 */
+int  qcd_batch_chain(qcd_batch *batch, int index, qcd_lattice **lat);
+
 /* ---- reference PRN buffer semantics (PRNG::produce, random.cpp:278-284;
         kernels xor128 prngcl_xor128.cl:100, pm prngcl_pm.cl:86,
         ranlux prngcl_ranlux.cl:534).  Host seed tables in, quads out:

@@ -6,6 +6,7 @@ driver.  Importing it does not load the library; the first call does, and it
 raises if the library is missing.
 """
 from ._abi import QCDError, header_symbols  # noqa: F401
+from .batch import Batch, polyakov_susceptibility  # noqa: F401
 from .lattice import Lattice, device_info, prng_produce  # noqa: F401
 
-__all__ = ["Lattice", "QCDError", "device_info", "prng_produce", "header_symbols"]
+__all__ = ["Batch", "Lattice", "QCDError", "device_info", "polyakov_susceptibility", "prng_produce", "header_symbols"]

@@ -39,6 +39,10 @@ class Params(ctypes.Structure):
     ]
 
 
+class ChainDesc(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int * 4), ("beta", ctypes.c_double), ("randseries", ctypes.c_uint)]
+
+
 class QCDError(RuntimeError):
     pass
 
@@ -75,6 +79,18 @@ _SIGNATURES = {
     "qcd_cycle_launch": [_P, ctypes.c_int],
     "qcd_cycle_time": [_P, ctypes.c_int, _D],
     "qcd_cycle_profile": [_P, ctypes.c_int, _D, _D, _D, _D],
+    "qcd_batch_create": [ctypes.POINTER(Params), ctypes.c_int, ctypes.POINTER(ChainDesc), ctypes.POINTER(_P)],
+    "qcd_batch_destroy": [_P],
+    "qcd_batch_info": [_P, _I, _I],
+    "qcd_batch_init": [_P, ctypes.c_int],
+    "qcd_batch_sweep": [_P, ctypes.c_int],
+    "qcd_batch_cycle_run": [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int],
+    "qcd_batch_cycle_capture": [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int],
+    "qcd_batch_cycle_launch": [_P, ctypes.c_int],
+    "qcd_batch_cycle_time": [_P, ctypes.c_int, _D],
+    "qcd_batch_sync": [_P],
+    "qcd_batch_autotune": [_P, ctypes.c_int, _I],
+    "qcd_batch_chain": [_P, ctypes.c_int, ctypes.POINTER(_P)],
     "qcd_prng_produce_host": [
         ctypes.c_int,
         ctypes.c_int,

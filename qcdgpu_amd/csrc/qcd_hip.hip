@@ -28,6 +28,7 @@
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -476,8 +477,11 @@ __device__ __forceinline__ uint32_t upd_block(const UpdArgs& a)
     return k * per + l;
 }
 
+// Body of the three-wavefront update for 64-link unit `blk` of a lattice
+// with `nunits` units (the fused-plaquette partial of the unit goes to
+// partials[(meas_mode - 1) * nunits + blk]).
 template <typename R, int N, int NREAL, bool OVER, int RK, bool MEAS>
-__global__ __launch_bounds__(kUpdThreads) void k_update(UpdArgs a)
+__device__ __forceinline__ void update3(const UpdArgs& a, uint32_t blk, uint32_t nunits)
 {
     using T = R;   // compute type of the SU(2) subgroup arithmetic
     constexpr int NC = 2 * N * N;   // reals per matrix
@@ -486,7 +490,6 @@ __global__ __launch_bounds__(kUpdThreads) void k_update(UpdArgs a)
     // derived from it live in scalar registers.
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int lane = threadIdx.x & 63;
-    const uint32_t blk = upd_block(a);
     const uint32_t i = blk * kLinks + lane;
     const bool active = i < a.g.vh;
     Table<R, N, NREAL> Tb{(typename Vec2<R>::T*)a.table, a.g.vh_pad};
@@ -562,8 +565,14 @@ __global__ __launch_bounds__(kUpdThreads) void k_update(UpdArgs a)
         trace = wave_sum(trace);
         // slot = 64-link unit, so the final sum runs in lattice order
         // whatever the launch order or kernel shape
-        if (lane == 0) a.partials[(size_t)(a.meas_mode - 1) * gridDim.x + blk] = trace;
+        if (lane == 0) a.partials[(size_t)(a.meas_mode - 1) * nunits + blk] = trace;
     }
+}
+
+template <typename R, int N, int NREAL, bool OVER, int RK, bool MEAS>
+__global__ __launch_bounds__(kUpdThreads) void k_update(UpdArgs a)
+{
+    update3<R, N, NREAL, OVER, RK, MEAS>(a, upd_block(a), gridDim.x);
 }
 
 // One wavefront per 64 links: each wavefront gathers and sums the three
@@ -585,13 +594,11 @@ __device__ __forceinline__ uint32_t upd_unit1(const UpdArgs& a, int w)
     return k * per + l;
 }
 
+// Body of the one-wavefront update for 64-link unit `unit` of `units`.
 template <typename R, int N, int NREAL, bool OVER, int RK, bool MEAS>
-__global__ __launch_bounds__(kWaves1 * kLinks) void k_update1(UpdArgs a, uint32_t units)
+__device__ __forceinline__ void update1(const UpdArgs& a, uint32_t unit, uint32_t units)
 {
-    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int lane = threadIdx.x & 63;
-    const uint32_t unit = upd_unit1(a, w);
-    if (unit >= units) return;   // whole wavefront: beyond the last 64-link unit
     const uint32_t i = unit * kLinks + lane;   // < units * 64 = number of streams
     const bool active = i < a.g.vh;
     Table<R, N, NREAL> Tb{(typename Vec2<R>::T*)a.table, a.g.vh_pad};
@@ -637,6 +644,15 @@ __global__ __launch_bounds__(kWaves1 * kLinks) void k_update1(UpdArgs a, uint32_
         trace = wave_sum(trace);
         if (lane == 0) a.partials[(size_t)(a.meas_mode - 1) * units + unit] = trace;
     }
+}
+
+template <typename R, int N, int NREAL, bool OVER, int RK, bool MEAS>
+__global__ __launch_bounds__(kWaves1 * kLinks) void k_update1(UpdArgs a, uint32_t units)
+{
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const uint32_t unit = upd_unit1(a, w);
+    if (unit >= units) return;   // whole wavefront: beyond the last 64-link unit
+    update1<R, N, NREAL, OVER, RK, MEAS>(a, unit, units);
 }
 
 // ---------------------------------------------------------------- PRNG kernels
@@ -686,10 +702,8 @@ __global__ __launch_bounds__(kBlock) void k_pm_produce(uint4* st, uint32_t n, ui
 // s[4q .. 4q+3], q = 6 = (23, 9, quad counter of the block, carry).  All
 // streams advance together, so the quad counter is uniform.
 template <int LUX>
-__global__ __launch_bounds__(kBlock) void k_ranlux_produce(float4* state, uint32_t n, uint32_t samples, float4* quads)
+__device__ __forceinline__ void ranlux_body(float4* state, uint32_t n, uint32_t samples, float4* quads, uint32_t g)
 {
-    const uint32_t g = blockIdx.x * kBlock + threadIdx.x;
-    if (g >= n) return;
     float s[24];
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
@@ -722,6 +736,14 @@ __global__ __launch_bounds__(kBlock) void k_ranlux_produce(float4* state, uint32
 #pragma unroll
     for (int q = 0; q < 6; ++q) state[(size_t)q * n + g] = make_float4(s[4 * q], s[4 * q + 1], s[4 * q + 2], s[4 * q + 3]);
     state[(size_t)6 * n + g] = make_float4(23.0f, 9.0f, (float)k, carry);
+}
+
+template <int LUX>
+__global__ __launch_bounds__(kBlock) void k_ranlux_produce(float4* state, uint32_t n, uint32_t samples, float4* quads)
+{
+    const uint32_t g = blockIdx.x * kBlock + threadIdx.x;
+    if (g >= n) return;
+    ranlux_body<LUX>(state, n, samples, quads, g);
 }
 
 void launch_ranlux_produce(int lux, float4* state, uint32_t n, uint32_t samples, float4* quads, hipStream_t st)
@@ -892,11 +914,12 @@ __global__ __launch_bounds__(kBlock) void k_reunitarize(void* table, Geom g)
 // ---------------------------------------------------------------- reductions
 // Workgroup partial of K doubles per thread: wave shuffles, then LDS, written
 // to partials[block * K + k].  Deterministic; no atomics.
+// Writes the workgroup's K sums to partials[slot * K + k].
 template <int K>
-__device__ __forceinline__ void block_partial(double (&v)[K], double* partials)
+__device__ __forceinline__ void block_partial(double (&v)[K], double* partials, uint32_t slot)
 {
     __shared__ double sm[K][kBlock / 64];
-    const uint32_t bid = blockIdx.y * gridDim.x + blockIdx.x;
+    const uint32_t bid = slot;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
@@ -947,7 +970,7 @@ __device__ __forceinline__ double block_sum_1(double v)
     return x;
 }
 
-__global__ __launch_bounds__(kBlock) void k_finalize(FinArgs f)
+__device__ __forceinline__ void finalize_body(const FinArgs& f)
 {
     const uint32_t t = threadIdx.x;
     if (f.fused || f.plq) {
@@ -1002,6 +1025,8 @@ __global__ __launch_bounds__(kBlock) void k_finalize(FinArgs f)
     }
 }
 
+__global__ __launch_bounds__(kBlock) void k_finalize(FinArgs f) { finalize_body(f); }
+
 // Plaquette: per site sum of ReTr U_p over the spatial planes (x,y), (x,z),
 // (y,z) and the temporal planes (x,t), (y,t), (z,t), products in FP64
 // (lattice_retrace_plaquette3, su3_measurements_cl.cl:527-549).
@@ -1031,7 +1056,7 @@ __global__ __launch_bounds__(kBlock) void k_plaquette(void* table, Geom g, doubl
             }
         }
     }
-    block_partial<2>(v, partials);
+    block_partial<2>(v, partials, blockIdx.y * gridDim.x + blockIdx.x);
 }
 
 // Polyakov loop per spatial site, product of the N4 time links in link
@@ -1098,13 +1123,14 @@ __global__ __launch_bounds__(kBlock) void k_wilson(void* table, Geom g, int wr, 
         }
     }
     double v[1] = {acc};
-    block_partial<1>(v, partials);
+    block_partial<1>(v, partials, blockIdx.x);
 }
 
+// Polyakov-loop body for workgroup `blk` (8 lanes per spatial site).
 template <typename R, int N, int NREAL>
-__global__ __launch_bounds__(kBlock) void k_polyakov(void* table, Geom g, double* partials)
+__device__ __forceinline__ void polyakov_body(void* table, const Geom& g, double* partials, uint32_t blk)
 {
-    const uint32_t tid = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t tid = blk * kBlock + threadIdx.x;
     const uint32_t j = tid >> 3;
     const int l8 = (int)(tid & 7u);
     const uint32_t vs = (uint32_t)g.n1 * g.n2 * g.n3;
@@ -1143,7 +1169,95 @@ __global__ __launch_bounds__(kBlock) void k_polyakov(void* table, Geom g, double
         v[2] = a2;
         v[3] = a2 * a2;
     }
-    block_partial<4>(v, partials);
+    block_partial<4>(v, partials, blk);
+}
+
+template <typename R, int N, int NREAL>
+__global__ __launch_bounds__(kBlock) void k_polyakov(void* table, Geom g, double* partials)
+{
+    polyakov_body<R, N, NREAL>(table, g, partials, blockIdx.x);
+}
+
+// ---------------------------------------------------------------- batched ensembles
+// Many independent lattices ("chains") updated by one launch per half-step.
+// ChainDesc holds what the kernels need for one chain; a host-built work map
+// gives each workgroup (or wavefront) its chain and 64-link unit, with
+// chain = ~0u marking idle entries.  The map is built so that each chain's
+// units run on one XCD.
+struct ChainDesc {
+    UpdArgs a;            // parity, mu, meas_mode, rng, rng_out are set per launch
+    uint4* rng_buf[2];    // XOR128 / PM states (alternating)
+    float4* rl_state;     // RANLUX states
+    uint32_t nunits;      // 64-link units = streams / 64
+    double* part_poly;    // Polyakov partials of this chain
+    FinArgs fin;          // finaliser arguments (sources selected per launch)
+};
+
+__device__ __forceinline__ UpdArgs chain_args(const ChainDesc* cd, uint32_t c, int parity, int mu, int meas_mode,
+                                              int rng_cur)
+{
+    UpdArgs a = cd[c].a;
+    a.parity = parity;
+    a.mu = mu;
+    a.meas_mode = meas_mode;
+    a.rng = cd[c].rng_buf[rng_cur];
+    a.rng_out = cd[c].rng_buf[rng_cur ^ 1];
+    return a;
+}
+
+template <typename R, int N, int NREAL, bool OVER, int RK, bool MEAS>
+__global__ __launch_bounds__(kUpdThreads) void k_update_batch3(const ChainDesc* cd, const uint2* map, int parity,
+                                                               int mu, int meas_mode, int rng_cur)
+{
+    const uint32_t c = __builtin_amdgcn_readfirstlane(map[blockIdx.x].x);
+    const uint32_t u = __builtin_amdgcn_readfirstlane(map[blockIdx.x].y);
+    if (c == ~0u) return;   // whole workgroup idle
+    const UpdArgs a = chain_args(cd, c, parity, mu, meas_mode, rng_cur);
+    update3<R, N, NREAL, OVER, RK, MEAS>(a, u, cd[c].nunits);
+}
+
+template <typename R, int N, int NREAL, bool OVER, int RK, bool MEAS>
+__global__ __launch_bounds__(kWaves1 * kLinks) void k_update_batch1(const ChainDesc* cd, const uint2* map, int parity,
+                                                                    int mu, int meas_mode, int rng_cur)
+{
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const uint32_t e = blockIdx.x * kWaves1 + (uint32_t)w;
+    const uint32_t c = __builtin_amdgcn_readfirstlane(map[e].x);
+    const uint32_t u = __builtin_amdgcn_readfirstlane(map[e].y);
+    if (c == ~0u) return;   // whole wavefront idle
+    const UpdArgs a = chain_args(cd, c, parity, mu, meas_mode, rng_cur);
+    update1<R, N, NREAL, OVER, RK, MEAS>(a, u, cd[c].nunits);
+}
+
+// RANLUX produce for every chain: one wavefront-sized workgroup per unit.
+template <int LUX>
+__global__ __launch_bounds__(kLinks) void k_ranlux_produce_batch(const ChainDesc* cd, const uint2* map, uint32_t samples)
+{
+    const uint32_t c = __builtin_amdgcn_readfirstlane(map[blockIdx.x].x);
+    const uint32_t u = __builtin_amdgcn_readfirstlane(map[blockIdx.x].y);
+    if (c == ~0u) return;
+    const uint32_t n = cd[c].nunits * kLinks;
+    ranlux_body<LUX>(cd[c].rl_state, n, samples, const_cast<float4*>(cd[c].a.prn), u * kLinks + threadIdx.x);
+}
+
+template <typename R, int N, int NREAL>
+__global__ __launch_bounds__(kBlock) void k_polyakov_batch(const ChainDesc* cd, const uint2* map)
+{
+    const uint32_t c = __builtin_amdgcn_readfirstlane(map[blockIdx.x].x);
+    const uint32_t b = __builtin_amdgcn_readfirstlane(map[blockIdx.x].y);
+    if (c == ~0u) return;
+    polyakov_body<R, N, NREAL>(cd[c].a.table, cd[c].a.g, cd[c].part_poly, b);
+}
+
+// One workgroup per chain.  flags: 1 fused plaquette, 4 Polyakov.
+__global__ __launch_bounds__(kBlock) void k_finalize_batch(const ChainDesc* cd, int flags)
+{
+    FinArgs f = cd[blockIdx.x].fin;
+    if (!(flags & 1)) f.fused = nullptr;
+    f.plq = nullptr;
+    if (!(flags & 4)) f.poly = nullptr;
+    f.wil = nullptr;
+    finalize_body(f);
 }
 
 }  // namespace
@@ -2100,6 +2214,504 @@ int qcd_prng_produce_host(int prng, int ranlux_luxury, unsigned n, unsigned samp
     HIPCHK(hipMemcpy(table, dt, sizeof(uint4) * n, hipMemcpyDeviceToHost));
     (void)hipFree(dt);
     (void)hipFree(dq);
+    return QCD_OK;
+}
+
+}  // extern "C"
+
+// ============================================================================
+// Batched ensembles (host)
+// ============================================================================
+// A batch owns independent lattices of possibly different extents and beta
+// (each a full qcd_lattice with its own table, streams and records) and
+// updates all of them with one launch per half-step and bucket.  Buckets are
+// the chains of equal spatial extent; each bucket has its own launch and its
+// own kernel shape per update kind, chosen by qcd_batch_autotune.
+struct qcd_batch {
+    qcd_params p;                          // common parameters (sizes, beta, seed per chain)
+    std::vector<qcd_lattice*> chains;
+    hipStream_t stream = nullptr;
+    ChainDesc* d_desc = nullptr;
+    uint2* d_map = nullptr;                // all work maps, concatenated
+    struct Bucket {
+        std::vector<int> members;
+        uint32_t off3 = 0, len3 = 0;       // three-wavefront map: one entry per workgroup
+        uint32_t off1 = 0, len1 = 0;       // one-wavefront map: one entry per wavefront
+        int shape[2] = {3, 3};             // [heatbath, overrelaxation]
+    };
+    std::vector<Bucket> buckets;
+    uint32_t off_units = 0, len_units = 0; // every (chain, unit): RANLUX produce
+    uint32_t off_poly = 0, len_poly = 0;   // every (chain, Polyakov workgroup)
+    int rng_cur = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+UpdArgs chain_base_args(const qcd_lattice* L)
+{
+    UpdArgs a{};
+    a.table = L->table;
+    a.g = L->g;
+    a.beta = L->p.beta / L->p.sun;
+    a.nhit = L->p.nhit;
+    a.sites = 2u * L->g.vh;
+    a.s_quads = (L->p.stream_mode == QCD_STREAM_REFERENCE) ? L->s_quads : 0u;
+    a.partials = L->part_fused;
+    a.xcd_map = 0;
+    a.ord_ng = a.ord_nt = a.ord_xs = 1;
+    a.jump = L->jump_cols;
+    a.pm_ak = L->pm_ak;
+    a.prn = L->prn_buf;
+    a.prn_stride = L->n_streams;
+    return a;
+}
+
+FinArgs chain_fin_args(const qcd_lattice* L)
+{
+    FinArgs f{};
+    f.fused = L->part_fused;
+    f.n_fused = L->nb_fused;
+    f.plq = L->part_plq;
+    f.n_plq = 2u * L->upd_blocks;
+    f.poly = L->part_poly;
+    f.n_poly = L->poly_blocks;
+    f.wil = L->part_wil;
+    f.n_wil = L->wil_blocks;
+    f.plq_rec = L->plq_rec;
+    f.poly_rec = L->poly_rec;
+    f.wil_rec = L->wil_rec;
+    f.rec_index = L->rec_index;
+    f.capacity = L->p.max_measurements;
+    return f;
+}
+
+// Work maps.  Chains of a bucket go to the XCD with the least work so far
+// (largest first); each XCD's list holds its chains' units in order.
+// Hardware workgroup b runs on XCD b mod 8.
+void build_maps(qcd_batch* B, std::vector<uint2>& map)
+{
+    const uint2 idle = make_uint2(~0u, 0u);
+    for (auto& bk : B->buckets) {
+        std::vector<int> order = bk.members;
+        std::sort(order.begin(), order.end(), [&](int a, int b) {
+            return B->chains[a]->nb_fused > B->chains[b]->nb_fused;
+        });
+        std::vector<std::vector<uint2>> xcd(8);
+        std::vector<uint64_t> load(8, 0);
+        for (int c : order) {
+            const int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            const uint32_t nu = B->chains[c]->nb_fused;
+            for (uint32_t u = 0; u < nu; ++u) xcd[k].push_back(make_uint2((uint32_t)c, u));
+            load[k] += nu;
+        }
+        size_t maxlen = 0;
+        for (auto& l : xcd) maxlen = std::max(maxlen, l.size());
+        bk.off3 = (uint32_t)map.size();
+        bk.len3 = (uint32_t)(8 * maxlen);
+        for (uint32_t b = 0; b < bk.len3; ++b) {
+            const auto& l = xcd[b % 8];
+            map.push_back(b / 8 < l.size() ? l[b / 8] : idle);
+        }
+        const size_t rows = (maxlen + kWaves1 - 1) / kWaves1;
+        bk.off1 = (uint32_t)map.size();
+        bk.len1 = (uint32_t)(8 * rows * kWaves1);
+        for (uint32_t b = 0; b < 8 * rows; ++b)
+            for (int w = 0; w < kWaves1; ++w) {
+                const auto& l = xcd[b % 8];
+                const size_t e = (b / 8) * kWaves1 + w;
+                map.push_back(e < l.size() ? l[e] : idle);
+            }
+    }
+    B->off_units = (uint32_t)map.size();
+    for (size_t c = 0; c < B->chains.size(); ++c)
+        for (uint32_t u = 0; u < B->chains[c]->nb_fused; ++u) map.push_back(make_uint2((uint32_t)c, u));
+    B->len_units = (uint32_t)map.size() - B->off_units;
+    B->off_poly = (uint32_t)map.size();
+    for (size_t c = 0; c < B->chains.size(); ++c)
+        for (uint32_t b = 0; b < B->chains[c]->poly_blocks; ++b) map.push_back(make_uint2((uint32_t)c, b));
+    B->len_poly = (uint32_t)map.size() - B->off_poly;
+}
+
+template <typename R, int N, int NREAL, bool OVER, int RK>
+void launch_batch_update(qcd_batch* B, const qcd_batch::Bucket& bk, int parity, int mu, int meas_mode, const LaunchOpt& o)
+{
+    const int shape = bk.shape[OVER ? 1 : 0];
+    if (shape == 1) {
+        const dim3 grid(bk.len1 / kWaves1), block(kWaves1 * kLinks);
+        const uint2* map = B->d_map + bk.off1;
+        if (meas_mode)
+            launch(k_update_batch1<R, N, NREAL, OVER, RK, true>, grid, block, B->stream, o, (const ChainDesc*)B->d_desc,
+                   map, parity, mu, meas_mode, B->rng_cur);
+        else
+            launch(k_update_batch1<R, N, NREAL, OVER, RK, false>, grid, block, B->stream, o, (const ChainDesc*)B->d_desc,
+                   map, parity, mu, meas_mode, B->rng_cur);
+    } else {
+        const dim3 grid(bk.len3), block(kUpdThreads);
+        const uint2* map = B->d_map + bk.off3;
+        if (meas_mode)
+            launch(k_update_batch3<R, N, NREAL, OVER, RK, true>, grid, block, B->stream, o, (const ChainDesc*)B->d_desc,
+                   map, parity, mu, meas_mode, B->rng_cur);
+        else
+            launch(k_update_batch3<R, N, NREAL, OVER, RK, false>, grid, block, B->stream, o, (const ChainDesc*)B->d_desc,
+                   map, parity, mu, meas_mode, B->rng_cur);
+    }
+}
+
+// One (parity, mu) half-step of every chain; `only` >= 0 restricts it to one
+// bucket (autotuning).
+template <typename R, int N, int NREAL>
+int batch_halfstep_rn(qcd_batch* B, int parity, int mu, int kind, int meas_mode, int only, const LaunchOpt& o)
+{
+    const qcd_params& p = B->p;
+    if (kind == QCD_HEATBATH && p.prng == QCD_PRNG_RANLUX) {
+        const uint32_t S = B->chains[0]->s_quads;
+        const dim3 grid(B->len_units), block(kLinks);
+        const uint2* map = B->d_map + B->off_units;
+        switch (p.ranlux_luxury) {
+            case 0: hipLaunchKernelGGL(k_ranlux_produce_batch<0>, grid, block, 0, B->stream, B->d_desc, map, S); break;
+            case 1: hipLaunchKernelGGL(k_ranlux_produce_batch<1>, grid, block, 0, B->stream, B->d_desc, map, S); break;
+            case 2: hipLaunchKernelGGL(k_ranlux_produce_batch<2>, grid, block, 0, B->stream, B->d_desc, map, S); break;
+            case 3: hipLaunchKernelGGL(k_ranlux_produce_batch<3>, grid, block, 0, B->stream, B->d_desc, map, S); break;
+            default: hipLaunchKernelGGL(k_ranlux_produce_batch<4>, grid, block, 0, B->stream, B->d_desc, map, S); break;
+        }
+    }
+    for (size_t k = 0; k < B->buckets.size(); ++k) {
+        if (only >= 0 && (int)k != only) continue;
+        const auto& bk = B->buckets[k];
+        if (kind == QCD_OVERRELAX) launch_batch_update<R, N, NREAL, true, RK_NONE>(B, bk, parity, mu, meas_mode, o);
+        else if (p.prng == QCD_PRNG_RANLUX) launch_batch_update<R, N, NREAL, false, RK_BUF>(B, bk, parity, mu, meas_mode, o);
+        else if (p.prng == QCD_PRNG_XOR128) launch_batch_update<R, N, NREAL, false, RK_XOR128>(B, bk, parity, mu, meas_mode, o);
+        else launch_batch_update<R, N, NREAL, false, RK_PM>(B, bk, parity, mu, meas_mode, o);
+    }
+    HIPCHK(hipGetLastError());
+    if (kind == QCD_HEATBATH && p.prng != QCD_PRNG_RANLUX) {
+        B->rng_cur ^= 1;   // every chain's states went to its other buffer
+        for (auto* L : B->chains) L->rng_cur = B->rng_cur;
+    }
+    return QCD_OK;
+}
+
+int batch_sweep(qcd_batch* B, int kind, bool fuse_plaquette)
+{
+    for (int parity = 1; parity >= 0; --parity)
+        for (int mu = 0; mu < 4; ++mu) {
+            const int meas = (fuse_plaquette && parity == 0 && mu >= 1) ? mu : 0;
+            const int rc = DISPATCH(B, batch_halfstep_rn, B, parity, mu, kind, meas, -1, LaunchOpt{});
+            if (rc) return rc;
+        }
+    return QCD_OK;
+}
+
+template <typename R, int N, int NREAL>
+int batch_polyakov_rn(qcd_batch* B)
+{
+    hipLaunchKernelGGL((k_polyakov_batch<R, N, NREAL>), dim3(B->len_poly), dim3(kBlock), 0, B->stream,
+                       (const ChainDesc*)B->d_desc, (const uint2*)(B->d_map + B->off_poly));
+    HIPCHK(hipGetLastError());
+    return QCD_OK;
+}
+
+// n_hb heatbath + n_or overrelaxation sweeps of every chain, then the
+// plaquette (fused into the last sweep) and Polyakov records of every chain.
+int batch_enqueue_cycle(qcd_batch* B, int n_hb, int n_or, int mask)
+{
+    const int total = n_hb + n_or;
+    const bool fuse = (mask & QCD_MEAS_PLAQUETTE) && total > 0;
+    for (int k = 0; k < total; ++k) {
+        const int rc = batch_sweep(B, k < n_hb ? QCD_HEATBATH : QCD_OVERRELAX, fuse && k == total - 1);
+        if (rc) return rc;
+    }
+    int flags = fuse ? 1 : 0;
+    if (mask & QCD_MEAS_POLYAKOV) {
+        const int rc = DISPATCH(B, batch_polyakov_rn, B);
+        if (rc) return rc;
+        flags |= 4;
+    }
+    if (flags) {
+        hipLaunchKernelGGL(k_finalize_batch, dim3((uint32_t)B->chains.size()), dim3(kBlock), 0, B->stream,
+                           (const ChainDesc*)B->d_desc, flags);
+        HIPCHK(hipGetLastError());
+    }
+    return QCD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int qcd_batch_create(const qcd_params* common, int nchains, const qcd_chain_desc* chains, qcd_batch** out)
+{
+    if (!common || !chains || !out || nchains < 1) return fail(QCD_E_ARG, "null argument or no chains");
+    if (common->prng != QCD_PRNG_XOR128 && common->prng != QCD_PRNG_PM && common->prng != QCD_PRNG_RANLUX)
+        return fail(QCD_E_UNSUPPORTED, "batched chains need per-site streams (XOR128, PM or RANLUX)");
+    *out = nullptr;
+    qcd_batch* B = new qcd_batch;
+    B->p = *common;
+    for (int c = 0; c < nchains; ++c) {
+        qcd_params q = *common;
+        for (int k = 0; k < 4; ++k) q.n[k] = chains[c].n[k];
+        q.beta = chains[c].beta;
+        q.randseries = chains[c].randseries;
+        qcd_lattice* L = nullptr;
+        const int rc = qcd_lattice_create(&q, &L);
+        if (rc) {
+            qcd_batch_destroy(B);
+            return rc;
+        }
+        B->chains.push_back(L);
+    }
+    // buckets: chains of equal spatial extent
+    for (int c = 0; c < nchains; ++c) {
+        const qcd_lattice* L = B->chains[c];
+        int found = -1;
+        for (size_t k = 0; k < B->buckets.size(); ++k) {
+            const qcd_lattice* M = B->chains[B->buckets[k].members[0]];
+            if (M->g.n1 == L->g.n1 && M->g.n2 == L->g.n2 && M->g.n3 == L->g.n3) found = (int)k;
+        }
+        if (found < 0) {
+            B->buckets.emplace_back();
+            found = (int)B->buckets.size() - 1;
+        }
+        B->buckets[found].members.push_back(c);
+    }
+    std::vector<uint2> map;
+    build_maps(B, map);
+    std::vector<ChainDesc> desc(nchains);
+    for (int c = 0; c < nchains; ++c) {
+        const qcd_lattice* L = B->chains[c];
+        desc[c].a = chain_base_args(L);
+        desc[c].rng_buf[0] = L->rng_buf[0];
+        desc[c].rng_buf[1] = L->rng_buf[1];
+        desc[c].rl_state = L->rl_state;
+        desc[c].nunits = L->nb_fused;
+        desc[c].part_poly = L->part_poly;
+        desc[c].fin = chain_fin_args(L);
+    }
+    hipError_t e;
+    if ((e = hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipMalloc((void**)&B->d_desc, sizeof(ChainDesc) * nchains)) != hipSuccess ||
+        (e = hipMalloc((void**)&B->d_map, sizeof(uint2) * map.size())) != hipSuccess ||
+        (e = hipMemcpy(B->d_desc, desc.data(), sizeof(ChainDesc) * nchains, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(B->d_map, map.data(), sizeof(uint2) * map.size(), hipMemcpyHostToDevice)) != hipSuccess) {
+        qcd_batch_destroy(B);
+        return fail(QCD_E_HIP, "batch set-up: %s", hipGetErrorString(e));
+    }
+    *out = B;
+    return QCD_OK;
+}
+
+int qcd_batch_destroy(qcd_batch* B)
+{
+    if (!B) return QCD_OK;
+    if (B->stream) (void)hipStreamSynchronize(B->stream);
+    if (B->exec) (void)hipGraphExecDestroy(B->exec);
+    if (B->graph) (void)hipGraphDestroy(B->graph);
+    for (auto* L : B->chains) qcd_lattice_destroy(L);
+    (void)hipFree(B->d_desc);
+    (void)hipFree(B->d_map);
+    if (B->stream) (void)hipStreamDestroy(B->stream);
+    delete B;
+    return QCD_OK;
+}
+
+int qcd_batch_sync(qcd_batch* B)
+{
+    if (!B) return fail(QCD_E_ARG, "null batch");
+    HIPCHK(hipStreamSynchronize(B->stream));
+    for (auto* L : B->chains) HIPCHK(hipStreamSynchronize(L->stream));
+    return QCD_OK;
+}
+
+int qcd_batch_init(qcd_batch* B, int start)
+{
+    if (!B) return fail(QCD_E_ARG, "null batch");
+    HIPCHK(hipStreamSynchronize(B->stream));
+    for (auto* L : B->chains) {
+        int rc = QCD_OK;
+        if (start == QCD_START_HOT) {
+            rc = qcd_prng_init(L);
+            if (!rc) rc = qcd_lattice_init(L, start);
+        } else if (start == QCD_START_COLD) {
+            rc = qcd_lattice_init(L, start);
+            if (!rc) rc = qcd_prng_init(L);
+        } else {
+            return fail(QCD_E_UNSUPPORTED, "batched chains start hot or cold");
+        }
+        if (rc) return rc;
+        if (L->rng_cur != 0) return fail(QCD_E_STATE, "internal: stream buffer parity");
+    }
+    B->rng_cur = 0;
+    for (auto* L : B->chains) HIPCHK(hipStreamSynchronize(L->stream));
+    return QCD_OK;
+}
+
+int qcd_batch_sweep(qcd_batch* B, int kind)
+{
+    if (!B) return fail(QCD_E_ARG, "null batch");
+    if (kind != QCD_HEATBATH && kind != QCD_OVERRELAX) return fail(QCD_E_ARG, "bad update kind");
+    return batch_sweep(B, kind, false);
+}
+
+int qcd_batch_cycle_run(qcd_batch* B, int n_hb, int n_or, int mask)
+{
+    if (!B) return fail(QCD_E_ARG, "null batch");
+    if (n_hb < 0 || n_or < 0 || n_hb + n_or < 1) return fail(QCD_E_ARG, "a cycle needs at least one sweep");
+    return batch_enqueue_cycle(B, n_hb, n_or, mask);
+}
+
+int qcd_batch_cycle_capture(qcd_batch* B, int n_hb, int n_or, int mask)
+{
+    if (!B) return fail(QCD_E_ARG, "null batch");
+    if (n_hb < 0 || n_or < 0 || n_hb + n_or < 1) return fail(QCD_E_ARG, "a cycle needs at least one sweep");
+    if (B->exec) {
+        (void)hipGraphExecDestroy(B->exec);
+        B->exec = nullptr;
+    }
+    if (B->graph) {
+        (void)hipGraphDestroy(B->graph);
+        B->graph = nullptr;
+    }
+    HIPCHK(hipStreamSynchronize(B->stream));
+    HIPCHK(hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = batch_enqueue_cycle(B, n_hb, n_or, mask);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(B->stream, &graph);
+    if (rc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(QCD_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    B->graph = graph;
+    HIPCHK(hipGraphInstantiate(&B->exec, graph, nullptr, nullptr, 0));
+    return QCD_OK;
+}
+
+int qcd_batch_cycle_launch(qcd_batch* B, int repeats)
+{
+    if (!B || !B->exec) return fail(QCD_E_STATE, "no captured cycle");
+    for (int r = 0; r < repeats; ++r) HIPCHK(hipGraphLaunch(B->exec, B->stream));
+    return QCD_OK;
+}
+
+int qcd_batch_cycle_time(qcd_batch* B, int repeats, double* ms)
+{
+    if (!B || !B->exec) return fail(QCD_E_STATE, "no captured cycle");
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    HIPCHK(hipStreamSynchronize(B->stream));
+    HIPCHK(hipEventRecord(a, B->stream));
+    for (int r = 0; r < repeats; ++r) HIPCHK(hipGraphLaunch(B->exec, B->stream));
+    HIPCHK(hipEventRecord(b, B->stream));
+    HIPCHK(hipEventSynchronize(b));
+    float f = 0.f;
+    HIPCHK(hipEventElapsedTime(&f, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (ms) *ms = f;
+    return QCD_OK;
+}
+
+// Times each bucket's half-steps with each kernel shape and keeps the faster
+// shape per bucket and update kind.  Link tables and generator states are
+// saved before and restored after, so tuning does not change any chain.
+int qcd_batch_autotune(qcd_batch* B, int repeats, int* shapes_out)
+{
+    if (!B) return fail(QCD_E_ARG, "null batch");
+    if (repeats < 1) return fail(QCD_E_ARG, "repeats must be >= 1");
+    HIPCHK(hipStreamSynchronize(B->stream));
+    // save
+    std::vector<void*> saved;
+    auto save = [&](const void* src, size_t bytes) -> bool {
+        void* d = nullptr;
+        if (!src || !bytes) {
+            saved.push_back(nullptr);
+            return true;
+        }
+        if (hipMalloc(&d, bytes) != hipSuccess) return false;
+        saved.push_back(d);
+        return hipMemcpyAsync(d, src, bytes, hipMemcpyDeviceToDevice, B->stream) == hipSuccess;
+    };
+    bool ok = true;
+    for (auto* L : B->chains) {
+        ok = ok && save(L->table, L->table_bytes);
+        ok = ok && save(L->rng_buf[0], sizeof(uint4) * L->n_streams);
+        ok = ok && save(L->rng_buf[1], sizeof(uint4) * L->n_streams);
+        ok = ok && save(L->rl_state, L->rl_state ? sizeof(float4) * 7 * L->n_streams : 0);
+    }
+    const int rng_cur0 = B->rng_cur;
+    hipEvent_t e0, e1;
+    int rc = QCD_OK;
+    if (ok && hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess) {
+        for (size_t k = 0; k < B->buckets.size() && !rc; ++k)
+            for (int kind = 0; kind < 2 && !rc; ++kind) {
+                float best = 1e30f;
+                int best_shape = 3;
+                for (int shape : {3, 1}) {
+                    B->buckets[k].shape[kind] = shape;
+                    // one untimed launch, then `repeats` timed ones
+                    rc = DISPATCH(B, batch_halfstep_rn, B, 1, 0, kind == 0 ? QCD_HEATBATH : QCD_OVERRELAX, 0, (int)k,
+                                  LaunchOpt{});
+                    (void)hipEventRecord(e0, B->stream);
+                    for (int r = 0; r < repeats && !rc; ++r)
+                        rc = DISPATCH(B, batch_halfstep_rn, B, r & 1, (r >> 1) & 3,
+                                      kind == 0 ? QCD_HEATBATH : QCD_OVERRELAX, 0, (int)k, LaunchOpt{});
+                    (void)hipEventRecord(e1, B->stream);
+                    (void)hipEventSynchronize(e1);
+                    float t = 0.f;
+                    (void)hipEventElapsedTime(&t, e0, e1);
+                    if (t < best) {
+                        best = t;
+                        best_shape = shape;
+                    }
+                }
+                B->buckets[k].shape[kind] = best_shape;
+            }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    } else {
+        rc = fail(QCD_E_NOMEM, "autotune: no room to save the chains");
+    }
+    // restore
+    size_t s = 0;
+    for (auto* L : B->chains) {
+        const void* dst[4] = {L->table, L->rng_buf[0], L->rng_buf[1], L->rl_state};
+        const size_t bytes[4] = {L->table_bytes, sizeof(uint4) * L->n_streams, sizeof(uint4) * L->n_streams,
+                                 L->rl_state ? sizeof(float4) * 7 * L->n_streams : 0};
+        for (int j = 0; j < 4; ++j, ++s)
+            if (s < saved.size() && saved[s] && bytes[j])
+                (void)hipMemcpyAsync((void*)dst[j], saved[s], bytes[j], hipMemcpyDeviceToDevice, B->stream);
+    }
+    (void)hipStreamSynchronize(B->stream);
+    for (void* d : saved) (void)hipFree(d);
+    B->rng_cur = rng_cur0;
+    for (auto* L : B->chains) L->rng_cur = rng_cur0;
+    if (rc) return rc;
+    if (shapes_out)
+        for (size_t k = 0; k < B->buckets.size(); ++k) {
+            shapes_out[2 * k] = B->buckets[k].shape[0];
+            shapes_out[2 * k + 1] = B->buckets[k].shape[1];
+        }
+    // a captured cycle keeps the shapes it was captured with
+    return QCD_OK;
+}
+
+int qcd_batch_info(qcd_batch* B, int* nchains, int* nbuckets)
+{
+    if (!B) return fail(QCD_E_ARG, "null batch");
+    if (nchains) *nchains = (int)B->chains.size();
+    if (nbuckets) *nbuckets = (int)B->buckets.size();
+    return QCD_OK;
+}
+
+int qcd_batch_chain(qcd_batch* B, int index, qcd_lattice** lat)
+{
+    if (!B || !lat) return fail(QCD_E_ARG, "null argument");
+    if (index < 0 || index >= (int)B->chains.size()) return fail(QCD_E_ARG, "chain %d out of range", index);
+    HIPCHK(hipStreamSynchronize(B->stream));
+    *lat = B->chains[index];
     return QCD_OK;
 }
 

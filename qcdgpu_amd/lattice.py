@@ -50,13 +50,26 @@ class Lattice:
         self.dims = tuple(int(d) for d in dims)
         self.volume = int(np.prod(self.dims))
         self._h = ctypes.c_void_p()
+        self._owned = True
         check(lib().qcd_lattice_create(ctypes.byref(p), ctypes.byref(self._h)))
+
+    @classmethod
+    def _view(cls, handle, params):
+        """A non-owning view of a lattice owned elsewhere (a batch chain)."""
+        self = cls.__new__(cls)
+        self.params = params
+        self.sun = params.sun
+        self.dims = tuple(int(params.n[k]) for k in range(4))
+        self.volume = int(np.prod(self.dims))
+        self._h = handle
+        self._owned = False
+        return self
 
     # -- lifetime -----------------------------------------------------------
     def close(self):
-        if self._h:
+        if self._h and self._owned:
             check(lib().qcd_lattice_destroy(self._h))
-            self._h = ctypes.c_void_p()
+        self._h = ctypes.c_void_p()
 
     def __del__(self):
         try:
