@@ -177,6 +177,74 @@ int  qcd_cycle_time(qcd_lattice *lat, int repeats, double *ms);
 int  qcd_cycle_profile(qcd_lattice *lat, int repeats, double *hb_us, double *or_us,
                        double *meas_us, double *total_ms);
 
+/* ---- checkpoints (model::lattice_save_state / lattice_load_state /
+        lattice_make_bin_header / lattice_load_bin_header,
+        suncl.cpp:1857-2079; generator replay suncl.cpp:5349-5351).        */
+int  qcd_lattice_get_params(const qcd_lattice *lat, qcd_params *out);
+/* PRNG::produce calls the reference would have made so far (its
+   PRNG_counter, random.cpp:283): 1 after qcd_prng_init, +4 for a hot start
+   (one produce before each of the four init kernels, suncl.cpp:5236-5248),
+   +1 per heatbath half-step.  Not kept for batch chains.                   */
+int  qcd_prng_counter(const qcd_lattice *lat, unsigned *counter);
+/* Re-seeds the streams (qcd_prng_init) and advances them to production
+   `counter` of a run that began with `start` (QCD_START_*), as the
+   reference's resume does (while (PRNG_counter > PRNG0->PRNG_counter)
+   produce()).  XOR128 / PM heatbath productions are applied as one jump.
+   Fast-layout XOR128 / PM streams cannot be replayed (QCD_E_UNSUPPORTED).  */
+int  qcd_prng_replay(qcd_lattice *lat, unsigned counter, int start);
+/* Exact state image: the table as stored on the device, the current
+   generator states, all records and their counts, the production counter
+   and the start flags, behind a head with the lattice parameters.  Loading
+   it into a lattice created with equal qcd_params continues the run
+   bit-identically (any other lattice: QCD_E_ARG).  Synchronous.            */
+int  qcd_lattice_state_bytes(const qcd_lattice *lat, size_t *bytes);
+int  qcd_lattice_state_save(qcd_lattice *lat, void *buf, size_t bytes);
+int  qcd_lattice_state_load(qcd_lattice *lat, const void *buf, size_t bytes);
+/* Replaces the records and their counts (the reference's buffer_init of the
+   measurement arrays from the loaded host copies, suncl.cpp:2061-2068);
+   counts beyond max_measurements are kept as counts, the extra records are
+   dropped as qcd_measure drops them.  plq: 2*count, poly: 4*count.         */
+int  qcd_write_measurements(qcd_lattice *lat, const double *plq, const double *poly, const double *wil,
+                            int count_plq, int count_poly, int count_wil);
+
+/* The reference's .qcg state file.  64 uint32 header words
+   (lattice_make_bin_header, suncl.cpp:1939-1993), then, as
+   lattice_save_state writes them (suncl.cpp:1889-1901): the measurement
+   partials (double2 x measurement_size_F; written as zeros), the action
+   sums (double2 x energies_size), the plaquette sums (double2 x
+   energies_size_F), the Wilson loop sums (double x energies_size, if
+   get_wilson), the Polyakov sums (Re, Im) (double2 x polyakov_loop_size,
+   if pl_level > 0) and the table in the reference layout (float4 / double4
+   x table_size: plane k of direction d at [gindex + (4k + d) * ROWSIZE];
+   SU(3) planes (Re u11, Re u12, Re u13, Re u23), (Im u11, Im u12, Im u13,
+   Im u23), (Re u21, Re u22, Im u21, Im u22); SU(2) (Re u11, Re u12, Im u11,
+   Im u12)).  Sizes use buffer_size_align with the reference driver's
+   64-item work-group cap (QCDGPU.cpp:74).  After the table this library
+   appends "QCDGPUX1", a uint64 byte count and the exact state image; the
+   reference stops reading before it.                                       */
+typedef struct {
+    unsigned init;            /* INIT (0 = continue a run)            word 4     */
+    unsigned start;           /* ints: QCD_START_*                    word 5     */
+    unsigned prng_counter;    /* production counter                   word 9
+                                 (the writer takes it from the lattice)         */
+    unsigned nav, nav_counter, niter, iter, iter_counter;   /* words 10-14      */
+    unsigned wilson_r, wilson_t;                            /* words 16-17      */
+    unsigned pl_level;                                      /* word 25          */
+    unsigned get_actions, get_plaquettes, get_wilson;       /* words 29-31      */
+} qcd_run_info;
+/* Writes the state file (atomically: a temporary file renamed into place).  */
+int  qcd_qcg_write(qcd_lattice *lat, const char *path, const qcd_run_info *info);
+/* Reads the header: the lattice parameters (storage, stream layout and
+   record capacity from the exact state image when the file has one, else
+   18 / 4 reals, reference streams and ITER records) and the run counters.
+   has_exact (optional) tells whether the image is present.  No device use. */
+int  qcd_qcg_read_header(const char *path, qcd_params *params, qcd_run_info *info, int *has_exact);
+/* Loads the file into `lat`: from the exact image when its parameters equal
+   the lattice's, else from the reference sections (table, plaquette,
+   Polyakov (Re, Im) and Wilson sums; P2 / P4 are not in them) with the
+   streams replayed to the header's production counter.                     */
+int  qcd_qcg_read(qcd_lattice *lat, const char *path, qcd_run_info *info);
+
 /* ---- batched ensembles (new; the reference runs one lattice per process and
         parallelises ensembles trivially, paper p.7 sec. 5).  Many independent
         chains with their own extents, beta and random series are updated by

@@ -7,6 +7,9 @@ raises if the library is missing.
 """
 from ._abi import QCDError, header_symbols  # noqa: F401
 from .batch import Batch, polyakov_susceptibility  # noqa: F401
-from .lattice import Lattice, device_info, prng_produce  # noqa: F401
+from .lattice import Lattice, device_info, prng_produce, read_qcg_header  # noqa: F401
 
-__all__ = ["Batch", "Lattice", "QCDError", "device_info", "polyakov_susceptibility", "prng_produce", "header_symbols"]
+__all__ = [
+    "Batch", "Lattice", "QCDError", "device_info", "polyakov_susceptibility", "prng_produce", "header_symbols",
+    "read_qcg_header",
+]

@@ -43,6 +43,21 @@ class ChainDesc(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int * 4), ("beta", ctypes.c_double), ("randseries", ctypes.c_uint)]
 
 
+class RunInfo(ctypes.Structure):
+    """qcd_run_info: the run counters of a .qcg header."""
+
+    _fields_ = [
+        (name, ctypes.c_uint)
+        for name in (
+            "init", "start", "prng_counter", "nav", "nav_counter", "niter", "iter", "iter_counter",
+            "wilson_r", "wilson_t", "pl_level", "get_actions", "get_plaquettes", "get_wilson",
+        )
+    ]
+
+    def as_dict(self):
+        return {name: int(getattr(self, name)) for name, _ in self._fields_}
+
+
 class QCDError(RuntimeError):
     pass
 
@@ -91,6 +106,16 @@ _SIGNATURES = {
     "qcd_batch_sync": [_P],
     "qcd_batch_autotune": [_P, ctypes.c_int, _I],
     "qcd_batch_chain": [_P, ctypes.c_int, ctypes.POINTER(_P)],
+    "qcd_lattice_get_params": [_P, ctypes.POINTER(Params)],
+    "qcd_prng_counter": [_P, ctypes.POINTER(ctypes.c_uint)],
+    "qcd_prng_replay": [_P, ctypes.c_uint, ctypes.c_int],
+    "qcd_lattice_state_bytes": [_P, ctypes.POINTER(ctypes.c_size_t)],
+    "qcd_lattice_state_save": [_P, ctypes.c_void_p, ctypes.c_size_t],
+    "qcd_lattice_state_load": [_P, ctypes.c_void_p, ctypes.c_size_t],
+    "qcd_write_measurements": [_P, _D, _D, _D, ctypes.c_int, ctypes.c_int, ctypes.c_int],
+    "qcd_qcg_write": [_P, ctypes.c_char_p, ctypes.POINTER(RunInfo)],
+    "qcd_qcg_read_header": [ctypes.c_char_p, ctypes.POINTER(Params), ctypes.POINTER(RunInfo), _I],
+    "qcd_qcg_read": [_P, ctypes.c_char_p, ctypes.POINTER(RunInfo)],
     "qcd_prng_produce_host": [
         ctypes.c_int,
         ctypes.c_int,

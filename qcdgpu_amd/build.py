@@ -42,9 +42,9 @@ def build_host(verbose=False):
 
 def build(verbose=False):
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    src = os.path.join(CSRC, "qcd_hip.hip")
+    srcs = [os.path.join(CSRC, "qcd_hip.hip"), os.path.join(CSRC, "qcd_qcg.cpp")]
     tmp = LIB + ".tmp"
-    cmd = [hipcc] + HIPCC_FLAGS + ["-o", tmp, src]
+    cmd = [hipcc] + HIPCC_FLAGS + ["-o", tmp] + srcs
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError("hipcc failed:\n" + res.stdout + res.stderr)

@@ -39,6 +39,7 @@
 
 #include "../../include/qcd_hip.h"
 #include "qcd_device.hpp"
+#include "qcd_internal.h"
 
 using namespace qcd;
 
@@ -676,6 +677,25 @@ __global__ __launch_bounds__(kBlock) void k_pm_skip(uint4* st, uint32_t n, uint3
     st[i] = s;
 }
 
+// Replay of many heatbath half-steps in one pass: apply a precomputed jump
+// (XOR128: GF(2) matrix columns; PM: multiplier) to every stream.
+__global__ __launch_bounds__(kBlock) void k_xor128_jump_apply(uint4* st, uint32_t n, const uint4* cols)
+{
+    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const uint4 s = st[i];
+    const uint2 lo = xor128_jump_half(s, cols, 0), hi = xor128_jump_half(s, cols, 1);
+    st[i] = make_uint4(lo.x, lo.y, hi.x, hi.y);
+}
+
+__global__ __launch_bounds__(kBlock) void k_pm_jump_apply(uint4* st, uint32_t n, uint32_t ak)
+{
+    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const uint4 s = st[i];
+    st[i] = make_uint4(pm_jump(s.x, ak), pm_jump(s.y, ak), pm_jump(s.z, ak), pm_jump(s.w, ak));
+}
+
 // Reference produce kernels (buffer semantics): quads[i * n + g].
 __global__ __launch_bounds__(kBlock) void k_xor128_produce(uint4* st, uint32_t n, uint32_t samples, float4* quads)
 {
@@ -1300,6 +1320,12 @@ struct qcd_lattice {
     int cyc_hb = 0, cyc_or = 0, cyc_mask = 0;
     bool rng_ready = false;
     bool gid_updates = false;   // INTS = 2: GID known-answer numbers in the heatbath
+    // PRNG::produce calls the reference would have made so far
+    // (PRNG::PRNG_counter, random.cpp:283): 1 for the initial produce, 4 for
+    // a hot start, 1 per heatbath half-step.  Resume replays the streams to
+    // this position (suncl.cpp:5349-5351).
+    uint32_t prng_counter = 0;
+    uint32_t cyc_prn = 0;       // productions in one replay of the captured cycle
     int tune_xcd_map = 2;       // QCD_TUNE_XCD_MAP
     int tune_blocks_per_cu = 0; // QCD_TUNE_BLOCKS_PER_CU (0 = no cap)
     int tune_update_shape = 3;  // QCD_TUNE_UPDATE_SHAPE: wavefronts per 64 links (3 or 1)
@@ -1428,6 +1454,7 @@ int launch_update_rn(qcd_lattice* L, int parity, int mu, int kind, int start_gid
             else return fail(QCD_E_UNSUPPORTED, "generator %d not available for fused updates", L->p.prng);
             L->rng_cur ^= 1;   // the kernel wrote the states into the other buffer
         }
+        ++L->prng_counter;
     }
     HIPCHK(hipGetLastError());
     return QCD_OK;
@@ -1524,6 +1551,11 @@ int ensure_device()
 }
 
 }  // namespace
+
+int qcdhip::set_error(int code, const char* msg)
+{
+    return fail(code, "%s", msg);
+}
 
 // ---------------------------------------------------------------- C ABI
 extern "C" {
@@ -1749,7 +1781,11 @@ int qcd_lattice_init(qcd_lattice* L, int start)
     if (start == QCD_START_HOT) {
         if (!L->rng_ready) return fail(QCD_E_STATE, "the hot start draws from the streams: call qcd_prng_init first");
         L->gid_updates = false;
-        return DISPATCH(L, launch_simple, L, 5, LaunchOpt{});
+        const int rc = DISPATCH(L, launch_simple, L, 5, LaunchOpt{});
+        // the reference produces before each of its four init kernels
+        // (suncl.cpp:5236-5248)
+        if (rc == QCD_OK) L->prng_counter += 4;
+        return rc;
     }
     return fail(QCD_E_ARG, "bad start type %d", start);
 }
@@ -1757,7 +1793,10 @@ int qcd_lattice_init(qcd_lattice* L, int start)
 int qcd_prng_init(qcd_lattice* L)
 {
     if (!L) return fail(QCD_E_ARG, "null lattice");
+    // RANDSERIES 0 takes the time as seed and keeps it as the series, so
+    // checkpoints can replay it (random.cpp:228-232)
     unsigned seed = L->p.randseries ? L->p.randseries : (unsigned)time(nullptr);
+    L->p.randseries = seed;
     if (L->p.prng == QCD_PRNG_RANLUX) {
         // RANLUX streams are always in the reference layout: a produce()
         // fills the quad buffer before every heatbath half-step.
@@ -1769,6 +1808,7 @@ int qcd_prng_init(qcd_lattice* L)
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(L->stream));
         L->rng_ready = true;
+        L->prng_counter = 1;
         return QCD_OK;
     }
     if (L->p.prng != QCD_PRNG_XOR128 && L->p.prng != QCD_PRNG_PM)
@@ -1794,6 +1834,7 @@ int qcd_prng_init(qcd_lattice* L)
     }
     HIPCHK(hipStreamSynchronize(L->stream));
     L->rng_ready = true;
+    L->prng_counter = 1;
     return QCD_OK;
 }
 
@@ -2058,7 +2099,11 @@ int qcd_cycle_capture(qcd_lattice* L, int n_hb, int n_or, int mask)
     }
     HIPCHK(hipStreamSynchronize(L->stream));
     HIPCHK(hipStreamBeginCapture(L->stream, hipStreamCaptureModeThreadLocal));
+    const uint32_t counter0 = L->prng_counter;
     int rc = enqueue_cycle(L, n_hb, n_or, mask, nullptr, nullptr);
+    // capture runs nothing: the productions count when the graph is launched
+    L->cyc_prn = L->prng_counter - counter0;
+    L->prng_counter = counter0;
     hipGraph_t graph = nullptr;
     hipError_t e = hipStreamEndCapture(L->stream, &graph);
     if (rc) {
@@ -2084,7 +2129,10 @@ int qcd_cycle_run(qcd_lattice* L, int n_hb, int n_or, int mask)
 int qcd_cycle_launch(qcd_lattice* L, int repeats)
 {
     if (!L || !L->exec) return fail(QCD_E_STATE, "no captured cycle");
-    for (int r = 0; r < repeats; ++r) HIPCHK(hipGraphLaunch(L->exec, L->stream));
+    for (int r = 0; r < repeats; ++r) {
+        HIPCHK(hipGraphLaunch(L->exec, L->stream));
+        L->prng_counter += L->cyc_prn;
+    }
     return QCD_OK;
 }
 
@@ -2096,7 +2144,10 @@ int qcd_cycle_time(qcd_lattice* L, int repeats, double* ms)
     HIPCHK(hipEventCreate(&b));
     HIPCHK(hipStreamSynchronize(L->stream));
     HIPCHK(hipEventRecord(a, L->stream));
-    for (int r = 0; r < repeats; ++r) HIPCHK(hipGraphLaunch(L->exec, L->stream));
+    for (int r = 0; r < repeats; ++r) {
+        HIPCHK(hipGraphLaunch(L->exec, L->stream));
+        L->prng_counter += L->cyc_prn;
+    }
     HIPCHK(hipEventRecord(b, L->stream));
     HIPCHK(hipEventSynchronize(b));
     float f = 0.f;
@@ -2214,6 +2265,290 @@ int qcd_prng_produce_host(int prng, int ranlux_luxury, unsigned n, unsigned samp
     HIPCHK(hipMemcpy(table, dt, sizeof(uint4) * n, hipMemcpyDeviceToHost));
     (void)hipFree(dt);
     (void)hipFree(dq);
+    return QCD_OK;
+}
+
+// ---------------------------------------------------------------- checkpoints
+// model::lattice_save_state (suncl.cpp:1857-1901) keeps the table and the
+// measurement buffers and resumes the generators by replaying PRNG_counter
+// productions from the seed (suncl.cpp:5349-5351).  The state image below
+// holds everything later updates and measurements read: the table as stored
+// on the device, the current generator states, the records and their
+// indices, the production counter and the start flags.  A lattice restored
+// from it continues bit-identically.
+
+namespace {
+
+constexpr char kStateMagic[8] = {'Q', 'C', 'D', 'S', 'T', 'A', 'T', '1'};
+constexpr size_t kStateHead = 128;   // bytes of the head qcd_lattice_state_save writes
+
+struct ByteWriter {
+    unsigned char* p;
+    void u32(uint32_t v) { memcpy(p, &v, 4); p += 4; }
+    void u64(uint64_t v) { memcpy(p, &v, 8); p += 8; }
+    void raw(const void* s, size_t n) { memcpy(p, s, n); p += n; }
+};
+struct ByteReader {
+    const unsigned char* p;
+    uint32_t u32() { uint32_t v; memcpy(&v, p, 4); p += 4; return v; }
+    uint64_t u64() { uint64_t v; memcpy(&v, p, 8); p += 8; return v; }
+};
+
+size_t rng_state_bytes(const qcd_lattice* L)
+{
+    return (L->p.prng == QCD_PRNG_RANLUX ? 7u : 1u) * sizeof(uint4) * (size_t)L->n_streams;
+}
+
+size_t state_image_bytes(const qcd_lattice* L)
+{
+    return kStateHead + L->table_bytes + rng_state_bytes(L) + sizeof(double) * 7 * (size_t)L->p.max_measurements;
+}
+
+uint64_t f64_bits(double x)
+{
+    uint64_t b;
+    memcpy(&b, &x, 8);
+    return b;
+}
+
+// The parameter fields in image order, with their names for error messages.
+void param_fields(const qcd_params& p, uint64_t v[16])
+{
+    const uint64_t f[16] = {(uint64_t)p.sun,        (uint64_t)p.n[0],          (uint64_t)p.n[1],
+                            (uint64_t)p.n[2],       (uint64_t)p.n[3],          (uint64_t)p.precision,
+                            (uint64_t)p.storage_reals, (uint64_t)p.nhit,       (uint64_t)p.nhitpar,
+                            (uint64_t)p.prng,       (uint64_t)p.ranlux_luxury, (uint64_t)p.stream_mode,
+                            (uint64_t)p.randseries, (uint64_t)p.max_measurements, f64_bits(p.beta), 0};
+    memcpy(v, f, sizeof(f));
+}
+const char* const kParamNames[15] = {"sun",  "n1",      "n2",         "n3",          "n4",
+                                     "precision", "storage_reals", "nhit", "nhitpar", "prng",
+                                     "ranlux_luxury", "stream_mode", "randseries", "max_measurements", "beta"};
+
+// Columns of k applications of the GF(2) map whose column b is cols[b].
+void gf2_apply(const uint4* m, const uint4& v, uint4& out)
+{
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+    uint4 o = make_uint4(0u, 0u, 0u, 0u);
+    for (int b = 0; b < 128; ++b)
+        if ((w[b >> 5] >> (b & 31)) & 1u) {
+            o.x ^= m[b].x;
+            o.y ^= m[b].y;
+            o.z ^= m[b].z;
+            o.w ^= m[b].w;
+        }
+    out = o;
+}
+void gf2_power(const uint4* cols, uint64_t k, uint4* out)
+{
+    std::vector<uint4> r(128), b(cols, cols + 128), t(128);
+    for (int j = 0; j < 128; ++j) {
+        r[j] = make_uint4(0u, 0u, 0u, 0u);
+        ((uint32_t*)&r[j])[j >> 5] = 1u << (j & 31);
+    }
+    while (k) {
+        if (k & 1) {
+            for (int j = 0; j < 128; ++j) gf2_apply(b.data(), r[j], t[j]);
+            r.swap(t);
+        }
+        k >>= 1;
+        if (k) {
+            for (int j = 0; j < 128; ++j) gf2_apply(b.data(), b[j], t[j]);
+            b.swap(t);
+        }
+    }
+    std::copy(r.begin(), r.end(), out);
+}
+
+}  // namespace
+
+int qcd_lattice_get_params(const qcd_lattice* L, qcd_params* out)
+{
+    if (!L || !out) return fail(QCD_E_ARG, "null argument");
+    *out = L->p;
+    return QCD_OK;
+}
+
+int qcd_prng_counter(const qcd_lattice* L, unsigned* counter)
+{
+    if (!L || !counter) return fail(QCD_E_ARG, "null argument");
+    *counter = L->prng_counter;
+    return QCD_OK;
+}
+
+int qcd_lattice_state_bytes(const qcd_lattice* L, size_t* bytes)
+{
+    if (!L || !bytes) return fail(QCD_E_ARG, "null argument");
+    *bytes = state_image_bytes(L);
+    return QCD_OK;
+}
+
+int qcd_lattice_state_save(qcd_lattice* L, void* buf, size_t bytes)
+{
+    if (!L || !buf) return fail(QCD_E_ARG, "null argument");
+    if (bytes != state_image_bytes(L))
+        return fail(QCD_E_ARG, "state buffer of %zu bytes, the lattice needs %zu", bytes, state_image_bytes(L));
+    HIPCHK(hipStreamSynchronize(L->stream));
+    int idx[4];
+    HIPCHK(hipMemcpy(idx, L->rec_index, sizeof(idx), hipMemcpyDeviceToHost));
+    ByteWriter w{(unsigned char*)buf};
+    w.raw(kStateMagic, 8);
+    w.u32((uint32_t)kStateHead);
+    uint64_t f[16];
+    param_fields(L->p, f);
+    for (int k = 0; k < 14; ++k) w.u32((uint32_t)f[k]);
+    w.u64(f[14]);
+    w.u32(L->prng_counter);
+    w.u32(L->rng_ready ? 1u : 0u);
+    w.u32(L->gid_updates ? 1u : 0u);
+    w.u32((uint32_t)L->wil_R);
+    w.u32((uint32_t)L->wil_T);
+    for (int k = 0; k < 4; ++k) w.u32((uint32_t)idx[k]);
+    w.u64(L->table_bytes);
+    w.u64(rng_state_bytes(L));
+    if ((size_t)(w.p - (unsigned char*)buf) != kStateHead) return fail(QCD_E_STATE, "internal: state head size");
+    HIPCHK(hipMemcpy(w.p, L->table, L->table_bytes, hipMemcpyDeviceToHost));
+    w.p += L->table_bytes;
+    const void* rng = (L->p.prng == QCD_PRNG_RANLUX) ? (const void*)L->rl_state : (const void*)L->rng_buf[L->rng_cur];
+    HIPCHK(hipMemcpy(w.p, rng, rng_state_bytes(L), hipMemcpyDeviceToHost));
+    w.p += rng_state_bytes(L);
+    const size_t m = (size_t)L->p.max_measurements;
+    HIPCHK(hipMemcpy(w.p, L->plq_rec, sizeof(double) * 2 * m, hipMemcpyDeviceToHost));
+    w.p += sizeof(double) * 2 * m;
+    HIPCHK(hipMemcpy(w.p, L->poly_rec, sizeof(double) * 4 * m, hipMemcpyDeviceToHost));
+    w.p += sizeof(double) * 4 * m;
+    HIPCHK(hipMemcpy(w.p, L->wil_rec, sizeof(double) * m, hipMemcpyDeviceToHost));
+    return QCD_OK;
+}
+
+int qcd_lattice_state_load(qcd_lattice* L, const void* buf, size_t bytes)
+{
+    if (!L || !buf) return fail(QCD_E_ARG, "null argument");
+    if (bytes != state_image_bytes(L))
+        return fail(QCD_E_ARG, "state image of %zu bytes, this lattice needs %zu", bytes, state_image_bytes(L));
+    ByteReader r{(const unsigned char*)buf};
+    if (memcmp(r.p, kStateMagic, 8) != 0) return fail(QCD_E_ARG, "not a lattice state image");
+    r.p += 8;
+    if (r.u32() != (uint32_t)kStateHead) return fail(QCD_E_ARG, "state image head of another layout");
+    uint64_t f[16];
+    param_fields(L->p, f);
+    for (int k = 0; k < 15; ++k) {
+        const uint64_t v = (k < 14) ? r.u32() : r.u64();
+        if (v != (k < 14 ? (uint64_t)(uint32_t)f[k] : f[k]))
+            return fail(QCD_E_ARG, "state image has another %s than this lattice", kParamNames[k]);
+    }
+    const uint32_t counter = r.u32(), ready = r.u32(), gid = r.u32(), wr = r.u32(), wt = r.u32();
+    int idx[4];
+    for (int k = 0; k < 4; ++k) idx[k] = (int)r.u32();
+    if (r.u64() != L->table_bytes || r.u64() != rng_state_bytes(L))
+        return fail(QCD_E_ARG, "state image sizes do not match this lattice");
+    HIPCHK(hipStreamSynchronize(L->stream));
+    HIPCHK(hipMemcpy(L->table, r.p, L->table_bytes, hipMemcpyHostToDevice));
+    r.p += L->table_bytes;
+    void* rng = (L->p.prng == QCD_PRNG_RANLUX) ? (void*)L->rl_state : (void*)L->rng_buf[L->rng_cur];
+    HIPCHK(hipMemcpy(rng, r.p, rng_state_bytes(L), hipMemcpyHostToDevice));
+    r.p += rng_state_bytes(L);
+    const size_t m = (size_t)L->p.max_measurements;
+    HIPCHK(hipMemcpy(L->plq_rec, r.p, sizeof(double) * 2 * m, hipMemcpyHostToDevice));
+    r.p += sizeof(double) * 2 * m;
+    HIPCHK(hipMemcpy(L->poly_rec, r.p, sizeof(double) * 4 * m, hipMemcpyHostToDevice));
+    r.p += sizeof(double) * 4 * m;
+    HIPCHK(hipMemcpy(L->wil_rec, r.p, sizeof(double) * m, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(L->rec_index, idx, sizeof(idx), hipMemcpyHostToDevice));
+    L->prng_counter = counter;
+    L->rng_ready = ready != 0;
+    L->gid_updates = gid != 0;
+    L->wil_R = (int)wr;
+    L->wil_T = (int)wt;
+    return QCD_OK;
+}
+
+int qcd_write_measurements(qcd_lattice* L, const double* plq, const double* poly, const double* wil, int count_plq,
+                           int count_poly, int count_wil)
+{
+    if (!L) return fail(QCD_E_ARG, "null lattice");
+    if (count_plq < 0 || count_poly < 0 || count_wil < 0) return fail(QCD_E_ARG, "negative record count");
+    if ((count_plq && !plq) || (count_poly && !poly) || (count_wil && !wil))
+        return fail(QCD_E_ARG, "null record array with a non-zero count");
+    const int m = L->p.max_measurements;
+    HIPCHK(hipStreamSynchronize(L->stream));
+    const int np = count_plq < m ? count_plq : m, nq = count_poly < m ? count_poly : m, nw = count_wil < m ? count_wil : m;
+    if (np) HIPCHK(hipMemcpy(L->plq_rec, plq, sizeof(double) * 2 * np, hipMemcpyHostToDevice));
+    if (nq) HIPCHK(hipMemcpy(L->poly_rec, poly, sizeof(double) * 4 * nq, hipMemcpyHostToDevice));
+    if (nw) HIPCHK(hipMemcpy(L->wil_rec, wil, sizeof(double) * nw, hipMemcpyHostToDevice));
+    const int idx[4] = {count_plq, count_poly, count_wil, 0};
+    HIPCHK(hipMemcpy(L->rec_index, idx, sizeof(idx), hipMemcpyHostToDevice));
+    return QCD_OK;
+}
+
+int qcd_prng_replay(qcd_lattice* L, unsigned counter, int start)
+{
+    if (!L) return fail(QCD_E_ARG, "null lattice");
+    if (start != QCD_START_HOT && start != QCD_START_COLD && start != QCD_START_GID)
+        return fail(QCD_E_ARG, "bad start type %d", start);
+    if (counter == 0) {
+        L->rng_ready = false;
+        L->prng_counter = 0;
+        return QCD_OK;
+    }
+    int rc = qcd_prng_init(L);   // counter 1: the initial produce
+    if (rc) return rc;
+    uint32_t left = counter - 1;
+    const bool ranlux = (L->p.prng == QCD_PRNG_RANLUX);
+    const uint32_t nb = align_up(L->n_streams, kBlock) / kBlock;
+    if (start == QCD_START_HOT && left > 0) {
+        // the hot start's draw: 8 links x (3 or 1) quads per stream
+        if (left < 4) return fail(QCD_E_ARG, "counter %u cannot follow a hot start (it takes 4 productions)", counter);
+        const uint32_t q = (L->p.sun == 3) ? 24u : 8u;
+        if (ranlux) {
+            launch_ranlux_produce(L->p.ranlux_luxury, L->rl_state, L->n_streams, q, nullptr, L->stream);
+        } else {
+            const uint32_t nv = L->g.vh, nbv = align_up(nv, kBlock) / kBlock;
+            if (L->p.prng == QCD_PRNG_XOR128)
+                hipLaunchKernelGGL(k_xor128_skip, dim3(nbv), dim3(kBlock), 0, L->stream, L->rng_buf[L->rng_cur], nv, q);
+            else
+                hipLaunchKernelGGL(k_pm_skip, dim3(nbv), dim3(kBlock), 0, L->stream, L->rng_buf[L->rng_cur], nv, q);
+        }
+        HIPCHK(hipGetLastError());
+        left -= 4;
+    }
+    if (left > 0) {
+        if (ranlux) {
+            for (uint32_t k = 0; k < left; ++k)
+                launch_ranlux_produce(L->p.ranlux_luxury, L->rl_state, L->n_streams, L->s_quads, nullptr, L->stream);
+            HIPCHK(hipGetLastError());
+        } else if (L->p.stream_mode != QCD_STREAM_REFERENCE) {
+            return fail(QCD_E_UNSUPPORTED,
+                        "fast-layout streams advance by the numbers consumed: they resume from a state image only");
+        } else if (L->p.prng == QCD_PRNG_XOR128) {
+            uint4 cols[128], pw[128];
+            HIPCHK(hipMemcpy(cols, L->jump_cols, sizeof(cols), hipMemcpyDeviceToHost));
+            gf2_power(cols, left, pw);
+            uint4* d = nullptr;
+            HIPCHK(hipMalloc((void**)&d, sizeof(pw)));
+            if (hipMemcpy(d, pw, sizeof(pw), hipMemcpyHostToDevice) != hipSuccess) {
+                (void)hipFree(d);
+                return fail(QCD_E_HIP, "jump matrix upload failed");
+            }
+            hipLaunchKernelGGL(k_xor128_jump_apply, dim3(nb), dim3(kBlock), 0, L->stream, L->rng_buf[L->rng_cur],
+                               L->n_streams, (const uint4*)d);
+            const hipError_t e1 = hipGetLastError(), e2 = hipStreamSynchronize(L->stream);
+            (void)hipFree(d);
+            if (e1 != hipSuccess || e2 != hipSuccess)
+                return fail(QCD_E_HIP, "stream jump: %s", hipGetErrorString(e1 != hipSuccess ? e1 : e2));
+        } else {
+            uint32_t ak = 1u, b = L->pm_ak;
+            for (uint32_t k = left; k; k >>= 1) {
+                if (k & 1u) ak = pm_mulmod(ak, b);
+                b = pm_mulmod(b, b);
+            }
+            hipLaunchKernelGGL(k_pm_jump_apply, dim3(nb), dim3(kBlock), 0, L->stream, L->rng_buf[L->rng_cur],
+                               L->n_streams, ak);
+            HIPCHK(hipGetLastError());
+        }
+    }
+    HIPCHK(hipStreamSynchronize(L->stream));
+    L->prng_counter = counter;
     return QCD_OK;
 }
 

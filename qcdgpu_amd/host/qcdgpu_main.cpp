@@ -22,7 +22,13 @@
 // (SINGLE/DOUBLE, the reference sets it in QCDGPU.cpp), NOR (overrelaxation
 // sweeps after every heatbath sweep, default 0), STORAGE (18 or 12 reals per
 // SU(3) link), STREAMS (REFERENCE or FAST generator layout), LUXURY is
-// carried in the PRNG name (RANLUX0..RANLUX4) as in the reference.
+// carried in the PRNG name (RANLUX0..RANLUX4) as in the reference; INIT,
+// STATEFILE, SAVESTATE and SAVEEVERY set what QCDGPU.cpp hardcodes for
+// checkpoints (INIT, fload, turnoff_config_save,
+// write_lattice_state_every_secs): with INIT = 0 the lattice, its generator
+// position and the counters come from the .qcg file, which overrides the
+// keys it holds, and the run continues where the file left it
+// (suncl.cpp:5345-5351).
 // GETWILSON / WILSONR / WILSONT measure the R x T Wilson loop.  Keys of
 // features outside this path (F_munu, twisted boundary conditions, BIGLAT)
 // are rejected when they ask for the feature.
@@ -57,6 +63,19 @@ struct Settings {
     bool double_precision = true;    // QCDGPU.cpp: model_precision_double
     int nor = 0, storage = 0;        // storage 0: 18 reals (SU(3)) / 4 (SU(2))
     bool fast_streams = false;
+    // checkpoints (QCDGPU.cpp:46,180,203,215): INIT 0 continues the run in
+    // STATEFILE (relative to OUTPUTPATH unless absolute); SAVESTATE writes
+    // .qcg files after the start measurement, every SAVEEVERY seconds (checked
+    // after each thermalisation sweep and each measurement) and at the end.
+    int init = 1;
+    std::string statefile = "su3-14-Aug-20-00-00-00.qcg";
+    bool save_state = false;
+    double save_every = 2.0;
+    // STOPAFTER k > 0: end the process (exit status 3) inside the iteration
+    // after measurement k, once its sweeps ran and before its measurement,
+    // leaving what a process killed there leaves: the checkpoints written so
+    // far and work that no checkpoint holds.  All device work completes.
+    int stop_after = 0;
 };
 
 std::string trim(const std::string& s)
@@ -115,7 +134,12 @@ bool apply(Settings& st, const std::string& key, const std::string& val)
             fprintf(stderr, "PRECISION must be SINGLE or DOUBLE (got '%s')\n", val.c_str());
             return false;
         }
-    } else if (key == "NOR") st.nor = (int)iv;
+    } else if (key == "INIT") st.init = (int)iv;
+    else if (key == "STATEFILE") st.statefile = val;
+    else if (key == "SAVESTATE") st.save_state = iv != 0;
+    else if (key == "SAVEEVERY") st.save_every = fv;
+    else if (key == "STOPAFTER") st.stop_after = (int)iv;
+    else if (key == "NOR") st.nor = (int)iv;
     else if (key == "STORAGE") st.storage = (int)iv;
     else if (key == "STREAMS") {
         const std::string u = upper(val);
@@ -187,6 +211,49 @@ bool to_params(const Settings& st, qcd_params& p, std::string& why)
     return true;
 }
 
+// lattice_load_bin_header (suncl.cpp:1995-2041): the state file's values
+// replace the keys it holds.
+void from_state(Settings& st, const qcd_params& p, const qcd_run_info& r)
+{
+    st.group = p.sun;
+    st.nd = 4;
+    for (int k = 0; k < 4; ++k) st.n[k] = p.n[k];
+    st.double_precision = (p.precision == QCD_FP64);
+    st.storage = p.storage_reals;
+    st.beta = p.beta;
+    st.phi = st.omega = 0.0;
+    st.fmunu = false;
+    st.nhit = p.nhit;
+    st.nhitpar = p.nhitpar;
+    st.prng = (p.prng == QCD_PRNG_XOR128) ? "XOR128"
+              : (p.prng == QCD_PRNG_PM)   ? "PM"
+                                          : "RANLUX" + std::to_string(p.ranlux_luxury);
+    st.randseries = p.randseries;
+    st.fast_streams = (p.stream_mode == QCD_STREAM_FAST);
+    st.ints = (int)r.start;
+    st.nav = (int)r.nav;
+    st.niter = (int)r.niter;
+    st.iter = (int)r.iter;
+    st.pl_level = (int)r.pl_level;
+    st.wilson = r.get_wilson != 0;
+    st.wilson_r = (int)r.wilson_r;
+    st.wilson_t = (int)r.wilson_t;
+}
+
+// <OUTPUTPATH><su3-|su2-><yy>-<Mon>-<dd>-<hh>-<mm>-<ss><ext>, the names of
+// lattice_write_results and lattice_save_state (suncl.cpp:1686-1690,1881-1884).
+std::string stamped_name(const Settings& st, const char* ext)
+{
+    const time_t t = time(nullptr);
+    char now[32];
+    strncpy(now, ctime(&t), sizeof(now) - 1);
+    now[sizeof(now) - 1] = 0;
+    char name[1024];
+    snprintf(name, sizeof(name), "%s%s%.2s-%.3s-%.2s-%.2s-%.2s-%.2s%s", st.outputpath.c_str(),
+             st.group == 3 ? "su3-" : "su2-", now + 22, now + 4, now + 8, now + 11, now + 14, now + 17, ext);
+    return name;
+}
+
 void print_params(const Settings& st)
 {
     printf("GROUP=%d\nND=%d\nL1=%d\nL2=%d\nL3=%d\nL4=%d\n", st.group, st.nd, st.n[0], st.n[1], st.n[2], st.n[3]);
@@ -198,6 +265,8 @@ void print_params(const Settings& st)
            st.storage ? st.storage : (st.group == 3 ? 18 : 4), st.fast_streams ? "FAST" : "REFERENCE");
     printf("OUTPUTPATH=%s\nFINISHPATH=%s\nPLATFORM=%d\nDEVICE=%d\n", st.outputpath.c_str(), st.finishpath.c_str(),
            st.platform, st.device);
+    printf("INIT=%d\nSTATEFILE=%s\nSAVESTATE=%d\nSAVEEVERY=%.3f\n", st.init, st.statefile.c_str(),
+           st.save_state ? 1 : 0, st.save_every);
 }
 
 #define CHECK(call)                                                                   \
@@ -236,9 +305,24 @@ int main(int argc, char** argv)
         else file = argv[k];
     }
     if (file && !read_init_file(file, st)) return 1;
-    qcd_params p;
-    std::string why;
-    const bool ok = to_params(st, p, why);
+    qcd_params p, p_file;
+    qcd_run_info run;
+    memset(&run, 0, sizeof(run));
+    std::string why, state_in;
+    if (st.init == 0) {
+        // lattice_load_state with LOAD_state 0: header only (suncl.cpp:2057-2059)
+        state_in = (!st.statefile.empty() && st.statefile[0] == '/') ? st.statefile : st.outputpath + st.statefile;
+        if (qcd_qcg_read_header(state_in.c_str(), &p_file, &run, nullptr) != QCD_OK) {
+            fprintf(stderr, "%s\n", qcd_last_error());
+            return 2;
+        }
+        from_state(st, p_file, run);
+    }
+    bool ok = to_params(st, p, why);
+    if (ok && st.init == 0) {
+        p.max_measurements = p_file.max_measurements;   // the file's record capacity
+        if (p.max_measurements < st.iter) p.max_measurements = st.iter;
+    }
     if (print_only) {
         print_params(st);
         if (!ok) printf("UNSUPPORTED=%s\n", why.c_str());
@@ -263,28 +347,83 @@ int main(int argc, char** argv)
     qcd_lattice* lat = nullptr;
     CHECK(qcd_lattice_create(&p, &lat));
     const int start = st.ints == 1 ? QCD_START_COLD : (st.ints == 2 ? QCD_START_GID : QCD_START_HOT);
-    if (start == QCD_START_HOT) {   // the Haar start draws from the streams
-        CHECK(qcd_prng_init(lat));
-        CHECK(qcd_lattice_init(lat, start));
-    } else {
-        CHECK(qcd_lattice_init(lat, start));
-        if (start != QCD_START_GID) CHECK(qcd_prng_init(lat));
-    }
     const int mask = QCD_MEAS_PLAQUETTE | (st.pl_level > 0 ? QCD_MEAS_POLYAKOV : 0) | (st.wilson ? QCD_MEAS_WILSON : 0);
     if (st.wilson) CHECK(qcd_lattice_set_wilson(lat, st.wilson_r, st.wilson_t));
     auto sweep = [&]() {
         CHECK(qcd_sweep(lat, QCD_HEATBATH));
         for (int k = 0; k < st.nor; ++k) CHECK(qcd_sweep(lat, QCD_OVERRELAX));
     };
+    // lattice_save_state at the reference's points (suncl.cpp:5342,5383-5387,
+    // 5487-5492,5497-5499); the counters say where a resumed run continues.
+    std::string last_state;
+    auto save_state = [&]() {
+        qcd_run_info ri = run;
+        ri.init = (unsigned)st.init;
+        ri.start = (unsigned)st.ints;
+        ri.nav = (unsigned)st.nav;
+        ri.niter = (unsigned)st.niter;
+        ri.iter = (unsigned)st.iter;
+        ri.wilson_r = (unsigned)st.wilson_r;
+        ri.wilson_t = (unsigned)st.wilson_t;
+        ri.pl_level = (unsigned)st.pl_level;
+        ri.get_actions = 1;
+        ri.get_plaquettes = 1;
+        ri.get_wilson = st.wilson ? 1u : 0u;
+        last_state = stamped_name(st, ".qcg");
+        CHECK(qcd_qcg_write(lat, last_state.c_str(), &ri));
+    };
+    auto t_save = std::chrono::steady_clock::now();
+    auto save_due = [&]() {
+        if (!st.save_state) return;
+        const auto now = std::chrono::steady_clock::now();
+        if (std::chrono::duration<double>(now - t_save).count() > st.save_every) {
+            save_state();
+            t_save = std::chrono::steady_clock::now();
+        }
+    };
     const auto w0 = std::chrono::steady_clock::now();
-    CHECK(qcd_measure(lat, mask));   // measurement 0: the start configuration
-    for (int i = 0; i < st.nav; ++i) sweep();
-    for (int i = 1; i < st.iter; ++i) {
+    if (st.init == 0) {
+        CHECK(qcd_qcg_read(lat, state_in.c_str(), &run));
+        printf("continuing %s: thermalisation %u of %d, measurement %u of %d\n", state_in.c_str(), run.nav_counter,
+               st.nav, run.iter_counter, st.iter);
+    } else {
+        if (start == QCD_START_HOT) {   // the Haar start draws from the streams
+            CHECK(qcd_prng_init(lat));
+            CHECK(qcd_lattice_init(lat, start));
+        } else {
+            CHECK(qcd_lattice_init(lat, start));
+            if (start != QCD_START_GID) CHECK(qcd_prng_init(lat));
+        }
+        CHECK(qcd_measure(lat, mask));   // measurement 0: the start configuration
+        run.nav_counter = 0;
+        run.iter_counter = 1;
+        if (st.save_state) save_state();
+    }
+    const int nav_start = (int)run.nav_counter, iter_start = (int)run.iter_counter;
+    for (int i = nav_start; i < st.nav; ++i) {
+        sweep();
+        ++run.nav_counter;
+        save_due();
+    }
+    for (int i = iter_start; i < st.iter; ++i) {
         for (int j = 0; j < st.niter; ++j) sweep();
+        if (st.stop_after > 0 && (int)run.iter_counter == st.stop_after + 1) {
+            CHECK(qcd_lattice_sync(lat));
+            printf("stopped before measurement %u\n", run.iter_counter);
+            CHECK(qcd_lattice_destroy(lat));
+            CHECK(qcd_device_close());
+            return 3;
+        }
         CHECK(qcd_measure(lat, mask));
+        ++run.iter_counter;
+        save_due();
     }
     CHECK(qcd_lattice_sync(lat));
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - w0).count();
+    if (st.save_state) {
+        save_state();
+        printf(" state: %s\n", last_state.c_str());
+    }
 
     std::vector<double> plq(2 * (size_t)st.iter), poly(4 * (size_t)st.iter), wil((size_t)st.iter);
     int np = 0, nq = 0, nw = 0;
@@ -345,7 +484,8 @@ int main(int argc, char** argv)
         }
     printf("\n %-18s %-22s %-22s\n", "quantity", "mean", "variance");
     for (auto* c : cols) printf(" %-18s % 16.13e % 16.13e\n", c->name, c->mean, c->variance);
-    const double sweeps = (double)st.nav + (double)(st.iter - 1) * st.niter;
+    // sweeps run by this process (a continued run starts at its counters)
+    const double sweeps = (double)(st.nav - nav_start) + (double)(st.iter - iter_start) * st.niter;
     printf("\n sweeps %.0f (%d overrelaxation sweeps each), %.3f s, %.1f sweeps/s, %.3e link updates/s\n", sweeps,
            st.nor, secs, sweeps / secs, sweeps * (1 + st.nor) * 4 * V / secs);
 
@@ -354,13 +494,10 @@ int main(int argc, char** argv)
     char timeend[32];
     strncpy(timeend, ctime(&t1), sizeof(timeend) - 1);
     timeend[sizeof(timeend) - 1] = 0;
-    char fname[1024];
-    snprintf(fname, sizeof(fname), "%s%s%.2s-%.3s-%.2s-%.2s-%.2s-%.2s.txt", st.outputpath.c_str(),
-             st.group == 3 ? "su3-" : "su2-", timeend + 22, timeend + 4, timeend + 8, timeend + 11, timeend + 14,
-             timeend + 17);
-    FILE* out = fopen(fname, "w");
+    const std::string fname = stamped_name(st, ".txt");
+    FILE* out = fopen(fname.c_str(), "w");
     if (!out) {
-        fprintf(stderr, "cannot write %s: %s\n", fname, strerror(errno));
+        fprintf(stderr, "cannot write %s: %s\n", fname.c_str(), strerror(errno));
         return 1;
     }
     fprintf(out, " SU(%d) lattice %dx%dx%dx%d, beta = %.13e\n", st.group, st.n[0], st.n[1], st.n[2], st.n[3], st.beta);
@@ -385,6 +522,6 @@ int main(int argc, char** argv)
         fprintf(out, "\n");
     }
     fclose(out);
-    printf(" results: %s\n", fname);
+    printf(" results: %s\n", fname.c_str());
     return 0;
 }

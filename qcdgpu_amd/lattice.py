@@ -15,6 +15,8 @@ import numpy as np
 from . import _abi
 from ._abi import check, lib
 
+_START = {"hot": _abi.START_HOT, "cold": _abi.START_COLD, "gid": _abi.START_GID}
+
 
 class Lattice:
     def __init__(
@@ -52,6 +54,30 @@ class Lattice:
         self._h = ctypes.c_void_p()
         self._owned = True
         check(lib().qcd_lattice_create(ctypes.byref(p), ctypes.byref(self._h)))
+
+    @classmethod
+    def from_params(cls, params):
+        """A lattice created from a filled _abi.Params (e.g. a .qcg header)."""
+        self = cls.__new__(cls)
+        p = _abi.Params()
+        ctypes.pointer(p)[0] = params
+        self.params = p
+        self.sun = p.sun
+        self.dims = tuple(int(p.n[k]) for k in range(4))
+        self.volume = int(np.prod(self.dims))
+        self._h = ctypes.c_void_p()
+        self._owned = True
+        check(lib().qcd_lattice_create(ctypes.byref(p), ctypes.byref(self._h)))
+        return self
+
+    @classmethod
+    def from_qcg(cls, path):
+        """model::lattice_init + lattice_load_state for INIT = 0: a lattice
+        with the parameters of the state file's header, loaded from it.
+        Returns (lattice, run counters)."""
+        params, _, _ = read_qcg_header(path)
+        lat = cls.from_params(params)
+        return lat, lat.load_qcg(path)
 
     @classmethod
     def _view(cls, handle, params):
@@ -225,6 +251,72 @@ class Lattice:
         check(lib().qcd_lattice_table_bytes(self._h, ctypes.byref(b)))
         return b.value
 
+    # -- checkpoints (model::lattice_save_state / lattice_load_state) ----------
+    def prng_counter(self):
+        """PRNG::produce calls the reference would have made so far."""
+        c = ctypes.c_uint()
+        check(lib().qcd_prng_counter(self._h, ctypes.byref(c)))
+        return c.value
+
+    def prng_replay(self, counter, start="cold"):
+        """Re-seed and advance the streams to production `counter` of a run
+        that began with `start` (suncl.cpp:5349-5351)."""
+        check(lib().qcd_prng_replay(self._h, int(counter), _START[start] if isinstance(start, str) else int(start)))
+
+    def save_state(self):
+        """Exact state image (uint8 array): table, streams, records, counters."""
+        n = ctypes.c_size_t()
+        check(lib().qcd_lattice_state_bytes(self._h, ctypes.byref(n)))
+        a = np.empty(n.value, dtype=np.uint8)
+        check(lib().qcd_lattice_state_save(self._h, a.ctypes.data_as(ctypes.c_void_p), a.size))
+        return a
+
+    def load_state(self, image):
+        a = np.ascontiguousarray(np.frombuffer(image, dtype=np.uint8))
+        check(lib().qcd_lattice_state_load(self._h, a.ctypes.data_as(ctypes.c_void_p), a.size))
+
+    def write_measurements(self, plq=None, poly=None, wilson=None):
+        """Replace the raw records (plq: (n, 2), poly: (n, 4), wilson: (n,))."""
+        D = ctypes.POINTER(ctypes.c_double)
+        arrs, counts = [], []
+        for x, w in ((plq, 2), (poly, 4), (wilson, 1)):
+            if x is None:
+                arrs.append(None)
+                counts.append(0)
+            else:
+                a = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, w)
+                arrs.append(a)
+                counts.append(a.shape[0])
+        ptr = [a.ctypes.data_as(D) if a is not None and a.size else None for a in arrs]
+        check(lib().qcd_write_measurements(self._h, ptr[0], ptr[1], ptr[2], *counts))
+
+    def save_qcg(self, path, **run):
+        """Write the reference's .qcg state file (suncl.cpp:1857-1993) plus
+        the exact state image.  `run` sets qcd_run_info fields (init, start,
+        nav, nav_counter, niter, iter, iter_counter, wilson_r, wilson_t,
+        pl_level, get_actions, get_plaquettes, get_wilson); by default ITER is
+        the record capacity and ITER_counter the number of plaquette records."""
+        plq, _ = self.raw_measurements()
+        info = _abi.RunInfo()
+        defaults = dict(init=1, start="cold", nav=0, nav_counter=0, niter=1, iter=self.params.max_measurements,
+                        iter_counter=len(plq), wilson_r=7, wilson_t=2, pl_level=3, get_actions=1,
+                        get_plaquettes=1, get_wilson=0)
+        unknown = set(run) - set(defaults) - {"prng_counter"}
+        if unknown:
+            raise ValueError("unknown run fields %s" % sorted(unknown))
+        defaults.update(run)
+        if isinstance(defaults["start"], str):
+            defaults["start"] = _START[defaults["start"]]
+        for k, v in defaults.items():
+            setattr(info, k, int(v))
+        check(lib().qcd_qcg_write(self._h, str(path).encode(), ctypes.byref(info)))
+
+    def load_qcg(self, path):
+        """Load a .qcg file into this lattice; returns the header's run counters."""
+        info = _abi.RunInfo()
+        check(lib().qcd_qcg_read(self._h, str(path).encode(), ctypes.byref(info)))
+        return info.as_dict()
+
     # -- the reference run loop -------------------------------------------------
     def simulate(self, nav=0, iters=2, niter=1, n_or=0, start="cold", wilson=False):
         """model::lattice_simulate: start, measure, NAV sweeps, then ITER-1
@@ -252,6 +344,17 @@ class Lattice:
             self.measure(wilson=wilson)
         self.sync()
         return self.measurements()
+
+
+def read_qcg_header(path):
+    """model::lattice_load_bin_header (suncl.cpp:1995-2041): the lattice
+    parameters and run counters of a .qcg file, and whether it carries the
+    exact state image.  Reads the file only (no device)."""
+    p = _abi.Params()
+    info = _abi.RunInfo()
+    ex = ctypes.c_int()
+    check(lib().qcd_qcg_read_header(str(path).encode(), ctypes.byref(p), ctypes.byref(info), ctypes.byref(ex)))
+    return p, info.as_dict(), bool(ex.value)
 
 
 def device_info():

@@ -1,11 +1,21 @@
-"""C2: one large lattice, SU(3) 64^4, beta 6.0, cold start, heatbath +
-overrelaxation as the headline workload, run in FP32 and in FP64 link storage
-from the identical start and the identical reference-layout XOR128 streams.
-Reports the per-link deviation of the FP32 run from the FP64 run after 1 and
-10 sweeps (max, median, 99th percentile of max |dU| over a link's entries)
-and the number of links whose Kennedy-Pendleton accept decision flipped
-(|dU| = O(1)), plus plaquette and Polyakov loop of both runs.  This is a
-report: the assertions only bound what rounding alone must satisfy."""
+"""C2: one large lattice, SU(3) 64^4, beta 6.0, cold start, run in FP32 and
+in FP64 link storage from the identical start and the identical
+reference-layout XOR128 streams.
+
+Reports the per-link deviation of the FP32 run from the FP64 run (max,
+median, 99th percentile of max |dU| over a link's entries) and the number
+of links that differ by O(1) (|dU| > 1e-3), plus plaquette and Polyakov
+loop of both runs, at three stages: after one heatbath sweep, after the
+first full cycle (1 heatbath + 4 overrelaxation sweeps) and after 10
+cycles.  O(1) differences start where a Kennedy-Pendleton accept decision
+flips under rounding; every later update whose staple holds a flipped link
+then draws from a different distribution (heatbath) or reflects about a
+different staple (overrelaxation), so they spread from there.
+
+This is a report.  The assertions bound what rounding alone must satisfy
+(median and 99th percentile after one heatbath sweep, plaquette agreement)
+and keep the O(1) set a small fraction, which a systematic FP32 error would
+not."""
 import numpy as np
 import pytest
 
@@ -38,30 +48,34 @@ def _deviation(a, b, chunk=1 << 20):
 def test_c2_fp32_vs_fp64_deviation():
     print("[case] C2 64^4 FP32 vs FP64", flush=True)
     lat32, lat64 = _lattice("fp32"), _lattice("fp64")
-    done = 0
+    stages = [("1 heatbath sweep", 1, 0), ("1 cycle (1 HB + 4 OR)", 0, 4), ("10 cycles", 9, 36)]
     try:
-        for k, target in enumerate((1, 10)):
-            while done < target:
-                for lat in (lat32, lat64):
-                    lat.sweep("hb")
-                    for _ in range(4):
-                        lat.sweep("or")
-                done += 1
+        for k, (name, n_hb, n_or) in enumerate(stages):
             for lat in (lat32, lat64):
+                if k < 2:
+                    for _ in range(n_hb):
+                        lat.sweep("hb")
+                    for _ in range(n_or):
+                        lat.sweep("or")
+                else:
+                    for _ in range(9):
+                        lat.sweep("hb")
+                        for _ in range(4):
+                            lat.sweep("or")
                 lat.measure()
             d = _deviation(lat32.export_links(), lat64.export_links())
-            flips = int((d > 1e-3).sum())
+            big = int((d > 1e-3).sum())
             q50, q99 = np.percentile(d, [50, 99])
             m32, m64 = lat32.measurements(), lat64.measurements()
-            print("  after %2d sweep(s): max %.3e  median %.3e  p99 %.3e  flipped links %d of %d"
-                  % (target, d.max(), q50, q99, flips, d.size))
+            print("  after %-22s: max %.3e  median %.3e  p99 %.3e  |dU| > 1e-3 on %d of %d links"
+                  % (name, d.max(), q50, q99, big, d.size))
             print("    plaquette fp32 %.10f fp64 %.10f | Polyakov fp32 %+.3e fp64 %+.3e"
                   % (m32["plq_total"][k] / 3, m64["plq_total"][k] / 3, m32["polyakov_re"][k], m64["polyakov_re"][k]),
                   flush=True)
-            if target == 1:
-                # one sweep: FP32 rounding of O(1e-7) per link, flips are rare
-                assert q50 < 1e-5 and q99 < 1e-4
-                assert flips < 1e-4 * d.size
+            if k == 0:
+                assert q50 < 1e-5 and q99 < 1e-4   # FP32 rounding, O(1e-7) per operation
+            if k <= 1:
+                assert big < 1e-2 * d.size
             assert abs(m32["plq_total"][k] - m64["plq_total"][k]) < 1e-3
             del d
     finally:
