@@ -646,6 +646,27 @@ __device__ __forceinline__ uint2 xor128_jump_half(uint4 s, const uint4* cols_g, 
 #endif
 }
 
+// The same jump four state bits at a time: tab[16 c + e] (c = 0..31, e =
+// 0..15) is the XOR of the columns of bits 4c .. 4c+3 selected by e, so the
+// jumped state is the XOR of 32 table entries, one per nibble of the state.
+// tab is the table staged in LDS; words 2*half and 2*half+1 as above.
+constexpr int kJumpCols = 128;   // uint4 columns
+constexpr int kJumpTab = 512;    // uint4 table entries after the columns
+__host__ __device__ __forceinline__ uint2 xor128_jump_tab_half(uint4 s, const uint4* tab, int half)
+{
+    const uint32_t w[4] = {s.x, s.y, s.z, s.w};
+    const uint2* t = reinterpret_cast<const uint2*>(tab) + half;
+    uint32_t oa = 0u, ob = 0u;
+#pragma unroll
+    for (int c = 0; c < 32; ++c) {
+        const uint32_t e = (w[c >> 3] >> (4 * (c & 7))) & 15u;
+        const uint2 v = t[2 * (16 * c + (int)e)];
+        oa ^= v.x;
+        ob ^= v.y;
+    }
+    return make_uint2(oa, ob);
+}
+
 // Park-Miller (Schrage form) for 0 < s < m is s -> 16807 s mod m, so k steps
 // are one multiplication by a_k = 16807^k mod m; the Schrage form maps 0 to m
 // and keeps m, which the jump reproduces.

@@ -510,6 +510,15 @@ __device__ __forceinline__ void update3(const UpdArgs& a, uint32_t blk, uint32_t
     if constexpr (kStream) {
         if ((w == 0 && active) || (w > 0 && a.s_quads)) rs = a.rng[i];
     }
+    // XOR128 jump table (8 KiB) staged in LDS by the whole workgroup; the
+    // barrier below publishes it.
+    __shared__ uint4 jtab[RK == RK_XOR128 ? kJumpTab : 1];
+    if constexpr (RK == RK_XOR128) {
+        if (a.s_quads) {
+            const uint4* src = a.jump + kJumpCols;
+            for (int k = (int)threadIdx.x; k < kJumpTab; k += kUpdThreads) jtab[k] = src[k];
+        }
+    }
     M<R, N> S;
     if (active) S = staple_nu<R, N, NREAL>(Tb, a.g, s, i, p, mu, nu);
     else m_zero(S);
@@ -529,7 +538,7 @@ __device__ __forceinline__ void update3(const UpdArgs& a, uint32_t blk, uint32_t
                 const int half = w - 1;
                 uint2 o;
                 if constexpr (RK == RK_XOR128) {
-                    o = xor128_jump_half(rs, a.jump, half);
+                    o = xor128_jump_tab_half(rs, jtab, half);
                 } else {
                     o = half ? make_uint2(pm_jump(rs.z, a.pm_ak), pm_jump(rs.w, a.pm_ak))
                              : make_uint2(pm_jump(rs.x, a.pm_ak), pm_jump(rs.y, a.pm_ak));
@@ -1293,7 +1302,7 @@ struct qcd_lattice {
     void* table = nullptr;
     size_t table_bytes = 0;
     uint4* rng_buf[2] = {nullptr, nullptr};   // generator states; rng_buf[rng_cur] is current
-    uint4* jump_cols = nullptr;               // XOR128 jump-ahead matrix (128 columns)
+    uint4* jump_cols = nullptr;               // XOR128 jump-ahead matrix: 128 columns, then the 4-bit table
     float4* rl_state = nullptr;               // RANLUX: 7 float4 per stream
     float4* prn_buf = nullptr;                // RANLUX: produced quads, prn_quads per stream
     uint32_t prn_quads = 0;
@@ -1653,7 +1662,7 @@ int qcd_lattice_create(const qcd_params* pp, qcd_lattice** out)
     if (hipMalloc(&L->table, L->table_bytes) != hipSuccess ||
         hipMalloc((void**)&L->rng_buf[0], sizeof(uint4) * L->n_streams) != hipSuccess ||
         hipMalloc((void**)&L->rng_buf[1], sizeof(uint4) * L->n_streams) != hipSuccess ||
-        hipMalloc((void**)&L->jump_cols, sizeof(uint4) * 128) != hipSuccess ||
+        hipMalloc((void**)&L->jump_cols, sizeof(uint4) * (kJumpCols + kJumpTab)) != hipSuccess ||
         hipMalloc((void**)&L->partials, sizeof(double) * part_doubles) != hipSuccess ||
         hipMalloc((void**)&L->rec_index, sizeof(int) * 4) != hipSuccess ||
         hipMalloc((void**)&L->plq_rec, sizeof(double) * 2 * p.max_measurements) != hipSuccess ||
@@ -1683,14 +1692,30 @@ int qcd_lattice_create(const qcd_params* pp, qcd_lattice** out)
         // XOR128 jump matrix for 4 * s_quads steps (four steps per quad),
         // Park-Miller multiplier for s_quads steps (pm_quad steps each of its
         // four lanes once).
-        uint4 cols[128];
-        for (int b = 0; b < 128; ++b) {
+        // Columns (image of state bit b = word b/32, bit b%32), then the
+        // four-bit table: entry [c][e] = XOR of the columns of the bits of
+        // chunk c (bits 4c .. 4c+3) set in e.
+        uint4 cols[kJumpCols + kJumpTab];
+        for (int b = 0; b < kJumpCols; ++b) {
             uint4 st = make_uint4(0u, 0u, 0u, 0u);
             uint32_t* w = (uint32_t*)&st;
             w[b / 32] = 1u << (b % 32);
             for (uint32_t k = 0; k < 4u * L->s_quads; ++k) (void)xor128_step(st);
             cols[b] = st;
         }
+        for (int c = 0; c < 32; ++c)
+            for (int e = 0; e < 16; ++e) {
+                uint4 acc = make_uint4(0u, 0u, 0u, 0u);
+                for (int k = 0; k < 4; ++k)
+                    if ((e >> k) & 1) {
+                        const uint4 v = cols[4 * c + k];
+                        acc.x ^= v.x;
+                        acc.y ^= v.y;
+                        acc.z ^= v.z;
+                        acc.w ^= v.w;
+                    }
+                cols[kJumpCols + 16 * c + e] = acc;
+            }
         if (hipMemcpy(L->jump_cols, cols, sizeof(cols), hipMemcpyHostToDevice) != hipSuccess) {
             qcd_lattice_destroy(L);
             return fail(QCD_E_HIP, "jump matrix upload failed");
