@@ -282,6 +282,37 @@ int  qcd_batch_autotune(qcd_batch *batch, int repeats, int *shapes_out);
    work is complete when this returns.  Do not destroy it.                 */
 int  qcd_batch_chain(qcd_batch *batch, int index, qcd_lattice **lat);
 
+/* ---- learned global proposal for SU(2) (new; no reference counterpart): a
+        gauge-equivariant flow of masked coupling layers on 8-channel
+        plaquette / staple features (two 3^4-stencil convolutions, 32 hidden
+        channels, tanh; BF16 MFMA with FP32 accumulation) and rational-
+        quadratic spline maps of the active loop's angle, for an independence
+        Metropolis step with the exact Wilson action; the plain heatbath stays
+        the fallback.  Links are float quaternions (a0, a1, a2, a3),
+        U = a0 + i a.sigma, in links[((b*4 + mu)*V + site)*4 + c].  log q is
+        the density with respect to normalised Haar measure; action is
+        beta * sum_P (1 - Tr U_P / 2).  Parameters: per layer W1[32][81*8],
+        b1[32], W2[16][81*32], b2[16] (FP32; stencil point st = ((ox*3+oy)*3
+        +oz)*3+ot with offsets o-1 along x,y,z,t, k = st*cin + channel).     */
+typedef struct {
+    int n[4];        /* lattice extents (site count divisible by 16)           */
+    int layers;      /* coupling layers (layer l: mu = l%4, mask (l/4)%2)       */
+    int hidden;      /* hidden channels: 32                                     */
+    int bins;        /* spline bins, 1..5                                       */
+    int max_batch;   /* configurations per sample / evaluate call               */
+} qcd_flow_desc;
+typedef struct qcd_flow qcd_flow;
+int  qcd_flow_create(const qcd_flow_desc *desc, qcd_flow **out);
+int  qcd_flow_destroy(qcd_flow *flow);
+int  qcd_flow_param_count(const qcd_flow *flow, size_t *count);
+int  qcd_flow_set_params(qcd_flow *flow, const float *params, size_t count);
+/* Draws `batch` Haar configurations (seeded counter hash), maps them through
+   the flow, returns the links, log q and the action (synchronous).        */
+int  qcd_flow_sample(qcd_flow *flow, unsigned long long seed, int batch, double beta, float *links,
+                     double *logq, double *action);
+/* log q and the action of given configurations (the inverse flow).         */
+int  qcd_flow_evaluate(qcd_flow *flow, const float *links, int batch, double beta, double *logq, double *action);
+
 /* ---- reference PRN buffer semantics (PRNG::produce, random.cpp:278-284;
         kernels xor128 prngcl_xor128.cl:100, pm prngcl_pm.cl:86,
         ranlux prngcl_ranlux.cl:534).  Host seed tables in, quads out:

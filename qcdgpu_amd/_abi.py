@@ -43,6 +43,11 @@ class ChainDesc(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int * 4), ("beta", ctypes.c_double), ("randseries", ctypes.c_uint)]
 
 
+class FlowDesc(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int * 4), ("layers", ctypes.c_int), ("hidden", ctypes.c_int), ("bins", ctypes.c_int),
+                ("max_batch", ctypes.c_int)]
+
+
 class RunInfo(ctypes.Structure):
     """qcd_run_info: the run counters of a .qcg header."""
 
@@ -116,6 +121,12 @@ _SIGNATURES = {
     "qcd_qcg_write": [_P, ctypes.c_char_p, ctypes.POINTER(RunInfo)],
     "qcd_qcg_read_header": [ctypes.c_char_p, ctypes.POINTER(Params), ctypes.POINTER(RunInfo), _I],
     "qcd_qcg_read": [_P, ctypes.c_char_p, ctypes.POINTER(RunInfo)],
+    "qcd_flow_create": [ctypes.POINTER(FlowDesc), ctypes.POINTER(_P)],
+    "qcd_flow_destroy": [_P],
+    "qcd_flow_param_count": [_P, ctypes.POINTER(ctypes.c_size_t)],
+    "qcd_flow_set_params": [_P, ctypes.POINTER(ctypes.c_float), ctypes.c_size_t],
+    "qcd_flow_sample": [_P, ctypes.c_ulonglong, ctypes.c_int, ctypes.c_double, ctypes.POINTER(ctypes.c_float), _D, _D],
+    "qcd_flow_evaluate": [_P, ctypes.POINTER(ctypes.c_float), ctypes.c_int, ctypes.c_double, _D, _D],
     "qcd_prng_produce_host": [
         ctypes.c_int,
         ctypes.c_int,

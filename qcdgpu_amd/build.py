@@ -42,7 +42,7 @@ def build_host(verbose=False):
 
 def build(verbose=False):
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    srcs = [os.path.join(CSRC, "qcd_hip.hip"), os.path.join(CSRC, "qcd_qcg.cpp")]
+    srcs = [os.path.join(CSRC, n) for n in ("qcd_hip.hip", "qcd_qcg.cpp", "qcd_flow.hip")]
     tmp = LIB + ".tmp"
     cmd = [hipcc] + HIPCC_FLAGS + ["-o", tmp] + srcs
     res = subprocess.run(cmd, capture_output=True, text=True)

@@ -1,0 +1,596 @@
+// qcd_flow.hip -- learned global proposal for SU(2) lattices: a
+// gauge-equivariant normalising flow of masked coupling layers, evaluated
+// with BF16 MFMA convolutions (FP32 accumulation), for an independence
+// Metropolis step with the exact Wilson action.  No counterpart in the
+// reference (new work); its plain heatbath is the fallback and the judge.
+//
+// Links: unit quaternions q = (a0, a1, a2, a3), U = a0 + i (a1 s1 + a2 s2 +
+// a3 s3) = [[a0 + i a3, a2 + i a1], [-a2 + i a1, a0 - i a3]], stored
+// links[(b * 4 + mu) * V + site] (float4), site = y + z N2 + t N2 N3 +
+// x N2 N3 N4 (the canonical order of the C ABI).
+//
+// Coupling layer l (direction mu = l % 4, mask parity m = (l / 4) % 2)
+// updates U_mu(x) at the active sites, sum_{nu != mu} x_nu + m even.  The
+// staple sum A(x) = sum_nu [U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+ +
+// U_nu(x+mu-nu)^+ U_mu(x-nu)^+ U_nu(x-nu)] holds frozen links only (the
+// U_mu(x +- nu) sit on inactive sites), A = k V with V in SU(2), and the
+// local action is -beta k (U V)_0.  W = U V = exp(i theta n.s) is gauge
+// covariant; the layer maps theta in [0, pi] by a monotone rational-quadratic
+// spline f whose parameters depend on frozen, gauge-invariant features, and
+// sets U' = exp(i f(theta) n.s) V^+.  Haar measure in (theta, n) is
+// sin^2(theta) d(theta) d(Omega), so log J = log f'(theta) + 2 log(sin
+// f(theta) / sin theta).  log q(U) = -sum_layers log J (Haar prior, density
+// 1).
+//
+// Conditioner per layer: features per site (channels-last, BF16, padded to
+// 8): k(x)/6 at active sites (0 elsewhere), (P_nu rho)_0 of the three
+// plaquettes in the planes orthogonal to mu, the active mask; then a 3^4
+// stencil convolution to 32 channels + bias, tanh; then a 3^4 stencil
+// convolution to 16 channels + bias (3K + 1 spline parameters used).
+// Stencil point st = ((ox * 3 + oy) * 3 + oz) * 3 + ot, offsets o - 1 along
+// x, y, z, t, periodic.  Convolutions are implicit GEMMs on
+// v_mfma_f32_16x16x16_bf16: rows = sites, K = (stencil point, input
+// channel), N = output channels.
+//
+// Spline (Durkan et al. 2019, rational-quadratic, K bins on [0, pi]): raw
+// outputs r[0..K) widths, r[K..2K) heights, r[2K..3K] derivatives; widths =
+// pi (eps + (1 - K eps) softmax), heights likewise, derivatives = eps_d +
+// (1 - eps_d) softplus(r) / ln 2 (all-zero outputs give the identity map).
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/qcd_hip.h"
+#include "qcd_internal.h"
+
+namespace {
+
+constexpr int kStencil = 81;
+constexpr int kCin0 = 8;     // input feature channels (5 used)
+constexpr int kHidden = 32;
+constexpr int kOut = 16;     // spline parameter channels (3K + 1 used)
+constexpr int kMaxBins = 5;
+constexpr float kPi = 3.14159265358979323846f;
+constexpr float kMinW = 1e-3f, kMinD = 1e-3f;
+
+typedef short s4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+struct FGeom {
+    int n1, n2, n3, n4;   // extents along x, y, z, t
+    uint32_t V;
+};
+
+__host__ __device__ inline uint16_t f2bf(float f)
+{   // round to nearest even
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__device__ inline float bf2f(uint16_t h)
+{
+    const uint32_t u = (uint32_t)h << 16;
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
+}
+
+__device__ inline void coords(const FGeom& g, uint32_t s, int c[4])
+{
+    c[1] = (int)(s % (uint32_t)g.n2);
+    uint32_t r = s / (uint32_t)g.n2;
+    c[2] = (int)(r % (uint32_t)g.n3);
+    r /= (uint32_t)g.n3;
+    c[3] = (int)(r % (uint32_t)g.n4);
+    c[0] = (int)(r / (uint32_t)g.n4);
+}
+__device__ inline uint32_t site(const FGeom& g, const int c[4])
+{
+    return (uint32_t)c[1] + (uint32_t)g.n2 * ((uint32_t)c[2] + (uint32_t)g.n3 * ((uint32_t)c[3] + (uint32_t)g.n4 * (uint32_t)c[0]));
+}
+__device__ inline int wrap(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
+__device__ inline uint32_t shifted(const FGeom& g, const int c[4], int d, int o)
+{
+    int e[4] = {c[0], c[1], c[2], c[3]};
+    const int n[4] = {g.n1, g.n2, g.n3, g.n4};
+    e[d] = wrap(e[d] + o, n[d]);
+    return site(g, e);
+}
+
+// quaternion algebra for U = a0 + i a.s
+__device__ inline float4 qmul(float4 a, float4 b)
+{
+    return make_float4(a.x * b.x - a.y * b.y - a.z * b.z - a.w * b.w,
+                       a.x * b.y + b.x * a.y - (a.z * b.w - a.w * b.z),
+                       a.x * b.z + b.x * a.z - (a.w * b.y - a.y * b.w),
+                       a.x * b.w + b.x * a.w - (a.y * b.z - a.z * b.y));
+}
+__device__ inline float4 qdag(float4 a) { return make_float4(a.x, -a.y, -a.z, -a.w); }
+__device__ inline float4 qadd(float4 a, float4 b) { return make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w); }
+
+__device__ inline float4 link(const float4* L, const FGeom& g, uint32_t b, int mu, uint32_t s)
+{
+    return L[((size_t)b * 4 + mu) * g.V + s];
+}
+
+// staple sum of U_mu(x) (see the file comment)
+__device__ float4 staple(const float4* L, const FGeom& g, uint32_t b, int mu, const int c[4])
+{
+    float4 A = make_float4(0.f, 0.f, 0.f, 0.f);
+    const uint32_t x = site(g, c);
+    const uint32_t xp = shifted(g, c, mu, 1);
+    for (int nu = 0; nu < 4; ++nu) {
+        if (nu == mu) continue;
+        const uint32_t xpn = shifted(g, c, nu, 1), xmn = shifted(g, c, nu, -1);
+        int cm[4] = {c[0], c[1], c[2], c[3]};
+        const int n[4] = {g.n1, g.n2, g.n3, g.n4};
+        cm[nu] = wrap(cm[nu] - 1, n[nu]);
+        const uint32_t xpmn = shifted(g, cm, mu, 1);   // x + mu - nu
+        const float4 f = qmul(qmul(link(L, g, b, nu, xp), qdag(link(L, g, b, mu, xpn))), qdag(link(L, g, b, nu, x)));
+        const float4 r = qmul(qmul(qdag(link(L, g, b, nu, xpmn)), qdag(link(L, g, b, mu, xmn))), link(L, g, b, nu, xmn));
+        A = qadd(A, qadd(f, r));
+    }
+    return A;
+}
+
+// (U_nu(x) U_rho(x+nu) U_nu(x+rho)^+ U_rho(x)^+)_0
+__device__ float plaq0(const float4* L, const FGeom& g, uint32_t b, int nu, int rho, const int c[4])
+{
+    const uint32_t x = site(g, c);
+    const float4 p = qmul(qmul(link(L, g, b, nu, x), link(L, g, b, rho, shifted(g, c, nu, 1))),
+                          qmul(qdag(link(L, g, b, nu, shifted(g, c, rho, 1))), qdag(link(L, g, b, rho, x))));
+    return p.x;
+}
+
+__device__ inline bool is_active(const int c[4], int mu, int par)
+{
+    int s = par;
+    for (int d = 0; d < 4; ++d)
+        if (d != mu) s += c[d];
+    return (s & 1) == 0;
+}
+
+// ---------------------------------------------------------------- kernels
+__global__ void k_flow_haar(float4* L, uint64_t n, uint64_t seed)
+{
+    const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    // splitmix64 of (seed, link index): two 64-bit outputs -> four uniforms
+    uint64_t z = seed + 0x9E3779B97F4A7C15ull * (j + 1);
+    float u[4];
+    for (int k = 0; k < 2; ++k) {
+        z += 0x9E3779B97F4A7C15ull;
+        uint64_t x = z;
+        x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+        x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+        x ^= x >> 31;
+        u[2 * k] = ((float)(uint32_t)(x >> 40) + 0.5f) * 5.9604644775390625e-8f;          // 24 bits, (0, 1)
+        u[2 * k + 1] = ((float)(uint32_t)((x >> 8) & 0xffffffu) + 0.5f) * 5.9604644775390625e-8f;
+    }
+    // Box-Muller: four Gaussians, normalised: uniform on S^3 = Haar on SU(2)
+    const float r1 = sqrtf(-2.f * logf(u[0])), r2 = sqrtf(-2.f * logf(u[2]));
+    float s1, c1, s2, c2;
+    sincosf(2.f * kPi * u[1], &s1, &c1);
+    sincosf(2.f * kPi * u[3], &s2, &c2);
+    float4 q = make_float4(r1 * c1, r1 * s1, r2 * c2, r2 * s2);
+    const float nrm = rsqrtf(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w);
+    L[j] = make_float4(q.x * nrm, q.y * nrm, q.z * nrm, q.w * nrm);
+}
+
+__global__ void k_flow_features(const float4* L, uint16_t* feat, FGeom g, uint32_t B, int mu, int par)
+{
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= B * g.V) return;
+    const uint32_t b = j / g.V, s = j % g.V;
+    int c[4];
+    coords(g, s, c);
+    const bool act = is_active(c, mu, par);
+    float f[kCin0] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (act) {
+        const float4 A = staple(L, g, b, mu, c);
+        f[0] = sqrtf(A.x * A.x + A.y * A.y + A.z * A.z + A.w * A.w) * (1.f / 6.f);
+        f[4] = 1.f;
+    }
+    int k = 1;
+    for (int nu = 0; nu < 4; ++nu)
+        for (int rho = nu + 1; rho < 4; ++rho)
+            if (nu != mu && rho != mu) f[k++] = plaq0(L, g, b, nu, rho, c);
+    uint16_t* o = feat + (size_t)j * kCin0;
+    for (int q = 0; q < kCin0; ++q) o[q] = f2bf(f[q]);
+}
+
+// One wavefront: a tile of 16 sites x 16 output channels.  in[(row) * cin +
+// channel] (BF16), wt[n * (81 cin) + k] (BF16, k = st * cin + channel),
+// out: tanh -> BF16 [row * cout + n] (act) or FP32 [row * cout + n].
+__global__ __launch_bounds__(256) void k_flow_conv(const uint16_t* in, int cin, const uint16_t* wt, const float* bias,
+                                                   int cout, uint16_t* out_bf, float* out_f, FGeom g, uint32_t B,
+                                                   int act)
+{
+    const uint32_t wave = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+    const int lane = threadIdx.x & 63;
+    const uint32_t ntile_m = B * g.V / 16, ntile_n = (uint32_t)cout / 16;
+    if (wave >= ntile_m * ntile_n) return;   // whole wavefront
+    const uint32_t tm = wave % ntile_m, tn = wave / ntile_m;
+    const uint32_t row = tm * 16 + (lane & 15);   // A row of this lane
+    const uint32_t b = row / g.V, s = row % g.V;
+    int c[4];
+    coords(g, s, c);
+    const int n[4] = {g.n1, g.n2, g.n3, g.n4};
+    const int kq = 4 * (lane >> 4);                // k offset of this lane's 4 values
+    const uint32_t ncol = tn * 16 + (lane & 15);   // B column of this lane
+    const int K = kStencil * cin;
+    const uint16_t* wrow = wt + (size_t)ncol * K;
+    f4 acc = {0.f, 0.f, 0.f, 0.f};
+    for (int k0 = 0; k0 < K; k0 += 16) {
+        const int k = k0 + kq;   // K is a multiple of 4: a lane's 4 values are all inside or all past K
+        s4 av = {0, 0, 0, 0}, bv = {0, 0, 0, 0};
+        if (k < K) {
+            const int st = k / cin, ch = k % cin;
+            const int o[4] = {st / 27 - 1, (st / 9) % 3 - 1, (st / 3) % 3 - 1, st % 3 - 1};
+            int e[4];
+            for (int d = 0; d < 4; ++d) e[d] = wrap(c[d] + o[d], n[d]);
+            const uint32_t nb = site(g, e);
+            av = *reinterpret_cast<const s4*>(in + ((size_t)b * g.V + nb) * cin + ch);
+            bv = *reinterpret_cast<const s4*>(wrow + k);
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
+    }
+    // lane holds D[4 * (lane / 16) + i][lane % 16]
+    for (int i = 0; i < 4; ++i) {
+        const uint32_t r = tm * 16 + 4 * (lane >> 4) + i;
+        float v = acc[i] + bias[ncol];
+        if (act) out_bf[(size_t)r * cout + ncol] = f2bf(tanhf(v));
+        else out_f[(size_t)r * cout + ncol] = v;
+    }
+}
+
+struct Spline {
+    float xk[kMaxBins + 1], yk[kMaxBins + 1], d[kMaxBins + 1];
+    int K;
+};
+
+__device__ inline float softplus(float x) { return x > 20.f ? x : log1pf(expf(x)); }
+
+__device__ void make_spline(const float* r, int K, Spline& sp)
+{
+    sp.K = K;
+    float mw = r[0], mh = r[K];
+    for (int i = 1; i < K; ++i) {
+        mw = fmaxf(mw, r[i]);
+        mh = fmaxf(mh, r[K + i]);
+    }
+    float ew[kMaxBins], eh[kMaxBins], sw = 0.f, sh = 0.f;
+    for (int i = 0; i < K; ++i) {
+        ew[i] = expf(r[i] - mw);
+        eh[i] = expf(r[K + i] - mh);
+        sw += ew[i];
+        sh += eh[i];
+    }
+    sp.xk[0] = sp.yk[0] = 0.f;
+    for (int i = 0; i < K; ++i) {
+        sp.xk[i + 1] = sp.xk[i] + kPi * (kMinW + (1.f - K * kMinW) * ew[i] / sw);
+        sp.yk[i + 1] = sp.yk[i] + kPi * (kMinW + (1.f - K * kMinW) * eh[i] / sh);
+    }
+    sp.xk[K] = sp.yk[K] = kPi;
+    for (int i = 0; i <= K; ++i) sp.d[i] = kMinD + (1.f - kMinD) * softplus(r[2 * K + i]) * 1.4426950408889634f;
+}
+
+// forward: y = f(x), dlog = log f'(x)
+__device__ float spline_fwd(const Spline& sp, float x, float& dlog)
+{
+    int k = 0;
+    while (k < sp.K - 1 && x >= sp.xk[k + 1]) ++k;
+    const float w = sp.xk[k + 1] - sp.xk[k], h = sp.yk[k + 1] - sp.yk[k], s = h / w;
+    const float xi = fminf(fmaxf((x - sp.xk[k]) / w, 0.f), 1.f), om = 1.f - xi;
+    const float den = s + (sp.d[k + 1] + sp.d[k] - 2.f * s) * xi * om;
+    const float y = sp.yk[k] + h * (s * xi * xi + sp.d[k] * xi * om) / den;
+    const float num = s * s * (sp.d[k + 1] * xi * xi + 2.f * s * xi * om + sp.d[k] * om * om);
+    dlog = logf(num) - 2.f * logf(den);
+    return y;
+}
+
+// inverse: x = f^-1(y), dlog = log f'(x)
+__device__ float spline_inv(const Spline& sp, float y, float& dlog)
+{
+    int k = 0;
+    while (k < sp.K - 1 && y >= sp.yk[k + 1]) ++k;
+    const float w = sp.xk[k + 1] - sp.xk[k], h = sp.yk[k + 1] - sp.yk[k], s = h / w;
+    const float dy = y - sp.yk[k], dd = sp.d[k + 1] + sp.d[k] - 2.f * s;
+    const float a = h * (s - sp.d[k]) + dy * dd;
+    const float bq = h * sp.d[k] - dy * dd;
+    const float cq = -s * dy;
+    const float disc = fmaxf(bq * bq - 4.f * a * cq, 0.f);
+    const float xi = fminf(fmaxf(2.f * cq / (-bq - sqrtf(disc)), 0.f), 1.f), om = 1.f - xi;
+    const float den = s + dd * xi * om;
+    const float num = s * s * (sp.d[k + 1] * xi * xi + 2.f * s * xi * om + sp.d[k] * om * om);
+    dlog = logf(num) - 2.f * logf(den);
+    return sp.xk[k] + xi * w;
+}
+
+// Coupling of layer (mu, par): forward (inverse = 0) maps theta -> f(theta),
+// inverse maps theta' -> f^-1(theta'); both add log J = log f'(theta) +
+// 2 log(sin f(theta) / sin theta) of the forward map to logj[row].
+__global__ void k_flow_couple(float4* L, const float* prm, float* logj, FGeom g, uint32_t B, int mu, int par, int K,
+                              int inverse)
+{
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= B * g.V) return;
+    const uint32_t b = j / g.V, s = j % g.V;
+    int c[4];
+    coords(g, s, c);
+    if (!is_active(c, mu, par)) return;
+    const float4 A = staple(L, g, b, mu, c);
+    const float k = sqrtf(A.x * A.x + A.y * A.y + A.z * A.z + A.w * A.w);
+    if (!(k > 1e-20f)) return;
+    const float ik = 1.f / k;
+    const float4 Vq = make_float4(A.x * ik, A.y * ik, A.z * ik, A.w * ik);
+    float4& U = L[((size_t)b * 4 + mu) * g.V + s];
+    const float4 W = qmul(U, Vq);
+    const float sn = sqrtf(W.y * W.y + W.z * W.z + W.w * W.w);
+    const float th = atan2f(sn, W.x);
+    Spline sp;
+    make_spline(prm + (size_t)j * kOut, K, sp);
+    float dlog, th2;
+    if (!inverse) th2 = spline_fwd(sp, th, dlog);
+    else th2 = spline_inv(sp, th, dlog);
+    const float th_in = inverse ? th2 : th, th_out = inverse ? th : th2;   // theta and f(theta)
+    const float si = sinf(th_in), so = sinf(th_out);
+    const float ratio = (si > 1e-6f && so > 1e-6f) ? so / si : expf(dlog);   // limit at the poles: f'(theta)
+    logj[j] += dlog + 2.f * logf(ratio);
+    float4 Wn;
+    if (sn > 1e-12f) {
+        const float f = sinf(th2) / sn;
+        Wn = make_float4(cosf(th2), W.y * f, W.z * f, W.w * f);
+    } else {
+        Wn = make_float4(cosf(th2) >= 0.f ? 1.f : -1.f, 0.f, 0.f, 0.f);
+    }
+    U = qmul(Wn, qdag(Vq));
+}
+
+// per-site sum of (1 - (U_p)_0) over the six plaquettes with corner x
+__global__ void k_flow_action_sites(const float4* L, double* out, FGeom g, uint32_t B)
+{
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= B * g.V) return;
+    const uint32_t b = j / g.V, s = j % g.V;
+    int c[4];
+    coords(g, s, c);
+    double acc = 0.0;
+    for (int nu = 0; nu < 4; ++nu)
+        for (int rho = nu + 1; rho < 4; ++rho) acc += 1.0 - (double)plaq0(L, g, b, nu, rho, c);
+    out[j] = acc;
+}
+
+// out[b] = sum of in[b * n + i] (one workgroup per row, fixed order)
+template <typename T>
+__global__ __launch_bounds__(256) void k_flow_rowsum(const T* in, uint32_t n, double* out)
+{
+    __shared__ double red[256];
+    const uint32_t b = blockIdx.x;
+    double acc = 0.0;
+    for (uint32_t i = threadIdx.x; i < n; i += 256) acc += (double)in[(size_t)b * n + i];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[b] = red[0];
+}
+
+int err(int code, const std::string& m) { return qcdhip::set_error(code, m.c_str()); }
+
+}  // namespace
+
+struct qcd_flow {
+    qcd_flow_desc d;
+    FGeom g;
+    size_t nparam = 0;
+    std::vector<float> host_params;
+    uint16_t* w1 = nullptr;    // [layers][32][81 * 8]
+    float* b1 = nullptr;       // [layers][32]
+    uint16_t* w2 = nullptr;    // [layers][16][81 * 32]
+    float* b2 = nullptr;       // [layers][16]
+    float4* links = nullptr;   // [max_batch][4][V]
+    uint16_t* feat = nullptr;  // [max_batch * V][8]
+    uint16_t* hid = nullptr;   // [max_batch * V][32]
+    float* prm = nullptr;      // [max_batch * V][16]
+    float* logj = nullptr;     // [max_batch * V]
+    double* act = nullptr;     // [max_batch * V]
+    double* sums = nullptr;    // [2][max_batch]
+    hipStream_t stream = nullptr;
+};
+
+namespace {
+
+constexpr size_t kW1 = (size_t)kHidden * kStencil * kCin0;
+constexpr size_t kW2 = (size_t)kOut * kStencil * kHidden;
+constexpr size_t kLayerParams = kW1 + kHidden + kW2 + kOut;
+
+#define FCHK(x)                                                                                         \
+    do {                                                                                                \
+        hipError_t e_ = (x);                                                                            \
+        if (e_ != hipSuccess) return err(QCD_E_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+int run_layers(qcd_flow* F, uint32_t B, bool inverse)
+{
+    const FGeom g = F->g;
+    const uint32_t n = B * g.V, nb = (n + 255) / 256;
+    FCHK(hipMemsetAsync(F->logj, 0, sizeof(float) * n, F->stream));
+    for (int step = 0; step < F->d.layers; ++step) {
+        const int l = inverse ? F->d.layers - 1 - step : step;
+        const int mu = l % 4, par = (l / 4) % 2;
+        hipLaunchKernelGGL(k_flow_features, dim3(nb), dim3(256), 0, F->stream, F->links, F->feat, g, B, mu, par);
+        const uint32_t w1n = (n / 16) * (kHidden / 16), w2n = (n / 16) * (kOut / 16);
+        hipLaunchKernelGGL(k_flow_conv, dim3((w1n + 3) / 4), dim3(256), 0, F->stream, F->feat, kCin0, F->w1 + l * kW1,
+                           F->b1 + l * kHidden, kHidden, F->hid, (float*)nullptr, g, B, 1);
+        hipLaunchKernelGGL(k_flow_conv, dim3((w2n + 3) / 4), dim3(256), 0, F->stream, F->hid, kHidden, F->w2 + l * kW2,
+                           F->b2 + l * kOut, kOut, (uint16_t*)nullptr, F->prm, g, B, 0);
+        hipLaunchKernelGGL(k_flow_couple, dim3(nb), dim3(256), 0, F->stream, F->links, F->prm, F->logj, g, B, mu, par,
+                           F->d.bins, inverse ? 1 : 0);
+    }
+    FCHK(hipGetLastError());
+    return QCD_OK;
+}
+
+// per-site action terms of the configurations now in F->links
+int action_sites(qcd_flow* F, uint32_t B)
+{
+    const uint32_t n = B * F->g.V, nb = (n + 255) / 256;
+    hipLaunchKernelGGL(k_flow_action_sites, dim3(nb), dim3(256), 0, F->stream, F->links, F->act, F->g, B);
+    FCHK(hipGetLastError());
+    return QCD_OK;
+}
+
+// log q from the accumulated log J, the action from the per-site terms
+int finish(qcd_flow* F, uint32_t B, double beta, double* logq, double* action)
+{
+    const FGeom g = F->g;
+    hipLaunchKernelGGL((k_flow_rowsum<float>), dim3(B), dim3(256), 0, F->stream, F->logj, g.V, F->sums);
+    hipLaunchKernelGGL((k_flow_rowsum<double>), dim3(B), dim3(256), 0, F->stream, F->act, g.V, F->sums + F->d.max_batch);
+    FCHK(hipGetLastError());
+    std::vector<double> h(2 * (size_t)F->d.max_batch);
+    FCHK(hipMemcpyAsync(h.data(), F->sums, sizeof(double) * h.size(), hipMemcpyDeviceToHost, F->stream));
+    FCHK(hipStreamSynchronize(F->stream));
+    for (uint32_t b = 0; b < B; ++b) {
+        if (logq) logq[b] = -h[b];
+        if (action) action[b] = beta * h[F->d.max_batch + b];
+    }
+    return QCD_OK;
+}
+
+int check_batch(const qcd_flow* F, int batch)
+{
+    if (batch < 1 || batch > F->d.max_batch)
+        return err(QCD_E_ARG, "batch " + std::to_string(batch) + " outside 1.." + std::to_string(F->d.max_batch));
+    if (F->nparam == 0 || F->host_params.empty()) return err(QCD_E_STATE, "flow parameters not set");
+    return QCD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int qcd_flow_create(const qcd_flow_desc* desc, qcd_flow** out)
+{
+    if (!desc || !out) return err(QCD_E_ARG, "null argument");
+    const qcd_flow_desc d = *desc;
+    for (int k = 0; k < 4; ++k)
+        if (d.n[k] < 2) return err(QCD_E_ARG, "flow lattice extents must be >= 2");
+    if (d.layers < 1 || d.layers > 64) return err(QCD_E_ARG, "layers must be 1..64");
+    if (d.hidden != kHidden) return err(QCD_E_UNSUPPORTED, "hidden channels must be 32");
+    if (d.bins < 1 || d.bins > kMaxBins) return err(QCD_E_ARG, "bins must be 1..5");
+    if (d.max_batch < 1) return err(QCD_E_ARG, "max_batch must be >= 1");
+    const uint64_t V = (uint64_t)d.n[0] * d.n[1] * d.n[2] * d.n[3];
+    if (V % 16) return err(QCD_E_ARG, "the flow needs a site count divisible by 16");
+    if (V * (uint64_t)d.max_batch > (1ull << 26)) return err(QCD_E_ARG, "max_batch x sites above 2^26");
+    qcd_flow* F = new qcd_flow();
+    F->d = d;
+    F->g = FGeom{d.n[0], d.n[1], d.n[2], d.n[3], (uint32_t)V};
+    F->nparam = kLayerParams * (size_t)d.layers;
+    const size_t n = V * (size_t)d.max_batch;
+    bool ok = hipStreamCreateWithFlags(&F->stream, hipStreamNonBlocking) == hipSuccess;
+    ok = ok && hipMalloc((void**)&F->w1, sizeof(uint16_t) * kW1 * d.layers) == hipSuccess;
+    ok = ok && hipMalloc((void**)&F->b1, sizeof(float) * kHidden * d.layers) == hipSuccess;
+    ok = ok && hipMalloc((void**)&F->w2, sizeof(uint16_t) * kW2 * d.layers) == hipSuccess;
+    ok = ok && hipMalloc((void**)&F->b2, sizeof(float) * kOut * d.layers) == hipSuccess;
+    ok = ok && hipMalloc((void**)&F->links, sizeof(float4) * 4 * n) == hipSuccess;
+    ok = ok && hipMalloc((void**)&F->feat, sizeof(uint16_t) * kCin0 * n) == hipSuccess;
+    ok = ok && hipMalloc((void**)&F->hid, sizeof(uint16_t) * kHidden * n) == hipSuccess;
+    ok = ok && hipMalloc((void**)&F->prm, sizeof(float) * kOut * n) == hipSuccess;
+    ok = ok && hipMalloc((void**)&F->logj, sizeof(float) * n) == hipSuccess;
+    ok = ok && hipMalloc((void**)&F->act, sizeof(double) * n) == hipSuccess;
+    ok = ok && hipMalloc((void**)&F->sums, sizeof(double) * 2 * d.max_batch) == hipSuccess;
+    if (!ok) {
+        qcd_flow_destroy(F);
+        return err(QCD_E_NOMEM, "flow buffers");
+    }
+    *out = F;
+    return QCD_OK;
+}
+
+int qcd_flow_destroy(qcd_flow* F)
+{
+    if (!F) return QCD_OK;
+    if (F->stream) (void)hipStreamSynchronize(F->stream);
+    (void)hipFree(F->w1);
+    (void)hipFree(F->b1);
+    (void)hipFree(F->w2);
+    (void)hipFree(F->b2);
+    (void)hipFree(F->links);
+    (void)hipFree(F->feat);
+    (void)hipFree(F->hid);
+    (void)hipFree(F->prm);
+    (void)hipFree(F->logj);
+    (void)hipFree(F->act);
+    (void)hipFree(F->sums);
+    if (F->stream) (void)hipStreamDestroy(F->stream);
+    delete F;
+    return QCD_OK;
+}
+
+int qcd_flow_param_count(const qcd_flow* F, size_t* n)
+{
+    if (!F || !n) return err(QCD_E_ARG, "null argument");
+    *n = F->nparam;
+    return QCD_OK;
+}
+
+int qcd_flow_set_params(qcd_flow* F, const float* p, size_t n)
+{
+    if (!F || !p) return err(QCD_E_ARG, "null argument");
+    if (n != F->nparam) return err(QCD_E_ARG, "expected " + std::to_string(F->nparam) + " parameters, got " + std::to_string(n));
+    const int Ln = F->d.layers;
+    std::vector<uint16_t> w1(kW1 * Ln), w2(kW2 * Ln);
+    std::vector<float> b1(kHidden * Ln), b2(kOut * Ln);
+    for (int l = 0; l < Ln; ++l) {
+        const float* q = p + l * kLayerParams;
+        for (size_t i = 0; i < kW1; ++i) w1[l * kW1 + i] = f2bf(q[i]);
+        for (int i = 0; i < kHidden; ++i) b1[l * kHidden + i] = q[kW1 + i];
+        for (size_t i = 0; i < kW2; ++i) w2[l * kW2 + i] = f2bf(q[kW1 + kHidden + i]);
+        for (int i = 0; i < kOut; ++i) b2[l * kOut + i] = q[kW1 + kHidden + kW2 + i];
+    }
+    FCHK(hipMemcpy(F->w1, w1.data(), sizeof(uint16_t) * w1.size(), hipMemcpyHostToDevice));
+    FCHK(hipMemcpy(F->w2, w2.data(), sizeof(uint16_t) * w2.size(), hipMemcpyHostToDevice));
+    FCHK(hipMemcpy(F->b1, b1.data(), sizeof(float) * b1.size(), hipMemcpyHostToDevice));
+    FCHK(hipMemcpy(F->b2, b2.data(), sizeof(float) * b2.size(), hipMemcpyHostToDevice));
+    F->host_params.assign(p, p + n);
+    return QCD_OK;
+}
+
+int qcd_flow_sample(qcd_flow* F, unsigned long long seed, int batch, double beta, float* links, double* logq,
+                    double* action)
+{
+    if (!F || !links) return err(QCD_E_ARG, "null argument");
+    int rc = check_batch(F, batch);
+    if (rc) return rc;
+    const uint64_t n = (uint64_t)batch * 4 * F->g.V;
+    hipLaunchKernelGGL(k_flow_haar, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, F->stream, F->links, n,
+                       (uint64_t)seed);
+    if ((rc = run_layers(F, (uint32_t)batch, false))) return rc;
+    if ((rc = action_sites(F, (uint32_t)batch))) return rc;   // of the proposals
+    if ((rc = finish(F, (uint32_t)batch, beta, logq, action))) return rc;
+    FCHK(hipMemcpy(links, F->links, sizeof(float4) * n, hipMemcpyDeviceToHost));
+    return QCD_OK;
+}
+
+int qcd_flow_evaluate(qcd_flow* F, const float* links, int batch, double beta, double* logq, double* action)
+{
+    if (!F || !links) return err(QCD_E_ARG, "null argument");
+    int rc = check_batch(F, batch);
+    if (rc) return rc;
+    const uint64_t n = (uint64_t)batch * 4 * F->g.V;
+    FCHK(hipMemcpyAsync(F->links, links, sizeof(float4) * n, hipMemcpyHostToDevice, F->stream));
+    if ((rc = action_sites(F, (uint32_t)batch))) return rc;   // of the given configurations, before the inverse
+    if ((rc = run_layers(F, (uint32_t)batch, true))) return rc;
+    return finish(F, (uint32_t)batch, beta, logq, action);
+}
+
+}  // extern "C"
