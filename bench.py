@@ -88,6 +88,38 @@ def copy_calibration(mib=256, repeats=20):
     print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "launches": repeats}))
 
 
+def c4(args):
+    """The learned-proposal workload (SU(2) 8^3 x 4, beta 2.2) as three child
+    processes: heatbath training configurations, forward-KL training (PyTorch
+    on the GPU), then the plain and learned chains.  This process never
+    touches the GPU; it relays the children's output and prints the chains'
+    JSON line last."""
+    import tempfile
+
+    work = tempfile.mkdtemp(prefix="qcd_c4_")
+    data, params = os.path.join(work, "data.npy"), os.path.join(work, "params.npy")
+    py = [sys.executable, "-u", "-m", "qcdgpu_amd.c4"]
+    steps = [
+        py + ["gen", "--out", data, "--n", str(args.c4_train)],
+        py + ["train", "--data", data, "--out", params, "--n", str(args.c4_train), "--steps", str(args.c4_train_steps)],
+        py + ["chain", "--params", params, "--steps", str(args.c4_steps)],
+    ]
+    last = None
+    for cmd in steps:
+        p = subprocess.Popen(cmd, cwd=REPO, stdout=subprocess.PIPE, text=True)
+        for line in p.stdout:
+            line = line.rstrip("\n")
+            print(line, file=sys.stderr, flush=True)
+            last = line
+        if p.wait() != 0:
+            sys.exit("bench.py --c4: step %s failed" % cmd[4])
+    for f in (data, params):
+        if os.path.exists(f):
+            os.remove(f)
+    os.rmdir(work)
+    print(last, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -103,9 +135,17 @@ def main():
         help="only run a plain 256 MiB device-to-device copy 20 times (the known-size "
         "kernel that scales the FETCH_SIZE / WRITE_SIZE counters)",
     )
+    ap.add_argument("--c4", action="store_true",
+                    help="run the learned-proposal workload instead (SU(2) 8^3x4, beta 2.2; several minutes)")
+    ap.add_argument("--c4-train", type=int, default=20000, help="heatbath training configurations")
+    ap.add_argument("--c4-train-steps", type=int, default=1000)
+    ap.add_argument("--c4-steps", type=int, default=4000, help="steps of each chain")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.c4:
+        c4(args)
         return
     if args.gpus != 1:
         print(

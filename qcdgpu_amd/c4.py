@@ -1,0 +1,181 @@
+"""C4 workload: learned proposal vs plain heatbath on SU(2) 8^3 x 4, beta 2.2.
+
+Three steps, each its own process (the training step runs PyTorch on the
+GPU, the others libqcdhip.so; no process loads both):
+
+  python -m qcdgpu_amd.c4 gen   --out data.npy    heatbath configurations
+  python -m qcdgpu_amd.c4 train --data data.npy --out params.npy
+  python -m qcdgpu_amd.c4 chain --params params.npy   plain vs learned chains
+
+`gen` runs the plain heatbath of the C0 form (Haar hot start, 1 Kennedy-
+Pendleton + 3 overrelaxation sweeps per configuration) and keeps every
+configuration after thermalisation.  `train` fits the flow by forward KL.
+`chain` runs two chains from the same start for the same number of steps:
+plain = 1 heatbath sweep per step; learned = 1 heatbath sweep and one
+independence-Metropolis step with the flow per step; both measure the
+plaquette, the Polyakov loop and the 2x2 Wilson loop every step.  It prints
+one JSON line: observables with binned errors and their difference in
+standard deviations, the Polyakov loop's tau_int, steps per second and
+independent configurations per second of each chain, and the acceptance.
+"""
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+DIMS = (8, 8, 8, 4)
+BETA = 2.2
+
+
+def cmd_gen(a):
+    import qcdgpu_amd as Q
+    from qcdgpu_amd import flow as QF
+
+    dims = tuple(a.dims)
+    out = np.lib.format.open_memmap(a.out, mode="w+", dtype=np.float32, shape=(a.n, 4, int(np.prod(dims)), 4))
+    t0 = time.perf_counter()
+    with Q.Lattice(sun=2, dims=dims, beta=a.beta, precision="fp32", prng="XOR128", randseries=a.seed,
+                   max_measurements=4) as lat:
+        lat.prng_init()
+        lat.init("hot")
+        for _ in range(a.therm):
+            lat.sweep("hb")
+            for _ in range(3):
+                lat.sweep("or")
+        for i in range(a.n):
+            lat.sweep("hb")
+            for _ in range(3):
+                lat.sweep("or")
+            out[i] = QF.to_quaternions(lat.export_links(), dims)
+            if (i + 1) % 5000 == 0:
+                print("  gen %d of %d (%.1f s)" % (i + 1, a.n, time.perf_counter() - t0), flush=True)
+    out.flush()
+    print(json.dumps({"gen_configs": a.n, "gen_seconds": round(time.perf_counter() - t0, 2)}), flush=True)
+
+
+def cmd_train(a):
+    import torch
+
+    from qcdgpu_amd import flow as QF
+
+    dev = "cuda" if torch.cuda.is_available() else "cpu"
+    data = np.load(a.data, mmap_mode="r")
+    X = torch.as_tensor(np.array(data[: a.n]), device=dev)
+    m = QF.FlowModel(dims=tuple(a.dims), layers=a.layers, bins=a.bins, device=dev, seed=a.seed)
+    with torch.no_grad():
+        base = float(-m.log_q(X[:64]).mean())
+    t0 = time.perf_counter()
+    hist = QF.train_forward_kl(m, X, steps=a.steps, batch=a.batch, lr=a.lr, seed=a.seed, log_every=max(a.steps // 10, 1))
+    secs = time.perf_counter() - t0
+    with torch.no_grad():
+        held = float(-m.log_q(X[-64:]).mean())
+    np.save(a.out, m.flat_params())
+    print(json.dumps({"train_configs": int(X.shape[0]), "train_steps": a.steps, "batch": a.batch,
+                      "train_seconds": round(secs, 1), "neg_log_q_identity": round(base, 3),
+                      "neg_log_q_trained": round(held, 3), "last_loss": round(hist[-1], 3)}), flush=True)
+
+
+def _stats(series, sweeps_per_s):
+    from qcdgpu_amd import analysis as A
+
+    m, e = A.binned_error(series, nbins=20)
+    tau = A.tau_int(series)[0]
+    return m, e, tau, sweeps_per_s / (2.0 * tau)
+
+
+def cmd_chain(a):
+    import qcdgpu_amd as Q
+    from qcdgpu_amd import flow as QF
+
+    dims = tuple(a.dims)
+    res = {}
+    flow = None
+    if a.params:
+        flow = QF.Flow(dims=dims, layers=a.layers, bins=a.bins, max_batch=a.batch)
+        flow.set_params(np.load(a.params))
+    for arm in ("plain", "learned"):
+        with Q.Lattice(sun=2, dims=dims, beta=a.beta, precision="fp32", prng="XOR128", randseries=a.seed,
+                       max_measurements=a.steps + 2) as lat:
+            lat.set_wilson(2, 2)
+            lat.prng_init()
+            lat.init("hot")
+            for _ in range(a.therm):
+                lat.sweep("hb")
+            lat.reset_measurements()
+            use = flow if arm == "learned" else None
+            lw = {}
+            acc, secs = QF.imh_chain(lat, use, a.beta, a.steps, hb_per_step=1, seed=a.seed + 17, wilson=True,
+                                     log_weights=lw)
+            m = lat.measurements()
+            plq = m["plq_total"] / 2.0
+            pol = m["polyakov_re"]
+            wil = lat.wilson_measurements() / 2.0
+            rate = a.steps / secs
+            out = {"steps_per_s": round(rate, 1), "acceptance": round(acc, 4)}
+            for name, s in (("plaquette", plq), ("polyakov", pol), ("wilson_2x2", wil)):
+                mu, err, tau, ind = _stats(s, rate)
+                out[name] = {"mean": mu, "err": err, "tau_int": round(tau, 2)}
+                if name == "polyakov":
+                    out["independent_configs_per_s"] = round(ind, 2)
+            out["tau_int_abs_polyakov"] = round(_stats(np.abs(pol), rate)[2], 2)
+            for k, v in lw.items():   # log w = -S - log q: the acceptance compares proposal with current
+                v = np.asarray(v)
+                out["log_w_" + k] = {"mean": round(float(v.mean()), 2), "std": round(float(v.std()), 2)}
+            res[arm] = out
+            print("  %s: %s" % (arm, json.dumps(out)), flush=True)
+    z = {}
+    for name in ("plaquette", "polyakov", "wilson_2x2"):
+        p, l = res["plain"][name], res["learned"][name]
+        z[name] = round((l["mean"] - p["mean"]) / np.hypot(p["err"], l["err"]), 2)
+    line = {
+        "metric": "independent configurations/s (Polyakov loop tau_int), learned + heatbath vs plain heatbath",
+        "value": res["learned"]["independent_configs_per_s"],
+        "unit": "configurations/s",
+        "plain": res["plain"]["independent_configs_per_s"],
+        "ratio": round(res["learned"]["independent_configs_per_s"] / res["plain"]["independent_configs_per_s"], 3),
+        "acceptance": res["learned"]["acceptance"],
+        "z_learned_minus_plain": z,
+        "config": {"workload": "SU(2) %dx%dx%dx%d beta=%.2f, FP32 links" % (dims + (a.beta,)), "steps": a.steps,
+                   "thermalisation": a.therm, "flow": {"layers": a.layers, "bins": a.bins, "hidden": 32}},
+        "arms": res,
+    }
+    print(json.dumps(line), flush=True)
+    if flow is not None:
+        flow.close()
+
+
+def main(argv=None):
+    p = argparse.ArgumentParser(prog="python -m qcdgpu_amd.c4")
+    sub = p.add_subparsers(dest="cmd", required=True)
+    g = sub.add_parser("gen")
+    g.add_argument("--out", required=True)
+    g.add_argument("--n", type=int, default=20000)
+    g.add_argument("--therm", type=int, default=200)
+    t = sub.add_parser("train")
+    t.add_argument("--data", required=True)
+    t.add_argument("--out", required=True)
+    t.add_argument("--n", type=int, default=20000)
+    t.add_argument("--steps", type=int, default=1000)
+    t.add_argument("--batch", type=int, default=16)
+    # each output of the second convolution sums 2592 inputs: Adam steps of
+    # 1e-3 per weight move the spline parameters by O(1)
+    t.add_argument("--lr", type=float, default=1e-4)
+    c = sub.add_parser("chain")
+    c.add_argument("--params", default=None)
+    c.add_argument("--steps", type=int, default=4000)
+    c.add_argument("--therm", type=int, default=200)
+    c.add_argument("--batch", type=int, default=64)
+    for q in (g, t, c):
+        q.add_argument("--dims", type=int, nargs=4, default=list(DIMS))
+        q.add_argument("--beta", type=float, default=BETA)
+        q.add_argument("--seed", type=int, default=54)
+        q.add_argument("--layers", type=int, default=8)
+        q.add_argument("--bins", type=int, default=4)
+    a = p.parse_args(argv)
+    {"gen": cmd_gen, "train": cmd_train, "chain": cmd_chain}[a.cmd](a)
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -339,18 +339,21 @@ def train_forward_kl(model, data, steps=1000, batch=32, lr=1e-3, seed=0, log_eve
         loss.backward()
         torch.nn.utils.clip_grad_norm_([model.params], 10.0)
         opt.step()
-        hist.append(float(loss))
+        hist.append(float(loss.detach()))
         if log_every and (it % log_every == 0 or it == steps - 1):
             print("  step %5d  -<log q> %.3f" % (it, hist[-1]), flush=True)
     return hist
 
 
 # ---------------------------------------------------------------- chains
-def imh_chain(lat, flow, beta, steps, hb_per_step=1, batch=None, seed=1, rng=None, measure=True, wilson=True):
+def imh_chain(lat, flow, beta, steps, hb_per_step=1, batch=None, seed=1, rng=None, measure=True, wilson=True,
+              log_weights=None):
     """Heatbath chain on `lat` (an SU(2) FP32 Lattice) with one
     independence-Metropolis step using `flow` (or none if flow is None)
     after every hb_per_step heatbath sweeps.  Returns (acceptance rate,
-    seconds)."""
+    seconds).  log_weights (a dict) collects log w = -S - log q of the
+    current states ("current") and of the proposals ("proposal"): the
+    acceptance is min(1, w(proposal) / w(current))."""
     rng = rng or np.random.default_rng(seed)
     batch = batch or (flow.max_batch if flow is not None else 1)
     pool, drawn, acc = [], 0, 0
@@ -368,6 +371,9 @@ def imh_chain(lat, flow, beta, steps, hb_per_step=1, batch=None, seed=1, rng=Non
                 pool = [(links[b], lq[b], S[b]) for b in range(batch)]
             prop, lq_p, S_p = pool.pop()
             log_a = (-S_p + lq_c[0]) - (-S_c[0] + lq_p)
+            if log_weights is not None:
+                log_weights.setdefault("current", []).append(-S_c[0] - lq_c[0])
+                log_weights.setdefault("proposal", []).append(-S_p - lq_p)
             if np.log(rng.random()) < log_a:
                 lat.import_links(to_canonical(prop))
                 acc += 1
