@@ -66,7 +66,7 @@ enum { QCD_STREAM_REFERENCE = 0, QCD_STREAM_FAST = 1 };
 enum { QCD_HEATBATH = 0, QCD_OVERRELAX = 1 };
 
 /* Measurements (bit mask). */
-enum { QCD_MEAS_PLAQUETTE = 1, QCD_MEAS_POLYAKOV = 2, QCD_MEAS_WILSON = 4 };
+enum { QCD_MEAS_PLAQUETTE = 1, QCD_MEAS_POLYAKOV = 2, QCD_MEAS_WILSON = 4, QCD_MEAS_POLYAKOV_SLICES = 8 };
 
 typedef struct {
     int      sun;            /* 2 or 3 (GROUP)                                  */
@@ -155,6 +155,13 @@ int  qcd_read_measurements(qcd_lattice *lat, double *plq, double *poly, int *cou
         three spatial directions (the reference divides by 3V).            */
 int  qcd_lattice_set_wilson(qcd_lattice *lat, int R, int T);
 int  qcd_read_wilson(qcd_lattice *lat, double *w, int *count);
+/* ---- Polyakov loop per slice (PL_level > 2: lattice_polyakov_diff_x/y/z +
+        reduce_polyakov_diff_*_double2, polyakov.cl:142-560): a
+        QCD_MEAS_POLYAKOV_SLICES measurement (it implies QCD_MEAS_POLYAKOV)
+        appends, for every slice x = 0..N1-1, y = 0..N2-1, z = 0..N3-1, the
+        sums (sum Re Tr P, sum Im Tr P) over the slice's spatial sites.
+        out[(k*(N1+N2+N3) + slice)*2 + c], slices in the order x, y, z.      */
+int  qcd_read_polyakov_slices(qcd_lattice *lat, double *out, int *count);
 int  qcd_reset_measurements(qcd_lattice *lat);
 
 /* ---- graphs: one captured cycle of n_hb heatbath sweeps, n_or

@@ -18,7 +18,7 @@ START_HOT, START_COLD, START_GID = 0, 1, 2
 PRNG_XOR128, PRNG_RANLUX, PRNG_PM = 1, 2, 3
 STREAM_REFERENCE, STREAM_FAST = 0, 1
 HEATBATH, OVERRELAX = 0, 1
-MEAS_PLAQUETTE, MEAS_POLYAKOV, MEAS_WILSON = 1, 2, 4
+MEAS_PLAQUETTE, MEAS_POLYAKOV, MEAS_WILSON, MEAS_POLYAKOV_SLICES = 1, 2, 4, 8
 TUNE_XCD_MAP, TUNE_BLOCKS_PER_CU, TUNE_UPDATE_SHAPE = 1, 2, 3
 
 
@@ -94,6 +94,7 @@ _SIGNATURES = {
     "qcd_reset_measurements": [_P],
     "qcd_lattice_set_wilson": [_P, ctypes.c_int, ctypes.c_int],
     "qcd_read_wilson": [_P, _D, _I],
+    "qcd_read_polyakov_slices": [_P, _D, _I],
     "qcd_cycle_capture": [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int],
     "qcd_cycle_run": [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int],
     "qcd_cycle_launch": [_P, ctypes.c_int],

@@ -982,8 +982,9 @@ struct FinArgs {
     double* plq_rec;
     double* poly_rec;
     double* wil_rec;
-    int* rec_index;   // [0] plaquette, [1] polyakov, [2] Wilson loop
+    int* rec_index;   // [0] plaquette, [1] polyakov, [2] Wilson loop, [3] Polyakov slices
     int capacity;
+    int slices = 0;   // k_poly_slices wrote record rec_index[3]: advance it
 };
 
 __device__ __forceinline__ double block_sum_1(double v)
@@ -1052,6 +1053,7 @@ __device__ __forceinline__ void finalize_body(const FinArgs& f)
             f.rec_index[2] = rec + 1;
         }
     }
+    if (f.slices && t == 0) f.rec_index[3] += 1;
 }
 
 __global__ __launch_bounds__(kBlock) void k_finalize(FinArgs f) { finalize_body(f); }
@@ -1157,7 +1159,8 @@ __global__ __launch_bounds__(kBlock) void k_wilson(void* table, Geom g, int wr, 
 
 // Polyakov-loop body for workgroup `blk` (8 lanes per spatial site).
 template <typename R, int N, int NREAL>
-__device__ __forceinline__ void polyakov_body(void* table, const Geom& g, double* partials, uint32_t blk)
+__device__ __forceinline__ void polyakov_body(void* table, const Geom& g, double* partials, uint32_t blk,
+                                              double* site_tr = nullptr)
 {
     const uint32_t tid = blk * kBlock + threadIdx.x;
     const uint32_t j = tid >> 3;
@@ -1197,14 +1200,60 @@ __device__ __forceinline__ void polyakov_body(void* table, const Geom& g, double
         v[1] = im;
         v[2] = a2;
         v[3] = a2 * a2;
+        if (site_tr) {   // per spatial site, for the slice sums
+            site_tr[2 * j] = re;
+            site_tr[2 * j + 1] = im;
+        }
     }
     block_partial<4>(v, partials, blk);
 }
 
 template <typename R, int N, int NREAL>
-__global__ __launch_bounds__(kBlock) void k_polyakov(void* table, Geom g, double* partials)
+__global__ __launch_bounds__(kBlock) void k_polyakov(void* table, Geom g, double* partials, double* site_tr)
 {
-    polyakov_body<R, N, NREAL>(table, g, partials, blockIdx.x);
+    polyakov_body<R, N, NREAL>(table, g, partials, blockIdx.x, site_tr);
+}
+
+// Polyakov loop per slice (lattice_polyakov_diff_x/y/z + reduce_polyakov_
+// diff_*_double2, polyakov.cl:142-560): workgroup b sums Tr P over the
+// spatial sites of slice x = b (b < N1), y = b - N1, or z = b - N1 - N2, in
+// a fixed order, into record rec_index[3] at slot b: (sum Re, sum Im).  The
+// finaliser advances rec_index[3].
+__global__ __launch_bounds__(kBlock) void k_poly_slices(const double* site_tr, Geom g, double* slice_rec, int cap,
+                                                        const int* rec_index)
+{
+    __shared__ double red[2][kBlock];
+    const uint32_t b = blockIdx.x, n1 = (uint32_t)g.n1, n2 = (uint32_t)g.n2, n3 = (uint32_t)g.n3;
+    const int d = b < n1 ? 0 : (b < n1 + n2 ? 1 : 2);
+    const uint32_t v = d == 0 ? b : (d == 1 ? b - n1 : b - n1 - n2);
+    const uint32_t count = n1 * n2 * n3 / (d == 0 ? n1 : (d == 1 ? n2 : n3));
+    double re = 0.0, im = 0.0;
+    for (uint32_t k = threadIdx.x; k < count; k += kBlock) {
+        uint32_t j;
+        if (d == 0) j = (k % n2) + n2 * ((k / n2) + n3 * v);        // x = v: y, z
+        else if (d == 1) j = v + n2 * ((k % n3) + n3 * (k / n3));   // y = v: z, x
+        else j = (k % n2) + n2 * (v + n3 * (k / n2));               // z = v: y, x
+        re += site_tr[2 * j];
+        im += site_tr[2 * j + 1];
+    }
+    red[0][threadIdx.x] = re;
+    red[1][threadIdx.x] = im;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + w];
+            red[1][threadIdx.x] += red[1][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const int rec = rec_index[3];
+        if (rec < cap) {
+            double* o = slice_rec + 2 * ((size_t)rec * (n1 + n2 + n3) + b);
+            o[0] = red[0][0];
+            o[1] = red[1][0];
+        }
+    }
 }
 
 // ---------------------------------------------------------------- batched ensembles
@@ -1320,6 +1369,9 @@ struct qcd_lattice {
     double* poly_rec = nullptr;
     double* wil_rec = nullptr;          // Wilson loop records
     double* part_wil = nullptr;         //   wil_blocks (k_wilson)
+    double* site_tr = nullptr;          // Tr P per spatial site (2 doubles), for slice sums
+    double* slice_rec = nullptr;        // Polyakov slice records: [rec][N1 + N2 + N3][2]
+    bool want_slices = false;           // the Polyakov launch being enqueued feeds the slice sums
     uint32_t wil_blocks = 0;
     int wil_R = 7, wil_T = 2;           // wilson_R, wilson_T defaults of QCDGPU.cpp
     uint32_t upd_blocks = 0, poly_blocks = 0;
@@ -1361,6 +1413,12 @@ int fail(int code, const char* fmt, ...)
     } while (0)
 
 uint32_t align_up(uint32_t x, uint32_t a) { return (1u + (x - 1u) / a) * a; }
+
+// Polyakov slice records: max_measurements x (N1 + N2 + N3) x (Re, Im)
+size_t slice_rec_doubles(const qcd_lattice* L)
+{
+    return (size_t)2 * (L->p.n[0] + L->p.n[1] + L->p.n[2]) * (size_t)L->p.max_measurements;
+}
 
 // Optional per-launch extras: fused plaquette measurement and a pair of HIP
 // events recorded at the start and end of the kernel itself
@@ -1499,7 +1557,11 @@ int launch_simple(qcd_lattice* L, int which, const LaunchOpt& o)
             break;
         case 4:
             launch(k_polyakov<R, N, NREAL>, dim3(L->poly_blocks), dim3(kBlock), L->stream, o, L->table, L->g,
-                   L->part_poly);
+                   L->part_poly, L->want_slices ? L->site_tr : (double*)nullptr);
+            if (L->want_slices)
+                hipLaunchKernelGGL(k_poly_slices, dim3((unsigned)(L->g.n1 + L->g.n2 + L->g.n3)), dim3(kBlock), 0,
+                                   L->stream, (const double*)L->site_tr, L->g, L->slice_rec, L->p.max_measurements,
+                                   (const int*)L->rec_index);
             break;
         case 6:
             hipLaunchKernelGGL((k_wilson<R, N, NREAL>), dim3(L->wil_blocks), dim3(kBlock), 0, L->stream, L->table,
@@ -1545,6 +1607,7 @@ int launch_finalize(qcd_lattice* L, int flags, const LaunchOpt& o)
     f.wil_rec = L->wil_rec;
     f.rec_index = L->rec_index;
     f.capacity = L->p.max_measurements;
+    f.slices = (flags & 16) ? 1 : 0;
     launch(k_finalize, dim3(1), dim3(kBlock), L->stream, o, f);
     HIPCHK(hipGetLastError());
     return QCD_OK;
@@ -1667,7 +1730,9 @@ int qcd_lattice_create(const qcd_params* pp, qcd_lattice** out)
         hipMalloc((void**)&L->rec_index, sizeof(int) * 4) != hipSuccess ||
         hipMalloc((void**)&L->plq_rec, sizeof(double) * 2 * p.max_measurements) != hipSuccess ||
         hipMalloc((void**)&L->poly_rec, sizeof(double) * 4 * p.max_measurements) != hipSuccess ||
-        hipMalloc((void**)&L->wil_rec, sizeof(double) * p.max_measurements) != hipSuccess) {
+        hipMalloc((void**)&L->wil_rec, sizeof(double) * p.max_measurements) != hipSuccess ||
+        hipMalloc((void**)&L->site_tr, sizeof(double) * 2 * vs) != hipSuccess ||
+        hipMalloc((void**)&L->slice_rec, sizeof(double) * slice_rec_doubles(L)) != hipSuccess) {
         qcd_lattice_destroy(L);
         return fail(QCD_E_NOMEM, "device allocation failed (table %zu bytes)", L->table_bytes);
     }
@@ -1749,6 +1814,8 @@ int qcd_lattice_destroy(qcd_lattice* L)
     (void)hipFree(L->plq_rec);
     (void)hipFree(L->poly_rec);
     (void)hipFree(L->wil_rec);
+    (void)hipFree(L->site_tr);
+    (void)hipFree(L->slice_rec);
     if (L->stream) (void)hipStreamDestroy(L->stream);
     delete L;
     return QCD_OK;
@@ -2020,9 +2087,12 @@ static int measure_impl(qcd_lattice* L, int mask, bool fused_plq, const LaunchOp
         flags |= 2;
     }
     if (rc) return rc;
-    if (mask & QCD_MEAS_POLYAKOV) {
+    if (mask & (QCD_MEAS_POLYAKOV | QCD_MEAS_POLYAKOV_SLICES)) {
+        L->want_slices = (mask & QCD_MEAS_POLYAKOV_SLICES) != 0;
         rc = DISPATCH(L, launch_simple, L, 4, o_poly);
+        L->want_slices = false;
         flags |= 4;
+        if (mask & QCD_MEAS_POLYAKOV_SLICES) flags |= 16;
     }
     if (rc) return rc;
     if (mask & QCD_MEAS_WILSON) {
@@ -2051,6 +2121,19 @@ int qcd_read_measurements(qcd_lattice* L, double* plq, double* poly, int* count_
     if (poly && nq) HIPCHK(hipMemcpy(poly, L->poly_rec, sizeof(double) * 4 * nq, hipMemcpyDeviceToHost));
     if (count_plq) *count_plq = idx[0];
     if (count_poly) *count_poly = idx[1];
+    return QCD_OK;
+}
+
+int qcd_read_polyakov_slices(qcd_lattice* L, double* out, int* count)
+{
+    if (!L) return fail(QCD_E_ARG, "null lattice");
+    int idx[4];
+    HIPCHK(hipMemcpyAsync(idx, L->rec_index, sizeof(idx), hipMemcpyDeviceToHost, L->stream));
+    HIPCHK(hipStreamSynchronize(L->stream));
+    const int n = idx[3] < L->p.max_measurements ? idx[3] : L->p.max_measurements;
+    const size_t per = 2 * (size_t)(L->g.n1 + L->g.n2 + L->g.n3);
+    if (out && n) HIPCHK(hipMemcpy(out, L->slice_rec, sizeof(double) * per * n, hipMemcpyDeviceToHost));
+    if (count) *count = idx[3];
     return QCD_OK;
 }
 
@@ -2326,7 +2409,8 @@ size_t rng_state_bytes(const qcd_lattice* L)
 
 size_t state_image_bytes(const qcd_lattice* L)
 {
-    return kStateHead + L->table_bytes + rng_state_bytes(L) + sizeof(double) * 7 * (size_t)L->p.max_measurements;
+    return kStateHead + L->table_bytes + rng_state_bytes(L) + sizeof(double) * 7 * (size_t)L->p.max_measurements +
+           sizeof(double) * slice_rec_doubles(L);
 }
 
 uint64_t f64_bits(double x)
@@ -2443,6 +2527,8 @@ int qcd_lattice_state_save(qcd_lattice* L, void* buf, size_t bytes)
     HIPCHK(hipMemcpy(w.p, L->poly_rec, sizeof(double) * 4 * m, hipMemcpyDeviceToHost));
     w.p += sizeof(double) * 4 * m;
     HIPCHK(hipMemcpy(w.p, L->wil_rec, sizeof(double) * m, hipMemcpyDeviceToHost));
+    w.p += sizeof(double) * m;
+    HIPCHK(hipMemcpy(w.p, L->slice_rec, sizeof(double) * slice_rec_doubles(L), hipMemcpyDeviceToHost));
     return QCD_OK;
 }
 
@@ -2479,6 +2565,8 @@ int qcd_lattice_state_load(qcd_lattice* L, const void* buf, size_t bytes)
     HIPCHK(hipMemcpy(L->poly_rec, r.p, sizeof(double) * 4 * m, hipMemcpyHostToDevice));
     r.p += sizeof(double) * 4 * m;
     HIPCHK(hipMemcpy(L->wil_rec, r.p, sizeof(double) * m, hipMemcpyHostToDevice));
+    r.p += sizeof(double) * m;
+    HIPCHK(hipMemcpy(L->slice_rec, r.p, sizeof(double) * slice_rec_doubles(L), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(L->rec_index, idx, sizeof(idx), hipMemcpyHostToDevice));
     L->prng_counter = counter;
     L->rng_ready = ready != 0;

@@ -33,6 +33,7 @@
 // features outside this path (F_munu, twisted boundary conditions, BIGLAT)
 // are rejected when they ask for the feature.
 
+#include <algorithm>
 #include <cerrno>
 #include <chrono>
 #include <cmath>
@@ -347,7 +348,8 @@ int main(int argc, char** argv)
     qcd_lattice* lat = nullptr;
     CHECK(qcd_lattice_create(&p, &lat));
     const int start = st.ints == 1 ? QCD_START_COLD : (st.ints == 2 ? QCD_START_GID : QCD_START_HOT);
-    const int mask = QCD_MEAS_PLAQUETTE | (st.pl_level > 0 ? QCD_MEAS_POLYAKOV : 0) | (st.wilson ? QCD_MEAS_WILSON : 0);
+    const int mask = QCD_MEAS_PLAQUETTE | (st.pl_level > 0 ? QCD_MEAS_POLYAKOV : 0) | (st.wilson ? QCD_MEAS_WILSON : 0) |
+                     (st.pl_level > 2 ? QCD_MEAS_POLYAKOV_SLICES : 0);
     if (st.wilson) CHECK(qcd_lattice_set_wilson(lat, st.wilson_r, st.wilson_t));
     auto sweep = [&]() {
         CHECK(qcd_sweep(lat, QCD_HEATBATH));
@@ -425,10 +427,17 @@ int main(int argc, char** argv)
         printf(" state: %s\n", last_state.c_str());
     }
 
-    std::vector<double> plq(2 * (size_t)st.iter), poly(4 * (size_t)st.iter), wil((size_t)st.iter);
-    int np = 0, nq = 0, nw = 0;
+    const size_t cap = (size_t)p.max_measurements;   // >= ITER (a continued run keeps the file's capacity)
+    const int nslice = st.n[0] + st.n[1] + st.n[2];
+    std::vector<double> plq(2 * cap), poly(4 * cap), wil(cap), slc(2 * (size_t)nslice * cap);
+    int np = 0, nq = 0, nw = 0, ns = 0;
     CHECK(qcd_read_measurements(lat, plq.data(), poly.data(), &np, &nq));
     if (st.wilson) CHECK(qcd_read_wilson(lat, wil.data(), &nw));
+    if (st.pl_level > 2) CHECK(qcd_read_polyakov_slices(lat, slc.data(), &ns));
+    np = std::min(np, (int)cap);
+    nq = std::min(nq, (int)cap);
+    nw = std::min(nw, (int)cap);
+    ns = std::min(ns, (int)cap);
     CHECK(qcd_lattice_destroy(lat));
     CHECK(qcd_device_close());
 
@@ -512,6 +521,40 @@ int main(int argc, char** argv)
     fprintf(out, " run time %.6f s\n", secs);
     fprintf(out, " ***************************************************\n");
     for (auto* c : cols) fprintf(out, " %-21s mean % 16.13e variance % 16.13e\n", c->name, c->mean, c->variance);
+    if (st.pl_level > 2 && ns > 0) {
+        // lattice_write_results' differentiated Polyakov loop block
+        // (suncl.cpp:1695-1751); every direction is normalised by N2 N3 N as
+        // in lattice_analysis (suncl.cpp:1516,1541,1566)
+        fprintf(out, " ***************************************************\n");
+        fprintf(out, "Differentiated Polyakov loop data (#, PL, PL_im, PL_variance, PL_im_variance):\n");
+        const double den = (double)st.n[1] * st.n[2] * N;
+        const char* lab[3] = {"X", "Y", "Z"};
+        int off = 0;
+        for (int d = 0; d < 3; ++d) {
+            std::vector<Series> re(st.n[d], Series{"PL"}), im(st.n[d], Series{"PL_im"});
+            for (int s = 0; s < st.n[d]; ++s) {
+                for (int i = 0; i < ns; ++i) {
+                    const double* r = &slc[2 * ((size_t)i * nslice + off + s)];
+                    re[s].v.push_back(r[0] / den);
+                    im[s].v.push_back(r[1] / den);
+                }
+                re[s].analyse();
+                im[s].analyse();
+            }
+            fprintf(out, "\n%-15s", lab[d]);
+            for (int s = 0; s < st.n[d]; ++s) fprintf(out, "%2u                   ", (unsigned)s);
+            fprintf(out, "\nPL             ");
+            for (int s = 0; s < st.n[d]; ++s) fprintf(out, "% 16.13e ", re[s].mean);
+            fprintf(out, "\nPL_im          ");
+            for (int s = 0; s < st.n[d]; ++s) fprintf(out, "% 16.13e ", im[s].mean);
+            fprintf(out, "\nPL_variance    ");
+            for (int s = 0; s < st.n[d]; ++s) fprintf(out, "% 16.13e ", re[s].variance);
+            fprintf(out, "\nPL_im_variance ");
+            for (int s = 0; s < st.n[d]; ++s) fprintf(out, "% 16.13e ", im[s].variance);
+            fprintf(out, "\n");
+            off += st.n[d];
+        }
+    }
     fprintf(out, " ***************************************************\n");
     fprintf(out, " Data fields:\n    #,  ");
     for (auto* c : cols) fprintf(out, "%-21s ", c->name);

@@ -147,13 +147,32 @@ class Lattice:
         check(lib().qcd_lattice_sync(self._h))
 
     # -- measurements ---------------------------------------------------------
-    def measure(self, plaquette=True, polyakov=True, wilson=False):
+    def measure(self, plaquette=True, polyakov=True, wilson=False, polyakov_slices=False):
         mask = (
             (_abi.MEAS_PLAQUETTE if plaquette else 0)
             | (_abi.MEAS_POLYAKOV if polyakov else 0)
             | (_abi.MEAS_WILSON if wilson else 0)
+            | (_abi.MEAS_POLYAKOV_SLICES if polyakov_slices else 0)
         )
         check(lib().qcd_measure(self._h, mask))
+
+    def polyakov_slices(self, raw=False):
+        """Polyakov loop per x, y and z slice for every polyakov_slices
+        measurement: dict x/y/z -> complex array (count, N_d).  Raw: sums of
+        Tr P over the slice; otherwise divided by N2 N3 N, the denominator
+        the reference's analysis uses for all three directions
+        (suncl.cpp:1516, 1541, 1566)."""
+        n1, n2, n3 = self.dims[:3]
+        cap = self.params.max_measurements
+        a = np.zeros((cap, n1 + n2 + n3, 2))
+        c = ctypes.c_int()
+        check(lib().qcd_read_polyakov_slices(self._h, a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                             ctypes.byref(c)))
+        a = a[: min(c.value, cap)]
+        z = a[..., 0] + 1j * a[..., 1]
+        if not raw:
+            z = z / (n2 * n3 * self.sun)
+        return {"x": z[:, :n1], "y": z[:, n1:n1 + n2], "z": z[:, n1 + n2:]}
 
     def set_wilson(self, R, T):
         """Extents of the Wilson loop measured with measure(wilson=True):
@@ -318,11 +337,12 @@ class Lattice:
         return info.as_dict()
 
     # -- the reference run loop -------------------------------------------------
-    def simulate(self, nav=0, iters=2, niter=1, n_or=0, start="cold", wilson=False):
+    def simulate(self, nav=0, iters=2, niter=1, n_or=0, start="cold", wilson=False, slices=False):
         """model::lattice_simulate: start, measure, NAV sweeps, then ITER-1
         measurements each after NITER sweeps (suncl.cpp:5236-5483).  Each sweep
         is one heatbath sweep plus `n_or` overrelaxation sweeps; wilson=True
-        adds the Wilson loop (set_wilson) to every measurement."""
+        adds the Wilson loop (set_wilson) to every measurement, slices=True the
+        Polyakov loop per slice (PL_level > 2)."""
         if start == "hot":
             self.prng_init()   # the Haar start draws from the streams
             self.init(start)
@@ -331,7 +351,7 @@ class Lattice:
             if start != "gid":
                 self.prng_init()
         self.reset_measurements()
-        self.measure(wilson=wilson)
+        self.measure(wilson=wilson, polyakov_slices=slices)
         for _ in range(nav):
             self.sweep("hb")
             for _ in range(n_or):
@@ -341,7 +361,7 @@ class Lattice:
                 self.sweep("hb")
                 for _ in range(n_or):
                     self.sweep("or")
-            self.measure(wilson=wilson)
+            self.measure(wilson=wilson, polyakov_slices=slices)
         self.sync()
         return self.measurements()
 

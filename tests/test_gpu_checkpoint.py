@@ -66,7 +66,9 @@ def _same_run(a, b):
 
 
 def _rng_slice(lat, img):
-    n = img.size - STATE_HEAD - lat.table_bytes() - 7 * 8 * lat.params.max_measurements
+    m = lat.params.max_measurements
+    slices = 2 * 8 * (lat.dims[0] + lat.dims[1] + lat.dims[2]) * m   # Polyakov slice records
+    n = img.size - STATE_HEAD - lat.table_bytes() - 7 * 8 * m - slices
     return slice(STATE_HEAD + lat.table_bytes(), STATE_HEAD + lat.table_bytes() + n)
 
 

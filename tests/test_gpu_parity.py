@@ -501,3 +501,46 @@ def test_wilson_loop_vs_oracle(O, sun, prec, R, T):
     assert len(w) == 1 and abs(w[0] - ref) <= rtol * abs(ref), (w, ref)
     if (R, T) == (1, 1):
         assert abs(w[0] - plq[-1][1]) <= rtol * abs(plq[-1][1])
+
+
+# ---------------------------------------------------------------- Polyakov slices
+def _polyakov_traces(links, dims, N):
+    """Tr prod_t U_T(x,y,z,t) per spatial site from canonical links, by
+    explicit complex products (site = y + z N2 + t N2 N3 + x N2 N3 N4)."""
+    n1, n2, n3, n4 = dims
+    U = links[..., 0] + 1j * links[..., 1]            # (V, 4, N, N)
+    UT = U[:, 3].reshape(n1, n4, n3, n2, N, N)         # x, t, z, y
+    P = np.broadcast_to(np.eye(N), (n1, n3, n2, N, N)).copy()
+    for t in range(n4):
+        P = P @ UT[:, t]
+    return np.trace(P, axis1=-2, axis2=-1)             # (x, z, y)
+
+
+@pytest.mark.parametrize("sun,prec", [(3, "fp64"), (2, "fp32")])
+def test_polyakov_slices_vs_explicit_products(sun, prec):
+    """Per-slice sums of Tr P (PL_level > 2) against explicit products on a
+    lattice with unequal spatial extents; the x slices add up to the total
+    Polyakov sum; a cold start gives N per site."""
+    _case("polyakov slices SU(%d) %s" % (sun, prec))
+    dims = (6, 4, 2, 4)
+    tol = 1e-12 if prec == "fp64" else 2e-5
+    with _lat(sun, dims, prec, prng="XOR128", max_measurements=8) as lat:
+        lat.init("cold")
+        lat.prng_init()
+        lat.reset_measurements()
+        lat.measure(polyakov_slices=True)
+        s0 = lat.polyakov_slices(raw=True)
+        assert np.allclose(s0["x"], sun * dims[1] * dims[2]) and np.allclose(s0["y"], sun * dims[0] * dims[2])
+        assert np.allclose(s0["z"], sun * dims[0] * dims[1])
+        for _ in range(2):
+            lat.sweep("hb")
+        lat.measure(polyakov_slices=True)
+        tr = _polyakov_traces(lat.export_links(), dims, sun)
+        s = lat.polyakov_slices(raw=True)
+        assert s["x"].shape == (2, 6) and s["y"].shape == (2, 4) and s["z"].shape == (2, 2)
+        want = {"x": tr.sum(axis=(1, 2)), "y": tr.sum(axis=(0, 1)), "z": tr.sum(axis=(0, 2))}
+        for k in "xyz":
+            assert np.max(np.abs(s[k][1] - want[k])) < tol * tr.size, k
+        _, poly = lat.raw_measurements()
+        assert abs(s["x"][1].sum().real - poly[1, 0]) < 1e-9 * tr.size
+        assert abs(s["y"][1].sum().imag - poly[1, 1]) < 1e-9 * tr.size

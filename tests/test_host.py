@@ -83,21 +83,31 @@ def test_host_run_matches_python_driver(tmp_path, prng, ints):
     assert r.returncode == 0, r.stderr
     out = [p for p in os.listdir(tmp_path) if p.startswith("su3-") and p.endswith(".txt")]
     assert len(out) == 1
-    rows = []
+    rows, pl_rows, section = [], {}, None
     for line in open(tmp_path / out[0]):
         parts = line.split()
         if len(parts) == 12 and parts[0].isdigit():
             rows.append([float(x) for x in parts[1:]])
+        if parts and parts[0] in ("X", "Y", "Z") and len(parts) == 9:
+            section = parts[0]
+        elif parts and parts[0] in ("PL", "PL_im") and section:
+            pl_rows[section, parts[0]] = np.array([float(x) for x in parts[1:]])
     rows = np.array(rows)
     assert rows.shape == (4, 11)   # Plq x3, Wilson_loop, S x3, Polyakov x4
+    assert len(pl_rows) == 6        # PL_LEVEL 3 (the default): the differentiated Polyakov loop block
     lux = int(prng[-1]) if prng.startswith("RANLUX") else 3
     name = "RANLUX" if prng.startswith("RANLUX") else prng
     with Q.Lattice(sun=3, dims=(8, 8, 8, 4), beta=5.9, precision="fp64", prng=name, ranlux_luxury=lux,
                    randseries=11, max_measurements=8) as lat:
         lat.set_wilson(2, 2)
-        m = lat.simulate(nav=1, iters=4, niter=2, n_or=2, start={0: "hot", 1: "cold"}[ints], wilson=True)
+        m = lat.simulate(nav=1, iters=4, niter=2, n_or=2, start={0: "hot", 1: "cold"}[ints], wilson=True,
+                         slices=True)
         wl = lat.wilson_measurements()
+        sl = lat.polyakov_slices()
     assert np.allclose(rows[:, 0], m["plq_spat"], rtol=1e-13, atol=0)
     assert np.allclose(rows[:, 1], m["plq_temp"], rtol=1e-13, atol=0)
     assert np.allclose(rows[:, 3], wl, rtol=1e-12, atol=1e-15)
     assert np.allclose(rows[:, 7], m["polyakov_re"], rtol=1e-12, atol=1e-15)
+    for d in "xyz":   # slice means without measurement 0, as lattice_data_analysis
+        assert np.allclose(pl_rows[d.upper(), "PL"], sl[d][1:].real.mean(axis=0), rtol=1e-11, atol=1e-14)
+        assert np.allclose(pl_rows[d.upper(), "PL_im"], sl[d][1:].imag.mean(axis=0), rtol=1e-11, atol=1e-14)
