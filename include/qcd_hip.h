@@ -66,7 +66,14 @@ enum { QCD_STREAM_REFERENCE = 0, QCD_STREAM_FAST = 1 };
 enum { QCD_HEATBATH = 0, QCD_OVERRELAX = 1 };
 
 /* Measurements (bit mask). */
-enum { QCD_MEAS_PLAQUETTE = 1, QCD_MEAS_POLYAKOV = 2, QCD_MEAS_WILSON = 4, QCD_MEAS_POLYAKOV_SLICES = 8 };
+enum {
+    QCD_MEAS_PLAQUETTE = 1,
+    QCD_MEAS_POLYAKOV = 2,
+    QCD_MEAS_WILSON = 4,
+    QCD_MEAS_POLYAKOV_SLICES = 8,
+    QCD_MEAS_FMUNU = 16,   /* F_munu, H field: planes (x,y), (x,z), (y,z)  */
+    QCD_MEAS_F0MU = 32     /* F_munu, E field: planes (x,t), (y,t), (z,t)  */
+};
 
 typedef struct {
     int      sun;            /* 2 or 3 (GROUP)                                  */
@@ -162,6 +169,22 @@ int  qcd_read_wilson(qcd_lattice *lat, double *w, int *count);
         sums (sum Re Tr P, sum Im Tr P) over the slice's spatial sites.
         out[(k*(N1+N2+N3) + slice)*2 + c], slices in the order x, y, z.      */
 int  qcd_read_polyakov_slices(qcd_lattice *lat, double *out, int *count);
+/* ---- field strength (get_Fmunu / get_F0mu: lattice_measurement_plq with
+        FMUNU / F0MU, sun_measurements_cl.cl:710-1173, trace_lambda* and
+        trace_sigma*, su3_measurements_cl.cl:357-460,
+        su2_measurements_cl.cl:89-123).  A QCD_MEAS_FMUNU or QCD_MEAS_F0MU
+        measurement (one of the two; either implies QCD_MEAS_PLAQUETTE)
+        stores, beside that plaquette record, for the two generators and the
+        three planes, sum over sites of F = -i Tr(P G) / 2 with
+        P = U_a(x) U_b(x+a) U_a(x+b)^+ U_b(x)^+ in FP64:
+        out[rec*12 + gen*6 + plane*2 + (0 re, 1 im)].  Records of plaquette
+        measurements taken without F hold zeros.  SU(2) imaginary parts are 0
+        (the reference's trace_sigma*).                                       */
+/* Generators (Fmunu_index1/2, suncl.cpp:2093-2113; init keys FMUNU1/2/4 and
+   FMUNU5/6/7): SU(3) gen1 in 1..4, gen2 in 5..8 (Gell-Mann lambda_k, default
+   3 and 8); SU(2) gen1 1 or 3, gen2 2 (Pauli sigma_k, default 3 and 2).   */
+int  qcd_lattice_set_fmunu(qcd_lattice *lat, int gen1, int gen2);
+int  qcd_read_fmunu(qcd_lattice *lat, double *out, int *count);
 int  qcd_reset_measurements(qcd_lattice *lat);
 
 /* ---- graphs: one captured cycle of n_hb heatbath sweeps, n_or
@@ -228,7 +251,12 @@ int  qcd_write_measurements(qcd_lattice *lat, const double *plq, const double *p
    Im u12)).  Sizes use buffer_size_align with the reference driver's
    64-item work-group cap (QCDGPU.cpp:74).  After the table this library
    appends "QCDGPUX1", a uint64 byte count and the exact state image; the
-   reference stops reading before it.                                       */
+   reference stops reading before it.  With F_munu (get_fmunu or get_f0mu)
+   the partials and plaquette sections are 7 times longer (suncl.cpp:
+   2266-2298) and the plaquette section holds, after the plaquette sums,
+   the F sums of the three planes for the first generator, then for the
+   second (double2 (re, im) per record; reduce_measurement_plq_double2,
+   sun_measurements_cl.cl:1312-1354).                                      */
 typedef struct {
     unsigned init;            /* INIT (0 = continue a run)            word 4     */
     unsigned start;           /* ints: QCD_START_*                    word 5     */
@@ -238,6 +266,10 @@ typedef struct {
     unsigned wilson_r, wilson_t;                            /* words 16-17      */
     unsigned pl_level;                                      /* word 25          */
     unsigned get_actions, get_plaquettes, get_wilson;       /* words 29-31      */
+    unsigned fmunu_index1, fmunu_index2, get_fmunu;         /* words 26-28      */
+    unsigned get_f0mu;        /* E-field F_munu: not a header word; a file
+                                 with F sections (energies_size_F = 7 x
+                                 energies_size, word 34) and get_fmunu 0       */
 } qcd_run_info;
 /* Writes the state file (atomically: a temporary file renamed into place).  */
 int  qcd_qcg_write(qcd_lattice *lat, const char *path, const qcd_run_info *info);

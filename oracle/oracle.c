@@ -25,6 +25,9 @@
  *   overrelaxation ............. no reference code; pinned by invariants
  *                                (ReTr(U A) preserved, involution)
  *   Polyakov loop .............. closed forms (cold start, gauge invariance)
+ *   F_munu ..................... closed form (constant abelian flux along
+ *                                each generator: F = V sin b) and the
+ *                                explicit Gell-Mann / Pauli matrix traces
  */
 #include "oracle.h"
 
@@ -831,6 +834,79 @@ void orc_plaquette(const orc_lattice *L, const double *links, double plq[2])
     }
     plq[0] = sp;
     plq[1] = tm;
+}
+
+/* F_munu (lattice_measurement_plq with FMUNU / F0MU,
+   sun_measurements_cl.cl:710-1173): for each site and plane the plaquette
+   P = U_a(x) U_b(x+a) U_a(x+b)^+ U_b(x)^+ and F = -i Tr(P G) / 2 for the
+   generator G, written out element by element as trace_lambda1..8
+   (su3_measurements_cl.cl:357-460) and trace_sigma1..3
+   (su2_measurements_cl.cl:89-123; SU(2) imaginary part 0 and no 1/2: its
+   P is stored as the first row (u1, u2) only).  Planes (x,y), (x,z), (y,z)
+   or, efield = 1, (x,t), (y,t), (z,t).
+   out[6*g + 2*plane + (0 re, 1 im)] summed over sites, g = 0 for gen1.    */
+static void plaq_matrix(const orc_lattice *L, const double *links, unsigned s, int a, int b, double *P)
+{
+    int N = L->sun;
+    double m3[18], m4[18], t1[18], t2[18];
+    unsigned xa = orc_neighbour(L, s, a, +1), xb = orc_neighbour(L, s, b, +1);
+    mat_dagger(N, CLINK(L, links, xb, a), m3);
+    mat_dagger(N, CLINK(L, links, s, b), m4);
+    mat_mul(N, CLINK(L, links, s, a), CLINK(L, links, xa, b), t1);
+    mat_mul(N, t1, m3, t2);
+    mat_mul(N, t2, m4, P);
+}
+
+static void trace_gen3(const double *P, int k, double *re, double *im)
+{
+#define RE(r, c) P[2 * ((r) * 3 + (c))]
+#define IM(r, c) P[2 * ((r) * 3 + (c)) + 1]
+    double a = 0.0, b = 0.0;
+    switch (k) {
+        case 1: a = IM(0, 1) + IM(1, 0); b = -RE(0, 1) - RE(1, 0); break;
+        case 2: a = RE(0, 1) - RE(1, 0); b = IM(0, 1) - IM(1, 0); break;
+        case 3: a = IM(0, 0) - IM(1, 1); b = -RE(0, 0) + RE(1, 1); break;
+        case 4: a = IM(0, 2) + IM(2, 0); b = -RE(0, 2) - RE(2, 0); break;
+        case 5: a = RE(0, 2) - RE(2, 0); b = IM(0, 2) - IM(2, 0); break;
+        case 6: a = IM(1, 2) + IM(2, 1); b = -RE(1, 2) - RE(2, 1); break;
+        case 7: a = RE(1, 2) - RE(2, 1); b = IM(1, 2) - IM(2, 1); break;
+        default:
+            a = (IM(0, 0) + IM(1, 1) - 2.0 * IM(2, 2)) * 0.57735026918962576451;
+            b = (-RE(0, 0) - RE(1, 1) + 2.0 * RE(2, 2)) * 0.57735026918962576451;
+    }
+#undef RE
+#undef IM
+    *re = a / 2;
+    *im = b / 2;
+}
+
+static void trace_gen2(const double *P, int k, double *re, double *im)
+{
+    /* first row u1 = P[0][0], u2 = P[0][1] */
+    *re = (k == 1) ? P[3] : (k == 2) ? P[2] : P[1];
+    *im = 0.0;
+}
+
+void orc_fmunu(const orc_lattice *L, const double *links, int efield, int gen1, int gen2, double out[12])
+{
+    static const int pa_h[3] = {0, 0, 1}, pb_h[3] = {1, 2, 2};
+    static const int pa_e[3] = {0, 1, 2}, pb_e[3] = {3, 3, 3};
+    const int *pa = efield ? pa_e : pa_h, *pb = efield ? pb_e : pb_h;
+    const int gens[2] = {gen1, gen2};
+    unsigned V = orc_sites(L);
+    double P[18];
+    for (int k = 0; k < 12; ++k) out[k] = 0.0;
+    for (unsigned s = 0; s < V; ++s)
+        for (int pl = 0; pl < 3; ++pl) {
+            plaq_matrix(L, links, s, pa[pl], pb[pl], P);
+            for (int g = 0; g < 2; ++g) {
+                double re, im;
+                if (L->sun == 3) trace_gen3(P, gens[g], &re, &im);
+                else trace_gen2(P, gens[g], &re, &im);
+                out[6 * g + 2 * pl] += re;
+                out[6 * g + 2 * pl + 1] += im;
+            }
+        }
 }
 
 /* R x T Wilson loop summed over sites and the three spatial directions d

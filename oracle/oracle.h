@@ -122,6 +122,10 @@ void     orc_plaquette(const orc_lattice *L, const double *links, double plq[2])
    (spatial extent R along x, y, z; temporal extent T along t).             */
 double   orc_wilson(const orc_lattice *L, const double *links, int R, int T);
 void     orc_polyakov(const orc_lattice *L, const double *links, double poly[4]);
+/* F_munu sums: out[6*g + 2*plane + (re, im)], generators gen1 (g = 0) and
+   gen2 (g = 1), H-field planes (x,y), (x,z), (y,z) or E-field (x,t), (y,t),
+   (z,t) for efield = 1.                                                    */
+void     orc_fmunu(const orc_lattice *L, const double *links, int efield, int gen1, int gen2, double out[12]);
 /* Sum over links of ReTr(U A) (A = staple), the local action density. */
 double   orc_link_action(const orc_lattice *L, const double *links, unsigned site, int mu);
 

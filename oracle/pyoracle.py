@@ -241,6 +241,13 @@ def plaquette(L, links):
     return out
 
 
+def fmunu(L, links, efield=False, gen1=3, gen2=8):
+    """F_munu sums as complex [generator][plane]."""
+    out = np.zeros(12)
+    lib().orc_fmunu(ctypes.byref(L), _p(links, _F64P), int(efield), int(gen1), int(gen2), _p(out, _F64P))
+    return (out[0::2] + 1j * out[1::2]).reshape(2, 3)
+
+
 def polyakov(L, links):
     out = np.zeros(4)
     lib().orc_polyakov(ctypes.byref(L), _p(links, _F64P), _p(out, _F64P))

@@ -12,6 +12,10 @@ Sources (SU(N)/...):
                         random/random.cpp:66,192-196
   double split          clinterface/clinterface.cpp:325-336 (LOW / HIGH words)
   section order         suncl/suncl.cpp:1889-1901 (lattice_save_state)
+  F_munu sections       suncl/suncl.cpp:2266-2298 (x MODEL_energies_size = 7,
+                        suncl/suncl.h:221), block order of
+                        reduce_measurement_plq_double2
+                        (suncl/sun_measurements_cl.cl:1312-1354)
   sizes                 suncl/suncl.cpp:2266-2298 with buffer_size_align
                         (clinterface/clinterface.cpp:1587-1590) at the
                         64-item work-group cap of QCDGPU.cpp:74
@@ -61,8 +65,9 @@ def sizes(group, dims, iters):
     }
 
 
-def make_header(group, dims, beta, precision, generator, **fields):
-    """The 64 header words; `fields` are FIELDS entries (counters, flags)."""
+def make_header(group, dims, beta, precision, generator, f_sections=False, **fields):
+    """The 64 header words; `fields` are FIELDS entries (counters, flags).
+    f_sections: the partials and plaquette sections carry F_munu (7 x)."""
     w = np.zeros(HEAD_WORDS, dtype=np.uint32)
     w[0], w[1] = string_word("QCDGPU", 0), string_word("QCDGPU", 4)
     w[2] = string_word("1.5", 0)
@@ -71,8 +76,10 @@ def make_header(group, dims, beta, precision, generator, **fields):
         init=1, start=1, randseries=54, generator=GENERATOR[generator], rl_nskip=NSKIP.get(generator, 24),
         prng_counter=0, nav=0, nav_counter=0, niter=1, iter=1, iter_counter=0, nhit=10, wilson_r=7, wilson_t=2,
         precision={"single": 1, "double": 2}[precision], pl_level=3, get_actions=1, get_plaquettes=1,
-        get_wilson=0, measurement_size_F=s["measurement_size_F"], energies_size=s["energies_size"],
-        energies_size_F=s["energies_size"], polyakov_loop_size=s["polyakov_loop_size"], group=group, nd=4,
+        get_wilson=0, measurement_size_F=s["measurement_size_F"] * (7 if f_sections else 1),
+        energies_size=s["energies_size"], energies_size_F=s["energies_size"] * (7 if f_sections else 1),
+        polyakov_loop_size=s["polyakov_loop_size"], group=group, nd=4, fmunu_index1=3,
+        fmunu_index2=8 if group == 3 else 2,
     )
     bits = np.array([beta], dtype=np.float64).view(np.uint32)
     vals["beta_lo"], vals["beta_hi"] = int(bits[0]), int(bits[1])
@@ -137,24 +144,34 @@ def links_from_planes(table, group, V, rowsize):
     return np.stack([L.real, L.imag], axis=-1)
 
 
-def encode(header, links, plq=(), poly=(), wilson=()):
+def _has_f(h):
+    return h["energies_size_F"] != h["energies_size"]
+
+
+def encode(header, links, plq=(), poly=(), wilson=(), fmunu=()):
     """A reference-layout file: header, partials (zeros), action sums from
-    the plaquette sums, plaquette, Wilson and Polyakov (Re, Im) sums, table
-    (float4 for precision word 1, double4 otherwise)."""
+    the plaquette sums, plaquette sums (then, with F sections, the F_munu
+    sums: fmunu[rec, generator, plane] complex), Wilson and Polyakov (Re, Im)
+    sums, table (float4 for precision word 1, double4 otherwise).  Section
+    sizes are the header's words."""
     h = parse_header(header)
     single = h["precision"] == 1
     group, dims = h["group"], h["full"]
     s = sizes(group, dims, h["iter"])
-    out = [np.asarray(header, dtype=np.uint32).tobytes(), np.zeros(2 * s["measurement_size_F"]).tobytes()]
-    en = s["energies_size"]
+    out = [np.asarray(header, dtype=np.uint32).tobytes(), np.zeros(2 * h["measurement_size_F"]).tobytes()]
+    en = h["energies_size"]
     plq = np.asarray(plq, dtype=np.float64).reshape(-1, 2)
     bet, V = h["beta"] / group, float(s["V"])
     act = np.zeros((en, 2))
     act[: len(plq)] = bet * (3.0 * group * V - plq)
     out.append(act.tobytes())
-    if h["get_plaquettes"]:
-        e = np.zeros((en, 2))
+    if h["get_plaquettes"] or h["get_fmunu"] or _has_f(h):
+        e = np.zeros((h["energies_size_F"], 2))
         e[: len(plq)] = plq
+        f = np.asarray(fmunu, dtype=np.complex128).reshape(-1, 6)
+        for b in range(6 if _has_f(h) else 0):   # block 1 + 3 * generator + plane
+            e[(1 + b) * en:(1 + b) * en + len(f), 0] = f[:, b].real
+            e[(1 + b) * en:(1 + b) * en + len(f), 1] = f[:, b].imag
         out.append(e.tobytes())
     if h["get_wilson"]:
         e = np.zeros(en)
@@ -162,7 +179,7 @@ def encode(header, links, plq=(), poly=(), wilson=()):
         e[: len(w)] = w
         out.append(e.tobytes())
     if h["pl_level"] > 0:
-        e = np.zeros((s["polyakov_loop_size"], 2))
+        e = np.zeros((h["polyakov_loop_size"], 2))
         p = np.asarray(poly, dtype=np.float64).reshape(-1, 2)
         e[: len(p)] = p
         out.append(e.tobytes())
@@ -186,14 +203,19 @@ def decode(data):
         pos += a.nbytes
         return a
 
-    out["partials"] = take(2 * s["measurement_size_F"], np.float64, (-1, 2))
-    out["actions"] = take(2 * s["energies_size"], np.float64, (-1, 2))
-    if h["get_plaquettes"]:
-        out["plaquettes"] = take(2 * s["energies_size"], np.float64, (-1, 2))
+    en = h["energies_size"]
+    out["partials"] = take(2 * h["measurement_size_F"], np.float64, (-1, 2))
+    out["actions"] = take(2 * en, np.float64, (-1, 2))
+    if h["get_plaquettes"] or h["get_fmunu"] or _has_f(h):
+        e = take(2 * h["energies_size_F"], np.float64, (-1, 2))
+        out["plaquettes"] = e[:en]
+        if _has_f(h):
+            z = e[en:, 0] + 1j * e[en:, 1]
+            out["fmunu"] = z.reshape(2, 3, en).transpose(2, 0, 1)   # [rec, generator, plane]
     if h["get_wilson"]:
-        out["wilson"] = take(s["energies_size"], np.float64, (-1,))
+        out["wilson"] = take(en, np.float64, (-1,))
     if h["pl_level"] > 0:
-        out["polyakov"] = take(2 * s["polyakov_loop_size"], np.float64, (-1, 2))
+        out["polyakov"] = take(2 * h["polyakov_loop_size"], np.float64, (-1, 2))
     single = h["precision"] == 1
     table = take(4 * s["table_size"], np.float32 if single else np.float64, (-1, 4))
     out["table"] = table

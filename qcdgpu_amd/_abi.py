@@ -19,6 +19,7 @@ PRNG_XOR128, PRNG_RANLUX, PRNG_PM = 1, 2, 3
 STREAM_REFERENCE, STREAM_FAST = 0, 1
 HEATBATH, OVERRELAX = 0, 1
 MEAS_PLAQUETTE, MEAS_POLYAKOV, MEAS_WILSON, MEAS_POLYAKOV_SLICES = 1, 2, 4, 8
+MEAS_FMUNU, MEAS_F0MU = 16, 32
 TUNE_XCD_MAP, TUNE_BLOCKS_PER_CU, TUNE_UPDATE_SHAPE = 1, 2, 3
 
 
@@ -56,6 +57,7 @@ class RunInfo(ctypes.Structure):
         for name in (
             "init", "start", "prng_counter", "nav", "nav_counter", "niter", "iter", "iter_counter",
             "wilson_r", "wilson_t", "pl_level", "get_actions", "get_plaquettes", "get_wilson",
+            "fmunu_index1", "fmunu_index2", "get_fmunu", "get_f0mu",
         )
     ]
 
@@ -95,6 +97,8 @@ _SIGNATURES = {
     "qcd_lattice_set_wilson": [_P, ctypes.c_int, ctypes.c_int],
     "qcd_read_wilson": [_P, _D, _I],
     "qcd_read_polyakov_slices": [_P, _D, _I],
+    "qcd_lattice_set_fmunu": [_P, ctypes.c_int, ctypes.c_int],
+    "qcd_read_fmunu": [_P, _D, _I],
     "qcd_cycle_capture": [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int],
     "qcd_cycle_run": [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int],
     "qcd_cycle_launch": [_P, ctypes.c_int],

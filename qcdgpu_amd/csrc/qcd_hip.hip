@@ -985,6 +985,9 @@ struct FinArgs {
     int* rec_index;   // [0] plaquette, [1] polyakov, [2] Wilson loop, [3] Polyakov slices
     int capacity;
     int slices = 0;   // k_poly_slices wrote record rec_index[3]: advance it
+    // F_munu (k_fmunu): n_plq slots of 12, recorded beside the plaquette
+    const double* fmunu = nullptr;
+    double* f_rec = nullptr;
 };
 
 __device__ __forceinline__ double block_sum_1(double v)
@@ -1019,11 +1022,20 @@ __device__ __forceinline__ void finalize_body(const FinArgs& f)
         }
         sp = block_sum_1(sp);
         tm = block_sum_1(tm);
+        double fm[12];
+        if (f.fmunu) {
+            for (int k = 0; k < 12; ++k) {
+                double a = 0.0;
+                for (uint32_t j = t; j < f.n_plq; j += kBlock) a += f.fmunu[12 * j + k];
+                fm[k] = block_sum_1(a);
+            }
+        }
         if (t == 0) {
             const int rec = f.rec_index[0];
             if (rec < f.capacity) {
                 f.plq_rec[2 * rec] = sp;
                 f.plq_rec[2 * rec + 1] = tm;
+                for (int k = 0; k < 12; ++k) f.f_rec[12 * rec + k] = f.fmunu ? fm[k] : 0.0;
             }
             f.rec_index[0] = rec + 1;
         }
@@ -1088,6 +1100,91 @@ __global__ __launch_bounds__(kBlock) void k_plaquette(void* table, Geom g, doubl
         }
     }
     block_partial<2>(v, partials, blockIdx.y * gridDim.x + blockIdx.x);
+}
+
+// Tr(P G_k) for generator k: Gell-Mann lambda_k (SU(3), k = 1..8) or Pauli
+// sigma_k (SU(2), k = 1..3).
+template <int N>
+__device__ __forceinline__ C<double> gen_trace(const M<double, N>& P, int k)
+{
+    auto add = [](C<double> a, C<double> b) { return C<double>{a.re + b.re, a.im + b.im}; };
+    auto sub = [](C<double> a, C<double> b) { return C<double>{a.re - b.re, a.im - b.im}; };
+    auto mul_i = [](C<double> a) { return C<double>{-a.im, a.re}; };
+    if constexpr (N == 3) {
+        switch (k) {
+            case 1: return add(P.e[0][1], P.e[1][0]);
+            case 2: return mul_i(sub(P.e[0][1], P.e[1][0]));
+            case 3: return sub(P.e[0][0], P.e[1][1]);
+            case 4: return add(P.e[0][2], P.e[2][0]);
+            case 5: return mul_i(sub(P.e[0][2], P.e[2][0]));
+            case 6: return add(P.e[1][2], P.e[2][1]);
+            case 7: return mul_i(sub(P.e[1][2], P.e[2][1]));
+            default: {
+                const double r3 = 0.57735026918962576451;   // 1 / sqrt(3)
+                const C<double> d = sub(add(P.e[0][0], P.e[1][1]), add(P.e[2][2], P.e[2][2]));
+                return C<double>{d.re * r3, d.im * r3};
+            }
+        }
+    } else {
+        switch (k) {
+            case 1: return add(P.e[0][1], P.e[1][0]);
+            case 2: return mul_i(sub(P.e[0][1], P.e[1][0]));
+            default: return sub(P.e[0][0], P.e[1][1]);
+        }
+    }
+}
+
+// F_munu measurement (lattice_measurement_plq with FMUNU / F0MU,
+// sun_measurements_cl.cl:710-1173; lattice_retrace_plaquette3_F /
+// lattice_retrace_plaquette2_F, su3_measurements_cl.cl:550-590,
+// su2_measurements_cl.cl:234-260): for the planes (x,y), (x,z), (y,z) (H
+// field) or (x,t), (y,t), (z,t) (E field, efield = 1), the plaquette
+// P = U_a(x) U_b(x+a) U_a(x+b)^+ U_b(x)^+ in FP64 and F = -i Tr(P G_k) / 2
+// for the generators g1 and g2 (SU(2): imaginary part 0, as the reference).
+// Block partials: 12 sums, [generator][plane][re, im].
+template <typename R, int N, int NREAL>
+__global__ __launch_bounds__(kBlock) void k_fmunu(void* table, Geom g, double* partials, int efield, int g1, int g2)
+{
+    const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    const int p = blockIdx.y, q = p ^ 1;
+    double v[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) v[k] = 0.0;
+    if (i < g.vh) {
+        Table<R, N, NREAL> T{(typename Vec2<R>::T*)table, g.vh_pad};
+        const Site s = site_of(g, i, p);
+        const int pa[3] = {efield ? 0 : 0, efield ? 1 : 0, efield ? 2 : 1};
+        const int pb[3] = {efield ? 3 : 1, efield ? 3 : 2, efield ? 3 : 2};
+        for (int pl = 0; pl < 3; ++pl) {
+            const int a = pa[pl], b = pb[pl];
+            const M<double, N> Ua = m_cast<double>(T.load_u(p, a, i));
+            const M<double, N> Ub = m_cast<double>(T.load_u(p, b, i));
+            const M<double, N> Ubxa = m_cast<double>(T.load_u(q, b, nb_index(g, s, a, +1)));
+            const M<double, N> Uaxb = m_cast<double>(T.load_u(q, a, nb_index(g, s, b, +1)));
+            const M<double, N> X = m_mul(Ua, Ubxa), Y = m_mul(Ub, Uaxb);
+            M<double, N> P;   // X Y^+
+#pragma unroll
+            for (int r = 0; r < N; ++r)
+#pragma unroll
+                for (int c = 0; c < N; ++c) {
+                    C<double> acc{0.0, 0.0};
+#pragma unroll
+                    for (int k = 0; k < N; ++k) {
+                        const C<double> x = X.e[r][k], y = Y.e[c][k];
+                        acc.re += x.re * y.re + x.im * y.im;
+                        acc.im += x.im * y.re - x.re * y.im;
+                    }
+                    P.e[r][c] = acc;
+                }
+            const int gens[2] = {g1, g2};
+            for (int gi = 0; gi < 2; ++gi) {
+                const C<double> z = gen_trace<N>(P, gens[gi]);
+                v[6 * gi + 2 * pl] += 0.5 * z.im;                       // Re(-i z / 2)
+                v[6 * gi + 2 * pl + 1] += (N == 2) ? 0.0 : -0.5 * z.re;  // Im(-i z / 2)
+            }
+        }
+    }
+    block_partial<12>(v, partials, blockIdx.y * gridDim.x + blockIdx.x);
 }
 
 // Polyakov loop per spatial site, product of the N4 time links in link
@@ -1372,6 +1469,10 @@ struct qcd_lattice {
     double* site_tr = nullptr;          // Tr P per spatial site (2 doubles), for slice sums
     double* slice_rec = nullptr;        // Polyakov slice records: [rec][N1 + N2 + N3][2]
     bool want_slices = false;           // the Polyakov launch being enqueued feeds the slice sums
+    double* part_f = nullptr;           // F_munu block partials: 2 x upd_blocks slots of 12
+    double* f_rec = nullptr;            // F_munu records: [rec][generator][plane][re, im]
+    int f_gen[2] = {3, 8};              // generators (SU(2): 3 and 2)
+    int f_efield = 0;                   // F0MU: planes (x,t), (y,t), (z,t)
     uint32_t wil_blocks = 0;
     int wil_R = 7, wil_T = 2;           // wilson_R, wilson_T defaults of QCDGPU.cpp
     uint32_t upd_blocks = 0, poly_blocks = 0;
@@ -1567,6 +1668,10 @@ int launch_simple(qcd_lattice* L, int which, const LaunchOpt& o)
             hipLaunchKernelGGL((k_wilson<R, N, NREAL>), dim3(L->wil_blocks), dim3(kBlock), 0, L->stream, L->table,
                                L->g, L->wil_R, L->wil_T, L->part_wil);
             break;
+        case 7:
+            hipLaunchKernelGGL((k_fmunu<R, N, NREAL>), dim3(L->upd_blocks, 2), dim3(kBlock), 0, L->stream, L->table,
+                               L->g, L->part_f, L->f_efield, L->f_gen[0], L->f_gen[1]);
+            break;
         case 5:
             if (L->p.prng == QCD_PRNG_XOR128) {
                 hipLaunchKernelGGL((k_init_hot<R, N, NREAL, RK_XOR128>), dim3(nb), dim3(kBlock), 0, L->stream, L->table,
@@ -1608,6 +1713,8 @@ int launch_finalize(qcd_lattice* L, int flags, const LaunchOpt& o)
     f.rec_index = L->rec_index;
     f.capacity = L->p.max_measurements;
     f.slices = (flags & 16) ? 1 : 0;
+    f.fmunu = (flags & 32) ? L->part_f : nullptr;
+    f.f_rec = L->f_rec;
     launch(k_finalize, dim3(1), dim3(kBlock), L->stream, o, f);
     HIPCHK(hipGetLastError());
     return QCD_OK;
@@ -1627,6 +1734,16 @@ int ensure_device()
 int qcdhip::set_error(int code, const char* msg)
 {
     return fail(code, "%s", msg);
+}
+
+int qcdhip::write_fmunu(qcd_lattice* L, const double* f, int count)
+{
+    if (!L || count < 0 || (count && !f)) return fail(QCD_E_ARG, "bad F_munu records");
+    const int m = L->p.max_measurements;
+    const int n = count < m ? count : m;
+    HIPCHK(hipStreamSynchronize(L->stream));
+    if (n) HIPCHK(hipMemcpy(L->f_rec, f, sizeof(double) * 12 * n, hipMemcpyHostToDevice));
+    return QCD_OK;
 }
 
 // ---------------------------------------------------------------- C ABI
@@ -1686,6 +1803,7 @@ int qcd_lattice_create(const qcd_params* pp, qcd_lattice** out)
 
     qcd_lattice* L = new qcd_lattice();
     L->p = p;
+    if (p.sun == 2) L->f_gen[1] = 2;   // SU(2): sigma_3 and sigma_2
     L->g.n1 = p.n[0];
     L->g.n2 = p.n[1];
     L->g.n3 = p.n[2];
@@ -1732,7 +1850,9 @@ int qcd_lattice_create(const qcd_params* pp, qcd_lattice** out)
         hipMalloc((void**)&L->poly_rec, sizeof(double) * 4 * p.max_measurements) != hipSuccess ||
         hipMalloc((void**)&L->wil_rec, sizeof(double) * p.max_measurements) != hipSuccess ||
         hipMalloc((void**)&L->site_tr, sizeof(double) * 2 * vs) != hipSuccess ||
-        hipMalloc((void**)&L->slice_rec, sizeof(double) * slice_rec_doubles(L)) != hipSuccess) {
+        hipMalloc((void**)&L->slice_rec, sizeof(double) * slice_rec_doubles(L)) != hipSuccess ||
+        hipMalloc((void**)&L->part_f, sizeof(double) * 12 * 2 * L->upd_blocks) != hipSuccess ||
+        hipMalloc((void**)&L->f_rec, sizeof(double) * 12 * p.max_measurements) != hipSuccess) {
         qcd_lattice_destroy(L);
         return fail(QCD_E_NOMEM, "device allocation failed (table %zu bytes)", L->table_bytes);
     }
@@ -1752,6 +1872,14 @@ int qcd_lattice_create(const qcd_params* pp, qcd_lattice** out)
     L->part_wil = L->part_poly + (size_t)4 * L->poly_blocks;
     (void)hipMemset(L->rec_index, 0, sizeof(int) * 4);
     (void)hipMemset(L->table, 0, L->table_bytes);
+    {   // records start at zero, so state images of equal runs are equal byte for byte
+        const size_t m = (size_t)p.max_measurements;
+        (void)hipMemset(L->plq_rec, 0, sizeof(double) * 2 * m);
+        (void)hipMemset(L->poly_rec, 0, sizeof(double) * 4 * m);
+        (void)hipMemset(L->wil_rec, 0, sizeof(double) * m);
+        (void)hipMemset(L->slice_rec, 0, sizeof(double) * slice_rec_doubles(L));
+        (void)hipMemset(L->f_rec, 0, sizeof(double) * 12 * m);
+    }
     {
         // Reference-stream skip of s_quads quads per heatbath half-step:
         // XOR128 jump matrix for 4 * s_quads steps (four steps per quad),
@@ -1816,6 +1944,8 @@ int qcd_lattice_destroy(qcd_lattice* L)
     (void)hipFree(L->wil_rec);
     (void)hipFree(L->site_tr);
     (void)hipFree(L->slice_rec);
+    (void)hipFree(L->part_f);
+    (void)hipFree(L->f_rec);
     if (L->stream) (void)hipStreamDestroy(L->stream);
     delete L;
     return QCD_OK;
@@ -2081,10 +2211,22 @@ static int measure_impl(qcd_lattice* L, int mask, bool fused_plq, const LaunchOp
 {
     int rc = QCD_OK;
     int flags = 0;
+    const int fmask = mask & (QCD_MEAS_FMUNU | QCD_MEAS_F0MU);
+    if (fmask == (QCD_MEAS_FMUNU | QCD_MEAS_F0MU)) return fail(QCD_E_ARG, "F_munu: H field or E field, not both");
+    if (fmask) {   // recorded beside the plaquette, from the standalone plaquette kernel
+        mask |= QCD_MEAS_PLAQUETTE;
+        fused_plq = false;
+    }
     if (fused_plq) flags |= 1;
     if ((mask & QCD_MEAS_PLAQUETTE) && !fused_plq) {
         rc = DISPATCH(L, launch_simple, L, 3, o_plq);
         flags |= 2;
+    }
+    if (rc) return rc;
+    if (fmask) {
+        L->f_efield = (fmask == QCD_MEAS_F0MU) ? 1 : 0;
+        rc = DISPATCH(L, launch_simple, L, 7, LaunchOpt{});
+        flags |= 32;
     }
     if (rc) return rc;
     if (mask & (QCD_MEAS_POLYAKOV | QCD_MEAS_POLYAKOV_SLICES)) {
@@ -2121,6 +2263,30 @@ int qcd_read_measurements(qcd_lattice* L, double* plq, double* poly, int* count_
     if (poly && nq) HIPCHK(hipMemcpy(poly, L->poly_rec, sizeof(double) * 4 * nq, hipMemcpyDeviceToHost));
     if (count_plq) *count_plq = idx[0];
     if (count_poly) *count_poly = idx[1];
+    return QCD_OK;
+}
+
+int qcd_lattice_set_fmunu(qcd_lattice* L, int gen1, int gen2)
+{
+    if (!L) return fail(QCD_E_ARG, "null lattice");
+    const bool ok = (L->p.sun == 3) ? (gen1 >= 1 && gen1 <= 4 && gen2 >= 5 && gen2 <= 8)
+                                    : ((gen1 == 1 || gen1 == 3) && gen2 == 2);
+    if (!ok)
+        return fail(QCD_E_ARG, "F_munu generators %d, %d: SU(3) takes 1..4 and 5..8, SU(2) 1 or 3 and 2", gen1, gen2);
+    L->f_gen[0] = gen1;
+    L->f_gen[1] = gen2;
+    return QCD_OK;
+}
+
+int qcd_read_fmunu(qcd_lattice* L, double* out, int* count)
+{
+    if (!L) return fail(QCD_E_ARG, "null lattice");
+    int idx[4];
+    HIPCHK(hipMemcpyAsync(idx, L->rec_index, sizeof(idx), hipMemcpyDeviceToHost, L->stream));
+    HIPCHK(hipStreamSynchronize(L->stream));
+    const int n = idx[0] < L->p.max_measurements ? idx[0] : L->p.max_measurements;
+    if (out && n) HIPCHK(hipMemcpy(out, L->f_rec, sizeof(double) * 12 * n, hipMemcpyDeviceToHost));
+    if (count) *count = idx[0];
     return QCD_OK;
 }
 
@@ -2173,7 +2339,8 @@ int qcd_reset_measurements(qcd_lattice* L)
 static int enqueue_cycle(qcd_lattice* L, int n_hb, int n_or, int mask, std::vector<hipEvent_t>* ev, int* ev_used)
 {
     const int total = n_hb + n_or;
-    const bool fuse = (mask & QCD_MEAS_PLAQUETTE) && total > 0;
+    // F_munu needs the standalone plaquette kernel
+    const bool fuse = (mask & QCD_MEAS_PLAQUETTE) && total > 0 && !(mask & (QCD_MEAS_FMUNU | QCD_MEAS_F0MU));
     for (int k = 0; k < total; ++k) {
         int rc = sweep_impl(L, k < n_hb ? QCD_HEATBATH : QCD_OVERRELAX, fuse && k == total - 1, ev, ev_used);
         if (rc) return rc;
@@ -2292,7 +2459,7 @@ int qcd_cycle_profile(qcd_lattice* L, int repeats, double* hb_us, double* or_us,
         HIPCHK(hipEventRecord(t1, L->stream));
         HIPCHK(hipEventSynchronize(t1));
         const int total = n_hb + n_or;
-        const bool fuse = (mask & QCD_MEAS_PLAQUETTE) && total > 0;
+        const bool fuse = (mask & QCD_MEAS_PLAQUETTE) && total > 0 && !(mask & (QCD_MEAS_FMUNU | QCD_MEAS_F0MU));
         for (int j = 0; j < used; ++j) {
             float f = 0.f;
             HIPCHK(hipEventElapsedTime(&f, ev[2 * j], ev[2 * j + 1]));
@@ -2410,7 +2577,7 @@ size_t rng_state_bytes(const qcd_lattice* L)
 size_t state_image_bytes(const qcd_lattice* L)
 {
     return kStateHead + L->table_bytes + rng_state_bytes(L) + sizeof(double) * 7 * (size_t)L->p.max_measurements +
-           sizeof(double) * slice_rec_doubles(L);
+           sizeof(double) * slice_rec_doubles(L) + sizeof(double) * 12 * (size_t)L->p.max_measurements;
 }
 
 uint64_t f64_bits(double x)
@@ -2529,6 +2696,8 @@ int qcd_lattice_state_save(qcd_lattice* L, void* buf, size_t bytes)
     HIPCHK(hipMemcpy(w.p, L->wil_rec, sizeof(double) * m, hipMemcpyDeviceToHost));
     w.p += sizeof(double) * m;
     HIPCHK(hipMemcpy(w.p, L->slice_rec, sizeof(double) * slice_rec_doubles(L), hipMemcpyDeviceToHost));
+    w.p += sizeof(double) * slice_rec_doubles(L);
+    HIPCHK(hipMemcpy(w.p, L->f_rec, sizeof(double) * 12 * m, hipMemcpyDeviceToHost));
     return QCD_OK;
 }
 
@@ -2567,6 +2736,8 @@ int qcd_lattice_state_load(qcd_lattice* L, const void* buf, size_t bytes)
     HIPCHK(hipMemcpy(L->wil_rec, r.p, sizeof(double) * m, hipMemcpyHostToDevice));
     r.p += sizeof(double) * m;
     HIPCHK(hipMemcpy(L->slice_rec, r.p, sizeof(double) * slice_rec_doubles(L), hipMemcpyHostToDevice));
+    r.p += sizeof(double) * slice_rec_doubles(L);
+    HIPCHK(hipMemcpy(L->f_rec, r.p, sizeof(double) * 12 * m, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(L->rec_index, idx, sizeof(idx), hipMemcpyHostToDevice));
     L->prng_counter = counter;
     L->rng_ready = ready != 0;
@@ -2589,6 +2760,7 @@ int qcd_write_measurements(qcd_lattice* L, const double* plq, const double* poly
     if (np) HIPCHK(hipMemcpy(L->plq_rec, plq, sizeof(double) * 2 * np, hipMemcpyHostToDevice));
     if (nq) HIPCHK(hipMemcpy(L->poly_rec, poly, sizeof(double) * 4 * nq, hipMemcpyHostToDevice));
     if (nw) HIPCHK(hipMemcpy(L->wil_rec, wil, sizeof(double) * nw, hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(L->f_rec, 0, sizeof(double) * 12 * m));   // restored records carry no F_munu
     const int idx[4] = {count_plq, count_poly, count_wil, 0};
     HIPCHK(hipMemcpy(L->rec_index, idx, sizeof(idx), hipMemcpyHostToDevice));
     return QCD_OK;
@@ -2732,6 +2904,7 @@ FinArgs chain_fin_args(const qcd_lattice* L)
     f.wil_rec = L->wil_rec;
     f.rec_index = L->rec_index;
     f.capacity = L->p.max_measurements;
+    f.f_rec = L->f_rec;
     return f;
 }
 

@@ -64,17 +64,19 @@ struct Layout {
     uint64_t V = 0;
     uint64_t rowsize = 0;       // lattice_table_row_size
     uint64_t table_size = 0;    // 4-vectors
-    uint64_t meas_size_F = 0;   // double2 (per-workgroup partials)
-    uint64_t energies_size = 0; // double2 / double per record slot
-    uint64_t poly_size = 0;     // double2
-    size_t elem = 0;            // bytes per 4-vector: 16 (float4) or 32 (double4)
+    uint64_t meas_size_F = 0;     // double2 (per-workgroup partials; x7 with F_munu)
+    uint64_t energies_size = 0;   // double2 / double per record slot
+    uint64_t energies_size_F = 0; // double2: plaquette section (x7 with F_munu)
+    uint64_t poly_size = 0;       // double2
+    size_t elem = 0;              // bytes per 4-vector: 16 (float4) or 32 (double4)
     int N = 3;
     uint32_t get_plq = 1, get_wilson = 0, pl_level = 0;
+    uint32_t f = 0;               // F_munu sections (get_Fmunu or get_F0mu)
 
     uint64_t body_bytes() const
     {
         uint64_t b = 16 * meas_size_F + 16 * energies_size;
-        if (get_plq) b += 16 * energies_size;
+        if (get_plq || f) b += 16 * energies_size_F;
         if (get_wilson) b += 8 * energies_size;
         if (pl_level > 0) b += 16 * poly_size;
         return b + (uint64_t)elem * table_size;
@@ -82,7 +84,7 @@ struct Layout {
 };
 
 Layout make_layout(int sun, const int n[4], int precision, unsigned iter, unsigned get_plq, unsigned get_wilson,
-                   unsigned pl_level)
+                   unsigned pl_level, unsigned f)
 {
     Layout l;
     l.N = sun;
@@ -90,9 +92,12 @@ Layout make_layout(int sun, const int n[4], int precision, unsigned iter, unsign
     l.rowsize = align64(l.V);
     const uint64_t elements = (sun == 3) ? 12 : 4;   // lattice_group_elements (suncl.cpp:751-753)
     l.table_size = align64(l.rowsize * 4 * elements / 4);
-    l.meas_size_F = align64((align64(l.V) + 63) / 64);   // ceil(exact_row_size / 64), no F_munu
+    const uint64_t kF = f ? 7 : 1;   // MODEL_energies_size (suncl.h:221) with F_munu
+    l.meas_size_F = kF * align64((align64(l.V) + 63) / 64);   // ceil(exact_row_size / 64)
     const uint64_t it = iter ? iter : 1;
     l.energies_size = align64(it);
+    l.energies_size_F = kF * l.energies_size;
+    l.f = f ? 1 : 0;
     l.poly_size = align64(it);
     l.elem = (precision == QCD_FP32) ? 16 : 32;
     l.get_plq = get_plq ? 1 : 0;
@@ -197,7 +202,10 @@ int header_to_params(const Header& h, qcd_params* p, qcd_run_info* info)
     memcpy(&q.beta, &bb, 8);
     if (w[21] || w[22] || w[23] || w[24])
         return err(QCD_E_UNSUPPORTED, "state file with PHI / OMEGA (twisted boundaries)");
-    if (w[28]) return err(QCD_E_UNSUPPORTED, "state file with F_munu measurements");
+    if (w[34] != w[33] && (uint64_t)w[34] != 7ull * w[33])
+        return err(QCD_E_ARG, "state file section sizes: energies_size_F is neither 1 nor 7 x energies_size");
+    const bool fsec = w[34] != w[33];
+    if (w[28] && !fsec) return err(QCD_E_ARG, "state file with get_Fmunu but no F_munu sections");
     q.storage_reals = (q.sun == 3) ? 18 : 4;
     q.nhit = (int)w[15];
     q.nhitpar = 1;
@@ -220,14 +228,32 @@ int header_to_params(const Header& h, qcd_params* p, qcd_run_info* info)
         info->get_actions = w[29];
         info->get_plaquettes = w[30];
         info->get_wilson = w[31];
+        info->fmunu_index1 = w[26];
+        info->fmunu_index2 = w[27];
+        info->get_fmunu = w[28] ? 1u : 0u;
+        info->get_f0mu = (fsec && !w[28]) ? 1u : 0u;
     }
     return QCD_OK;
 }
 
+// Section sizes come from the header words, as lattice_load_bin_header
+// reads them (suncl.cpp:2031-2034).
 Layout header_layout(const Header& h, const qcd_params& q)
 {
-    Layout l = make_layout(q.sun, q.n, q.precision, h.w[13], h.w[30], h.w[31], h.w[25]);
+    Layout l = make_layout(q.sun, q.n, q.precision, h.w[13], h.w[30], h.w[31], h.w[25], 0);
+    l.meas_size_F = h.w[32];
+    l.energies_size = h.w[33];
+    l.energies_size_F = h.w[34];
+    l.poly_size = h.w[35];
+    l.f = (h.w[34] != h.w[33]) ? 1 : 0;
     return l;
+}
+
+// Generators of a file with F_munu sections (header words 26-27).
+int apply_fmunu(qcd_lattice* lat, const qcd_run_info& ri)
+{
+    if (!ri.get_fmunu && !ri.get_f0mu) return QCD_OK;
+    return qcd_lattice_set_fmunu(lat, (int)ri.fmunu_index1, (int)ri.fmunu_index2);
 }
 
 // Parameter fields of an exact image head (see qcd_lattice_state_save):
@@ -302,8 +328,9 @@ int qcd_qcg_write(qcd_lattice* lat, const char* path, const qcd_run_info* info)
     if (rc) return rc;
     unsigned counter = 0;
     if ((rc = qcd_prng_counter(lat, &counter))) return rc;
+    const unsigned fsec = (info->get_fmunu || info->get_f0mu) ? 1u : 0u;
     const Layout lay = make_layout(p.sun, p.n, p.precision, info->iter, info->get_plaquettes, info->get_wilson,
-                                   info->pl_level);
+                                   info->pl_level, fsec);
     const int N = p.sun;
 
     // lattice_make_bin_header (suncl.cpp:1939-1993)
@@ -330,14 +357,17 @@ int qcd_qcg_write(qcd_lattice* lat, const char* path, const qcd_run_info* info)
     memcpy(&bb, &p.beta, 8);
     h.w[19] = (uint32_t)bb;                           // convert_to_uint_LOW / HIGH
     h.w[20] = (uint32_t)(bb >> 32);
-    // 21-24: PHI, OMEGA = 0; 26-28: F_munu indices and flag = 0
+    // 21-24: PHI, OMEGA = 0
     h.w[25] = info->pl_level;
+    h.w[26] = info->fmunu_index1;
+    h.w[27] = info->fmunu_index2;
+    h.w[28] = info->get_fmunu ? 1u : 0u;
     h.w[29] = info->get_actions;
     h.w[30] = info->get_plaquettes;
     h.w[31] = info->get_wilson;
     h.w[32] = (uint32_t)lay.meas_size_F;
     h.w[33] = (uint32_t)lay.energies_size;
-    h.w[34] = (uint32_t)lay.energies_size;   // energies_size_F without F_munu
+    h.w[34] = (uint32_t)lay.energies_size_F;
     h.w[35] = (uint32_t)lay.poly_size;
     h.w[36] = (uint32_t)N;
     h.w[37] = 4;
@@ -349,6 +379,9 @@ int qcd_qcg_write(qcd_lattice* lat, const char* path, const qcd_run_info* info)
     int np = 0, nq = 0, nw = 0;
     if ((rc = qcd_read_measurements(lat, plq.data(), poly.data(), &np, &nq))) return rc;
     if ((rc = qcd_read_wilson(lat, wil.data(), &nw))) return rc;
+    std::vector<double> fr(12 * cap);
+    int nf = 0;
+    if ((rc = qcd_read_fmunu(lat, fr.data(), &nf))) return rc;
     np = np < (int)cap ? np : (int)cap;
     nq = nq < (int)cap ? nq : (int)cap;
     nw = nw < (int)cap ? nw : (int)cap;
@@ -376,11 +409,14 @@ int qcd_qcg_write(qcd_lattice* lat, const char* path, const qcd_run_info* info)
         }
         ok = ok && put(f, e.data(), sizeof(double) * e.size());
     }
-    if (lay.get_plq) {   // lattice_energies_plq: (sum ReTr spatial, temporal)
-        std::vector<double> e(2 * lay.energies_size, 0.0);
+    if (lay.get_plq || lay.f) {   // lattice_energies_plq: (sum ReTr spatial, temporal), then F_munu
+        std::vector<double> e(2 * lay.energies_size_F, 0.0);
         for (int i = 0; i < np && (uint64_t)i < lay.energies_size; ++i) {
             e[2 * i] = plq[2 * i];
             e[2 * i + 1] = plq[2 * i + 1];
+            if (lay.f)
+                for (int b = 0; b < 6; ++b)   // block 1 + 3 * generator + plane
+                    for (int c = 0; c < 2; ++c) e[2 * ((1 + b) * lay.energies_size + i) + c] = fr[12 * i + 2 * b + c];
         }
         ok = ok && put(f, e.data(), sizeof(double) * e.size());
     }
@@ -488,6 +524,7 @@ int qcd_qcg_read(qcd_lattice* lat, const char* path, qcd_run_info* info)
         if (!get(fh.f, img.data(), img.size())) return err(QCD_E_ARG, "short state image");
         if (image_params(img.data(), img.size(), &e) && same_params(e, lp)) {
             if ((rc = qcd_lattice_state_load(lat, img.data(), img.size()))) return rc;
+            if ((rc = apply_fmunu(lat, ri))) return rc;
             if (info) *info = ri;
             return QCD_OK;
         }
@@ -500,8 +537,8 @@ int qcd_qcg_read(qcd_lattice* lat, const char* path, qcd_run_info* info)
     std::vector<double> en(2 * lay.energies_size);
     ok = ok && get(fh.f, en.data(), sizeof(double) * en.size());   // action sums: derived, not kept
     std::vector<double> eplq, ewil, epoly;
-    if (lay.get_plq) {
-        eplq.resize(2 * lay.energies_size);
+    if (lay.get_plq || lay.f) {
+        eplq.resize(2 * lay.energies_size_F);
         ok = ok && get(fh.f, eplq.data(), sizeof(double) * eplq.size());
     }
     if (lay.get_wilson) {
@@ -535,7 +572,7 @@ int qcd_qcg_read(qcd_lattice* lat, const char* path, qcd_run_info* info)
     if ((rc = qcd_lattice_init(lat, start == QCD_START_GID ? QCD_START_GID : QCD_START_COLD))) return rc;
     if ((rc = qcd_lattice_import(lat, links.data()))) return rc;
     const int n = (int)ri.iter_counter;
-    const int np = lay.get_plq ? (int)std::min<uint64_t>((uint64_t)n, lay.energies_size) : 0;
+    const int np = (lay.get_plq || lay.f) ? (int)std::min<uint64_t>((uint64_t)n, lay.energies_size) : 0;
     const int nw = lay.get_wilson ? (int)std::min<uint64_t>((uint64_t)n, lay.energies_size) : 0;
     const int nq = lay.pl_level > 0 ? (int)std::min<uint64_t>((uint64_t)n, lay.poly_size) : 0;
     std::vector<double> poly4(4 * (size_t)(nq ? nq : 1), 0.0);
@@ -546,6 +583,14 @@ int qcd_qcg_read(qcd_lattice* lat, const char* path, qcd_run_info* info)
     if ((rc = qcd_write_measurements(lat, np ? eplq.data() : nullptr, nq ? poly4.data() : nullptr,
                                      nw ? ewil.data() : nullptr, np, nq, nw)))
         return rc;
+    if (lay.f && np) {
+        std::vector<double> fr(12 * (size_t)np);
+        for (int i = 0; i < np; ++i)
+            for (int b = 0; b < 6; ++b)
+                for (int c = 0; c < 2; ++c) fr[12 * i + 2 * b + c] = eplq[2 * ((1 + b) * lay.energies_size + i) + c];
+        if ((rc = qcdhip::write_fmunu(lat, fr.data(), np))) return rc;
+    }
+    if ((rc = apply_fmunu(lat, ri))) return rc;
     if (ri.wilson_r >= 1 && ri.wilson_t >= 1 &&
         (rc = qcd_lattice_set_wilson(lat, (int)ri.wilson_r, (int)ri.wilson_t)))
         return rc;

@@ -29,9 +29,12 @@
 // position and the counters come from the .qcg file, which overrides the
 // keys it holds, and the run continues where the file left it
 // (suncl.cpp:5345-5351).
-// GETWILSON / WILSONR / WILSONT measure the R x T Wilson loop.  Keys of
-// features outside this path (F_munu, twisted boundary conditions, BIGLAT)
-// are rejected when they ask for the feature.
+// GETWILSON / WILSONR / WILSONT measure the R x T Wilson loop.  FMUNU (the
+// default, as in the reference's GPU build, suncl.cpp:761) and F0MU measure
+// the field strength for the H or E field, TURNOFFFMUNU neither; FMUNU1/2/4
+// and FMUNU5/6/7 choose the generators (suncl.cpp:412-453, 2093-2113).  Keys
+// of features outside this path (twisted boundary conditions, BIGLAT) are
+// rejected when they ask for the feature.
 
 #include <algorithm>
 #include <cerrno>
@@ -58,7 +61,9 @@ struct Settings {
     std::string prng = "RANLUX3";
     unsigned randseries = 54;
     int pl_level = 3;
-    bool wilson = false, fmunu = false;
+    bool wilson = false;
+    int fmunu = 1;                   // 0 off, 1 H field (FMUNU), 2 E field (F0MU)
+    int fgen1 = 0, fgen2 = 0;        // FMUNU1/2/4 and FMUNU5/6/7 (0: lambda_3 / lambda_8)
     int wilson_r = 7, wilson_t = 2;  // QCDGPU.cpp:202-203
     std::string outputpath = "./", finishpath = "./";
     bool double_precision = true;    // QCDGPU.cpp: model_precision_double
@@ -93,6 +98,20 @@ std::string upper(std::string s)
     return s;
 }
 
+// Fmunu_index1 / Fmunu_index2 (model::Fmunu_defaults, suncl.cpp:2093-2113):
+// SU(3) lambda_3 or the FMUNU1/2/4 choice, lambda_8 or FMUNU5/6/7; SU(2)
+// sigma_3 or sigma_1 (FMUNU1), and sigma_2.
+void fmunu_indices(const Settings& st, int& g1, int& g2)
+{
+    if (st.group == 2) {
+        g1 = (st.fgen1 == 1) ? 1 : 3;
+        g2 = 2;
+    } else {
+        g1 = st.fgen1 ? st.fgen1 : 3;
+        g2 = st.fgen2 ? st.fgen2 : 8;
+    }
+}
+
 // Returns false (and prints why) on an unusable value.
 bool apply(Settings& st, const std::string& key, const std::string& val)
 {
@@ -125,8 +144,16 @@ bool apply(Settings& st, const std::string& key, const std::string& val)
     else if (key == "GETWILSON") st.wilson = true;
     else if (key == "WILSONR") st.wilson_r = (int)iv;
     else if (key == "WILSONT") st.wilson_t = (int)iv;
-    else if (key == "FMUNU" || key == "F0MU") st.fmunu = true;
-    else if (key == "TURNOFFFMUNU") st.fmunu = false;
+    // the F_munu keys act by their presence; the value is not read
+    else if (key == "FMUNU") st.fmunu = 1;
+    else if (key == "F0MU") st.fmunu = 2;
+    else if (key == "TURNOFFFMUNU") st.fmunu = 0;
+    else if (key == "FMUNU1") st.fgen1 = 1;
+    else if (key == "FMUNU2") st.fgen1 = 2;
+    else if (key == "FMUNU4") st.fgen1 = 4;
+    else if (key == "FMUNU5") st.fgen2 = 5;
+    else if (key == "FMUNU6") st.fgen2 = 6;
+    else if (key == "FMUNU7") st.fgen2 = 7;
     else if (key == "PRECISION") {
         const std::string u = upper(val);
         if (u == "SINGLE" || u == "1") st.double_precision = false;
@@ -186,7 +213,6 @@ bool to_params(const Settings& st, qcd_params& p, std::string& why)
     if (st.group != 2 && st.group != 3) return why = "GROUP must be 2 or 3", false;
     if (st.phi != 0.0 || st.omega != 0.0) return why = "PHI/OMEGA (twisted boundary conditions) are not supported", false;
     if (st.wilson && (st.wilson_r < 1 || st.wilson_t < 1)) return why = "WILSONR and WILSONT must be >= 1", false;
-    if (st.fmunu) return why = "FMUNU/F0MU (field-strength measurements) are not supported", false;
     if (st.iter < 1 || st.niter < 0 || st.nav < 0 || st.nor < 0) return why = "ITER >= 1, NITER, NAV, NOR >= 0", false;
     memset(&p, 0, sizeof(p));
     p.sun = st.group;
@@ -223,7 +249,9 @@ void from_state(Settings& st, const qcd_params& p, const qcd_run_info& r)
     st.storage = p.storage_reals;
     st.beta = p.beta;
     st.phi = st.omega = 0.0;
-    st.fmunu = false;
+    st.fmunu = r.get_fmunu ? 1 : (r.get_f0mu ? 2 : 0);
+    st.fgen1 = (p.sun == 3 && r.fmunu_index1 != 3) || (p.sun == 2 && r.fmunu_index1 == 1) ? (int)r.fmunu_index1 : 0;
+    st.fgen2 = (p.sun == 3 && r.fmunu_index2 != 8) ? (int)r.fmunu_index2 : 0;
     st.nhit = p.nhit;
     st.nhitpar = p.nhitpar;
     st.prng = (p.prng == QCD_PRNG_XOR128) ? "XOR128"
@@ -262,6 +290,9 @@ void print_params(const Settings& st)
     printf("BETA=%.13e\nPHI=%.13e\nOMEGA=%.13e\nINTS=%d\n", st.beta, st.phi, st.omega, st.ints);
     printf("PRNG=%s\nRANDSERIES=%u\nPL_LEVEL=%d\n", st.prng.c_str(), st.randseries, st.pl_level);
     printf("GETWILSON=%d\nWILSONR=%d\nWILSONT=%d\n", st.wilson ? 1 : 0, st.wilson_r, st.wilson_t);
+    int g1, g2;
+    fmunu_indices(st, g1, g2);
+    printf("FMUNU=%s\nFMUNU_INDEX1=%d\nFMUNU_INDEX2=%d\n", st.fmunu == 1 ? "H" : (st.fmunu == 2 ? "E" : "OFF"), g1, g2);
     printf("PRECISION=%s\nNOR=%d\nSTORAGE=%d\nSTREAMS=%s\n", st.double_precision ? "DOUBLE" : "SINGLE", st.nor,
            st.storage ? st.storage : (st.group == 3 ? 18 : 4), st.fast_streams ? "FAST" : "REFERENCE");
     printf("OUTPUTPATH=%s\nFINISHPATH=%s\nPLATFORM=%d\nDEVICE=%d\n", st.outputpath.c_str(), st.finishpath.c_str(),
@@ -280,7 +311,7 @@ void print_params(const Settings& st)
     } while (0)
 
 struct Series {
-    const char* name;
+    std::string name;
     std::vector<double> v;
     double mean = 0.0, variance = 0.0;
     void analyse()
@@ -349,8 +380,12 @@ int main(int argc, char** argv)
     CHECK(qcd_lattice_create(&p, &lat));
     const int start = st.ints == 1 ? QCD_START_COLD : (st.ints == 2 ? QCD_START_GID : QCD_START_HOT);
     const int mask = QCD_MEAS_PLAQUETTE | (st.pl_level > 0 ? QCD_MEAS_POLYAKOV : 0) | (st.wilson ? QCD_MEAS_WILSON : 0) |
-                     (st.pl_level > 2 ? QCD_MEAS_POLYAKOV_SLICES : 0);
+                     (st.pl_level > 2 ? QCD_MEAS_POLYAKOV_SLICES : 0) |
+                     (st.fmunu == 1 ? QCD_MEAS_FMUNU : (st.fmunu == 2 ? QCD_MEAS_F0MU : 0));
     if (st.wilson) CHECK(qcd_lattice_set_wilson(lat, st.wilson_r, st.wilson_t));
+    int fgen[2];
+    fmunu_indices(st, fgen[0], fgen[1]);
+    if (st.fmunu) CHECK(qcd_lattice_set_fmunu(lat, fgen[0], fgen[1]));
     auto sweep = [&]() {
         CHECK(qcd_sweep(lat, QCD_HEATBATH));
         for (int k = 0; k < st.nor; ++k) CHECK(qcd_sweep(lat, QCD_OVERRELAX));
@@ -371,6 +406,10 @@ int main(int argc, char** argv)
         ri.get_actions = 1;
         ri.get_plaquettes = 1;
         ri.get_wilson = st.wilson ? 1u : 0u;
+        ri.fmunu_index1 = (unsigned)fgen[0];
+        ri.fmunu_index2 = (unsigned)fgen[1];
+        ri.get_fmunu = st.fmunu == 1 ? 1u : 0u;
+        ri.get_f0mu = st.fmunu == 2 ? 1u : 0u;
         last_state = stamped_name(st, ".qcg");
         CHECK(qcd_qcg_write(lat, last_state.c_str(), &ri));
     };
@@ -434,6 +473,10 @@ int main(int argc, char** argv)
     CHECK(qcd_read_measurements(lat, plq.data(), poly.data(), &np, &nq));
     if (st.wilson) CHECK(qcd_read_wilson(lat, wil.data(), &nw));
     if (st.pl_level > 2) CHECK(qcd_read_polyakov_slices(lat, slc.data(), &ns));
+    std::vector<double> fr(12 * cap);
+    int nf = 0;
+    if (st.fmunu) CHECK(qcd_read_fmunu(lat, fr.data(), &nf));
+    nf = std::min(nf, (int)cap);
     np = std::min(np, (int)cap);
     nq = std::min(nq, (int)cap);
     nw = std::min(nw, (int)cap);
@@ -480,10 +523,44 @@ int main(int argc, char** argv)
         cols.push_back(&s_p4);
     }
     for (auto* c : cols) c->analyse();
+    // F_munu (lattice_analysis, suncl.cpp:1614-1677): the six planes of each
+    // generator divided by V, then |F| = sqrt of the sum of squares over the
+    // three planes (analysis::lattice_data_analysis_joint3,
+    // data_analysis.cpp:186-212); columns xy, xz, yz (xt, yt, zt for the E
+    // field) re / im of the first generator, of the second, then the four
+    // |F| columns.
+    std::vector<Series> fcols;
+    if (st.fmunu && nf > 0) {
+        const char* pl_h[3] = {"xy", "xz", "yz"};
+        const char* pl_e[3] = {"xt", "yt", "zt"};
+        const char* reim[2] = {"re", "im"};
+        for (int gi = 0; gi < 2; ++gi)
+            for (int pl = 0; pl < 3; ++pl)
+                for (int c = 0; c < 2; ++c) {
+                    Series se{std::string("Fmunu_") + (st.fmunu == 1 ? pl_h[pl] : pl_e[pl]) + "_" +
+                              std::to_string(fgen[gi]) + "_" + reim[c]};
+                    for (int i = 0; i < nf; ++i) se.v.push_back(fr[12 * i + 6 * gi + 2 * pl + c] / V);
+                    fcols.push_back(se);
+                }
+        for (int gi = 0; gi < 2; ++gi)
+            for (int c = 0; c < 2; ++c) {
+                Series se{std::string("Fmunu_abs_") + std::to_string(fgen[gi]) + "_" + reim[c]};
+                for (int i = 0; i < nf; ++i) {
+                    double a = 0.0;
+                    for (int pl = 0; pl < 3; ++pl) {
+                        const double x = fcols[6 * gi + 2 * pl + c].v[i];
+                        a += x * x;
+                    }
+                    se.v.push_back(sqrt(a));
+                }
+                fcols.push_back(se);
+            }
+        for (auto& c : fcols) c.analyse();
+    }
 
     // lattice_print_measurements: first five and the last measurement
     printf("\n #        ");
-    for (auto* c : cols) printf(" %-11s", c->name);
+    for (auto* c : cols) printf(" %-11s", c->name.c_str());
     printf("\n");
     for (int i = 0; i < st.iter; ++i)
         if (i < 5 || i == st.iter - 1) {
@@ -492,7 +569,8 @@ int main(int argc, char** argv)
             printf("\n");
         }
     printf("\n %-18s %-22s %-22s\n", "quantity", "mean", "variance");
-    for (auto* c : cols) printf(" %-18s % 16.13e % 16.13e\n", c->name, c->mean, c->variance);
+    for (auto* c : cols) printf(" %-18s % 16.13e % 16.13e\n", c->name.c_str(), c->mean, c->variance);
+    for (auto& c : fcols) printf(" %-18s % 16.13e % 16.13e\n", c.name.c_str(), c.mean, c.variance);
     // sweeps run by this process (a continued run starts at its counters)
     const double sweeps = (double)(st.nav - nav_start) + (double)(st.iter - iter_start) * st.niter;
     printf("\n sweeps %.0f (%d overrelaxation sweeps each), %.3f s, %.1f sweeps/s, %.3e link updates/s\n", sweeps,
@@ -517,10 +595,13 @@ int main(int argc, char** argv)
             st.fast_streams ? "fast" : "reference", st.randseries);
     fprintf(out, " start                       : %s\n", st.ints == 1 ? "cold" : (st.ints == 2 ? "gid" : "hot (Haar)"));
     fprintf(out, " NAV %d, ITER %d, NITER %d, NHIT %d, NOR %d\n", st.nav, st.iter, st.niter, st.nhit, st.nor);
+    if (st.fmunu) fprintf(out, " %s(%d, %d)\n", st.fmunu == 1 ? "FMUNU" : "F0MU", fgen[0], fgen[1]);
     fprintf(out, " started %s finished %s", timestart, timeend);
     fprintf(out, " run time %.6f s\n", secs);
     fprintf(out, " ***************************************************\n");
-    for (auto* c : cols) fprintf(out, " %-21s mean % 16.13e variance % 16.13e\n", c->name, c->mean, c->variance);
+    for (auto* c : cols) fprintf(out, " %-21s mean % 16.13e variance % 16.13e\n", c->name.c_str(), c->mean, c->variance);
+    for (auto& c : fcols)
+        fprintf(out, " %-21s mean % 16.13e variance % 16.13e\n", c.name.c_str(), c.mean, c.variance);
     if (st.pl_level > 2 && ns > 0) {
         // lattice_write_results' differentiated Polyakov loop block
         // (suncl.cpp:1695-1751); every direction is normalised by N2 N3 N as
@@ -557,11 +638,13 @@ int main(int argc, char** argv)
     }
     fprintf(out, " ***************************************************\n");
     fprintf(out, " Data fields:\n    #,  ");
-    for (auto* c : cols) fprintf(out, "%-21s ", c->name);
+    for (auto* c : cols) fprintf(out, "%-21s ", c->name.c_str());
+    for (auto& c : fcols) fprintf(out, "%-21s ", c.name.c_str());
     fprintf(out, "\n ***************************************************\n");
     for (int i = 0; i < st.iter; ++i) {
         fprintf(out, "%5i", i);
         for (auto* c : cols) fprintf(out, " % 16.13e    ", c->v[i]);
+        for (auto& c : fcols) fprintf(out, " % 16.13e    ", c.v[i]);
         fprintf(out, "\n");
     }
     fclose(out);

@@ -147,14 +147,39 @@ class Lattice:
         check(lib().qcd_lattice_sync(self._h))
 
     # -- measurements ---------------------------------------------------------
-    def measure(self, plaquette=True, polyakov=True, wilson=False, polyakov_slices=False):
+    def measure(self, plaquette=True, polyakov=True, wilson=False, polyakov_slices=False, fmunu=None):
+        """fmunu: None, "H" (planes xy, xz, yz) or "E" (xt, yt, zt); it
+        implies the plaquette."""
+        if fmunu not in (None, "H", "E"):
+            raise ValueError("fmunu must be None, 'H' or 'E'")
         mask = (
             (_abi.MEAS_PLAQUETTE if plaquette else 0)
             | (_abi.MEAS_POLYAKOV if polyakov else 0)
             | (_abi.MEAS_WILSON if wilson else 0)
             | (_abi.MEAS_POLYAKOV_SLICES if polyakov_slices else 0)
+            | {None: 0, "H": _abi.MEAS_FMUNU, "E": _abi.MEAS_F0MU}[fmunu]
         )
         check(lib().qcd_measure(self._h, mask))
+
+    def set_fmunu(self, gen1, gen2):
+        """Generators of the F_munu measurement: SU(3) lambda_gen1 (1..4) and
+        lambda_gen2 (5..8), default 3 and 8; SU(2) sigma_gen1 (1 or 3) and
+        sigma_2 (suncl.cpp:2093-2113)."""
+        check(lib().qcd_lattice_set_fmunu(self._h, int(gen1), int(gen2)))
+
+    def fmunu(self, raw=False):
+        """F_munu records, one per plaquette measurement: array (count, 2, 3)
+        complex, [generator][plane], planes (x,y), (x,z), (y,z) for the H
+        field or (x,t), (y,t), (z,t) for the E field.  Raw: sums over sites
+        of -i Tr(P G) / 2; otherwise divided by V, as the reference's
+        analysis (suncl.cpp:1614-1623)."""
+        cap = self.params.max_measurements
+        a = np.zeros((cap, 2, 3, 2))
+        c = ctypes.c_int()
+        check(lib().qcd_read_fmunu(self._h, a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.byref(c)))
+        a = a[: min(c.value, cap)]
+        z = a[..., 0] + 1j * a[..., 1]
+        return z if raw else z / float(self.volume)
 
     def polyakov_slices(self, raw=False):
         """Polyakov loop per x, y and z slice for every polyakov_slices
@@ -313,13 +338,16 @@ class Lattice:
         """Write the reference's .qcg state file (suncl.cpp:1857-1993) plus
         the exact state image.  `run` sets qcd_run_info fields (init, start,
         nav, nav_counter, niter, iter, iter_counter, wilson_r, wilson_t,
-        pl_level, get_actions, get_plaquettes, get_wilson); by default ITER is
-        the record capacity and ITER_counter the number of plaquette records."""
+        pl_level, get_actions, get_plaquettes, get_wilson, fmunu_index1,
+        fmunu_index2, get_fmunu, get_f0mu); by default ITER is the record
+        capacity and ITER_counter the number of plaquette records.  With
+        get_fmunu or get_f0mu the file carries the F_munu records."""
         plq, _ = self.raw_measurements()
         info = _abi.RunInfo()
         defaults = dict(init=1, start="cold", nav=0, nav_counter=0, niter=1, iter=self.params.max_measurements,
                         iter_counter=len(plq), wilson_r=7, wilson_t=2, pl_level=3, get_actions=1,
-                        get_plaquettes=1, get_wilson=0)
+                        get_plaquettes=1, get_wilson=0, fmunu_index1=3, fmunu_index2=8 if self.sun == 3 else 2,
+                        get_fmunu=0, get_f0mu=0)
         unknown = set(run) - set(defaults) - {"prng_counter"}
         if unknown:
             raise ValueError("unknown run fields %s" % sorted(unknown))
@@ -337,12 +365,13 @@ class Lattice:
         return info.as_dict()
 
     # -- the reference run loop -------------------------------------------------
-    def simulate(self, nav=0, iters=2, niter=1, n_or=0, start="cold", wilson=False, slices=False):
+    def simulate(self, nav=0, iters=2, niter=1, n_or=0, start="cold", wilson=False, slices=False, fmunu=None):
         """model::lattice_simulate: start, measure, NAV sweeps, then ITER-1
         measurements each after NITER sweeps (suncl.cpp:5236-5483).  Each sweep
         is one heatbath sweep plus `n_or` overrelaxation sweeps; wilson=True
         adds the Wilson loop (set_wilson) to every measurement, slices=True the
-        Polyakov loop per slice (PL_level > 2)."""
+        Polyakov loop per slice (PL_level > 2), fmunu="H"/"E" the field
+        strength (get_Fmunu / get_F0mu)."""
         if start == "hot":
             self.prng_init()   # the Haar start draws from the streams
             self.init(start)
@@ -351,7 +380,7 @@ class Lattice:
             if start != "gid":
                 self.prng_init()
         self.reset_measurements()
-        self.measure(wilson=wilson, polyakov_slices=slices)
+        self.measure(wilson=wilson, polyakov_slices=slices, fmunu=fmunu)
         for _ in range(nav):
             self.sweep("hb")
             for _ in range(n_or):
@@ -361,7 +390,7 @@ class Lattice:
                 self.sweep("hb")
                 for _ in range(n_or):
                     self.sweep("or")
-            self.measure(wilson=wilson, polyakov_slices=slices)
+            self.measure(wilson=wilson, polyakov_slices=slices, fmunu=fmunu)
         self.sync()
         return self.measurements()
 

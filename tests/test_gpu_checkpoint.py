@@ -68,7 +68,8 @@ def _same_run(a, b):
 def _rng_slice(lat, img):
     m = lat.params.max_measurements
     slices = 2 * 8 * (lat.dims[0] + lat.dims[1] + lat.dims[2]) * m   # Polyakov slice records
-    n = img.size - STATE_HEAD - lat.table_bytes() - 7 * 8 * m - slices
+    fmunu = 12 * 8 * m                                                # F_munu records
+    n = img.size - STATE_HEAD - lat.table_bytes() - 7 * 8 * m - slices - fmunu
     return slice(STATE_HEAD + lat.table_bytes(), STATE_HEAD + lat.table_bytes() + n)
 
 
@@ -255,7 +256,7 @@ def _rows(path):
     rows = []
     for line in open(path):
         parts = line.split()
-        if len(parts) == 12 and parts[0].isdigit():
+        if len(parts) == 28 and parts[0].isdigit():   # 11 columns and 16 of F_munu (on by default)
             rows.append(parts[1:])
     return rows
 
@@ -293,3 +294,42 @@ def test_host_stop_and_resume(tmp_path):
     assert len(out_a) == 1 and len(out_c) == 1
     rows_a, rows_c = _rows(da / out_a[0]), _rows(dc / out_c[0])
     assert len(rows_a) == 12 and rows_a == rows_c
+
+
+@pytest.mark.parametrize("field", ["H", "E"])
+def test_qcg_fmunu_sections(tmp_path, field):
+    """F_munu records travel in the reference's longer plaquette section
+    (7 x energies_size: plaquettes, then the planes of each generator) and
+    in the exact image; a load restores them and the generators."""
+    _case("qcg fmunu %s" % field)
+    cfg = dict(sun=3, prec="fp32", storage=12, prng="XOR128")
+    full, cut = tmp_path / "f.qcg", tmp_path / "c.qcg"
+    with _lat(**cfg) as a, _lat(**cfg) as b:
+        _start(a, "hot")
+        a.set_fmunu(2, 6)
+        a.measure(fmunu=field)
+        _cycles(a, 1)
+        a.measure(fmunu=field)
+        F = a.fmunu(raw=True)
+        assert F.shape[0] >= 2 and np.any(F[-1])
+        a.save_qcg(full, start="hot", iter=16, fmunu_index1=2, fmunu_index2=6,
+                   get_fmunu=int(field == "H"), get_f0mu=int(field == "E"))
+        d = qcg.decode(full.read_bytes())
+        h = d["header"]
+        assert h["energies_size_F"] == 7 * h["energies_size"] and h["get_fmunu"] == int(field == "H")
+        assert (h["fmunu_index1"], h["fmunu_index2"]) == (2, 6)
+        n = F.shape[0]
+        plq, _ = a.raw_measurements()
+        assert np.array_equal(d["plaquettes"][:n], plq[:n])
+        assert np.array_equal(d["fmunu"][:n], F) and not np.any(d["fmunu"][n:])
+        cut.write_bytes(full.read_bytes()[: d["table_end"]])
+        info = b.load_qcg(cut)
+        assert (info["get_fmunu"], info["get_f0mu"]) == (int(field == "H"), int(field == "E"))
+        assert np.array_equal(b.fmunu(raw=True), F)
+        c, _ = Q.Lattice.from_qcg(full)        # exact image
+        with c:
+            assert np.array_equal(c.fmunu(raw=True), F)
+            for lat in (a, b, c):               # generators restored from the header
+                lat.measure(fmunu=field)
+            assert np.array_equal(b.fmunu(raw=True)[-1], a.fmunu(raw=True)[-1])
+            assert np.array_equal(c.fmunu(raw=True)[-1], a.fmunu(raw=True)[-1])

@@ -13,6 +13,8 @@ import numpy as np
 import pytest
 
 import qcdgpu_amd as Q
+from qcdgpu_amd import _abi
+from qcdgpu_amd._abi import check, lib
 
 pytestmark = pytest.mark.gpu
 
@@ -544,3 +546,45 @@ def test_polyakov_slices_vs_explicit_products(sun, prec):
         _, poly = lat.raw_measurements()
         assert abs(s["x"][1].sum().real - poly[1, 0]) < 1e-9 * tr.size
         assert abs(s["y"][1].sum().imag - poly[1, 1]) < 1e-9 * tr.size
+
+
+@pytest.mark.parametrize("sun,prec,storage", [(3, "fp64", None), (3, "fp32", None), (3, "fp32", 12),
+                                              (2, "fp64", None), (2, "fp32", None)])
+def test_fmunu_vs_oracle(O, sun, prec, storage):
+    """F_munu (H and E field, every generator choice) on the GPU's own links
+    vs the oracle on the exported copy; F records sit beside the plaquette
+    records (zeros where the plaquette was measured without F); the
+    plaquette itself is unchanged by measuring F."""
+    _case("fmunu sun=%d %s storage=%s" % (sun, prec, storage))
+    dims = (6, 4, 4, 6)
+    beta = 5.9 if sun == 3 else 2.3
+    V = int(np.prod(dims))
+    gens = [(3, 8), (1, 5), (2, 6), (4, 7)] if sun == 3 else [(3, 2), (1, 2)]
+    tol = (1e-12 if prec == "fp64" else 1e-5) * V
+    with _lat(sun, dims, prec, storage=storage, beta=beta, seed=5) as lat:
+        lat.prng_init()
+        lat.init("hot")
+        for _ in range(2):
+            lat.sweep("hb")
+        lat.reset_measurements()
+        lat.measure()                    # record 0: no F
+        links = lat.export_links()
+        L = O.lattice(sun, dims, beta, 10)
+        want = []
+        for g1, g2 in gens:
+            lat.set_fmunu(g1, g2)
+            for field in ("H", "E"):
+                lat.measure(fmunu=field)
+                want.append(O.fmunu(L, links, field == "E", g1, g2))
+        F = lat.fmunu(raw=True)
+        plq, _ = lat.raw_measurements()
+    assert F.shape == (1 + 2 * len(gens), 2, 3)
+    assert not np.any(F[0])
+    for k, w in enumerate(want):
+        assert np.max(np.abs(F[1 + k] - w)) < tol, (k, F[1 + k], w)
+    assert np.all(plq[1:] == plq[0])     # the same links: identical plaquette sums
+    with _lat(sun, dims, prec) as lat2:
+        with pytest.raises(Q.QCDError, match="generators"):
+            lat2.set_fmunu(5, 3)         # gen1 out of range
+        with pytest.raises(Q.QCDError, match="not both"):   # H and E in one measurement
+            check(lib().qcd_measure(lat2._h, _abi.MEAS_FMUNU | _abi.MEAS_F0MU))

@@ -52,12 +52,32 @@ def test_reference_init_syntax(tmp_path):
 
 
 @pytest.mark.parametrize(
-    "line", ["PHI = 3.0", "OMEGA = 0.5", "FMUNU = 1", "PRNG = RANMAR", "ND = 3", "GROUP = 4",
+    "line", ["PHI = 3.0", "OMEGA = 0.5", "PRNG = RANMAR", "ND = 3", "GROUP = 4",
              "GETWILSON = 1\nWILSONR = 0"]
 )
 def test_unsupported_features_rejected(tmp_path, line):
     rc, kv = _params(line + "\n", tmp_path)
     assert rc == 2 and "UNSUPPORTED" in kv
+
+
+@pytest.mark.parametrize(
+    "text, want",
+    [
+        ("", ("H", "3", "8")),                              # get_Fmunu is on by default (suncl.cpp:761)
+        ("F0MU = 1\n", ("E", "3", "8")),
+        ("FMUNU = 0\nTURNOFFFMUNU = 1\n", ("OFF", "3", "8")),
+        ("FMUNU1 = 1\nFMUNU6 = 1\n", ("H", "1", "6")),
+        ("FMUNU1 = 1\nFMUNU4 = 1\nFMUNU7 = 1\nFMUNU5 = 1\n", ("H", "4", "5")),   # a later key replaces
+        ("GROUP = 2\nFMUNU1 = 1\nFMUNU6 = 1\n", ("H", "1", "2")),   # SU(2): sigma_1 or sigma_3, sigma_2
+        ("GROUP = 2\nFMUNU2 = 1\n", ("H", "3", "2")),
+    ],
+)
+def test_fmunu_keys(tmp_path, text, want):
+    """model::lattice_get_init_file F_munu keys (suncl.cpp:412-453) act by
+    presence; Fmunu_defaults picks the generators (suncl.cpp:2093-2113)."""
+    rc, kv = _params(text, tmp_path)
+    assert rc == 0
+    assert (kv["FMUNU"], kv["FMUNU_INDEX1"], kv["FMUNU_INDEX2"]) == want
 
 
 def test_bad_precision_value(tmp_path):
@@ -66,44 +86,55 @@ def test_bad_precision_value(tmp_path):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("prng,ints", [("XOR128", 1), ("RANLUX3", 0)])
-def test_host_run_matches_python_driver(tmp_path, prng, ints):
-    """qcdgpu_hip on a small lattice writes the same plaquette and Polyakov
-    series as qcdgpu_amd.Lattice.simulate with the same parameters."""
+@pytest.mark.parametrize("prng,ints,fkeys", [("XOR128", 1, ""), ("RANLUX3", 0, "F0MU = 1\nFMUNU1 = 1\nFMUNU7 = 1\n")])
+def test_host_run_matches_python_driver(tmp_path, prng, ints, fkeys):
+    """qcdgpu_hip on a small lattice writes the same plaquette, Wilson loop,
+    Polyakov and F_munu series as qcdgpu_amd.Lattice.simulate with the same
+    parameters (F_munu is on by default, as in the reference)."""
     import qcdgpu_amd as Q
 
-    print("[case] host run %s ints=%d" % (prng, ints), flush=True)
+    print("[case] host run %s ints=%d %s" % (prng, ints, fkeys.split("\n")[0]), flush=True)
     f = tmp_path / "init.dat"
     f.write_text(
         "GROUP = 3\nLS = 8\nLT = 4\nITER = 4\nNITER = 2\nNAV = 1\nBETA = 5.9\nINTS = %d\nPRNG = %s\n"
         "RANDSERIES = 11\nPRECISION = DOUBLE\nNOR = 2\nGETWILSON = 1\nWILSONR = 2\nWILSONT = 2\n"
-        "OUTPUTPATH = %s/\n" % (ints, prng, tmp_path)
+        "OUTPUTPATH = %s/\n%s" % (ints, prng, tmp_path, fkeys)
     )
+    efield = "F0MU" in fkeys
+    g1, g2 = (1, 7) if fkeys else (3, 8)
     r = subprocess.run([BIN, str(f)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     out = [p for p in os.listdir(tmp_path) if p.startswith("su3-") and p.endswith(".txt")]
     assert len(out) == 1
-    rows, pl_rows, section = [], {}, None
-    for line in open(tmp_path / out[0]):
+    rows, pl_rows, section, fields = [], {}, None, None
+    lines = open(tmp_path / out[0]).read().splitlines()
+    for k, line in enumerate(lines):
         parts = line.split()
-        if len(parts) == 12 and parts[0].isdigit():
+        if line.strip() == "Data fields:":
+            fields = lines[k + 1].split()[1:]
+        if len(parts) == 28 and parts[0].isdigit():
             rows.append([float(x) for x in parts[1:]])
         if parts and parts[0] in ("X", "Y", "Z") and len(parts) == 9:
             section = parts[0]
         elif parts and parts[0] in ("PL", "PL_im") and section:
             pl_rows[section, parts[0]] = np.array([float(x) for x in parts[1:]])
     rows = np.array(rows)
-    assert rows.shape == (4, 11)   # Plq x3, Wilson_loop, S x3, Polyakov x4
+    assert rows.shape == (4, 27)   # Plq x3, Wilson_loop, S x3, Polyakov x4, F_munu x16
+    planes = ("xt", "yt", "zt") if efield else ("xy", "xz", "yz")
+    assert fields[11] == "Fmunu_%s_%d_re" % (planes[0], g1) and fields[17] == "Fmunu_%s_%d_re" % (planes[0], g2)
+    assert fields[26] == "Fmunu_abs_%d_im" % g2
     assert len(pl_rows) == 6        # PL_LEVEL 3 (the default): the differentiated Polyakov loop block
     lux = int(prng[-1]) if prng.startswith("RANLUX") else 3
     name = "RANLUX" if prng.startswith("RANLUX") else prng
     with Q.Lattice(sun=3, dims=(8, 8, 8, 4), beta=5.9, precision="fp64", prng=name, ranlux_luxury=lux,
                    randseries=11, max_measurements=8) as lat:
         lat.set_wilson(2, 2)
+        lat.set_fmunu(g1, g2)
         m = lat.simulate(nav=1, iters=4, niter=2, n_or=2, start={0: "hot", 1: "cold"}[ints], wilson=True,
-                         slices=True)
+                         slices=True, fmunu="E" if efield else "H")
         wl = lat.wilson_measurements()
         sl = lat.polyakov_slices()
+        F = lat.fmunu()
     assert np.allclose(rows[:, 0], m["plq_spat"], rtol=1e-13, atol=0)
     assert np.allclose(rows[:, 1], m["plq_temp"], rtol=1e-13, atol=0)
     assert np.allclose(rows[:, 3], wl, rtol=1e-12, atol=1e-15)
@@ -111,3 +142,10 @@ def test_host_run_matches_python_driver(tmp_path, prng, ints):
     for d in "xyz":   # slice means without measurement 0, as lattice_data_analysis
         assert np.allclose(pl_rows[d.upper(), "PL"], sl[d][1:].real.mean(axis=0), rtol=1e-11, atol=1e-14)
         assert np.allclose(pl_rows[d.upper(), "PL_im"], sl[d][1:].imag.mean(axis=0), rtol=1e-11, atol=1e-14)
+    for gi in range(2):
+        for pl in range(3):
+            assert np.allclose(rows[:, 11 + 6 * gi + 2 * pl], F[:, gi, pl].real, rtol=1e-12, atol=1e-15)
+            assert np.allclose(rows[:, 12 + 6 * gi + 2 * pl], F[:, gi, pl].imag, rtol=1e-12, atol=1e-15)
+        # |F| = sqrt of the sum over planes of squares (lattice_data_analysis_joint3)
+        assert np.allclose(rows[:, 23 + 2 * gi], np.sqrt((F[:, gi].real ** 2).sum(axis=1)), rtol=1e-12, atol=1e-15)
+        assert np.allclose(rows[:, 24 + 2 * gi], np.sqrt((F[:, gi].imag ** 2).sum(axis=1)), rtol=1e-12, atol=1e-15)

@@ -84,7 +84,8 @@ def test_library_reads_reference_header(tmp_path):
     assert p.beta == 5.9 and p.storage_reals == 18 and p.stream_mode == _abi.STREAM_REFERENCE
     assert p.max_measurements == 10
     assert info == dict(init=1, start=0, prng_counter=45, nav=3, nav_counter=3, niter=2, iter=10, iter_counter=6,
-                        wilson_r=3, wilson_t=1, pl_level=2, get_actions=1, get_plaquettes=1, get_wilson=1)
+                        wilson_r=3, wilson_t=1, pl_level=2, get_actions=1, get_plaquettes=1, get_wilson=1,
+                        fmunu_index1=3, fmunu_index2=8, get_fmunu=0, get_f0mu=0)
     # RANLUX luxury and PM codes, SU(2), double precision
     h2 = qcg.make_header(2, (2, 2, 2, 2), 2.3, "double", "RANLUX2", iter=3)
     p2, _, _ = read_qcg_header(_write(tmp_path, "b.qcg", h2, haar(rng, 16, 2)))
@@ -94,12 +95,45 @@ def test_library_reads_reference_header(tmp_path):
     assert read_qcg_header(_write(tmp_path, "c.qcg", h3, haar(rng, 16, 3)))[0].prng == _abi.PRNG_PM
 
 
+@pytest.mark.parametrize("field", ["H", "E"])
+def test_fmunu_sections(tmp_path, field):
+    """With F_munu the partials and plaquette sections are 7 x longer
+    (suncl.cpp:2266-2298); the plaquette section holds the plaquette sums,
+    then F for the three planes of the first generator and of the second
+    (reduce_measurement_plq_double2, sun_measurements_cl.cl:1312-1354).  An
+    E-field run writes get_Fmunu 0 with the longer sections."""
+    rng = np.random.default_rng(7)
+    dims = (4, 4, 2, 2)
+    links = haar(rng, 64, 3)
+    h = qcg.make_header(3, dims, 5.9, "double", "XOR128", f_sections=True, iter=5, iter_counter=5,
+                        fmunu_index1=2, fmunu_index2=6, get_fmunu=1 if field == "H" else 0)
+    plq = rng.normal(size=(5, 2))
+    F = rng.normal(size=(5, 2, 3)) + 1j * rng.normal(size=(5, 2, 3))
+    data = qcg.encode(h, links, plq=plq, poly=rng.normal(size=(5, 2)), fmunu=F)
+    s = qcg.sizes(3, dims, 5)
+    assert len(data) == 4 * 64 + 16 * 7 * s["measurement_size_F"] + 16 * s["energies_size"] * (1 + 7) \
+        + 16 * s["polyakov_loop_size"] + 32 * s["table_size"]
+    d = qcg.decode(data)
+    assert np.array_equal(d["plaquettes"][:5], plq) and np.array_equal(d["fmunu"][:5], F)
+    # block 1 + 3 g + plane starts at (1 + 3 g + plane) * energies_size double2
+    raw = np.frombuffer(data, dtype=np.float64, offset=4 * 64 + 16 * 7 * s["measurement_size_F"]
+                        + 16 * s["energies_size"]).reshape(-1, 2)
+    en = s["energies_size"]
+    assert raw[(1 + 3 * 1 + 2) * en + 4, 0] == F[4, 1, 2].real
+    path = tmp_path / "f.qcg"
+    path.write_bytes(data)
+    _, info, _ = read_qcg_header(path)
+    assert (info["fmunu_index1"], info["fmunu_index2"]) == (2, 6)
+    assert (info["get_fmunu"], info["get_f0mu"]) == ((1, 0) if field == "H" else (0, 1))
+
+
 @pytest.mark.parametrize(
     "change, message",
     [
         (lambda w: w.__setitem__(0, 0), "prefix"),
         (lambda w: w.__setitem__(2, qcg.string_word("1.4", 0)), "version"),
-        (lambda w: w.__setitem__(qcg.WORD["get_fmunu"], 1), "F_munu"),
+        (lambda w: w.__setitem__(qcg.WORD["get_fmunu"], 1), "get_Fmunu but no F_munu sections"),
+        (lambda w: w.__setitem__(qcg.WORD["energies_size_F"], 3 * w[qcg.WORD["energies_size"]]), "energies_size_F"),
         (lambda w: w.__setitem__(qcg.WORD["phi_hi"], 0x3FF00000), "PHI"),
         (lambda w: w.__setitem__(42, 6), "multi-device"),
         (lambda w: w.__setitem__(qcg.WORD["generator"], 8), "generator"),
