@@ -161,6 +161,22 @@ int  qcd_read_measurements(qcd_lattice *lat, double *plq, double *poly, int *cou
         QCD_MEAS_WILSON measurement appends sum ReTr W over sites and the
         three spatial directions (the reference divides by 3V).            */
 int  qcd_lattice_set_wilson(qcd_lattice *lat, int R, int T);
+/* ---- twisted boundary conditions (PHI / OMEGA, the reference's -D TBC:
+        lattice_table_3 / lattice_table_2, su3_matrix_memory.cl:57-98,
+        su2_matrix_memory.cl:56-88; PHI / OMEGA in lattice_parameters[1..2],
+        suncl.cpp:3941-3951).  Every read of a Y link of a site with
+        x = N1 - 1 -- in the updates (the link itself and the staples), the
+        plaquette, the Wilson loop and F_munu -- sees T U, with
+        T = diag(e^{i(phi+omega)/2}, e^{i(omega-phi)/2}, e^{-i omega}) for
+        SU(3) and diag(e^{i phi/2}, e^{-i phi/2}) for SU(2) (omega unused);
+        an update stores its result as computed, without removing T, as the
+        reference does.  phi = omega = 0 (the default) turns the twist off.
+        Set it before qcd_cycle_capture (a captured cycle keeps the values
+        it was captured with); while it is on, cycles measure the plaquette
+        with the standalone kernel.  Export, import and the state image hold
+        the stored links.                                                    */
+int  qcd_lattice_set_twist(qcd_lattice *lat, double phi, double omega);
+int  qcd_lattice_get_twist(qcd_lattice *lat, double *phi, double *omega);
 int  qcd_read_wilson(qcd_lattice *lat, double *w, int *count);
 /* ---- Polyakov loop per slice (PL_level > 2: lattice_polyakov_diff_x/y/z +
         reduce_polyakov_diff_*_double2, polyakov.cl:142-560): a
@@ -270,6 +286,9 @@ typedef struct {
     unsigned get_f0mu;        /* E-field F_munu: not a header word; a file
                                  with F sections (energies_size_F = 7 x
                                  energies_size, word 34) and get_fmunu 0       */
+    double   phi, omega;      /* PHI, OMEGA (words 21-24; the reader reports
+                                 them and applies them to the lattice, the
+                                 writer takes them from the lattice)           */
 } qcd_run_info;
 /* Writes the state file (atomically: a temporary file renamed into place).  */
 int  qcd_qcg_write(qcd_lattice *lat, const char *path, const qcd_run_info *info);

@@ -274,3 +274,54 @@ def parity_site(L, i, parity):
 
 def neighbour(L, site, d, sign):
     return lib().orc_neighbour(ctypes.byref(L), site, d, sign)
+
+
+# ---------------------------------------------------------------- twisted boundary
+def twist_matrix(sun, phi, omega=0.0):
+    """The matrix T that lattice_table_3 / lattice_table_2 under TBC
+    (su3_matrix_memory.cl:57-98, su2_matrix_memory.cl:56-88) multiplies the
+    Y link at x = N1 - 1 by, from the left: SU(3) scales the stored first row
+    by e^{i(phi+omega)/2} and the second by e^{i(omega-phi)/2} (the rebuilt
+    third row then carries e^{-i omega}); SU(2) multiplies by
+    Omega = diag(e^{i phi/2}, e^{-i phi/2}) (omega unused)."""
+    if sun == 3:
+        d = np.exp(1j * np.array([(phi + omega) / 2, (omega - phi) / 2, -omega]))
+    else:
+        d = np.exp(1j * np.array([phi / 2, -phi / 2]))
+    return np.diag(d)
+
+
+def _x_of_sites(L):
+    """x coordinate of every site in the oracle's order (x slowest)."""
+    V = int(np.prod(list(L.n)))
+    return np.arange(V) // (L.n[1] * L.n[2] * L.n[3])
+
+
+def twisted_view(L, links, phi, omega=0.0):
+    """The links as every read under TBC sees them: Y links (direction 1) of
+    the sites with x = N1 - 1 replaced by T U; all others unchanged."""
+    out = np.array(links, dtype=np.float64, copy=True)
+    if phi == 0.0 and omega == 0.0:
+        return out
+    T = twist_matrix(L.sun, phi, omega)
+    sel = _x_of_sites(L) == L.n[0] - 1
+    U = out[sel, 1, ..., 0] + 1j * out[sel, 1, ..., 1]
+    W = T @ U
+    out[sel, 1, ..., 0], out[sel, 1, ..., 1] = W.real, W.imag
+    return out
+
+
+def halfstep_twisted(L, links, parity, mu, mode, rng, phi, omega=0.0):
+    """One (parity, mu) half-step under TBC: every read sees the twisted view
+    (the link being updated included), the results are stored as computed
+    (update_{even,odd}_Y with lattice_table_3 / lattice_store_3,
+    suncl.cl:505-552).  A half-step reads no link it writes (staples of
+    (parity, mu) links hold only direction-mu links of the other parity), so
+    running the plain half-step on the view and copying its (parity, mu)
+    links back is that rule exactly."""
+    view = twisted_view(L, links, phi, omega)
+    rejected = halfstep(L, view, parity, mu, mode, rng)
+    V = int(np.prod(list(L.n)))
+    sites = np.array([parity_site(L, i, parity) for i in range(V // 2)])
+    links[sites, mu] = view[sites, mu]
+    return rejected

@@ -59,10 +59,10 @@ class RunInfo(ctypes.Structure):
             "wilson_r", "wilson_t", "pl_level", "get_actions", "get_plaquettes", "get_wilson",
             "fmunu_index1", "fmunu_index2", "get_fmunu", "get_f0mu",
         )
-    ]
+    ] + [("phi", ctypes.c_double), ("omega", ctypes.c_double)]
 
     def as_dict(self):
-        return {name: int(getattr(self, name)) for name, _ in self._fields_}
+        return {name: (float if t is ctypes.c_double else int)(getattr(self, name)) for name, t in self._fields_}
 
 
 class QCDError(RuntimeError):
@@ -98,6 +98,8 @@ _SIGNATURES = {
     "qcd_read_wilson": [_P, _D, _I],
     "qcd_read_polyakov_slices": [_P, _D, _I],
     "qcd_lattice_set_fmunu": [_P, ctypes.c_int, ctypes.c_int],
+    "qcd_lattice_set_twist": [_P, ctypes.c_double, ctypes.c_double],
+    "qcd_lattice_get_twist": [_P, _D, _D],
     "qcd_read_fmunu": [_P, _D, _I],
     "qcd_cycle_capture": [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int],
     "qcd_cycle_run": [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int],

@@ -399,6 +399,16 @@ struct Geom {
     FastDiv dh, d3, d4;   // division by h, N3, N4
 };
 
+// Twisted boundary: the Y link of every site with x = N1 - 1 (compact index
+// >= from in either parity, x being the slowest coordinate) is read as
+// diag(d) U.  Passed to the kernels after all other arguments, so that the
+// argument layout of the untwisted kernels is unchanged.
+struct Twist {
+    int on = 0;
+    uint32_t from = 0xffffffffu;
+    double d[6] = {1.0, 0.0, 1.0, 0.0, 1.0, 0.0};   // (re, im) of the diagonal entries
+};
+
 struct Site {
     int x, y, z, t;
 };
@@ -586,6 +596,32 @@ template <typename R, int N, int NREAL> struct Table {
         }
     }
 };
+
+// Twisted boundary conditions (lattice_table_3 / lattice_table_2 under TBC,
+// su3_matrix_memory.cl:57-98, su2_matrix_memory.cl:56-88): the reference
+// scales the stored rows of the Y link at x = N1 - 1 by e^{i(phi+omega)/2}
+// and e^{i(omega-phi)/2} (SU(3); row 3, rebuilt from them, picks up
+// e^{-i omega}), or multiplies by diag(e^{i phi/2}, e^{-i phi/2}) (SU(2)),
+// every time the link is read; stores write the link as computed.  Here the
+// whole matrix is scaled row by row after the load (the same matrix: for
+// two-row storage the rebuilt third row of the scaled rows equals the
+// third row scaled by e^{-i omega}).
+template <typename R, int N>
+__device__ __forceinline__ void twist_rows(M<R, N>& a, const Twist& t)
+{
+#pragma unroll
+    for (int r = 0; r < N; ++r) {
+        const C<R> f{(R)t.d[2 * r], (R)t.d[2 * r + 1]};
+#pragma unroll
+        for (int c = 0; c < N; ++c) a.e[r][c] = cmul(f, a.e[r][c]);
+    }
+}
+// The link of direction dir at compact index idx as every kernel reads it.
+template <typename R, int N>
+__device__ __forceinline__ void tbc(M<R, N>& a, const Twist& t, int dir, uint32_t idx)
+{
+    if (t.on && dir == 1 && idx >= t.from) twist_rows(a, t);
+}
 
 // ------------------------------------------------------------------ generators
 // XOR128 (Marsaglia 2003), one step as prngcl_xor128.cl:62-74.

@@ -74,6 +74,7 @@ struct UpdArgs {
     int meas_mode;
     int xcd_map;          // 0: hardware order, 1: one x-slab per XCD, 2: 1 + cross-section-first order
     uint32_t ord_ng, ord_nt, ord_xs;   // xcd_map 2: z-groups, N4, x-values per XCD slab
+    Twist tw;             // twisted boundary (read only by the TW instantiations)
 };
 
 // Hardware dispatches workgroup b to XCD (b mod 8).  Remap so that each XCD
@@ -289,10 +290,11 @@ __device__ __forceinline__ C<R> ua_entry(const M<R, 3>& U, const M<R, 3>& A, int
 // One direction's share of the staple sum,
 //   S_nu = U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+ + U_nu(x+mu-nu)^+ U_mu(x-nu)^+ U_nu(x-nu)
 // (suncpp/Update/sun_update.h:44-77; su3_update_cl.cl:341-565).  nu, mu and
-// the parity p are wavefront-uniform.
-template <typename R, int N, int NREAL>
+// the parity p are wavefront-uniform.  TW: apply the twisted boundary (a
+// separate instantiation, so the untwisted kernels carry none of its code).
+template <typename R, int N, int NREAL, bool TW = false>
 __device__ __forceinline__ M<R, N> staple_nu(const Table<R, N, NREAL>& T, const Geom& g, Site s, uint32_t i, int p,
-                                             int mu, int nu)
+                                             int mu, int nu, const Twist& tw = Twist{})
 {
     using Tab = Table<R, N, NREAL>;
     const int q = p ^ 1;
@@ -320,6 +322,16 @@ __device__ __forceinline__ M<R, N> staple_nu(const Table<R, N, NREAL>& T, const 
     Tab::finish(b1);
     Tab::finish(b2);
     Tab::finish(b3);
+    if constexpr (TW) {
+        if (mu == 1 || nu == 1) {   // wavefront-uniform
+            tbc(f2, tw, mu, i_pnu);
+            tbc(b1, tw, mu, i_mnu);
+            tbc(f1, tw, nu, i_pmu);
+            tbc(b3, tw, nu, i_mnu);
+            tbc(f3, tw, nu, i);
+            tbc(b2, tw, nu, i_pmu_mnu);
+        }
+    }
     M<R, N> S;
     if constexpr (N == 3) {
         const M<R, 3> F = su3_fwd_term(f1, f2, f3);
@@ -364,7 +376,7 @@ constexpr int kUpdThreads = 3 * kLinks;
 // Returns ReTr(U' * Alow) for the fused plaquette (0 unless MEAS).  `rs` is
 // the generator state at the start of the half-step; in the fast stream
 // layout the state after consumption is written to rng_out.
-template <typename R, int N, int NREAL, bool OVER, int RK, bool MEAS>
+template <typename R, int N, int NREAL, bool OVER, int RK, bool MEAS, bool TW = false>
 __device__ __forceinline__ double update_link(const UpdArgs& a, const Table<R, N, NREAL>& Tb, uint32_t i, const Site& s,
                                               const M<R, N>& A, const M<R, N>& Alow, uint4 rs)
 {
@@ -372,6 +384,7 @@ __device__ __forceinline__ double update_link(const UpdArgs& a, const Table<R, N
     const int p = a.parity, mu = a.mu;
     double trace = 0.0;
     M<R, N> U = Tb.load_u(p, mu, i);
+    if constexpr (TW) tbc(U, a.tw, mu, i);   // stored back as updated, the twist kept (as the reference)
     SrcGid sg{};
     SrcXor sx{};
     SrcPm sp{};
@@ -481,7 +494,7 @@ __device__ __forceinline__ uint32_t upd_block(const UpdArgs& a)
 // Body of the three-wavefront update for 64-link unit `blk` of a lattice
 // with `nunits` units (the fused-plaquette partial of the unit goes to
 // partials[(meas_mode - 1) * nunits + blk]).
-template <typename R, int N, int NREAL, bool OVER, int RK, bool MEAS>
+template <typename R, int N, int NREAL, bool OVER, int RK, bool MEAS, bool TW = false>
 __device__ __forceinline__ void update3(const UpdArgs& a, uint32_t blk, uint32_t nunits)
 {
     using T = R;   // compute type of the SU(2) subgroup arithmetic
@@ -520,7 +533,7 @@ __device__ __forceinline__ void update3(const UpdArgs& a, uint32_t blk, uint32_t
         }
     }
     M<R, N> S;
-    if (active) S = staple_nu<R, N, NREAL>(Tb, a.g, s, i, p, mu, nu);
+    if (active) S = staple_nu<R, N, NREAL, TW>(Tb, a.g, s, i, p, mu, nu, a.tw);
     else m_zero(S);
     if (w > 0) {
 #pragma unroll
@@ -570,7 +583,7 @@ __device__ __forceinline__ void update3(const UpdArgs& a, uint32_t blk, uint32_t
     m_acc(A, S2);
 
     double trace = 0.0;
-    if (active) trace = update_link<R, N, NREAL, OVER, RK, MEAS>(a, Tb, i, s, A, Alow, rs);
+    if (active) trace = update_link<R, N, NREAL, OVER, RK, MEAS, TW>(a, Tb, i, s, A, Alow, rs);
     if constexpr (MEAS) {
         trace = wave_sum(trace);
         // slot = 64-link unit, so the final sum runs in lattice order
@@ -579,10 +592,10 @@ __device__ __forceinline__ void update3(const UpdArgs& a, uint32_t blk, uint32_t
     }
 }
 
-template <typename R, int N, int NREAL, bool OVER, int RK, bool MEAS>
+template <typename R, int N, int NREAL, bool OVER, int RK, bool MEAS, bool TW = false>
 __global__ __launch_bounds__(kUpdThreads) void k_update(UpdArgs a)
 {
-    update3<R, N, NREAL, OVER, RK, MEAS>(a, upd_block(a), gridDim.x);
+    update3<R, N, NREAL, OVER, RK, MEAS, TW>(a, upd_block(a), gridDim.x);
 }
 
 // One wavefront per 64 links: each wavefront gathers and sums the three
@@ -1073,8 +1086,8 @@ __global__ __launch_bounds__(kBlock) void k_finalize(FinArgs f) { finalize_body(
 // Plaquette: per site sum of ReTr U_p over the spatial planes (x,y), (x,z),
 // (y,z) and the temporal planes (x,t), (y,t), (z,t), products in FP64
 // (lattice_retrace_plaquette3, su3_measurements_cl.cl:527-549).
-template <typename R, int N, int NREAL>
-__global__ __launch_bounds__(kBlock) void k_plaquette(void* table, Geom g, double* partials)
+template <typename R, int N, int NREAL, bool TW = false>
+__global__ __launch_bounds__(kBlock) void k_plaquette(void* table, Geom g, double* partials, Twist tw)
 {
     const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     const int p = blockIdx.y, q = p ^ 1;
@@ -1082,16 +1095,21 @@ __global__ __launch_bounds__(kBlock) void k_plaquette(void* table, Geom g, doubl
     if (i < g.vh) {
         Table<R, N, NREAL> T{(typename Vec2<R>::T*)table, g.vh_pad};
         Site s = site_of(g, i, p);
+        auto ld = [&](int pp, int d, uint32_t idx) {
+            M<R, N> m = T.load_u(pp, d, idx);
+            if constexpr (TW) tbc(m, tw, d, idx);
+            return m_cast<double>(m);
+        };
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            M<double, N> Ua = m_cast<double>(T.load_u(p, a, i));
+            M<double, N> Ua = ld(p, a, i);
             const uint32_t i_pa = nb_index(g, s, a, +1);
 #pragma unroll
             for (int b = a + 1; b < 4; ++b) {
                 const uint32_t i_pb = nb_index(g, s, b, +1);
-                M<double, N> Ub = m_cast<double>(T.load_u(p, b, i));
-                M<double, N> Ubxa = m_cast<double>(T.load_u(q, b, i_pa));
-                M<double, N> Uaxb = m_cast<double>(T.load_u(q, a, i_pb));
+                M<double, N> Ub = ld(p, b, i);
+                M<double, N> Ubxa = ld(q, b, i_pa);
+                M<double, N> Uaxb = ld(q, a, i_pb);
                 // ReTr[Ua(x) Ub(x+a) Ua(x+b)^+ Ub(x)^+] = ReTr[(Ua Ubxa)(Ub Uaxb)^+]
                 double tr = m_retr_nd(m_mul(Ua, Ubxa), m_mul(Ub, Uaxb));
                 if (b == 3) v[1] += tr;
@@ -1142,8 +1160,9 @@ __device__ __forceinline__ C<double> gen_trace(const M<double, N>& P, int k)
 // P = U_a(x) U_b(x+a) U_a(x+b)^+ U_b(x)^+ in FP64 and F = -i Tr(P G_k) / 2
 // for the generators g1 and g2 (SU(2): imaginary part 0, as the reference).
 // Block partials: 12 sums, [generator][plane][re, im].
-template <typename R, int N, int NREAL>
-__global__ __launch_bounds__(kBlock) void k_fmunu(void* table, Geom g, double* partials, int efield, int g1, int g2)
+template <typename R, int N, int NREAL, bool TW = false>
+__global__ __launch_bounds__(kBlock) void k_fmunu(void* table, Geom g, double* partials, int efield, int g1, int g2,
+                                                  Twist tw)
 {
     const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     const int p = blockIdx.y, q = p ^ 1;
@@ -1157,10 +1176,15 @@ __global__ __launch_bounds__(kBlock) void k_fmunu(void* table, Geom g, double* p
         const int pb[3] = {efield ? 3 : 1, efield ? 3 : 2, efield ? 3 : 2};
         for (int pl = 0; pl < 3; ++pl) {
             const int a = pa[pl], b = pb[pl];
-            const M<double, N> Ua = m_cast<double>(T.load_u(p, a, i));
-            const M<double, N> Ub = m_cast<double>(T.load_u(p, b, i));
-            const M<double, N> Ubxa = m_cast<double>(T.load_u(q, b, nb_index(g, s, a, +1)));
-            const M<double, N> Uaxb = m_cast<double>(T.load_u(q, a, nb_index(g, s, b, +1)));
+            auto ld = [&](int pp, int d, uint32_t idx) {
+                M<R, N> m = T.load_u(pp, d, idx);
+                if constexpr (TW) tbc(m, tw, d, idx);
+                return m_cast<double>(m);
+            };
+            const M<double, N> Ua = ld(p, a, i);
+            const M<double, N> Ub = ld(p, b, i);
+            const M<double, N> Ubxa = ld(q, b, nb_index(g, s, a, +1));
+            const M<double, N> Uaxb = ld(q, a, nb_index(g, s, b, +1));
             const M<double, N> X = m_mul(Ua, Ubxa), Y = m_mul(Ub, Uaxb);
             M<double, N> P;   // X Y^+
 #pragma unroll
@@ -1212,19 +1236,26 @@ __device__ __forceinline__ M<R, N> shfl_down_m(const M<R, N>& a, int d)
 // Rm = U_d(x+Tt) .. U_d(x+Tt+(R-1)d), Tm = U_t(x+Rd) .. U_t(x+Rd+(T-1)t),
 // Lm = U_d(x) .. U_d(x+(R-1)d).  Products in link precision, traces and
 // sums in FP64; one partial per workgroup.
-template <typename R, int N, int NREAL>
-__device__ __forceinline__ M<R, N> line_product(const Table<R, N, NREAL>& T, const Geom& g, Site s, int dir, int len)
+template <typename R, int N, int NREAL, bool TW>
+__device__ __forceinline__ M<R, N> line_product(const Table<R, N, NREAL>& T, const Geom& g, const Twist& tw, Site s,
+                                                int dir, int len)
 {
-    M<R, N> P = T.load((s.x + s.y + s.z + s.t) & 1, dir, cindex(g, s.x, s.y, s.z, s.t));
+    auto ld = [&](const Site& c) {
+        const uint32_t idx = cindex(g, c.x, c.y, c.z, c.t);
+        M<R, N> m = T.load((c.x + c.y + c.z + c.t) & 1, dir, idx);
+        if constexpr (TW) tbc(m, tw, dir, idx);
+        return m;
+    };
+    M<R, N> P = ld(s);
     for (int k = 1; k < len; ++k) {
         shift(g, s, dir, +1);
-        P = m_mul(P, T.load((s.x + s.y + s.z + s.t) & 1, dir, cindex(g, s.x, s.y, s.z, s.t)));
+        P = m_mul(P, ld(s));
     }
     return P;
 }
 
-template <typename R, int N, int NREAL>
-__global__ __launch_bounds__(kBlock) void k_wilson(void* table, Geom g, int wr, int wt, double* partials)
+template <typename R, int N, int NREAL, bool TW = false>
+__global__ __launch_bounds__(kBlock) void k_wilson(void* table, Geom g, int wr, int wt, double* partials, Twist tw)
 {
     const uint32_t site = blockIdx.x * kBlock + threadIdx.x;
     double acc = 0.0;
@@ -1238,15 +1269,15 @@ __global__ __launch_bounds__(kBlock) void k_wilson(void* table, Geom g, int wr, 
         r /= (uint32_t)g.n3;
         x.t = (int)(r % (uint32_t)g.n4);
         x.x = (int)(r / (uint32_t)g.n4);
-        const M<R, N> B = line_product(T, g, x, 3, wt);
+        const M<R, N> B = line_product<R, N, NREAL, TW>(T, g, tw, x, 3, wt);
         Site xt = x;
         for (int k = 0; k < wt; ++k) shift(g, xt, 3, +1);
         for (int d = 0; d < 3; ++d) {
             Site xr = x;
             for (int k = 0; k < wr; ++k) shift(g, xr, d, +1);
-            const M<R, N> Rm = line_product(T, g, xt, d, wr);
-            const M<R, N> Tm = line_product(T, g, xr, 3, wt);
-            const M<R, N> Lm = line_product(T, g, x, d, wr);
+            const M<R, N> Rm = line_product<R, N, NREAL, TW>(T, g, tw, xt, d, wr);
+            const M<R, N> Tm = line_product<R, N, NREAL, TW>(T, g, tw, xr, 3, wt);
+            const M<R, N> Lm = line_product<R, N, NREAL, TW>(T, g, tw, x, d, wr);
             acc += m_retr_nd(m_mul_nd(m_mul(B, Rm), Tm), Lm);
         }
     }
@@ -1473,6 +1504,8 @@ struct qcd_lattice {
     double* f_rec = nullptr;            // F_munu records: [rec][generator][plane][re, im]
     int f_gen[2] = {3, 8};              // generators (SU(2): 3 and 2)
     int f_efield = 0;                   // F0MU: planes (x,t), (y,t), (z,t)
+    double tw_phi = 0.0, tw_omega = 0.0;   // PHI, OMEGA of the twisted boundary
+    Twist tw;                              // what the kernels apply, derived from them
     uint32_t wil_blocks = 0;
     int wil_R = 7, wil_T = 2;           // wilson_R, wilson_T defaults of QCDGPU.cpp
     uint32_t upd_blocks = 0, poly_blocks = 0;
@@ -1542,6 +1575,10 @@ template <typename R, int N, int NREAL, bool OVER, int RK>
 void launch_update_t(qcd_lattice* L, const UpdArgs& a, const LaunchOpt& o)
 {
     const uint32_t nb = align_up(L->g.vh, kLinks) / kLinks;   // 64-link units
+    if (a.tw.on) {   // twisted boundary: three-wavefront shape, plaquette never fused
+        launch(k_update<R, N, NREAL, OVER, RK, false, true>, dim3(nb), dim3(kUpdThreads), L->stream, o, a);
+        return;
+    }
     if (L->tune_update_shape == 1) {
         const uint32_t grid = align_up(nb, kWaves1) / kWaves1;
         if (o.meas_mode)
@@ -1576,6 +1613,7 @@ int launch_update_rn(qcd_lattice* L, int parity, int mu, int kind, int start_gid
     a.meas_mode = o.meas_mode;
     a.xcd_map = L->tune_xcd_map;
     a.ord_ng = a.ord_nt = a.ord_xs = 1;
+    a.tw = L->tw;
     if (a.xcd_map == 2) {
         // Cross-section order needs whole y-rows per block, whole z-groups
         // and whole x-slabs per XCD; otherwise fall back to mode 1.
@@ -1593,7 +1631,7 @@ int launch_update_rn(qcd_lattice* L, int parity, int mu, int kind, int start_gid
         }
         if (!ok) a.xcd_map = 1;
     }
-    if (L->tune_update_shape == 1 && a.xcd_map != 0) {
+    if (L->tune_update_shape == 1 && !a.tw.on && a.xcd_map != 0) {
         // four 64-link units per workgroup: the XCD slabs need whole workgroups
         const uint32_t nb = align_up(L->g.vh, kLinks) / kLinks;
         if (nb % (8u * kWaves1)) a.xcd_map = 0;
@@ -1653,8 +1691,12 @@ int launch_simple(qcd_lattice* L, int which, const LaunchOpt& o)
             hipLaunchKernelGGL((k_reunitarize<R, N, NREAL>), dim3(nb), dim3(kBlock), 0, L->stream, L->table, L->g);
             break;
         case 3:
-            launch(k_plaquette<R, N, NREAL>, dim3(L->upd_blocks, 2), dim3(kBlock), L->stream, o, L->table, L->g,
-                   L->part_plq);
+            if (L->tw.on)
+                launch(k_plaquette<R, N, NREAL, true>, dim3(L->upd_blocks, 2), dim3(kBlock), L->stream, o, L->table,
+                       L->g, L->part_plq, L->tw);
+            else
+                launch(k_plaquette<R, N, NREAL>, dim3(L->upd_blocks, 2), dim3(kBlock), L->stream, o, L->table, L->g,
+                       L->part_plq, L->tw);
             break;
         case 4:
             launch(k_polyakov<R, N, NREAL>, dim3(L->poly_blocks), dim3(kBlock), L->stream, o, L->table, L->g,
@@ -1665,12 +1707,20 @@ int launch_simple(qcd_lattice* L, int which, const LaunchOpt& o)
                                    (const int*)L->rec_index);
             break;
         case 6:
-            hipLaunchKernelGGL((k_wilson<R, N, NREAL>), dim3(L->wil_blocks), dim3(kBlock), 0, L->stream, L->table,
-                               L->g, L->wil_R, L->wil_T, L->part_wil);
+            if (L->tw.on)
+                hipLaunchKernelGGL((k_wilson<R, N, NREAL, true>), dim3(L->wil_blocks), dim3(kBlock), 0, L->stream,
+                                   L->table, L->g, L->wil_R, L->wil_T, L->part_wil, L->tw);
+            else
+                hipLaunchKernelGGL((k_wilson<R, N, NREAL>), dim3(L->wil_blocks), dim3(kBlock), 0, L->stream, L->table,
+                                   L->g, L->wil_R, L->wil_T, L->part_wil, L->tw);
             break;
         case 7:
-            hipLaunchKernelGGL((k_fmunu<R, N, NREAL>), dim3(L->upd_blocks, 2), dim3(kBlock), 0, L->stream, L->table,
-                               L->g, L->part_f, L->f_efield, L->f_gen[0], L->f_gen[1]);
+            if (L->tw.on)
+                hipLaunchKernelGGL((k_fmunu<R, N, NREAL, true>), dim3(L->upd_blocks, 2), dim3(kBlock), 0, L->stream,
+                                   L->table, L->g, L->part_f, L->f_efield, L->f_gen[0], L->f_gen[1], L->tw);
+            else
+                hipLaunchKernelGGL((k_fmunu<R, N, NREAL>), dim3(L->upd_blocks, 2), dim3(kBlock), 0, L->stream,
+                                   L->table, L->g, L->part_f, L->f_efield, L->f_gen[0], L->f_gen[1], L->tw);
             break;
         case 5:
             if (L->p.prng == QCD_PRNG_XOR128) {
@@ -2278,6 +2328,35 @@ int qcd_lattice_set_fmunu(qcd_lattice* L, int gen1, int gen2)
     return QCD_OK;
 }
 
+int qcd_lattice_set_twist(qcd_lattice* L, double phi, double omega)
+{
+    if (!L) return fail(QCD_E_ARG, "null lattice");
+    if (!std::isfinite(phi) || !std::isfinite(omega)) return fail(QCD_E_ARG, "PHI and OMEGA must be finite");
+    L->tw_phi = phi;
+    L->tw_omega = omega;
+    Twist& t = L->tw;
+    // SU(3): diag(e^{i(phi+omega)/2}, e^{i(omega-phi)/2}, e^{-i omega}); SU(2):
+    // diag(e^{i phi/2}, e^{-i phi/2}), OMEGA unused (su2_matrix_memory.cl:56-88)
+    const double ang[3] = {L->p.sun == 3 ? 0.5 * (phi + omega) : 0.5 * phi,
+                           L->p.sun == 3 ? 0.5 * (omega - phi) : -0.5 * phi, L->p.sun == 3 ? -omega : 0.0};
+    for (int r = 0; r < 3; ++r) {
+        t.d[2 * r] = cos(ang[r]);
+        t.d[2 * r + 1] = sin(ang[r]);
+    }
+    // the reference builds with TBC when PHI or OMEGA is non-zero (suncl.cpp:2415)
+    t.on = (L->p.sun == 3 ? (phi != 0.0 || omega != 0.0) : phi != 0.0) ? 1 : 0;
+    t.from = t.on ? L->g.vh - L->g.vh / (uint32_t)L->g.n1 : 0xffffffffu;   // first site with x = N1 - 1
+    return QCD_OK;
+}
+
+int qcd_lattice_get_twist(qcd_lattice* L, double* phi, double* omega)
+{
+    if (!L) return fail(QCD_E_ARG, "null lattice");
+    if (phi) *phi = L->tw_phi;
+    if (omega) *omega = L->tw_omega;
+    return QCD_OK;
+}
+
 int qcd_read_fmunu(qcd_lattice* L, double* out, int* count)
 {
     if (!L) return fail(QCD_E_ARG, "null lattice");
@@ -2333,14 +2412,21 @@ int qcd_reset_measurements(qcd_lattice* L)
     return QCD_OK;
 }
 
+// Whether a cycle's plaquette rides on its last sweep.  F_munu needs the
+// standalone plaquette kernel, and so does a twisted boundary (the fused sum
+// takes the updated link as stored, the twisted Y links are read scaled).
+static bool cycle_fuses(const qcd_lattice* L, int mask, int total)
+{
+    return (mask & QCD_MEAS_PLAQUETTE) && total > 0 && !(mask & (QCD_MEAS_FMUNU | QCD_MEAS_F0MU)) && !L->tw.on;
+}
+
 // A measurement cycle: n_hb heatbath sweeps, n_or overrelaxation sweeps,
 // then the measurements.  The plaquette rides on the last sweep's even
 // Y/Z/T half-steps when there is a sweep; the Polyakov loop is its own kernel.
 static int enqueue_cycle(qcd_lattice* L, int n_hb, int n_or, int mask, std::vector<hipEvent_t>* ev, int* ev_used)
 {
     const int total = n_hb + n_or;
-    // F_munu needs the standalone plaquette kernel
-    const bool fuse = (mask & QCD_MEAS_PLAQUETTE) && total > 0 && !(mask & (QCD_MEAS_FMUNU | QCD_MEAS_F0MU));
+    const bool fuse = cycle_fuses(L, mask, total);
     for (int k = 0; k < total; ++k) {
         int rc = sweep_impl(L, k < n_hb ? QCD_HEATBATH : QCD_OVERRELAX, fuse && k == total - 1, ev, ev_used);
         if (rc) return rc;
@@ -2459,7 +2545,7 @@ int qcd_cycle_profile(qcd_lattice* L, int repeats, double* hb_us, double* or_us,
         HIPCHK(hipEventRecord(t1, L->stream));
         HIPCHK(hipEventSynchronize(t1));
         const int total = n_hb + n_or;
-        const bool fuse = (mask & QCD_MEAS_PLAQUETTE) && total > 0 && !(mask & (QCD_MEAS_FMUNU | QCD_MEAS_F0MU));
+        const bool fuse = cycle_fuses(L, mask, total);
         for (int j = 0; j < used; ++j) {
             float f = 0.f;
             HIPCHK(hipEventElapsedTime(&f, ev[2 * j], ev[2 * j + 1]));

@@ -15,6 +15,7 @@
 
 #include <algorithm>
 #include <cerrno>
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -200,8 +201,11 @@ int header_to_params(const Header& h, qcd_params* p, qcd_run_info* info)
     }
     uint64_t bb = (uint64_t)w[19] | ((uint64_t)w[20] << 32);
     memcpy(&q.beta, &bb, 8);
-    if (w[21] || w[22] || w[23] || w[24])
-        return err(QCD_E_UNSUPPORTED, "state file with PHI / OMEGA (twisted boundaries)");
+    double phi, omega;
+    const uint64_t pb = (uint64_t)w[21] | ((uint64_t)w[22] << 32), ob = (uint64_t)w[23] | ((uint64_t)w[24] << 32);
+    memcpy(&phi, &pb, 8);
+    memcpy(&omega, &ob, 8);
+    if (!std::isfinite(phi) || !std::isfinite(omega)) return err(QCD_E_ARG, "state file PHI / OMEGA not finite");
     if (w[34] != w[33] && (uint64_t)w[34] != 7ull * w[33])
         return err(QCD_E_ARG, "state file section sizes: energies_size_F is neither 1 nor 7 x energies_size");
     const bool fsec = w[34] != w[33];
@@ -232,6 +236,8 @@ int header_to_params(const Header& h, qcd_params* p, qcd_run_info* info)
         info->fmunu_index2 = w[27];
         info->get_fmunu = w[28] ? 1u : 0u;
         info->get_f0mu = (fsec && !w[28]) ? 1u : 0u;
+        info->phi = phi;
+        info->omega = omega;
     }
     return QCD_OK;
 }
@@ -249,10 +255,12 @@ Layout header_layout(const Header& h, const qcd_params& q)
     return l;
 }
 
-// Generators of a file with F_munu sections (header words 26-27).
-int apply_fmunu(qcd_lattice* lat, const qcd_run_info& ri)
+// Settings the header carries: the F_munu generators of a file with F
+// sections (words 26-27) and the twist (words 21-24).
+int apply_settings(qcd_lattice* lat, const qcd_run_info& ri)
 {
-    if (!ri.get_fmunu && !ri.get_f0mu) return QCD_OK;
+    int rc = qcd_lattice_set_twist(lat, ri.phi, ri.omega);
+    if (rc || (!ri.get_fmunu && !ri.get_f0mu)) return rc;
     return qcd_lattice_set_fmunu(lat, (int)ri.fmunu_index1, (int)ri.fmunu_index2);
 }
 
@@ -357,7 +365,14 @@ int qcd_qcg_write(qcd_lattice* lat, const char* path, const qcd_run_info* info)
     memcpy(&bb, &p.beta, 8);
     h.w[19] = (uint32_t)bb;                           // convert_to_uint_LOW / HIGH
     h.w[20] = (uint32_t)(bb >> 32);
-    // 21-24: PHI, OMEGA = 0
+    double phi = 0.0, omega = 0.0;
+    if ((rc = qcd_lattice_get_twist(lat, &phi, &omega))) return rc;
+    memcpy(&bb, &phi, 8);
+    h.w[21] = (uint32_t)bb;
+    h.w[22] = (uint32_t)(bb >> 32);
+    memcpy(&bb, &omega, 8);
+    h.w[23] = (uint32_t)bb;
+    h.w[24] = (uint32_t)(bb >> 32);
     h.w[25] = info->pl_level;
     h.w[26] = info->fmunu_index1;
     h.w[27] = info->fmunu_index2;
@@ -524,7 +539,7 @@ int qcd_qcg_read(qcd_lattice* lat, const char* path, qcd_run_info* info)
         if (!get(fh.f, img.data(), img.size())) return err(QCD_E_ARG, "short state image");
         if (image_params(img.data(), img.size(), &e) && same_params(e, lp)) {
             if ((rc = qcd_lattice_state_load(lat, img.data(), img.size()))) return rc;
-            if ((rc = apply_fmunu(lat, ri))) return rc;
+            if ((rc = apply_settings(lat, ri))) return rc;
             if (info) *info = ri;
             return QCD_OK;
         }
@@ -590,7 +605,7 @@ int qcd_qcg_read(qcd_lattice* lat, const char* path, qcd_run_info* info)
                 for (int c = 0; c < 2; ++c) fr[12 * i + 2 * b + c] = eplq[2 * ((1 + b) * lay.energies_size + i) + c];
         if ((rc = qcdhip::write_fmunu(lat, fr.data(), np))) return rc;
     }
-    if ((rc = apply_fmunu(lat, ri))) return rc;
+    if ((rc = apply_settings(lat, ri))) return rc;
     if (ri.wilson_r >= 1 && ri.wilson_t >= 1 &&
         (rc = qcd_lattice_set_wilson(lat, (int)ri.wilson_r, (int)ri.wilson_t)))
         return rc;

@@ -33,8 +33,8 @@
 // default, as in the reference's GPU build, suncl.cpp:761) and F0MU measure
 // the field strength for the H or E field, TURNOFFFMUNU neither; FMUNU1/2/4
 // and FMUNU5/6/7 choose the generators (suncl.cpp:412-453, 2093-2113).  Keys
-// of features outside this path (twisted boundary conditions, BIGLAT) are
-// rejected when they ask for the feature.
+// of features outside this path (BIGLAT) are rejected when they ask for the
+// feature.  PHI / OMEGA set the twisted boundary (the reference's TBC).
 
 #include <algorithm>
 #include <cerrno>
@@ -211,7 +211,7 @@ bool to_params(const Settings& st, qcd_params& p, std::string& why)
 {
     if (st.nd != 4) return why = "ND must be 4", false;
     if (st.group != 2 && st.group != 3) return why = "GROUP must be 2 or 3", false;
-    if (st.phi != 0.0 || st.omega != 0.0) return why = "PHI/OMEGA (twisted boundary conditions) are not supported", false;
+    if (!std::isfinite(st.phi) || !std::isfinite(st.omega)) return why = "PHI and OMEGA must be finite", false;
     if (st.wilson && (st.wilson_r < 1 || st.wilson_t < 1)) return why = "WILSONR and WILSONT must be >= 1", false;
     if (st.iter < 1 || st.niter < 0 || st.nav < 0 || st.nor < 0) return why = "ITER >= 1, NITER, NAV, NOR >= 0", false;
     memset(&p, 0, sizeof(p));
@@ -248,7 +248,8 @@ void from_state(Settings& st, const qcd_params& p, const qcd_run_info& r)
     st.double_precision = (p.precision == QCD_FP64);
     st.storage = p.storage_reals;
     st.beta = p.beta;
-    st.phi = st.omega = 0.0;
+    st.phi = r.phi;
+    st.omega = r.omega;
     st.fmunu = r.get_fmunu ? 1 : (r.get_f0mu ? 2 : 0);
     st.fgen1 = (p.sun == 3 && r.fmunu_index1 != 3) || (p.sun == 2 && r.fmunu_index1 == 1) ? (int)r.fmunu_index1 : 0;
     st.fgen2 = (p.sun == 3 && r.fmunu_index2 != 8) ? (int)r.fmunu_index2 : 0;
@@ -386,6 +387,7 @@ int main(int argc, char** argv)
     int fgen[2];
     fmunu_indices(st, fgen[0], fgen[1]);
     if (st.fmunu) CHECK(qcd_lattice_set_fmunu(lat, fgen[0], fgen[1]));
+    CHECK(qcd_lattice_set_twist(lat, st.phi, st.omega));   // twisted boundary when either is non-zero
     auto sweep = [&]() {
         CHECK(qcd_sweep(lat, QCD_HEATBATH));
         for (int k = 0; k < st.nor; ++k) CHECK(qcd_sweep(lat, QCD_OVERRELAX));
@@ -596,6 +598,9 @@ int main(int argc, char** argv)
     fprintf(out, " start                       : %s\n", st.ints == 1 ? "cold" : (st.ints == 2 ? "gid" : "hot (Haar)"));
     fprintf(out, " NAV %d, ITER %d, NITER %d, NHIT %d, NOR %d\n", st.nav, st.iter, st.niter, st.nhit, st.nor);
     if (st.fmunu) fprintf(out, " %s(%d, %d)\n", st.fmunu == 1 ? "FMUNU" : "F0MU", fgen[0], fgen[1]);
+    // model_make_header (suncl.cpp:789-791)
+    fprintf(out, " PHI   (lambda_3)            : %16.13e\n", st.phi);
+    fprintf(out, " OMEGA (lambda_8)            : %16.13e\n", st.omega);
     fprintf(out, " started %s finished %s", timestart, timeend);
     fprintf(out, " run time %.6f s\n", secs);
     fprintf(out, " ***************************************************\n");

@@ -199,6 +199,17 @@ class Lattice:
             z = z / (n2 * n3 * self.sun)
         return {"x": z[:, :n1], "y": z[:, n1:n1 + n2], "z": z[:, n1 + n2:]}
 
+    def set_twist(self, phi, omega=0.0):
+        """Twisted boundary (PHI, OMEGA): Y links at x = N1 - 1 are read as
+        T U, T = diag(e^{i(phi+omega)/2}, e^{i(omega-phi)/2}, e^{-i omega})
+        (SU(2): diag(e^{i phi/2}, e^{-i phi/2})), su3_matrix_memory.cl:57-98."""
+        check(lib().qcd_lattice_set_twist(self._h, float(phi), float(omega)))
+
+    def twist(self):
+        phi, omega = ctypes.c_double(), ctypes.c_double()
+        check(lib().qcd_lattice_get_twist(self._h, ctypes.byref(phi), ctypes.byref(omega)))
+        return phi.value, omega.value
+
     def set_wilson(self, R, T):
         """Extents of the Wilson loop measured with measure(wilson=True):
         R along x, y, z and T along t (reference defaults 7 and 2)."""

@@ -333,3 +333,33 @@ def test_qcg_fmunu_sections(tmp_path, field):
                 lat.measure(fmunu=field)
             assert np.array_equal(b.fmunu(raw=True)[-1], a.fmunu(raw=True)[-1])
             assert np.array_equal(c.fmunu(raw=True)[-1], a.fmunu(raw=True)[-1])
+
+
+def test_qcg_twist(tmp_path):
+    """PHI / OMEGA travel in header words 21-24 (lattice_make_bin_header,
+    suncl.cpp:1967-1970) and are applied on load, by the exact image and by
+    the reference sections; the continued runs stay identical."""
+    _case("qcg twist")
+    cfg = dict(sun=3, prec="fp32", storage=12, prng="XOR128")
+    full, cut = tmp_path / "t.qcg", tmp_path / "c.qcg"
+    with _lat(**cfg) as a, _lat(**cfg) as b:
+        a.set_twist(3.0, 0.5)
+        _start(a, "cold")
+        _cycles(a, 2)
+        a.save_qcg(full, iter=16)
+        w = np.frombuffer(full.read_bytes()[:256], dtype=np.uint32)
+        assert w[21:25].view(np.float64).tolist() == [3.0, 0.5]
+        assert Q.read_qcg_header(full)[1]["phi"] == 3.0
+        d = qcg.decode(full.read_bytes())
+        cut.write_bytes(full.read_bytes()[: d["table_end"]])
+        b.load_qcg(cut)
+        c, _ = Q.Lattice.from_qcg(full)
+        with c:
+            assert b.twist() == (3.0, 0.5) == c.twist()
+            for lat in (a, b, c):
+                _cycles(lat, 2)
+            assert np.array_equal(a.export_links(), b.export_links())
+            assert np.array_equal(a.export_links(), c.export_links())
+            pa, _ = a.raw_measurements()
+            assert np.array_equal(pa, c.raw_measurements()[0])
+            assert np.array_equal(pa[-2:], b.raw_measurements()[0][-2:])

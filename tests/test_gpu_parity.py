@@ -123,10 +123,11 @@ def test_ranlux_produce_bit_exact(O, lux):
         assert (tb.reshape(-1).view(np.uint32) == ta.reshape(-1).view(np.uint32)).all()
 
 
-def _oracle_reference_stream_sweeps(O, L, links, prng, seed, nsweeps):
+def _oracle_reference_stream_sweeps(O, L, links, prng, seed, nsweeps, twist=None):
     """The reference's GPU path restated: seed table for PRNG_instances =
     PRNGSTEP, one produce() at initialisation, then one produce() of S quads
-    before every half-step; thread i reads quads[i + k*PRNGSTEP]."""
+    before every half-step; thread i reads quads[i + k*PRNGSTEP].  twist:
+    (phi, omega) of the twisted boundary."""
     V = int(np.prod(list(L.n)))
     prngstep = -(-(V // 2) // 64) * 64
     total = -(-(3 * prngstep * (L.nhit + 1)) // 64) * 64
@@ -147,7 +148,10 @@ def _oracle_reference_stream_sweeps(O, L, links, prng, seed, nsweeps):
         for parity in (1, 0):
             for mu in range(4):
                 q = produce(table, S)
-                rejected += O.halfstep(L, links, parity, mu, O.HB, O.QuadRng(q, prngstep))
+                if twist:
+                    rejected += O.halfstep_twisted(L, links, parity, mu, O.HB, O.QuadRng(q, prngstep), *twist)
+                else:
+                    rejected += O.halfstep(L, links, parity, mu, O.HB, O.QuadRng(q, prngstep))
     return rejected
 
 
@@ -588,3 +592,61 @@ def test_fmunu_vs_oracle(O, sun, prec, storage):
             lat2.set_fmunu(5, 3)         # gen1 out of range
         with pytest.raises(Q.QCDError, match="not both"):   # H and E in one measurement
             check(lib().qcd_measure(lat2._h, _abi.MEAS_FMUNU | _abi.MEAS_F0MU))
+
+
+TBC_CASES = [
+    (3, (6, 4, 2, 4), 5.9, "XOR128", 54, None, (3.0, 0.5)),
+    (3, (4, 4, 4, 4), 6.0, "PM", 7, 12, (0.9, -1.7)),
+    (2, (6, 4, 2, 4), 2.3, "XOR128", 1, None, (3.0, 0.0)),
+    (3, (6, 4, 2, 4), 5.9, "RANLUX3", 5, None, (0.0, 2.2)),
+]
+
+
+@pytest.mark.parametrize("case", TBC_CASES, ids=["su%d-%s-%s" % (c[0], c[3], c[5] or "full") for c in TBC_CASES])
+def test_twisted_sweeps_and_measurements_vs_oracle(O, case):
+    """Twisted boundary (PHI, OMEGA): two heatbath sweeps on the reference
+    streams and one overrelaxation sweep against the oracle's twisted
+    half-steps (FP64), then the plaquette, Wilson loop, F_munu (H and E) and
+    Polyakov loop against the oracle on the twisted view of the same links;
+    the cycle's plaquette (not fused while twisted) equals the standalone
+    measurement."""
+    sun, dims, beta, prng, seed, storage, (phi, omega) = case
+    _case("twisted %s" % (case,))
+    L = O.lattice(sun, dims, beta, 10)
+    ref = O.cold(L)
+    _oracle_reference_stream_sweeps(O, L, ref, prng, seed, 2, twist=(phi, omega))
+    for parity in (1, 0):
+        for mu in range(4):
+            O.halfstep_twisted(L, ref, parity, mu, O.OR, O.GidRng(), phi, omega)
+    gens = (3, 8) if sun == 3 else (3, 2)
+    with _lat(sun, dims, "fp64", storage=storage, beta=beta, prng=prng, seed=seed) as lat:
+        lat.set_twist(phi, omega)
+        assert lat.twist() == (phi, omega)
+        lat.init("cold")
+        lat.prng_init()
+        lat.sweep("hb")
+        lat.sweep("hb")
+        lat.sweep("or")
+        got = lat.export_links()
+        lat.set_wilson(2, 1)
+        lat.reset_measurements()
+        lat.measure(wilson=True, fmunu="H")
+        lat.measure(fmunu="E")
+        plq, poly = lat.raw_measurements()
+        wil = lat.wilson_measurements(raw=True)
+        F = lat.fmunu(raw=True)
+        lat.run_cycle(n_hb=1, n_or=0)   # cycle: plaquette from the standalone kernel while twisted
+        lat.measure()
+        plq2, _ = lat.raw_measurements()
+    err = np.abs(got - ref).max()
+    print("  max|dU| = %.3e" % err)
+    assert err <= 1e-12
+    view = O.twisted_view(L, got, phi, omega)
+    V = int(np.prod(dims))
+    assert np.allclose(plq[0], O.plaquette(L, view), rtol=1e-12, atol=0)
+    assert not np.allclose(plq[0], O.plaquette(L, got), rtol=1e-9, atol=0)   # the twist matters
+    assert abs(wil[0] - O.wilson(L, view, 2, 1)) < 1e-12 * V
+    assert np.max(np.abs(F[0] - O.fmunu(L, view, False, *gens))) < 1e-12 * V
+    assert np.max(np.abs(F[1] - O.fmunu(L, view, True, *gens))) < 1e-12 * V
+    assert np.allclose(poly[0], O.polyakov(L, got), rtol=1e-12, atol=1e-9)
+    assert np.array_equal(plq2[2], plq2[3])   # the cycle's record == standalone on the same links

@@ -52,7 +52,7 @@ def test_reference_init_syntax(tmp_path):
 
 
 @pytest.mark.parametrize(
-    "line", ["PHI = 3.0", "OMEGA = 0.5", "PRNG = RANMAR", "ND = 3", "GROUP = 4",
+    "line", ["PRNG = RANMAR", "ND = 3", "GROUP = 4", "PHI = nan",
              "GETWILSON = 1\nWILSONR = 0"]
 )
 def test_unsupported_features_rejected(tmp_path, line):
@@ -80,13 +80,21 @@ def test_fmunu_keys(tmp_path, text, want):
     assert (kv["FMUNU"], kv["FMUNU_INDEX1"], kv["FMUNU_INDEX2"]) == want
 
 
+def test_twist_keys(tmp_path):
+    """PHI / OMEGA (suncl.cpp:372-373) are read as numbers; the reference's
+    own init.dat sets PHI = 3."""
+    rc, kv = _params("PHI = 3.0000000000000e+00\nOMEGA = -0.25\n", tmp_path)
+    assert rc == 0 and float(kv["PHI"]) == 3.0 and float(kv["OMEGA"]) == -0.25
+
+
 def test_bad_precision_value(tmp_path):
     rc, _ = _params("PRECISION = HALF\n", tmp_path)
     assert rc == 1
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("prng,ints,fkeys", [("XOR128", 1, ""), ("RANLUX3", 0, "F0MU = 1\nFMUNU1 = 1\nFMUNU7 = 1\n")])
+@pytest.mark.parametrize("prng,ints,fkeys", [("XOR128", 1, ""), ("RANLUX3", 0, "F0MU = 1\nFMUNU1 = 1\nFMUNU7 = 1\n"),
+                                             ("PM", 0, "PHI = 3.0\nOMEGA = 0.5\n")])
 def test_host_run_matches_python_driver(tmp_path, prng, ints, fkeys):
     """qcdgpu_hip on a small lattice writes the same plaquette, Wilson loop,
     Polyakov and F_munu series as qcdgpu_amd.Lattice.simulate with the same
@@ -101,7 +109,8 @@ def test_host_run_matches_python_driver(tmp_path, prng, ints, fkeys):
         "OUTPUTPATH = %s/\n%s" % (ints, prng, tmp_path, fkeys)
     )
     efield = "F0MU" in fkeys
-    g1, g2 = (1, 7) if fkeys else (3, 8)
+    g1, g2 = (1, 7) if "FMUNU1" in fkeys else (3, 8)
+    twist = (3.0, 0.5) if "PHI" in fkeys else (0.0, 0.0)
     r = subprocess.run([BIN, str(f)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     out = [p for p in os.listdir(tmp_path) if p.startswith("su3-") and p.endswith(".txt")]
@@ -121,6 +130,7 @@ def test_host_run_matches_python_driver(tmp_path, prng, ints, fkeys):
     rows = np.array(rows)
     assert rows.shape == (4, 27)   # Plq x3, Wilson_loop, S x3, Polyakov x4, F_munu x16
     planes = ("xt", "yt", "zt") if efield else ("xy", "xz", "yz")
+    assert any(ln.startswith(" PHI") and float(ln.split(":")[1]) == twist[0] for ln in lines)
     assert fields[11] == "Fmunu_%s_%d_re" % (planes[0], g1) and fields[17] == "Fmunu_%s_%d_re" % (planes[0], g2)
     assert fields[26] == "Fmunu_abs_%d_im" % g2
     assert len(pl_rows) == 6        # PL_LEVEL 3 (the default): the differentiated Polyakov loop block
@@ -130,6 +140,7 @@ def test_host_run_matches_python_driver(tmp_path, prng, ints, fkeys):
                    randseries=11, max_measurements=8) as lat:
         lat.set_wilson(2, 2)
         lat.set_fmunu(g1, g2)
+        lat.set_twist(*twist)
         m = lat.simulate(nav=1, iters=4, niter=2, n_or=2, start={0: "hot", 1: "cold"}[ints], wilson=True,
                          slices=True, fmunu="E" if efield else "H")
         wl = lat.wilson_measurements()

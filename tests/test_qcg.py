@@ -85,7 +85,7 @@ def test_library_reads_reference_header(tmp_path):
     assert p.max_measurements == 10
     assert info == dict(init=1, start=0, prng_counter=45, nav=3, nav_counter=3, niter=2, iter=10, iter_counter=6,
                         wilson_r=3, wilson_t=1, pl_level=2, get_actions=1, get_plaquettes=1, get_wilson=1,
-                        fmunu_index1=3, fmunu_index2=8, get_fmunu=0, get_f0mu=0)
+                        fmunu_index1=3, fmunu_index2=8, get_fmunu=0, get_f0mu=0, phi=0.0, omega=0.0)
     # RANLUX luxury and PM codes, SU(2), double precision
     h2 = qcg.make_header(2, (2, 2, 2, 2), 2.3, "double", "RANLUX2", iter=3)
     p2, _, _ = read_qcg_header(_write(tmp_path, "b.qcg", h2, haar(rng, 16, 2)))
@@ -134,7 +134,7 @@ def test_fmunu_sections(tmp_path, field):
         (lambda w: w.__setitem__(2, qcg.string_word("1.4", 0)), "version"),
         (lambda w: w.__setitem__(qcg.WORD["get_fmunu"], 1), "get_Fmunu but no F_munu sections"),
         (lambda w: w.__setitem__(qcg.WORD["energies_size_F"], 3 * w[qcg.WORD["energies_size"]]), "energies_size_F"),
-        (lambda w: w.__setitem__(qcg.WORD["phi_hi"], 0x3FF00000), "PHI"),
+        (lambda w: w.__setitem__(qcg.WORD["phi_hi"], 0x7FF80000), "PHI / OMEGA not finite"),
         (lambda w: w.__setitem__(42, 6), "multi-device"),
         (lambda w: w.__setitem__(qcg.WORD["generator"], 8), "generator"),
         (lambda w: w.__setitem__(qcg.WORD["nd"], 3), "ND"),
