@@ -72,7 +72,8 @@ enum {
     QCD_MEAS_WILSON = 4,
     QCD_MEAS_POLYAKOV_SLICES = 8,
     QCD_MEAS_FMUNU = 16,   /* F_munu, H field: planes (x,y), (x,z), (y,z)  */
-    QCD_MEAS_F0MU = 32     /* F_munu, E field: planes (x,t), (y,t), (z,t)  */
+    QCD_MEAS_F0MU = 32,    /* F_munu, E field: planes (x,t), (y,t), (z,t)  */
+    QCD_MEAS_ACTION_SLICES = 64   /* plaquette sums per x / y / z slice      */
 };
 
 typedef struct {
@@ -185,6 +186,19 @@ int  qcd_read_wilson(qcd_lattice *lat, double *w, int *count);
         sums (sum Re Tr P, sum Im Tr P) over the slice's spatial sites.
         out[(k*(N1+N2+N3) + slice)*2 + c], slices in the order x, y, z.      */
 int  qcd_read_polyakov_slices(qcd_lattice *lat, double *out, int *count);
+/* ---- action per slice (get_actions_diff: lattice_action_diff_x/y/z +
+        reduce_action_diff_*_double2, sun_measurements_cl.cl:186-708; the
+        reference's QCDGPU.cpp:218 sets get_actions_diff = false and no
+        init.dat key turns it on).  A QCD_MEAS_ACTION_SLICES measurement
+        (it implies QCD_MEAS_PLAQUETTE, measured with the standalone kernel)
+        stores, beside that plaquette record, for every slice x = 0..N1-1,
+        y = 0..N2-1, z = 0..N3-1 the sums over the slice's sites of the
+        three spatial and of the three temporal ReTr U_p:
+        out[(rec*(N1+N2+N3) + slice)*2 + (0 spatial, 1 temporal)]; records
+        of plaquette measurements without it hold zeros.  The reference's
+        S per slice is beta/N (N - sum / (3 * sites in the slice)).  The
+        first such measurement allocates 16 bytes per site.                  */
+int  qcd_read_action_slices(qcd_lattice *lat, double *out, int *count);
 /* ---- field strength (get_Fmunu / get_F0mu: lattice_measurement_plq with
         FMUNU / F0MU, sun_measurements_cl.cl:710-1173, trace_lambda* and
         trace_sigma*, su3_measurements_cl.cl:357-460,

@@ -836,6 +836,19 @@ void orc_plaquette(const orc_lattice *L, const double *links, double plq[2])
     plq[1] = tm;
 }
 
+/* Per-site spatial and temporal plaquette sums (the per-site values the
+   action per slice adds up, lattice_action_diff_x/y/z,
+   sun_measurements_cl.cl:186-615): out[2*site + 0/1], site order of
+   orc_site.                                                                 */
+void orc_plaquette_sites(const orc_lattice *L, const double *links, double *out)
+{
+    unsigned V = orc_sites(L);
+    for (unsigned s = 0; s < V; ++s) {
+        out[2 * s] = plaq(L, links, s, 0, 1) + plaq(L, links, s, 0, 2) + plaq(L, links, s, 1, 2);
+        out[2 * s + 1] = plaq(L, links, s, 0, 3) + plaq(L, links, s, 1, 3) + plaq(L, links, s, 2, 3);
+    }
+}
+
 /* F_munu (lattice_measurement_plq with FMUNU / F0MU,
    sun_measurements_cl.cl:710-1173): for each site and plane the plaquette
    P = U_a(x) U_b(x+a) U_a(x+b)^+ U_b(x)^+ and F = -i Tr(P G) / 2 for the

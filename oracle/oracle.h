@@ -117,6 +117,8 @@ void     orc_reunitarize(const orc_lattice *L, double *links);
 /* Measurements (FP64): sums over sites, not normalised.
    plq[0] = sum of ReTr over the 3 spatial planes, plq[1] over the 3 temporal. */
 void     orc_plaquette(const orc_lattice *L, const double *links, double plq[2]);
+/* Per-site (spatial, temporal) plaquette sums, out[2*site + c].            */
+void     orc_plaquette_sites(const orc_lattice *L, const double *links, double *out);
 /* poly[0..3] = sum over spatial sites of ReTr P, ImTr P, |Tr P|^2, |Tr P|^4. */
 /* Sum over sites and spatial directions of ReTr of the R x T Wilson loop
    (spatial extent R along x, y, z; temporal extent T along t).             */

@@ -248,6 +248,21 @@ def fmunu(L, links, efield=False, gen1=3, gen2=8):
     return (out[0::2] + 1j * out[1::2]).reshape(2, 3)
 
 
+def plaquette_sites(L, links):
+    """(V, 2): per-site sums of the spatial / temporal ReTr U_p."""
+    V = int(np.prod(list(L.n)))
+    out = np.zeros((V, 2))
+    lib().orc_plaquette_sites(ctypes.byref(L), _p(np.ascontiguousarray(links), _F64P), _p(out, _F64P))
+    return out
+
+
+def action_slices(L, links):
+    """Per x / y / z slice sums of plaquette_sites: dict -> (N_d, 2)."""
+    n1, n2, n3, n4 = list(L.n)
+    a = plaquette_sites(L, links).reshape(n1, n4, n3, n2, 2)   # site order: x slowest, then t, z, y
+    return {"x": a.sum(axis=(1, 2, 3)), "y": a.sum(axis=(0, 1, 2)), "z": a.sum(axis=(0, 1, 3))}
+
+
 def polyakov(L, links):
     out = np.zeros(4)
     lib().orc_polyakov(ctypes.byref(L), _p(links, _F64P), _p(out, _F64P))

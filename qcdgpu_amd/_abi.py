@@ -19,7 +19,7 @@ PRNG_XOR128, PRNG_RANLUX, PRNG_PM = 1, 2, 3
 STREAM_REFERENCE, STREAM_FAST = 0, 1
 HEATBATH, OVERRELAX = 0, 1
 MEAS_PLAQUETTE, MEAS_POLYAKOV, MEAS_WILSON, MEAS_POLYAKOV_SLICES = 1, 2, 4, 8
-MEAS_FMUNU, MEAS_F0MU = 16, 32
+MEAS_FMUNU, MEAS_F0MU, MEAS_ACTION_SLICES = 16, 32, 64
 TUNE_XCD_MAP, TUNE_BLOCKS_PER_CU, TUNE_UPDATE_SHAPE = 1, 2, 3
 
 
@@ -101,6 +101,7 @@ _SIGNATURES = {
     "qcd_lattice_set_twist": [_P, ctypes.c_double, ctypes.c_double],
     "qcd_lattice_get_twist": [_P, _D, _D],
     "qcd_read_fmunu": [_P, _D, _I],
+    "qcd_read_action_slices": [_P, _D, _I],
     "qcd_cycle_capture": [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int],
     "qcd_cycle_run": [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int],
     "qcd_cycle_launch": [_P, ctypes.c_int],

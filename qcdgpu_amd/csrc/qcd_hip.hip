@@ -997,7 +997,10 @@ struct FinArgs {
     double* wil_rec;
     int* rec_index;   // [0] plaquette, [1] polyakov, [2] Wilson loop, [3] Polyakov slices
     int capacity;
-    int slices = 0;   // k_poly_slices wrote record rec_index[3]: advance it
+    int slices = 0;   // k_slices wrote record rec_index[3]: advance it
+    double* a_rec = nullptr;   // action slice records beside the plaquette record
+    int a_n = 0;               // N1 + N2 + N3
+    int aslices = 0;           // k_slices wrote this plaquette record's action slices
     // F_munu (k_fmunu): n_plq slots of 12, recorded beside the plaquette
     const double* fmunu = nullptr;
     double* f_rec = nullptr;
@@ -1049,6 +1052,8 @@ __device__ __forceinline__ void finalize_body(const FinArgs& f)
                 f.plq_rec[2 * rec] = sp;
                 f.plq_rec[2 * rec + 1] = tm;
                 for (int k = 0; k < 12; ++k) f.f_rec[12 * rec + k] = f.fmunu ? fm[k] : 0.0;
+                if (f.a_rec && !f.aslices)   // a plaquette record without action slices
+                    for (int k = 0; k < 2 * f.a_n; ++k) f.a_rec[(size_t)2 * f.a_n * rec + k] = 0.0;
             }
             f.rec_index[0] = rec + 1;
         }
@@ -1087,7 +1092,7 @@ __global__ __launch_bounds__(kBlock) void k_finalize(FinArgs f) { finalize_body(
 // (y,z) and the temporal planes (x,t), (y,t), (z,t), products in FP64
 // (lattice_retrace_plaquette3, su3_measurements_cl.cl:527-549).
 template <typename R, int N, int NREAL, bool TW = false>
-__global__ __launch_bounds__(kBlock) void k_plaquette(void* table, Geom g, double* partials, Twist tw)
+__global__ __launch_bounds__(kBlock) void k_plaquette(void* table, Geom g, double* partials, Twist tw, double* site_pl)
 {
     const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     const int p = blockIdx.y, q = p ^ 1;
@@ -1115,6 +1120,12 @@ __global__ __launch_bounds__(kBlock) void k_plaquette(void* table, Geom g, doubl
                 if (b == 3) v[1] += tr;
                 else v[0] += tr;
             }
+        }
+        if (site_pl) {   // this site's spatial and temporal sums, site order y, z, t, x (x slowest)
+            const uint32_t j = (uint32_t)s.y + (uint32_t)g.n2 * ((uint32_t)s.z + (uint32_t)g.n3 *
+                                                                 ((uint32_t)s.t + (uint32_t)g.n4 * (uint32_t)s.x));
+            site_pl[2 * j] = v[0];
+            site_pl[2 * j + 1] = v[1];
         }
     }
     block_partial<2>(v, partials, blockIdx.y * gridDim.x + blockIdx.x);
@@ -1342,27 +1353,35 @@ __global__ __launch_bounds__(kBlock) void k_polyakov(void* table, Geom g, double
     polyakov_body<R, N, NREAL>(table, g, partials, blockIdx.x, site_tr);
 }
 
-// Polyakov loop per slice (lattice_polyakov_diff_x/y/z + reduce_polyakov_
-// diff_*_double2, polyakov.cl:142-560): workgroup b sums Tr P over the
-// spatial sites of slice x = b (b < N1), y = b - N1, or z = b - N1 - N2, in
-// a fixed order, into record rec_index[3] at slot b: (sum Re, sum Im).  The
-// finaliser advances rec_index[3].
-__global__ __launch_bounds__(kBlock) void k_poly_slices(const double* site_tr, Geom g, double* slice_rec, int cap,
-                                                        const int* rec_index)
+// Sums per x, y and z slice of a per-site pair of values, in a fixed order:
+// workgroup b takes slice x = b (b < N1), y = b - N1, or z = b - N1 - N2 and
+// writes record rec_index[slot] at position b.  Sites are indexed
+// y + N2 (z + N3 (t + nt x)); nt = 1 for the spatial sites of the Polyakov
+// loop (lattice_polyakov_diff_x/y/z + reduce_polyakov_diff_*_double2,
+// polyakov.cl:142-560: (sum Re Tr P, sum Im Tr P)), nt = N4 for the
+// plaquette sums of the action per slice (lattice_action_diff_x/y/z +
+// reduce_action_diff_*_double2, sun_measurements_cl.cl:186-708).  The
+// finaliser advances the record index.
+__global__ __launch_bounds__(kBlock) void k_slices(const double* site, Geom g, uint32_t nt, double* slice_rec, int cap,
+                                                   const int* rec_index, int slot)
 {
     __shared__ double red[2][kBlock];
     const uint32_t b = blockIdx.x, n1 = (uint32_t)g.n1, n2 = (uint32_t)g.n2, n3 = (uint32_t)g.n3;
     const int d = b < n1 ? 0 : (b < n1 + n2 ? 1 : 2);
     const uint32_t v = d == 0 ? b : (d == 1 ? b - n1 : b - n1 - n2);
-    const uint32_t count = n1 * n2 * n3 / (d == 0 ? n1 : (d == 1 ? n2 : n3));
+    const uint32_t count = n1 * n2 * n3 * nt / (d == 0 ? n1 : (d == 1 ? n2 : n3));
     double re = 0.0, im = 0.0;
     for (uint32_t k = threadIdx.x; k < count; k += kBlock) {
         uint32_t j;
-        if (d == 0) j = (k % n2) + n2 * ((k / n2) + n3 * v);        // x = v: y, z
-        else if (d == 1) j = v + n2 * ((k % n3) + n3 * (k / n3));   // y = v: z, x
-        else j = (k % n2) + n2 * (v + n3 * (k / n2));               // z = v: y, x
-        re += site_tr[2 * j];
-        im += site_tr[2 * j + 1];
+        if (d == 0) {          // x = v: y, z, t
+            j = (k % n2) + n2 * ((k / n2) % n3 + n3 * (k / (n2 * n3) + nt * v));
+        } else if (d == 1) {   // y = v: z, t, x
+            j = v + n2 * ((k % n3) + n3 * ((k / n3) % nt + nt * (k / (n3 * nt))));
+        } else {               // z = v: y, t, x
+            j = (k % n2) + n2 * (v + n3 * ((k / n2) % nt + nt * (k / (n2 * nt))));
+        }
+        re += site[2 * j];
+        im += site[2 * j + 1];
     }
     red[0][threadIdx.x] = re;
     red[1][threadIdx.x] = im;
@@ -1375,7 +1394,7 @@ __global__ __launch_bounds__(kBlock) void k_poly_slices(const double* site_tr, G
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        const int rec = rec_index[3];
+        const int rec = rec_index[slot];
         if (rec < cap) {
             double* o = slice_rec + 2 * ((size_t)rec * (n1 + n2 + n3) + b);
             o[0] = red[0][0];
@@ -1500,6 +1519,9 @@ struct qcd_lattice {
     double* site_tr = nullptr;          // Tr P per spatial site (2 doubles), for slice sums
     double* slice_rec = nullptr;        // Polyakov slice records: [rec][N1 + N2 + N3][2]
     bool want_slices = false;           // the Polyakov launch being enqueued feeds the slice sums
+    double* site_pl = nullptr;          // per-site plaquette sums (2 doubles), allocated on first use
+    double* a_rec = nullptr;            // action slice records: [rec][N1 + N2 + N3][spatial, temporal]
+    bool want_aslices = false;          // the plaquette launch being enqueued feeds the action slices
     double* part_f = nullptr;           // F_munu block partials: 2 x upd_blocks slots of 12
     double* f_rec = nullptr;            // F_munu records: [rec][generator][plane][re, im]
     int f_gen[2] = {3, 8};              // generators (SU(2): 3 and 2)
@@ -1691,20 +1713,27 @@ int launch_simple(qcd_lattice* L, int which, const LaunchOpt& o)
             hipLaunchKernelGGL((k_reunitarize<R, N, NREAL>), dim3(nb), dim3(kBlock), 0, L->stream, L->table, L->g);
             break;
         case 3:
+        {
+            double* sp = L->want_aslices ? L->site_pl : nullptr;
             if (L->tw.on)
                 launch(k_plaquette<R, N, NREAL, true>, dim3(L->upd_blocks, 2), dim3(kBlock), L->stream, o, L->table,
-                       L->g, L->part_plq, L->tw);
+                       L->g, L->part_plq, L->tw, sp);
             else
                 launch(k_plaquette<R, N, NREAL>, dim3(L->upd_blocks, 2), dim3(kBlock), L->stream, o, L->table, L->g,
-                       L->part_plq, L->tw);
+                       L->part_plq, L->tw, sp);
+            if (sp)
+                hipLaunchKernelGGL(k_slices, dim3((unsigned)(L->g.n1 + L->g.n2 + L->g.n3)), dim3(kBlock), 0,
+                                   L->stream, (const double*)sp, L->g, (uint32_t)L->g.n4, L->a_rec,
+                                   L->p.max_measurements, (const int*)L->rec_index, 0);
+        }
             break;
         case 4:
             launch(k_polyakov<R, N, NREAL>, dim3(L->poly_blocks), dim3(kBlock), L->stream, o, L->table, L->g,
                    L->part_poly, L->want_slices ? L->site_tr : (double*)nullptr);
             if (L->want_slices)
-                hipLaunchKernelGGL(k_poly_slices, dim3((unsigned)(L->g.n1 + L->g.n2 + L->g.n3)), dim3(kBlock), 0,
-                                   L->stream, (const double*)L->site_tr, L->g, L->slice_rec, L->p.max_measurements,
-                                   (const int*)L->rec_index);
+                hipLaunchKernelGGL(k_slices, dim3((unsigned)(L->g.n1 + L->g.n2 + L->g.n3)), dim3(kBlock), 0,
+                                   L->stream, (const double*)L->site_tr, L->g, 1u, L->slice_rec,
+                                   L->p.max_measurements, (const int*)L->rec_index, 3);
             break;
         case 6:
             if (L->tw.on)
@@ -1765,6 +1794,9 @@ int launch_finalize(qcd_lattice* L, int flags, const LaunchOpt& o)
     f.slices = (flags & 16) ? 1 : 0;
     f.fmunu = (flags & 32) ? L->part_f : nullptr;
     f.f_rec = L->f_rec;
+    f.a_rec = L->a_rec;
+    f.a_n = L->g.n1 + L->g.n2 + L->g.n3;
+    f.aslices = (flags & 64) ? 1 : 0;
     launch(k_finalize, dim3(1), dim3(kBlock), L->stream, o, f);
     HIPCHK(hipGetLastError());
     return QCD_OK;
@@ -1901,6 +1933,7 @@ int qcd_lattice_create(const qcd_params* pp, qcd_lattice** out)
         hipMalloc((void**)&L->wil_rec, sizeof(double) * p.max_measurements) != hipSuccess ||
         hipMalloc((void**)&L->site_tr, sizeof(double) * 2 * vs) != hipSuccess ||
         hipMalloc((void**)&L->slice_rec, sizeof(double) * slice_rec_doubles(L)) != hipSuccess ||
+        hipMalloc((void**)&L->a_rec, sizeof(double) * slice_rec_doubles(L)) != hipSuccess ||
         hipMalloc((void**)&L->part_f, sizeof(double) * 12 * 2 * L->upd_blocks) != hipSuccess ||
         hipMalloc((void**)&L->f_rec, sizeof(double) * 12 * p.max_measurements) != hipSuccess) {
         qcd_lattice_destroy(L);
@@ -1928,6 +1961,7 @@ int qcd_lattice_create(const qcd_params* pp, qcd_lattice** out)
         (void)hipMemset(L->poly_rec, 0, sizeof(double) * 4 * m);
         (void)hipMemset(L->wil_rec, 0, sizeof(double) * m);
         (void)hipMemset(L->slice_rec, 0, sizeof(double) * slice_rec_doubles(L));
+        (void)hipMemset(L->a_rec, 0, sizeof(double) * slice_rec_doubles(L));
         (void)hipMemset(L->f_rec, 0, sizeof(double) * 12 * m);
     }
     {
@@ -1994,6 +2028,8 @@ int qcd_lattice_destroy(qcd_lattice* L)
     (void)hipFree(L->wil_rec);
     (void)hipFree(L->site_tr);
     (void)hipFree(L->slice_rec);
+    (void)hipFree(L->a_rec);
+    (void)hipFree(L->site_pl);
     (void)hipFree(L->part_f);
     (void)hipFree(L->f_rec);
     if (L->stream) (void)hipStreamDestroy(L->stream);
@@ -2263,14 +2299,24 @@ static int measure_impl(qcd_lattice* L, int mask, bool fused_plq, const LaunchOp
     int flags = 0;
     const int fmask = mask & (QCD_MEAS_FMUNU | QCD_MEAS_F0MU);
     if (fmask == (QCD_MEAS_FMUNU | QCD_MEAS_F0MU)) return fail(QCD_E_ARG, "F_munu: H field or E field, not both");
-    if (fmask) {   // recorded beside the plaquette, from the standalone plaquette kernel
+    if (fmask || (mask & QCD_MEAS_ACTION_SLICES)) {   // recorded beside the plaquette, from the standalone kernel
         mask |= QCD_MEAS_PLAQUETTE;
         fused_plq = false;
     }
+    if ((mask & QCD_MEAS_ACTION_SLICES) && !L->site_pl) {
+        // per-site sums: allocated on first use (16 bytes per site)
+        if (hipMalloc((void**)&L->site_pl, sizeof(double) * 4 * (size_t)L->g.vh) != hipSuccess) {
+            L->site_pl = nullptr;
+            return fail(QCD_E_NOMEM, "action slices: per-site buffer");
+        }
+    }
     if (fused_plq) flags |= 1;
     if ((mask & QCD_MEAS_PLAQUETTE) && !fused_plq) {
+        L->want_aslices = (mask & QCD_MEAS_ACTION_SLICES) != 0;
         rc = DISPATCH(L, launch_simple, L, 3, o_plq);
+        L->want_aslices = false;
         flags |= 2;
+        if (mask & QCD_MEAS_ACTION_SLICES) flags |= 64;
     }
     if (rc) return rc;
     if (fmask) {
@@ -2369,6 +2415,19 @@ int qcd_read_fmunu(qcd_lattice* L, double* out, int* count)
     return QCD_OK;
 }
 
+int qcd_read_action_slices(qcd_lattice* L, double* out, int* count)
+{
+    if (!L) return fail(QCD_E_ARG, "null lattice");
+    int idx[4];
+    HIPCHK(hipMemcpyAsync(idx, L->rec_index, sizeof(idx), hipMemcpyDeviceToHost, L->stream));
+    HIPCHK(hipStreamSynchronize(L->stream));
+    const int n = idx[0] < L->p.max_measurements ? idx[0] : L->p.max_measurements;
+    const size_t per = 2 * (size_t)(L->g.n1 + L->g.n2 + L->g.n3);
+    if (out && n) HIPCHK(hipMemcpy(out, L->a_rec, sizeof(double) * per * n, hipMemcpyDeviceToHost));
+    if (count) *count = idx[0];
+    return QCD_OK;
+}
+
 int qcd_read_polyakov_slices(qcd_lattice* L, double* out, int* count)
 {
     if (!L) return fail(QCD_E_ARG, "null lattice");
@@ -2417,7 +2476,8 @@ int qcd_reset_measurements(qcd_lattice* L)
 // takes the updated link as stored, the twisted Y links are read scaled).
 static bool cycle_fuses(const qcd_lattice* L, int mask, int total)
 {
-    return (mask & QCD_MEAS_PLAQUETTE) && total > 0 && !(mask & (QCD_MEAS_FMUNU | QCD_MEAS_F0MU)) && !L->tw.on;
+    return (mask & QCD_MEAS_PLAQUETTE) && total > 0 &&
+           !(mask & (QCD_MEAS_FMUNU | QCD_MEAS_F0MU | QCD_MEAS_ACTION_SLICES)) && !L->tw.on;
 }
 
 // A measurement cycle: n_hb heatbath sweeps, n_or overrelaxation sweeps,
@@ -2663,7 +2723,7 @@ size_t rng_state_bytes(const qcd_lattice* L)
 size_t state_image_bytes(const qcd_lattice* L)
 {
     return kStateHead + L->table_bytes + rng_state_bytes(L) + sizeof(double) * 7 * (size_t)L->p.max_measurements +
-           sizeof(double) * slice_rec_doubles(L) + sizeof(double) * 12 * (size_t)L->p.max_measurements;
+           2 * sizeof(double) * slice_rec_doubles(L) + sizeof(double) * 12 * (size_t)L->p.max_measurements;
 }
 
 uint64_t f64_bits(double x)
@@ -2784,6 +2844,8 @@ int qcd_lattice_state_save(qcd_lattice* L, void* buf, size_t bytes)
     HIPCHK(hipMemcpy(w.p, L->slice_rec, sizeof(double) * slice_rec_doubles(L), hipMemcpyDeviceToHost));
     w.p += sizeof(double) * slice_rec_doubles(L);
     HIPCHK(hipMemcpy(w.p, L->f_rec, sizeof(double) * 12 * m, hipMemcpyDeviceToHost));
+    w.p += sizeof(double) * 12 * m;
+    HIPCHK(hipMemcpy(w.p, L->a_rec, sizeof(double) * slice_rec_doubles(L), hipMemcpyDeviceToHost));
     return QCD_OK;
 }
 
@@ -2824,6 +2886,8 @@ int qcd_lattice_state_load(qcd_lattice* L, const void* buf, size_t bytes)
     HIPCHK(hipMemcpy(L->slice_rec, r.p, sizeof(double) * slice_rec_doubles(L), hipMemcpyHostToDevice));
     r.p += sizeof(double) * slice_rec_doubles(L);
     HIPCHK(hipMemcpy(L->f_rec, r.p, sizeof(double) * 12 * m, hipMemcpyHostToDevice));
+    r.p += sizeof(double) * 12 * m;
+    HIPCHK(hipMemcpy(L->a_rec, r.p, sizeof(double) * slice_rec_doubles(L), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(L->rec_index, idx, sizeof(idx), hipMemcpyHostToDevice));
     L->prng_counter = counter;
     L->rng_ready = ready != 0;
@@ -2847,6 +2911,7 @@ int qcd_write_measurements(qcd_lattice* L, const double* plq, const double* poly
     if (nq) HIPCHK(hipMemcpy(L->poly_rec, poly, sizeof(double) * 4 * nq, hipMemcpyHostToDevice));
     if (nw) HIPCHK(hipMemcpy(L->wil_rec, wil, sizeof(double) * nw, hipMemcpyHostToDevice));
     HIPCHK(hipMemset(L->f_rec, 0, sizeof(double) * 12 * m));   // restored records carry no F_munu
+    HIPCHK(hipMemset(L->a_rec, 0, sizeof(double) * slice_rec_doubles(L)));   // nor action slices
     const int idx[4] = {count_plq, count_poly, count_wil, 0};
     HIPCHK(hipMemcpy(L->rec_index, idx, sizeof(idx), hipMemcpyHostToDevice));
     return QCD_OK;
@@ -2991,6 +3056,8 @@ FinArgs chain_fin_args(const qcd_lattice* L)
     f.rec_index = L->rec_index;
     f.capacity = L->p.max_measurements;
     f.f_rec = L->f_rec;
+    f.a_rec = L->a_rec;
+    f.a_n = L->g.n1 + L->g.n2 + L->g.n3;
     return f;
 }
 

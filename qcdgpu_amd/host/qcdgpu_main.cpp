@@ -82,6 +82,10 @@ struct Settings {
     // leaving what a process killed there leaves: the checkpoints written so
     // far and work that no checkpoint holds.  All device work completes.
     int stop_after = 0;
+    // ACTIONSDIFF = 1: the action per x / y / z slice and the results file's
+    // "Differentiated S data" block (get_actions_diff, suncl.cpp:1376-1480,
+    // 1753-1790; the reference hardcodes it off, QCDGPU.cpp:218).
+    bool actions_diff = false;
 };
 
 std::string trim(const std::string& s)
@@ -167,6 +171,7 @@ bool apply(Settings& st, const std::string& key, const std::string& val)
     else if (key == "SAVESTATE") st.save_state = iv != 0;
     else if (key == "SAVEEVERY") st.save_every = fv;
     else if (key == "STOPAFTER") st.stop_after = (int)iv;
+    else if (key == "ACTIONSDIFF") st.actions_diff = iv != 0;
     else if (key == "NOR") st.nor = (int)iv;
     else if (key == "STORAGE") st.storage = (int)iv;
     else if (key == "STREAMS") {
@@ -298,6 +303,7 @@ void print_params(const Settings& st)
            st.storage ? st.storage : (st.group == 3 ? 18 : 4), st.fast_streams ? "FAST" : "REFERENCE");
     printf("OUTPUTPATH=%s\nFINISHPATH=%s\nPLATFORM=%d\nDEVICE=%d\n", st.outputpath.c_str(), st.finishpath.c_str(),
            st.platform, st.device);
+    printf("ACTIONSDIFF=%d\n", st.actions_diff ? 1 : 0);
     printf("INIT=%d\nSTATEFILE=%s\nSAVESTATE=%d\nSAVEEVERY=%.3f\n", st.init, st.statefile.c_str(),
            st.save_state ? 1 : 0, st.save_every);
 }
@@ -382,7 +388,8 @@ int main(int argc, char** argv)
     const int start = st.ints == 1 ? QCD_START_COLD : (st.ints == 2 ? QCD_START_GID : QCD_START_HOT);
     const int mask = QCD_MEAS_PLAQUETTE | (st.pl_level > 0 ? QCD_MEAS_POLYAKOV : 0) | (st.wilson ? QCD_MEAS_WILSON : 0) |
                      (st.pl_level > 2 ? QCD_MEAS_POLYAKOV_SLICES : 0) |
-                     (st.fmunu == 1 ? QCD_MEAS_FMUNU : (st.fmunu == 2 ? QCD_MEAS_F0MU : 0));
+                     (st.fmunu == 1 ? QCD_MEAS_FMUNU : (st.fmunu == 2 ? QCD_MEAS_F0MU : 0)) |
+                     (st.actions_diff ? QCD_MEAS_ACTION_SLICES : 0);
     if (st.wilson) CHECK(qcd_lattice_set_wilson(lat, st.wilson_r, st.wilson_t));
     int fgen[2];
     fmunu_indices(st, fgen[0], fgen[1]);
@@ -479,6 +486,10 @@ int main(int argc, char** argv)
     int nf = 0;
     if (st.fmunu) CHECK(qcd_read_fmunu(lat, fr.data(), &nf));
     nf = std::min(nf, (int)cap);
+    std::vector<double> asl(2 * (size_t)nslice * cap);
+    int na = 0;
+    if (st.actions_diff) CHECK(qcd_read_action_slices(lat, asl.data(), &na));
+    na = std::min(na, (int)cap);
     np = std::min(np, (int)cap);
     nq = std::min(nq, (int)cap);
     nw = std::min(nw, (int)cap);
@@ -637,6 +648,35 @@ int main(int argc, char** argv)
             for (int s = 0; s < st.n[d]; ++s) fprintf(out, "% 16.13e ", re[s].variance);
             fprintf(out, "\nPL_im_variance ");
             for (int s = 0; s < st.n[d]; ++s) fprintf(out, "% 16.13e ", im[s].variance);
+            fprintf(out, "\n");
+            off += st.n[d];
+        }
+    }
+    if (st.actions_diff && na > 0) {
+        // lattice_write_results' differentiated S block (suncl.cpp:1753-1790):
+        // per slice S_total = (S_s + S_t) / 2 (lattice_data_analysis_joint,
+        // data_analysis.cpp:147-171) with S_s / S_t = (beta/N) (N - sum /
+        // (3 * sites in the slice)) (suncl.cpp:1376-1480)
+        fprintf(out, " ***************************************************\n");
+        fprintf(out, "Differentiated S data (#, S_total, S_variance):\n");
+        const char* lab[3] = {"X", "Y", "Z"};
+        int off = 0;
+        for (int d = 0; d < 3; ++d) {
+            const double cnt = 3.0 * V / st.n[d];
+            std::vector<Series> tot(st.n[d], Series{"S_total_diff"});
+            for (int s = 0; s < st.n[d]; ++s) {
+                for (int i = 0; i < na; ++i) {
+                    const double* r = &asl[2 * ((size_t)i * nslice + off + s)];
+                    tot[s].v.push_back(0.5 * (bet * (N - r[0] / cnt) + bet * (N - r[1] / cnt)));
+                }
+                tot[s].analyse();
+            }
+            fprintf(out, "\n%-15s", lab[d]);
+            for (int s = 0; s < st.n[d]; ++s) fprintf(out, "%2u                   ", (unsigned)s);
+            fprintf(out, "\nS             ");
+            for (int s = 0; s < st.n[d]; ++s) fprintf(out, "% 16.13e ", tot[s].mean);
+            fprintf(out, "\nS_variance    ");
+            for (int s = 0; s < st.n[d]; ++s) fprintf(out, "% 16.13e ", tot[s].variance);
             fprintf(out, "\n");
             off += st.n[d];
         }

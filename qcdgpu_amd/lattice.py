@@ -147,9 +147,10 @@ class Lattice:
         check(lib().qcd_lattice_sync(self._h))
 
     # -- measurements ---------------------------------------------------------
-    def measure(self, plaquette=True, polyakov=True, wilson=False, polyakov_slices=False, fmunu=None):
+    def measure(self, plaquette=True, polyakov=True, wilson=False, polyakov_slices=False, fmunu=None,
+                action_slices=False):
         """fmunu: None, "H" (planes xy, xz, yz) or "E" (xt, yt, zt); it
-        implies the plaquette."""
+        and action_slices imply the plaquette."""
         if fmunu not in (None, "H", "E"):
             raise ValueError("fmunu must be None, 'H' or 'E'")
         mask = (
@@ -158,8 +159,30 @@ class Lattice:
             | (_abi.MEAS_WILSON if wilson else 0)
             | (_abi.MEAS_POLYAKOV_SLICES if polyakov_slices else 0)
             | {None: 0, "H": _abi.MEAS_FMUNU, "E": _abi.MEAS_F0MU}[fmunu]
+            | (_abi.MEAS_ACTION_SLICES if action_slices else 0)
         )
         check(lib().qcd_measure(self._h, mask))
+
+    def action_slices(self, raw=False):
+        """Action per x, y and z slice for every plaquette record: dict
+        x/y/z -> array (count, N_d, 2) of (spatial, temporal).  Raw: sums of
+        ReTr U_p over the slice's three spatial / three temporal plaquettes
+        per site; otherwise the reference's S_diff_s / S_diff_t,
+        beta/N (N - sum / (3 * sites in the slice)) (suncl.cpp:1376-1480)."""
+        n1, n2, n3, n4 = self.dims
+        cap = self.params.max_measurements
+        a = np.zeros((cap, n1 + n2 + n3, 2))
+        c = ctypes.c_int()
+        check(lib().qcd_read_action_slices(self._h, a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                           ctypes.byref(c)))
+        a = a[: min(c.value, cap)]
+        out = {"x": a[:, :n1], "y": a[:, n1:n1 + n2], "z": a[:, n1 + n2:]}
+        if not raw:
+            V, N = float(self.volume), float(self.sun)
+            bet = self.params.beta / N
+            for k, n in zip("xyz", (n1, n2, n3)):
+                out[k] = bet * (N - out[k] / (3.0 * V / n))
+        return out
 
     def set_fmunu(self, gen1, gen2):
         """Generators of the F_munu measurement: SU(3) lambda_gen1 (1..4) and
@@ -376,13 +399,15 @@ class Lattice:
         return info.as_dict()
 
     # -- the reference run loop -------------------------------------------------
-    def simulate(self, nav=0, iters=2, niter=1, n_or=0, start="cold", wilson=False, slices=False, fmunu=None):
+    def simulate(self, nav=0, iters=2, niter=1, n_or=0, start="cold", wilson=False, slices=False, fmunu=None,
+                 action_slices=False):
         """model::lattice_simulate: start, measure, NAV sweeps, then ITER-1
         measurements each after NITER sweeps (suncl.cpp:5236-5483).  Each sweep
         is one heatbath sweep plus `n_or` overrelaxation sweeps; wilson=True
         adds the Wilson loop (set_wilson) to every measurement, slices=True the
         Polyakov loop per slice (PL_level > 2), fmunu="H"/"E" the field
-        strength (get_Fmunu / get_F0mu)."""
+        strength (get_Fmunu / get_F0mu), action_slices=True the action per
+        slice (get_actions_diff)."""
         if start == "hot":
             self.prng_init()   # the Haar start draws from the streams
             self.init(start)
@@ -391,7 +416,7 @@ class Lattice:
             if start != "gid":
                 self.prng_init()
         self.reset_measurements()
-        self.measure(wilson=wilson, polyakov_slices=slices, fmunu=fmunu)
+        self.measure(wilson=wilson, polyakov_slices=slices, fmunu=fmunu, action_slices=action_slices)
         for _ in range(nav):
             self.sweep("hb")
             for _ in range(n_or):
@@ -401,7 +426,7 @@ class Lattice:
                 self.sweep("hb")
                 for _ in range(n_or):
                     self.sweep("or")
-            self.measure(wilson=wilson, polyakov_slices=slices, fmunu=fmunu)
+            self.measure(wilson=wilson, polyakov_slices=slices, fmunu=fmunu, action_slices=action_slices)
         self.sync()
         return self.measurements()
 

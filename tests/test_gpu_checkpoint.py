@@ -69,7 +69,8 @@ def _rng_slice(lat, img):
     m = lat.params.max_measurements
     slices = 2 * 8 * (lat.dims[0] + lat.dims[1] + lat.dims[2]) * m   # Polyakov slice records
     fmunu = 12 * 8 * m                                                # F_munu records
-    n = img.size - STATE_HEAD - lat.table_bytes() - 7 * 8 * m - slices - fmunu
+    actions = slices                                                  # action slice records
+    n = img.size - STATE_HEAD - lat.table_bytes() - 7 * 8 * m - slices - fmunu - actions
     return slice(STATE_HEAD + lat.table_bytes(), STATE_HEAD + lat.table_bytes() + n)
 
 

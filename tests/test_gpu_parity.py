@@ -650,3 +650,39 @@ def test_twisted_sweeps_and_measurements_vs_oracle(O, case):
     assert np.max(np.abs(F[1] - O.fmunu(L, view, True, *gens))) < 1e-12 * V
     assert np.allclose(poly[0], O.polyakov(L, got), rtol=1e-12, atol=1e-9)
     assert np.array_equal(plq2[2], plq2[3])   # the cycle's record == standalone on the same links
+
+
+@pytest.mark.parametrize("sun,prec,twist", [(3, "fp64", None), (3, "fp32", None), (2, "fp64", None),
+                                            (3, "fp64", (3.0, 0.5))])
+def test_action_slices_vs_oracle(O, sun, prec, twist):
+    """Plaquette sums per x / y / z slice (get_actions_diff) on a lattice with
+    unequal extents against the oracle's per-site sums; the slices of each
+    direction add up to the plaquette record; a plaquette record without
+    them holds zeros; the S normalisation of Lattice.action_slices."""
+    _case("action slices sun=%d %s twist=%s" % (sun, prec, twist))
+    dims = (6, 4, 2, 4)
+    beta = 5.9 if sun == 3 else 2.3
+    tol = 1e-12 if prec == "fp64" else 1e-5
+    with _lat(sun, dims, prec, beta=beta, seed=9, max_measurements=8) as lat:
+        if twist:
+            lat.set_twist(*twist)
+        lat.prng_init()
+        lat.init("hot")
+        lat.sweep("hb")
+        lat.reset_measurements()
+        lat.measure()
+        lat.measure(action_slices=True)
+        links = lat.export_links()
+        raw = lat.action_slices(raw=True)
+        S = lat.action_slices()
+        plq, _ = lat.raw_measurements()
+    L = O.lattice(sun, dims, beta, 10)
+    view = O.twisted_view(L, links, *twist) if twist else links
+    want = O.action_slices(L, view)
+    V = int(np.prod(dims))
+    for k, n in zip("xyz", dims[:3]):
+        assert raw[k].shape == (2, n, 2)
+        assert not np.any(raw[k][0])
+        assert np.max(np.abs(raw[k][1] - want[k])) < tol * 3 * sun * V / n, k
+        assert np.allclose(raw[k][1].sum(axis=0), plq[1], rtol=1e-12, atol=0)
+        assert np.allclose(S[k][1], beta / sun * (sun - want[k] / (3.0 * V / n)), rtol=0, atol=tol * 10)

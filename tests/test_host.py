@@ -94,7 +94,7 @@ def test_bad_precision_value(tmp_path):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("prng,ints,fkeys", [("XOR128", 1, ""), ("RANLUX3", 0, "F0MU = 1\nFMUNU1 = 1\nFMUNU7 = 1\n"),
-                                             ("PM", 0, "PHI = 3.0\nOMEGA = 0.5\n")])
+                                             ("PM", 0, "PHI = 3.0\nOMEGA = 0.5\nACTIONSDIFF = 1\n")])
 def test_host_run_matches_python_driver(tmp_path, prng, ints, fkeys):
     """qcdgpu_hip on a small lattice writes the same plaquette, Wilson loop,
     Polyakov and F_munu series as qcdgpu_amd.Lattice.simulate with the same
@@ -115,7 +115,7 @@ def test_host_run_matches_python_driver(tmp_path, prng, ints, fkeys):
     assert r.returncode == 0, r.stderr
     out = [p for p in os.listdir(tmp_path) if p.startswith("su3-") and p.endswith(".txt")]
     assert len(out) == 1
-    rows, pl_rows, section, fields = [], {}, None, None
+    rows, pl_rows, s_rows, section, fields = [], {}, {}, None, None
     lines = open(tmp_path / out[0]).read().splitlines()
     for k, line in enumerate(lines):
         parts = line.split()
@@ -127,6 +127,8 @@ def test_host_run_matches_python_driver(tmp_path, prng, ints, fkeys):
             section = parts[0]
         elif parts and parts[0] in ("PL", "PL_im") and section:
             pl_rows[section, parts[0]] = np.array([float(x) for x in parts[1:]])
+        elif parts and parts[0] in ("S", "S_variance") and section:
+            s_rows[section, parts[0]] = np.array([float(x) for x in parts[1:]])
     rows = np.array(rows)
     assert rows.shape == (4, 27)   # Plq x3, Wilson_loop, S x3, Polyakov x4, F_munu x16
     planes = ("xt", "yt", "zt") if efield else ("xy", "xz", "yz")
@@ -142,7 +144,8 @@ def test_host_run_matches_python_driver(tmp_path, prng, ints, fkeys):
         lat.set_fmunu(g1, g2)
         lat.set_twist(*twist)
         m = lat.simulate(nav=1, iters=4, niter=2, n_or=2, start={0: "hot", 1: "cold"}[ints], wilson=True,
-                         slices=True, fmunu="E" if efield else "H")
+                         slices=True, fmunu="E" if efield else "H", action_slices="ACTIONSDIFF" in fkeys)
+        asl = lat.action_slices()
         wl = lat.wilson_measurements()
         sl = lat.polyakov_slices()
         F = lat.fmunu()
@@ -160,3 +163,11 @@ def test_host_run_matches_python_driver(tmp_path, prng, ints, fkeys):
         # |F| = sqrt of the sum over planes of squares (lattice_data_analysis_joint3)
         assert np.allclose(rows[:, 23 + 2 * gi], np.sqrt((F[:, gi].real ** 2).sum(axis=1)), rtol=1e-12, atol=1e-15)
         assert np.allclose(rows[:, 24 + 2 * gi], np.sqrt((F[:, gi].imag ** 2).sum(axis=1)), rtol=1e-12, atol=1e-15)
+    if "ACTIONSDIFF" in fkeys:   # S_total = (S_s + S_t) / 2 per slice, mean and variance without measurement 0
+        assert len(s_rows) == 6
+        for d in "xyz":
+            tot = 0.5 * (asl[d][1:, :, 0] + asl[d][1:, :, 1])
+            assert np.allclose(s_rows[d.upper(), "S"], tot.mean(axis=0), rtol=1e-11, atol=1e-14)
+            assert np.allclose(s_rows[d.upper(), "S_variance"], tot.var(axis=0), rtol=1e-8, atol=1e-18)
+    else:
+        assert not s_rows
