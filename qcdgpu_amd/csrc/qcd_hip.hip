@@ -298,42 +298,43 @@ __device__ __forceinline__ M<R, N> staple_nu(const Table<R, N, NREAL>& T, const 
 {
     using Tab = Table<R, N, NREAL>;
     const int q = p ^ 1;
-    // All six gathers are issued as one batch, grouped by plane, before any
-    // arithmetic (the scheduling barrier keeps the compiler from splitting
-    // the batch to save registers): wavefronts that share a neighbour line
-    // then request it at the same point of their programs, while it is
-    // still in L2.
+    // The six gathers go out as two batches of three (forward, backward
+    // term): three matrices in flight per wavefront keep the register count
+    // low enough for six to eight wavefronts per SIMD, which hides more
+    // latency than a single batch of six at four wavefronts per SIMD.
     const int cmu = coord(s, mu), cnu = coord(s, nu);
     const uint32_t o = (uint32_t)s.y & 1u;
     const uint32_t i_pmu = step_fwd(g, i, mu, cmu, o);
     const uint32_t i_pnu = step_fwd(g, i, nu, cnu, o);
     const uint32_t i_mnu = step_bwd(g, i, nu, cnu, o);
     const uint32_t i_pmu_mnu = step_bwd(g, i_pmu, nu, cnu, o);
-    M<R, N> f2 = T.load_raw_u(q, mu, i_pnu);
-    M<R, N> b1 = T.load_raw_u(q, mu, i_mnu);
-    M<R, N> f1 = T.load_raw_u(q, nu, i_pmu);
-    M<R, N> b3 = T.load_raw_u(q, nu, i_mnu);
-    M<R, N> f3 = T.load_raw_u(p, nu, i);
-    M<R, N> b2 = T.load_raw_u(p, nu, i_pmu_mnu);
-    __builtin_amdgcn_sched_barrier(0);
-    Tab::finish(f1);
-    Tab::finish(f2);
-    Tab::finish(f3);
-    Tab::finish(b1);
-    Tab::finish(b2);
-    Tab::finish(b3);
-    if constexpr (TW) {
-        if (mu == 1 || nu == 1) {   // wavefront-uniform
-            tbc(f2, tw, mu, i_pnu);
-            tbc(b1, tw, mu, i_mnu);
-            tbc(f1, tw, nu, i_pmu);
-            tbc(b3, tw, nu, i_mnu);
-            tbc(f3, tw, nu, i);
-            tbc(b2, tw, nu, i_pmu_mnu);
-        }
-    }
     M<R, N> S;
-    if constexpr (N == 3) {
+    if constexpr (N == 3 && NREAL == 18 && sizeof(R) == 4) {
+        // FP32, 18 reals: the forward and then the backward three gathers as
+        // two batches, scheduled freely (the peak holds three matrices in
+        // flight, 78 VGPRs: six wavefronts per SIMD instead of four).
+        M<R, N> f2 = T.load_raw_u(q, mu, i_pnu);
+        M<R, N> f1 = T.load_raw_u(q, nu, i_pmu);
+        M<R, N> f3 = T.load_raw_u(p, nu, i);
+        Tab::finish(f1);
+        Tab::finish(f2);
+        Tab::finish(f3);
+        M<R, N> b1 = T.load_raw_u(q, mu, i_mnu);
+        M<R, N> b3 = T.load_raw_u(q, nu, i_mnu);
+        M<R, N> b2 = T.load_raw_u(p, nu, i_pmu_mnu);
+        Tab::finish(b1);
+        Tab::finish(b2);
+        Tab::finish(b3);
+        if constexpr (TW) {
+            if (mu == 1 || nu == 1) {   // wavefront-uniform
+                tbc(f2, tw, mu, i_pnu);
+                tbc(b1, tw, mu, i_mnu);
+                tbc(f1, tw, nu, i_pmu);
+                tbc(b3, tw, nu, i_mnu);
+                tbc(f3, tw, nu, i);
+                tbc(b2, tw, nu, i_pmu_mnu);
+            }
+        }
         const M<R, 3> F = su3_fwd_term(f1, f2, f3);
         const M<R, 3> B = su3_bwd_term(b1, b2, b3);
 #pragma unroll
@@ -341,11 +342,57 @@ __device__ __forceinline__ M<R, N> staple_nu(const Table<R, N, NREAL>& T, const 
 #pragma unroll
             for (int c = 0; c < 3; ++c) S.e[r][c] = cadd(F.e[r][c], B.e[r][c]);
     } else {
-        m_zero(S);
-        const M<R, N> tf = m_mul_nd(f1, f2);
-        m_acc_nd(S, tf, f3);
-        const M<R, N> tb = m_mul(b1, b2);
-        m_acc_dn(S, tb, b3);
+        // Other storages: the forward term is gathered and formed before the
+        // backward gathers are issued (the scheduling barrier keeps the two
+        // apart), so at most three matrices and one partial sum are live.
+        {
+            M<R, N> f2 = T.load_raw_u(q, mu, i_pnu);
+            M<R, N> f1 = T.load_raw_u(q, nu, i_pmu);
+            M<R, N> f3 = T.load_raw_u(p, nu, i);
+            Tab::finish(f1);
+            Tab::finish(f2);
+            Tab::finish(f3);
+            if constexpr (TW) {
+                if (mu == 1 || nu == 1) {
+                    tbc(f2, tw, mu, i_pnu);
+                    tbc(f1, tw, nu, i_pmu);
+                    tbc(f3, tw, nu, i);
+                }
+            }
+            if constexpr (N == 3) {
+                S = su3_fwd_term(f1, f2, f3);
+            } else {
+                m_zero(S);
+                const M<R, N> tf = m_mul_nd(f1, f2);
+                m_acc_nd(S, tf, f3);
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        {
+            M<R, N> b1 = T.load_raw_u(q, mu, i_mnu);
+            M<R, N> b3 = T.load_raw_u(q, nu, i_mnu);
+            M<R, N> b2 = T.load_raw_u(p, nu, i_pmu_mnu);
+            Tab::finish(b1);
+            Tab::finish(b2);
+            Tab::finish(b3);
+            if constexpr (TW) {
+                if (mu == 1 || nu == 1) {
+                    tbc(b1, tw, mu, i_mnu);
+                    tbc(b3, tw, nu, i_mnu);
+                    tbc(b2, tw, nu, i_pmu_mnu);
+                }
+            }
+            if constexpr (N == 3) {
+                const M<R, 3> B = su3_bwd_term(b1, b2, b3);
+#pragma unroll
+                for (int r = 0; r < 3; ++r)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) S.e[r][c] = cadd(S.e[r][c], B.e[r][c]);
+            } else {
+                const M<R, N> tb = m_mul(b1, b2);
+                m_acc_dn(S, tb, b3);
+            }
+        }
     }
     return S;
 }
