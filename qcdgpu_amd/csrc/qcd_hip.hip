@@ -1083,6 +1083,9 @@ __device__ __forceinline__ void finalize_body(const FinArgs& f)
                 tm += f.plq[2 * j + 1];
             }
         }
+        // every thread reads the record index before the sums' barriers;
+        // thread 0 advances it after them
+        const int rec = f.rec_index[0];
         sp = block_sum_1(sp);
         tm = block_sum_1(tm);
         double fm[12];
@@ -1093,17 +1096,16 @@ __device__ __forceinline__ void finalize_body(const FinArgs& f)
                 fm[k] = block_sum_1(a);
             }
         }
-        if (t == 0) {
-            const int rec = f.rec_index[0];
-            if (rec < f.capacity) {
+        if (rec < f.capacity) {
+            if (t == 0) {
                 f.plq_rec[2 * rec] = sp;
                 f.plq_rec[2 * rec + 1] = tm;
                 for (int k = 0; k < 12; ++k) f.f_rec[12 * rec + k] = f.fmunu ? fm[k] : 0.0;
-                if (f.a_rec && !f.aslices)   // a plaquette record without action slices
-                    for (int k = 0; k < 2 * f.a_n; ++k) f.a_rec[(size_t)2 * f.a_n * rec + k] = 0.0;
             }
-            f.rec_index[0] = rec + 1;
+            if (f.a_rec && !f.aslices)   // a plaquette record without action slices
+                for (int k = (int)t; k < 2 * f.a_n; k += kBlock) f.a_rec[(size_t)2 * f.a_n * rec + k] = 0.0;
         }
+        if (t == 0) f.rec_index[0] = rec + 1;
     }
     if (f.poly) {
         double v[4] = {0.0, 0.0, 0.0, 0.0};
