@@ -171,3 +171,45 @@ def test_host_run_matches_python_driver(tmp_path, prng, ints, fkeys):
             assert np.allclose(s_rows[d.upper(), "S_variance"], tot.var(axis=0), rtol=1e-8, atol=1e-18)
     else:
         assert not s_rows
+
+
+GOLDEN = os.path.join(REPO, "tests", "golden")
+
+
+def test_reference_init_dat_accepted(tmp_path):
+    """The reference's own SU(N)/init.dat (a copy in tests/golden/): SU(2)
+    16^4, beta 3.5, RANLUX3, Wilson loop 10 x 3, PL_LEVEL 3, PHI = 3 (the
+    twisted boundary) and FMUNU -- every key it sets is taken."""
+    r = subprocess.run([BIN, "--print-params", os.path.join(GOLDEN, "reference_init.dat")], capture_output=True,
+                       text=True)
+    kv = dict(line.split("=", 1) for line in r.stdout.splitlines() if "=" in line)
+    assert r.returncode == 0 and "UNSUPPORTED" not in kv, r.stdout
+    assert kv["GROUP"] == "2" and [kv["L%d" % k] for k in range(1, 5)] == ["16"] * 4
+    assert float(kv["BETA"]) == 3.5 and kv["PRNG"] == "RANLUX3" and kv["ITER"] == "50" and kv["NAV"] == "50"
+    assert (kv["WILSONR"], kv["WILSONT"], kv["PL_LEVEL"]) == ("10", "3", "3")
+    assert float(kv["PHI"]) == 3.0 and kv["FMUNU"] == "H" and kv["GETWILSON"] == "1"
+
+
+@pytest.mark.gpu
+def test_reference_init_dat_runs(tmp_path):
+    """The reference's init.dat, run as it stands (540 sweeps of SU(2) 16^4
+    with the twist, Wilson loop, Polyakov slices and F_munu), only
+    OUTPUTPATH pointed at a writable folder (a later key replaces an earlier
+    one, as in the reference's parser)."""
+    print("[case] reference init.dat", flush=True)
+    f = tmp_path / "init.dat"
+    f.write_text(open(os.path.join(GOLDEN, "reference_init.dat")).read() + "\n\tOUTPUTPATH = %s/\n" % tmp_path)
+    r = subprocess.run([BIN, str(f)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    out = [p for p in os.listdir(tmp_path) if p.startswith("su2-") and p.endswith(".txt")]
+    assert len(out) == 1
+    text = open(tmp_path / out[0]).read()
+    rows = [ln.split() for ln in text.splitlines() if ln.split() and ln.split()[0].isdigit() and len(ln.split()) == 28]
+    assert len(rows) == 50
+    a = np.array([[float(x) for x in r_[1:]] for r_ in rows])
+    assert np.all(np.isfinite(a))
+    assert np.all(np.abs(a[:, :3]) <= 2.0)                       # Plq_spat / temp / total: mean ReTr U_p, |.| <= N
+    assert 1.0 < a[1:, 2].mean() < 2.0                           # ordered phase at beta = 3.5
+    phi = [ln for ln in text.splitlines() if ln.startswith(" PHI")]
+    assert len(phi) == 1 and float(phi[0].split(":")[1]) == 3.0
+    assert "Differentiated Polyakov loop data" in text and "FMUNU(3, 2)" in text
