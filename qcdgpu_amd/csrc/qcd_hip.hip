@@ -1154,7 +1154,9 @@ __global__ __launch_bounds__(kBlock) void k_plaquette(void* table, Geom g, doubl
             if constexpr (TW) tbc(m, tw, d, idx);
             return m_cast<double>(m);
         };
-#pragma unroll
+        // the outer loop stays rolled: unrolled, the FP64 matrices of all six
+        // planes are live at once (256 VGPRs, spills in the twisted form)
+#pragma unroll 1
         for (int a = 0; a < 3; ++a) {
             M<double, N> Ua = ld(p, a, i);
             const uint32_t i_pa = nb_index(g, s, a, +1);
