@@ -46,8 +46,12 @@ enum {
 /* Link precision (model::precision, QCDGPU.cpp:179). */
 enum { QCD_FP32 = 0, QCD_FP64 = 1 };
 
-/* Start types (model::ints: model_start_hot/cold/gid, suncl.h:67). */
-enum { QCD_START_HOT = 0, QCD_START_COLD = 1, QCD_START_GID = 2 };
+/* Start types (model::ints: model_start_hot/cold/gid, suncl.h:67).
+   QCD_START_HOT is a Haar-random start (no counterpart in the reference);
+   QCD_START_RANDOM is the reference's model_start_hot itself
+   (lattice_init_hot_X..T, suncl.cl:56-161), written as ints = 0 in .qcg
+   headers. */
+enum { QCD_START_HOT = 0, QCD_START_COLD = 1, QCD_START_GID = 2, QCD_START_RANDOM = 3 };
 
 /* Generators (PRNG::PRNG_generators, random.h:13-27, the subset on the path). */
 enum { QCD_PRNG_XOR128 = 1, QCD_PRNG_RANLUX = 2, QCD_PRNG_PM = 3 };
@@ -126,7 +130,23 @@ int  qcd_lattice_table_bytes(const qcd_lattice *lat, size_t *bytes);
         QCD_START_HOT draws Haar-distributed links from the per-site streams
         (the reference's lattice_init_hot_* is not Haar, su2_update_cl.cl:110),
         so it needs qcd_prng_init first; stream i fills the links of the two
-        sites with compact index i (parity 0 then 1, directions 0..3).       */
+        sites with compact index i (parity 0 then 1, directions 0..3).
+        QCD_START_RANDOM is the reference's hot start
+        (suncl.cpp:5234-5248): it seeds the streams itself (qcd_prng_init
+        is not needed), sizes every produce() of the run as the reference's
+        hot runs do (suncl.cpp:2304-2305: S_hot = ceil(align64(4 ROWSIZE +
+        3 PRNGSTEP (NHIT+1)) / PRNGSTEP) quads per stream, ROWSIZE =
+        align64(V)), and fills direction d from the (d+2)-th produce():
+        site s (lexicographic index y + N2 (z + N3 (t + N4 x))) takes M. Di
+        Pierro's random SU(2) element of quad s (lattice_random2,
+        su2_update_cl.cl:110-146), or for SU(3) the product E12(A) E13(B)
+        E23(C) of the elements of components x, y, z of quads s,
+        s + ROWSIZE, s + 2 ROWSIZE (lattice_random3,
+        su3_update_cl.cl:165-225).  Counter 5 afterwards.  Needs
+        QCD_STREAM_REFERENCE (RANLUX always is).  The other starts return
+        the lattice to the cold runs' produce() size (re-seeding streams
+        seeded under the other size); a change of size drops a captured
+        cycle.                                                             */
 int  qcd_lattice_init(qcd_lattice *lat, int start);
 
 /* ---- PRNG (PRNG::initialize, random.cpp:216-277): seeds the per-site
@@ -249,7 +269,8 @@ int  qcd_prng_counter(const qcd_lattice *lat, unsigned *counter);
 /* Re-seeds the streams (qcd_prng_init) and advances them to production
    `counter` of a run that began with `start` (QCD_START_*), as the
    reference's resume does (while (PRNG_counter > PRNG0->PRNG_counter)
-   produce()).  XOR128 / PM heatbath productions are applied as one jump.
+   produce()), with the produce() size of that start (QCD_START_RANDOM:
+   the hot runs').  XOR128 / PM heatbath productions are applied as one jump.
    Fast-layout XOR128 / PM streams cannot be replayed (QCD_E_UNSUPPORTED).  */
 int  qcd_prng_replay(qcd_lattice *lat, unsigned counter, int start);
 /* Exact state image: the table as stored on the device, the current

@@ -125,6 +125,67 @@ def hot(L, quads, stride):
     return a
 
 
+def di_pierro(u, v, w):
+    """lattice_random2 (su2_update_cl.cl:110-146): M. Di Pierro's map of
+    three uniform numbers to SU(2), alpha = pi u, phi = 2 pi v,
+    cos theta = 2 w - 1, quaternion a = (cos alpha, sin alpha sin theta
+    cos phi, sin alpha sin theta sin phi, sin alpha cos theta) as
+    [[a0 + i a3, a2 + i a1], [-a2 + i a1, a0 - i a3]].  Arrays in, (..., 2, 2)
+    complex out, in float64."""
+    u, v, w = (np.asarray(x, dtype=np.float64) for x in (u, v, w))
+    alpha, phi, c = np.pi * u, 2 * np.pi * v, 2 * w - 1
+    t = np.sin(alpha) * np.sqrt(1 - c * c)
+    a0, a1, a2, a3 = np.cos(alpha), t * np.cos(phi), t * np.sin(phi), np.sin(alpha) * c
+    m = np.empty(u.shape + (2, 2), dtype=np.complex128)
+    m[..., 0, 0] = a0 + 1j * a3
+    m[..., 0, 1] = a2 + 1j * a1
+    m[..., 1, 0] = -a2 + 1j * a1
+    m[..., 1, 1] = a0 - 1j * a3
+    return m
+
+
+def random_start(L, bufs, rowsize):
+    """The reference's hot start (lattice_init_hot_X..T, suncl.cl:56-161).
+    bufs[d]: the quads of the produce() before direction d, flat ((samples *
+    instances, 4), quad q of the reference's buffer at row q).  The link of
+    site s (this oracle's site order, which is the reference's table index)
+    takes quad s (SU(2)) or quads s, s + rowsize, s + 2 rowsize (SU(3)): the
+    SU(2) elements A, B, C of components x, y, z (u from the first quad, v
+    from the second, w from the third) multiplied as E12(A) E13(B) E23(C)
+    (lattice_random3, su3_update_cl.cl:165-225)."""
+    V = int(np.prod(list(L.n)))
+    N = L.sun
+    s = np.arange(V)
+    U = np.zeros((V, 4, N, N), dtype=np.complex128)
+    for d in range(4):
+        b = np.asarray(bufs[d], dtype=np.float32).reshape(-1, 4)
+        if N == 2:
+            q = b[s]
+            U[:, d] = di_pierro(q[:, 0], q[:, 1], q[:, 2])
+            continue
+        q0, q1, q2 = b[s], b[s + rowsize], b[s + 2 * rowsize]
+        m = np.tile(np.eye(3, dtype=np.complex128), (V, 1, 1))
+        for c, (i, j) in enumerate(((0, 1), (0, 2), (1, 2))):
+            w = di_pierro(q0[:, c], q1[:, c], q2[:, c])
+            e = np.tile(np.eye(3, dtype=np.complex128), (V, 1, 1))
+            e[:, i, i], e[:, i, j], e[:, j, i], e[:, j, j] = w[:, 0, 0], w[:, 0, 1], w[:, 1, 0], w[:, 1, 1]
+            m = m @ e
+        U[:, d] = m
+    return np.ascontiguousarray(np.stack([U.real, U.imag], axis=-1))
+
+
+def produce_quads(L, hot_run):
+    """Quads per instance of one produce() (suncl.cpp:2302-2307) with
+    PRNG_instances = PRNGSTEP = align64(V/2); a hot run adds ROWSIZE * ND
+    PRNs, ROWSIZE = align64(V)."""
+    V = int(np.prod(list(L.n)))
+    prngstep = -(-(V // 2) // 64) * 64
+    rowsize = -(-V // 64) * 64
+    total = 3 * prngstep * (L.nhit + 1) + (4 * rowsize if hot_run else 0)
+    total = -(-total // 64) * 64
+    return 1 + (total - 1) // prngstep
+
+
 def rand_fill(seed, count):
     out = np.zeros(count, dtype=np.int32)
     lib().orc_rand_fill(seed, count, _p(out, _I32P))

@@ -14,7 +14,7 @@ HEADER = os.path.join(REPO, "include", "qcd_hip.h")
 
 QCD_OK = 0
 FP32, FP64 = 0, 1
-START_HOT, START_COLD, START_GID = 0, 1, 2
+START_HOT, START_COLD, START_GID, START_RANDOM = 0, 1, 2, 3
 PRNG_XOR128, PRNG_RANLUX, PRNG_PM = 1, 2, 3
 STREAM_REFERENCE, STREAM_FAST = 0, 1
 HEATBATH, OVERRELAX = 0, 1

@@ -944,6 +944,78 @@ __global__ __launch_bounds__(kBlock) void k_init_hot(void* table, Geom g, uint4*
     if constexpr (RK != RK_BUF) rng[i] = st;
 }
 
+// One SU(2) element from three uniform numbers by M. Di Pierro's map
+// (lattice_random2, su2_update_cl.cl:110-146): alpha = pi u, phi = 2 pi v,
+// cos theta = 2 w - 1, quaternion a = (cos alpha, sin alpha sin theta cos phi,
+// sin alpha sin theta sin phi, sin alpha cos theta), as the matrix
+// [[a0 + i a3, a2 + i a1], [-a2 + i a1, a0 - i a3]].  In the link precision.
+template <typename R> __device__ __forceinline__ M<R, 2> di_pierro(float u, float v, float w)
+{
+    const R alpha = R(3.1415926535897932384626433832795) * (R)u;
+    const R phi = R(6.2831853071795864769252867665590) * (R)v;
+    const R costh = R(2) * (R)w - R(1);
+    R sa, ca, sp, cp;
+    if constexpr (sizeof(R) == 4) {
+        sincosf(alpha, &sa, &ca);
+        sincosf(phi, &sp, &cp);
+    } else {
+        sincos(alpha, &sa, &ca);
+        sincos(phi, &sp, &cp);
+    }
+    const R t = sa * sqrt(R(1) - costh * costh);
+    const R a1 = t * cp, a2 = t * sp, a3 = sa * costh;
+    M<R, 2> m;
+    m.e[0][0] = {ca, a3};
+    m.e[0][1] = {a2, a1};
+    m.e[1][0] = {-a2, a1};
+    m.e[1][1] = {ca, -a3};
+    return m;
+}
+
+// w embedded in rows / columns p < q of the SU(3) unit matrix.
+template <typename R> __device__ __forceinline__ M<R, 3> embed3(const M<R, 2>& w, int p, int q)
+{
+    M<R, 3> m;
+    m_unit(m);
+    m.e[p][p] = w.e[0][0];
+    m.e[p][q] = w.e[0][1];
+    m.e[q][p] = w.e[1][0];
+    m.e[q][q] = w.e[1][1];
+    return m;
+}
+
+// The reference's random ("hot") start, one direction d per launch
+// (lattice_init_hot_X..T, suncl.cl:56-161): buf is the flat quad buffer of
+// one produce(); the link of site s (lexicographic index y + N2 (z + N3
+// (t + N4 x)), the reference's table index) reads quad s (SU(2)) or quads
+// s + k rowsize, k = 0..2 (SU(3)).  SU(3) (lattice_random3,
+// su3_update_cl.cl:165-225): the SU(2) elements A, B, C of components x, y, z
+// (u from the first quad, v from the second, w from the third) in the product
+// E12(A) E13(B) E23(C) of subgroup embeddings.
+template <typename R, int N, int NREAL>
+__global__ __launch_bounds__(kBlock) void k_init_random(void* table, Geom g, const float4* buf, uint32_t rowsize,
+                                                        int d)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= g.vh) return;
+    Table<R, N, NREAL> T{(typename Vec2<R>::T*)table, g.vh_pad};
+    for (int p = 0; p < 2; ++p) {
+        const Site s = site_of(g, i, p);
+        const uint32_t gid = (uint32_t)s.y + (uint32_t)g.n2 * ((uint32_t)s.z + (uint32_t)g.n3 *
+                                                                 ((uint32_t)s.t + (uint32_t)g.n4 * (uint32_t)s.x));
+        if constexpr (N == 2) {
+            const float4 q = buf[gid];
+            T.store_u(p, d, i, di_pierro<R>(q.x, q.y, q.z));
+        } else {
+            const float4 q0 = buf[gid], q1 = buf[gid + rowsize], q2 = buf[gid + 2u * rowsize];
+            const M<R, 3> a = embed3(di_pierro<R>(q0.x, q1.x, q2.x), 0, 1);
+            const M<R, 3> b = embed3(di_pierro<R>(q0.y, q1.y, q2.y), 0, 2);
+            const M<R, 3> c = embed3(di_pierro<R>(q0.z, q1.z, q2.z), 1, 2);
+            T.store_u(p, d, i, m_mul(m_mul(a, b), c));
+        }
+    }
+}
+
 // lattice_init_gid (suncl.cl:190-347) / lattice_matrixGID
 // (suncpp/su3/algebra_su3.cpp:64-80, su2/algebra_su2.cpp:54-64).
 template <typename R, int N, int NREAL>
@@ -1587,6 +1659,7 @@ struct qcd_lattice {
     hipGraphExec_t exec = nullptr;
     int cyc_hb = 0, cyc_or = 0, cyc_mask = 0;
     bool rng_ready = false;
+    bool hot_run = false;       // produce() sized as in the reference's hot runs (QCD_START_RANDOM)
     bool gid_updates = false;   // INTS = 2: GID known-answer numbers in the heatbath
     // PRNG::produce calls the reference would have made so far
     // (PRNG::PRNG_counter, random.cpp:283): 1 for the initial produce, 4 for
@@ -1825,6 +1898,16 @@ int launch_simple(qcd_lattice* L, int which, const LaunchOpt& o)
     return QCD_OK;
 }
 
+template <typename R, int N, int NREAL>
+int launch_random(qcd_lattice* L, const float4* buf, uint32_t rowsize, int d)
+{
+    const uint32_t nb = align_up(L->g.vh, kBlock) / kBlock;
+    hipLaunchKernelGGL((k_init_random<R, N, NREAL>), dim3(nb), dim3(kBlock), 0, L->stream, L->table, L->g, buf,
+                       rowsize, d);
+    HIPCHK(hipGetLastError());
+    return QCD_OK;
+}
+
 // flags: 1 = fused plaquette partials, 2 = k_plaquette partials, 4 = Polyakov.
 int launch_finalize(qcd_lattice* L, int flags, const LaunchOpt& o)
 {
@@ -1915,6 +1998,89 @@ int qcd_device_info(char* name, int name_len, int* compute_units, size_t* global
     return QCD_OK;
 }
 
+// Quads per stream of one produce() (suncl.cpp:2302-2307, random.cpp:219-223)
+// with PRNG_instances = PRNGSTEP = align64(V/2): align64(3 PRNGSTEP (NHIT+1)
+// NHITPar) PRNs, or, in a run that starts hot (model_start_hot),
+// align64(ROWSIZE ND + 3 PRNGSTEP (NHIT+1)) with ROWSIZE = align64(V), over
+// PRNGSTEP instances, rounded up.
+static uint32_t produce_quads(const qcd_params& p, uint32_t vh, bool hot_run)
+{
+    auto align64 = [](uint64_t x) { return (x + 63u) / 64u * 64u; };
+    const uint64_t prngstep = align64(vh);
+    const uint64_t rowsize = align64(2ull * vh);
+    const uint64_t upd = 3ull * prngstep * (uint64_t)(p.nhit + 1);
+    const uint64_t total = align64(hot_run ? rowsize * 4u + upd : (uint64_t)p.nhitpar * upd);
+    return (uint32_t)(1 + (total - 1) / prngstep);
+}
+
+// Reference-stream skip of s_quads quads per produce(): XOR128 jump matrix
+// for 4 * s_quads steps (four steps per quad), Park-Miller multiplier for
+// s_quads steps (pm_quad steps each of its four lanes once).  Columns (image
+// of state bit b = word b/32, bit b%32), then the four-bit table: entry
+// [c][e] = XOR of the columns of the bits of chunk c (bits 4c .. 4c+3) set
+// in e.
+static int build_jumps(qcd_lattice* L)
+{
+    uint4 cols[kJumpCols + kJumpTab];
+    for (int b = 0; b < kJumpCols; ++b) {
+        uint4 st = make_uint4(0u, 0u, 0u, 0u);
+        uint32_t* w = (uint32_t*)&st;
+        w[b / 32] = 1u << (b % 32);
+        for (uint32_t k = 0; k < 4u * L->s_quads; ++k) (void)xor128_step(st);
+        cols[b] = st;
+    }
+    for (int c = 0; c < 32; ++c)
+        for (int e = 0; e < 16; ++e) {
+            uint4 acc = make_uint4(0u, 0u, 0u, 0u);
+            for (int k = 0; k < 4; ++k)
+                if ((e >> k) & 1) {
+                    const uint4 v = cols[4 * c + k];
+                    acc.x ^= v.x;
+                    acc.y ^= v.y;
+                    acc.z ^= v.z;
+                    acc.w ^= v.w;
+                }
+            cols[kJumpCols + 16 * c + e] = acc;
+        }
+    if (hipMemcpy(L->jump_cols, cols, sizeof(cols), hipMemcpyHostToDevice) != hipSuccess)
+        return fail(QCD_E_HIP, "jump matrix upload failed");
+    uint32_t ak = 1;
+    for (uint32_t k = 0; k < L->s_quads; ++k) ak = pm_mulmod(ak, 16807u);
+    L->pm_ak = ak;
+    return QCD_OK;
+}
+
+// Switches the produce() size between the cold/GID runs' and the hot runs'
+// (QCD_START_RANDOM).  A change rebuilds the jump tables, grows the RANLUX
+// quad buffer when needed and drops a captured cycle (its launches hold the
+// old size); the caller re-seeds streams seeded under the old size.
+static int set_sizing(qcd_lattice* L, bool hot_run, bool* changed)
+{
+    *changed = false;
+    const uint32_t S = produce_quads(L->p, L->g.vh, hot_run);
+    if (S == L->s_quads && hot_run == L->hot_run) return QCD_OK;
+    HIPCHK(hipStreamSynchronize(L->stream));
+    *changed = true;
+    L->hot_run = hot_run;
+    L->s_quads = S;
+    if (L->exec) {
+        (void)hipGraphExecDestroy(L->exec);
+        L->exec = nullptr;
+    }
+    if (L->graph) {
+        (void)hipGraphDestroy(L->graph);
+        L->graph = nullptr;
+    }
+    if (L->p.prng == QCD_PRNG_RANLUX && L->prn_quads < S) {
+        (void)hipFree(L->prn_buf);
+        L->prn_buf = nullptr;
+        L->prn_quads = S;
+        if (hipMalloc((void**)&L->prn_buf, sizeof(float4) * (size_t)S * L->n_streams) != hipSuccess)
+            return fail(QCD_E_NOMEM, "RANLUX quad buffer of %u quads per stream", S);
+    }
+    return build_jumps(L);
+}
+
 int qcd_lattice_create(const qcd_params* pp, qcd_lattice** out)
 {
     if (!pp || !out) return fail(QCD_E_ARG, "null argument");
@@ -1957,12 +2123,8 @@ int qcd_lattice_create(const qcd_params* pp, qcd_lattice** out)
     L->upd_blocks = align_up(L->g.vh, kBlock) / kBlock;
     uint32_t vs = (uint32_t)p.n[0] * p.n[1] * p.n[2];
     L->poly_blocks = align_up(8u * vs, kBlock) / kBlock;   // 8 lanes per spatial site
-    // Reference PRNG sizing (suncl.cpp:2302-2310, random.cpp:219-223) with
-    // PRNG_instances = PRNGSTEP = align64(V/2).
-    uint32_t prngstep = align_up(L->g.vh, 64);
-    uint64_t total = (uint64_t)align_up((uint32_t)((uint64_t)p.nhitpar * 3u * prngstep * (p.nhit + 1)), 64);
-    L->n_streams = prngstep;
-    L->s_quads = (uint32_t)(1 + (total - 1) / prngstep);
+    L->n_streams = align_up(L->g.vh, 64);
+    L->s_quads = produce_quads(p, L->g.vh, false);
 
     hipError_t e;
     if ((e = hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking)) != hipSuccess) {
@@ -2015,42 +2177,9 @@ int qcd_lattice_create(const qcd_params* pp, qcd_lattice** out)
         (void)hipMemset(L->a_rec, 0, sizeof(double) * slice_rec_doubles(L));
         (void)hipMemset(L->f_rec, 0, sizeof(double) * 12 * m);
     }
-    {
-        // Reference-stream skip of s_quads quads per heatbath half-step:
-        // XOR128 jump matrix for 4 * s_quads steps (four steps per quad),
-        // Park-Miller multiplier for s_quads steps (pm_quad steps each of its
-        // four lanes once).
-        // Columns (image of state bit b = word b/32, bit b%32), then the
-        // four-bit table: entry [c][e] = XOR of the columns of the bits of
-        // chunk c (bits 4c .. 4c+3) set in e.
-        uint4 cols[kJumpCols + kJumpTab];
-        for (int b = 0; b < kJumpCols; ++b) {
-            uint4 st = make_uint4(0u, 0u, 0u, 0u);
-            uint32_t* w = (uint32_t*)&st;
-            w[b / 32] = 1u << (b % 32);
-            for (uint32_t k = 0; k < 4u * L->s_quads; ++k) (void)xor128_step(st);
-            cols[b] = st;
-        }
-        for (int c = 0; c < 32; ++c)
-            for (int e = 0; e < 16; ++e) {
-                uint4 acc = make_uint4(0u, 0u, 0u, 0u);
-                for (int k = 0; k < 4; ++k)
-                    if ((e >> k) & 1) {
-                        const uint4 v = cols[4 * c + k];
-                        acc.x ^= v.x;
-                        acc.y ^= v.y;
-                        acc.z ^= v.z;
-                        acc.w ^= v.w;
-                    }
-                cols[kJumpCols + 16 * c + e] = acc;
-            }
-        if (hipMemcpy(L->jump_cols, cols, sizeof(cols), hipMemcpyHostToDevice) != hipSuccess) {
-            qcd_lattice_destroy(L);
-            return fail(QCD_E_HIP, "jump matrix upload failed");
-        }
-        uint32_t ak = 1;
-        for (uint32_t k = 0; k < L->s_quads; ++k) ak = pm_mulmod(ak, 16807u);
-        L->pm_ak = ak;
+    if ((rc = build_jumps(L)) != QCD_OK) {
+        qcd_lattice_destroy(L);
+        return rc;
     }
     if (hipDeviceSynchronize() != hipSuccess) {
         qcd_lattice_destroy(L);
@@ -2124,9 +2253,67 @@ int qcd_lattice_table_bytes(const qcd_lattice* L, size_t* bytes)
     return QCD_OK;
 }
 
+// The reference's hot start (suncl.cpp:5234-5248): the run's streams are
+// seeded with the hot runs' produce() size, then one produce() before each
+// of the four direction kernels; only the first `need` quads of every
+// stream feed the links, the rest of each produce() is skipped.
+static int random_start(qcd_lattice* L)
+{
+    if (L->p.prng != QCD_PRNG_XOR128 && L->p.prng != QCD_PRNG_PM && L->p.prng != QCD_PRNG_RANLUX)
+        return fail(QCD_E_UNSUPPORTED, "generator %d not available for the random start", L->p.prng);
+    if (L->p.prng != QCD_PRNG_RANLUX && L->p.stream_mode != QCD_STREAM_REFERENCE)
+        return fail(QCD_E_UNSUPPORTED, "the random start reads produce() buffers: it needs QCD_STREAM_REFERENCE");
+    bool changed = false;
+    int rc = set_sizing(L, true, &changed);
+    if (rc) return rc;
+    if ((rc = qcd_prng_init(L))) return rc;   // seeding and the initial produce()
+    L->gid_updates = false;
+    const uint32_t n = L->n_streams, S = L->s_quads;
+    const uint64_t V = 2ull * L->g.vh;
+    const uint64_t rowsize = (V + 63u) / 64u * 64u;
+    const uint64_t used = (L->p.sun == 3) ? V + 2u * rowsize : V;   // quads one direction reads
+    const uint32_t need = (uint32_t)((used + n - 1) / n);
+    if (need > S) return fail(QCD_E_STATE, "internal: random start reads %u of %u quads", need, S);
+    float4* buf = L->prn_buf;   // RANLUX: prn_quads >= S quads per stream
+    if (L->p.prng != QCD_PRNG_RANLUX)
+        HIPCHK(hipMalloc((void**)&buf, sizeof(float4) * (size_t)need * n));
+    const uint32_t nb = align_up(n, kBlock) / kBlock;
+    for (int d = 0; d < 4 && rc == QCD_OK; ++d) {
+        if (L->p.prng == QCD_PRNG_RANLUX) {
+            launch_ranlux_produce(L->p.ranlux_luxury, L->rl_state, n, need, buf, L->stream);
+            if (S > need) launch_ranlux_produce(L->p.ranlux_luxury, L->rl_state, n, S - need, nullptr, L->stream);
+        } else if (L->p.prng == QCD_PRNG_XOR128) {
+            hipLaunchKernelGGL(k_xor128_produce, dim3(nb), dim3(kBlock), 0, L->stream, L->rng_buf[L->rng_cur], n, need,
+                               buf);
+            hipLaunchKernelGGL(k_xor128_skip, dim3(nb), dim3(kBlock), 0, L->stream, L->rng_buf[L->rng_cur], n, S - need);
+        } else {
+            hipLaunchKernelGGL(k_pm_produce, dim3(nb), dim3(kBlock), 0, L->stream, L->rng_buf[L->rng_cur], n, need, buf);
+            hipLaunchKernelGGL(k_pm_skip, dim3(nb), dim3(kBlock), 0, L->stream, L->rng_buf[L->rng_cur], n, S - need);
+        }
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) rc = fail(QCD_E_HIP, "random start produce: %s", hipGetErrorString(e));
+        if (rc == QCD_OK) rc = DISPATCH(L, launch_random, L, (const float4*)buf, (uint32_t)rowsize, d);
+        if (rc == QCD_OK) ++L->prng_counter;
+    }
+    const hipError_t e = hipStreamSynchronize(L->stream);
+    if (L->p.prng != QCD_PRNG_RANLUX) (void)hipFree(buf);
+    if (rc == QCD_OK && e != hipSuccess) rc = fail(QCD_E_HIP, "random start: %s", hipGetErrorString(e));
+    return rc;
+}
+
 int qcd_lattice_init(qcd_lattice* L, int start)
 {
     if (!L) return fail(QCD_E_ARG, "null lattice");
+    if (start == QCD_START_RANDOM) return random_start(L);
+    if (start != QCD_START_COLD && start != QCD_START_GID && start != QCD_START_HOT)
+        return fail(QCD_E_ARG, "bad start type %d", start);
+    {   // these starts run with the cold runs' produce() size; streams seeded
+        // under the hot runs' size are seeded again
+        bool changed = false;
+        int rc = set_sizing(L, false, &changed);
+        if (rc) return rc;
+        if (changed && L->rng_ready && (rc = qcd_prng_init(L))) return rc;
+    }
     // The GID known-answer start (INTS = 2) also switches the heatbath to the
     // GID numbers, as -D GID_UPD does in the reference (suncl.cpp:3278-3279).
     if (start == QCD_START_COLD) {
@@ -2146,7 +2333,7 @@ int qcd_lattice_init(qcd_lattice* L, int start)
         if (rc == QCD_OK) L->prng_counter += 4;
         return rc;
     }
-    return fail(QCD_E_ARG, "bad start type %d", start);
+    return QCD_OK;
 }
 
 int qcd_prng_init(qcd_lattice* L)
@@ -2872,7 +3059,7 @@ int qcd_lattice_state_save(qcd_lattice* L, void* buf, size_t bytes)
     for (int k = 0; k < 14; ++k) w.u32((uint32_t)f[k]);
     w.u64(f[14]);
     w.u32(L->prng_counter);
-    w.u32(L->rng_ready ? 1u : 0u);
+    w.u32((L->rng_ready ? 1u : 0u) | (L->hot_run ? 2u : 0u));   // start flags
     w.u32(L->gid_updates ? 1u : 0u);
     w.u32((uint32_t)L->wil_R);
     w.u32((uint32_t)L->wil_T);
@@ -2921,6 +3108,10 @@ int qcd_lattice_state_load(qcd_lattice* L, const void* buf, size_t bytes)
     for (int k = 0; k < 4; ++k) idx[k] = (int)r.u32();
     if (r.u64() != L->table_bytes || r.u64() != rng_state_bytes(L))
         return fail(QCD_E_ARG, "state image sizes do not match this lattice");
+    if (ready & ~3u) return fail(QCD_E_ARG, "state image with unknown start flags %#x", ready);
+    bool changed = false;
+    int rc = set_sizing(L, (ready & 2u) != 0, &changed);   // the run's produce() size
+    if (rc) return rc;
     HIPCHK(hipStreamSynchronize(L->stream));
     HIPCHK(hipMemcpy(L->table, r.p, L->table_bytes, hipMemcpyHostToDevice));
     r.p += L->table_bytes;
@@ -2941,7 +3132,7 @@ int qcd_lattice_state_load(qcd_lattice* L, const void* buf, size_t bytes)
     HIPCHK(hipMemcpy(L->a_rec, r.p, sizeof(double) * slice_rec_doubles(L), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(L->rec_index, idx, sizeof(idx), hipMemcpyHostToDevice));
     L->prng_counter = counter;
-    L->rng_ready = ready != 0;
+    L->rng_ready = (ready & 1u) != 0;
     L->gid_updates = gid != 0;
     L->wil_R = (int)wr;
     L->wil_T = (int)wt;
@@ -2971,16 +3162,24 @@ int qcd_write_measurements(qcd_lattice* L, const double* plq, const double* poly
 int qcd_prng_replay(qcd_lattice* L, unsigned counter, int start)
 {
     if (!L) return fail(QCD_E_ARG, "null lattice");
-    if (start != QCD_START_HOT && start != QCD_START_COLD && start != QCD_START_GID)
+    if (start != QCD_START_HOT && start != QCD_START_COLD && start != QCD_START_GID && start != QCD_START_RANDOM)
         return fail(QCD_E_ARG, "bad start type %d", start);
+    if (start == QCD_START_RANDOM && L->p.prng != QCD_PRNG_RANLUX && L->p.stream_mode != QCD_STREAM_REFERENCE)
+        return fail(QCD_E_UNSUPPORTED, "the random start reads produce() buffers: it needs QCD_STREAM_REFERENCE");
+    bool changed = false;
+    int rc = set_sizing(L, start == QCD_START_RANDOM, &changed);
+    if (rc) return rc;
     if (counter == 0) {
         L->rng_ready = false;
         L->prng_counter = 0;
         return QCD_OK;
     }
-    int rc = qcd_prng_init(L);   // counter 1: the initial produce
+    rc = qcd_prng_init(L);   // counter 1: the initial produce
     if (rc) return rc;
     uint32_t left = counter - 1;
+    // a random start's four productions are ordinary ones of the hot runs' size
+    if (start == QCD_START_RANDOM && left > 0 && left < 4)
+        return fail(QCD_E_ARG, "counter %u cannot follow a hot start (it takes 4 productions)", counter);
     const bool ranlux = (L->p.prng == QCD_PRNG_RANLUX);
     const uint32_t nb = align_up(L->n_streams, kBlock) / kBlock;
     if (start == QCD_START_HOT && left > 0) {

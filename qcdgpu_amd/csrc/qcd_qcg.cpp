@@ -286,6 +286,17 @@ bool image_params(const unsigned char* head, size_t n, qcd_params* q)
     return true;
 }
 
+// Start flags of an exact image (after the parameter fields and the
+// production counter): bit 0 streams seeded, bit 1 produce() sized as in the
+// reference's hot runs.
+bool image_start_flags(const unsigned char* head, size_t n, uint32_t* flags)
+{
+    constexpr size_t at = 8 + 4 + 14 * 4 + 8 + 4;
+    if (n < at + 4 || memcmp(head, "QCDSTAT1", 8) != 0) return false;
+    memcpy(flags, head + at, 4);
+    return true;
+}
+
 bool same_params(const qcd_params& a, const qcd_params& b)
 {
     bool s = a.sun == b.sun && a.precision == b.precision && a.storage_reals == b.storage_reals &&
@@ -347,7 +358,7 @@ int qcd_qcg_write(qcd_lattice* lat, const char* path, const qcd_run_info* info)
     h.w[1] = str_word(kPrefix, 4);
     h.w[2] = str_word(kVersion, 0);
     h.w[4] = info->init;
-    h.w[5] = info->start;
+    h.w[5] = (info->start == QCD_START_RANDOM) ? (uint32_t)QCD_START_HOT : info->start;   // model_start_hot
     h.w[6] = p.randseries;
     h.w[7] = generator_code(p.prng, p.ranlux_luxury);
     h.w[8] = rl_nskip(p.prng, p.ranlux_luxury);
@@ -533,16 +544,24 @@ int qcd_qcg_read(qcd_lattice* lat, const char* path, qcd_run_info* info)
     for (int k = 0; k < 4; ++k) fits = fits && lp.n[k] == q.n[k];
     if (!fits) return err(QCD_E_ARG, std::string(path) + " holds another lattice (group, sizes, precision, beta, "
                                                          "NHIT or generator differ)");
+    // A hot (ints = 0) run is the reference's random start unless the file's
+    // own image says it began with the Haar start (cold-size productions).
+    int start = (int)ri.start;
     if (ext) {
         std::vector<unsigned char> img((size_t)ext);
         qcd_params e;
         if (!get(fh.f, img.data(), img.size())) return err(QCD_E_ARG, "short state image");
+        uint32_t flags = 0;
+        if (start == QCD_START_HOT && !(image_start_flags(img.data(), img.size(), &flags) && !(flags & 2u)))
+            start = QCD_START_RANDOM;
         if (image_params(img.data(), img.size(), &e) && same_params(e, lp)) {
             if ((rc = qcd_lattice_state_load(lat, img.data(), img.size()))) return rc;
             if ((rc = apply_settings(lat, ri))) return rc;
             if (info) *info = ri;
             return QCD_OK;
         }
+    } else if (start == QCD_START_HOT) {
+        start = QCD_START_RANDOM;   // a reference file
     }
     // reference sections (lattice_load_state, suncl.cpp:2060-2072)
     const int N = q.sun;
@@ -583,7 +602,6 @@ int qcd_qcg_read(qcd_lattice* lat, const char* path, qcd_run_info* info)
             for (int d = 0; d < 4; ++d)
                 from_planes(N, &pl[(s * 4 + d) * 4 * planes], &links[((s * 4 + d) * N * N) * 2]);
     }
-    const int start = (int)ri.start;
     if ((rc = qcd_lattice_init(lat, start == QCD_START_GID ? QCD_START_GID : QCD_START_COLD))) return rc;
     if ((rc = qcd_lattice_import(lat, links.data()))) return rc;
     const int n = (int)ri.iter_counter;

@@ -58,6 +58,7 @@ struct Settings {
     int iter = 2, niter = 1, nav = 0, nhit = 10, nhitpar = 1;
     double beta = 6.5, phi = 0.0, omega = 0.0;
     int ints = 1;                    // 0 hot, 1 cold, 2 gid (model::convert_uint_to_start)
+    bool haar = false;               // HAAR (no reference key): INTS 0 starts from Haar links
     std::string prng = "RANLUX3";
     unsigned randseries = 54;
     int pl_level = 3;
@@ -140,6 +141,7 @@ bool apply(Settings& st, const std::string& key, const std::string& val)
     else if (key == "OMEGA") st.omega = fv;
     else if (key == "NAV") st.nav = (int)iv;
     else if (key == "INTS") st.ints = (iv == 1 || iv == 2) ? (int)iv : 0;   // anything else: hot
+    else if (key == "HAAR") st.haar = iv != 0;
     else if (key == "PRNG") st.prng = upper(val);
     else if (key == "RANDSERIES") st.randseries = (unsigned)iv;
     else if (key == "PL_LEVEL") st.pl_level = (int)iv;
@@ -293,7 +295,8 @@ void print_params(const Settings& st)
 {
     printf("GROUP=%d\nND=%d\nL1=%d\nL2=%d\nL3=%d\nL4=%d\n", st.group, st.nd, st.n[0], st.n[1], st.n[2], st.n[3]);
     printf("ITER=%d\nNITER=%d\nNAV=%d\nNHIT=%d\nNHITPAR=%d\n", st.iter, st.niter, st.nav, st.nhit, st.nhitpar);
-    printf("BETA=%.13e\nPHI=%.13e\nOMEGA=%.13e\nINTS=%d\n", st.beta, st.phi, st.omega, st.ints);
+    printf("BETA=%.13e\nPHI=%.13e\nOMEGA=%.13e\nINTS=%d\nHAAR=%d\n", st.beta, st.phi, st.omega, st.ints,
+           st.haar ? 1 : 0);
     printf("PRNG=%s\nRANDSERIES=%u\nPL_LEVEL=%d\n", st.prng.c_str(), st.randseries, st.pl_level);
     printf("GETWILSON=%d\nWILSONR=%d\nWILSONT=%d\n", st.wilson ? 1 : 0, st.wilson_r, st.wilson_t);
     int g1, g2;
@@ -385,7 +388,8 @@ int main(int argc, char** argv)
 
     qcd_lattice* lat = nullptr;
     CHECK(qcd_lattice_create(&p, &lat));
-    const int start = st.ints == 1 ? QCD_START_COLD : (st.ints == 2 ? QCD_START_GID : QCD_START_HOT);
+    const int start = st.ints == 1 ? QCD_START_COLD
+                                   : (st.ints == 2 ? QCD_START_GID : (st.haar ? QCD_START_HOT : QCD_START_RANDOM));
     const int mask = QCD_MEAS_PLAQUETTE | (st.pl_level > 0 ? QCD_MEAS_POLYAKOV : 0) | (st.wilson ? QCD_MEAS_WILSON : 0) |
                      (st.pl_level > 2 ? QCD_MEAS_POLYAKOV_SLICES : 0) |
                      (st.fmunu == 1 ? QCD_MEAS_FMUNU : (st.fmunu == 2 ? QCD_MEAS_F0MU : 0)) |
@@ -442,7 +446,7 @@ int main(int argc, char** argv)
             CHECK(qcd_lattice_init(lat, start));
         } else {
             CHECK(qcd_lattice_init(lat, start));
-            if (start != QCD_START_GID) CHECK(qcd_prng_init(lat));
+            if (start == QCD_START_COLD) CHECK(qcd_prng_init(lat));   // the random start seeds its own
         }
         CHECK(qcd_measure(lat, mask));   // measurement 0: the start configuration
         run.nav_counter = 0;
@@ -606,7 +610,7 @@ int main(int argc, char** argv)
             p.storage_reals);
     fprintf(out, " PRN generator               : %s (%s streams), random series %u\n", st.prng.c_str(),
             st.fast_streams ? "fast" : "reference", st.randseries);
-    fprintf(out, " start                       : %s\n", st.ints == 1 ? "cold" : (st.ints == 2 ? "gid" : "hot (Haar)"));
+    fprintf(out, " start                       : %s\n", st.ints == 1 ? "cold" : (st.ints == 2 ? "gid" : (st.haar ? "hot (Haar)" : "hot")));
     fprintf(out, " NAV %d, ITER %d, NITER %d, NHIT %d, NOR %d\n", st.nav, st.iter, st.niter, st.nhit, st.nor);
     if (st.fmunu) fprintf(out, " %s(%d, %d)\n", st.fmunu == 1 ? "FMUNU" : "F0MU", fgen[0], fgen[1]);
     // model_make_header (suncl.cpp:789-791)

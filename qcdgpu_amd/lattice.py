@@ -15,7 +15,7 @@ import numpy as np
 from . import _abi
 from ._abi import check, lib
 
-_START = {"hot": _abi.START_HOT, "cold": _abi.START_COLD, "gid": _abi.START_GID}
+_START = {"hot": _abi.START_HOT, "cold": _abi.START_COLD, "gid": _abi.START_GID, "random": _abi.START_RANDOM}
 
 
 class Lattice:
@@ -115,8 +115,10 @@ class Lattice:
 
     # -- starts, PRNG, host copies -----------------------------------------
     def init(self, start="cold"):
-        code = {"hot": _abi.START_HOT, "cold": _abi.START_COLD, "gid": _abi.START_GID}[start]
-        check(lib().qcd_lattice_init(self._h, code))
+        """start: "cold", "gid", "hot" (Haar links from the streams; call
+        prng_init first) or "random" (the reference's hot start, which seeds
+        the streams itself, suncl.cpp:5234-5248)."""
+        check(lib().qcd_lattice_init(self._h, _START[start]))
 
     def prng_init(self):
         check(lib().qcd_prng_init(self._h))
@@ -413,7 +415,7 @@ class Lattice:
             self.init(start)
         else:
             self.init(start)
-            if start != "gid":
+            if start not in ("gid", "random"):   # the random start seeds the streams itself
                 self.prng_init()
         self.reset_measurements()
         self.measure(wilson=wilson, polyakov_slices=slices, fmunu=fmunu, action_slices=action_slices)

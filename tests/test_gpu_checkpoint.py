@@ -42,7 +42,7 @@ def _start(lat, start):
         lat.init("hot")
     else:
         lat.init(start)
-        if start != "gid":
+        if start == "cold":
             lat.prng_init()
     lat.reset_measurements()
 
@@ -110,7 +110,7 @@ def test_state_image_refuses_another_lattice():
 
 
 @pytest.mark.parametrize("prng", ["XOR128", "PM", "RANLUX1"])
-@pytest.mark.parametrize("start", ["cold", "hot"])
+@pytest.mark.parametrize("start", ["cold", "hot", "random"])
 def test_prng_replay_matches_run(prng, start):
     """Streams re-seeded and advanced to a run's production counter equal
     that run's streams; 6^3 x 4 has 432 links per half-step, so the last
@@ -125,7 +125,7 @@ def test_prng_replay_matches_run(prng, start):
             a.sweep("or")
         a.halfstep(1, 0, "hb")
         c = a.prng_counter()
-        assert c == 1 + (4 if start == "hot" else 0) + 8 * 3 + 1
+        assert c == 1 + (0 if start == "cold" else 4) + 8 * 3 + 1
         b.prng_replay(c, start)
         assert b.prng_counter() == c
         ia, ib = a.save_state(), b.save_state()
@@ -216,9 +216,9 @@ def test_qcg_resume_exact(tmp_path, cfg):
 
 
 FALLBACK = [
-    (dict(sun=3, prec="fp32", storage=12, prng="XOR128"), "hot"),
+    (dict(sun=3, prec="fp32", storage=12, prng="XOR128"), "random"),
     (dict(sun=2, prec="fp64", storage=4, prng="PM"), "cold"),
-    (dict(sun=3, prec="fp64", storage=12, prng="RANLUX3"), "hot"),
+    (dict(sun=3, prec="fp64", storage=12, prng="RANLUX3"), "random"),
 ]
 
 
@@ -226,8 +226,10 @@ FALLBACK = [
 def test_qcg_resume_from_reference_sections(tmp_path, cfg, start):
     """A file with only the reference's sections (the extension cut off):
     table, plaquette / Polyakov (Re, Im) / Wilson sums and the production
-    counter, with the streams replayed.  Two-row storage holds exactly what
-    the file holds, so the continued run is bit-identical."""
+    counter, with the streams replayed (a hot, ints = 0, file is the
+    reference's random start, replayed with the hot runs' produce() size).
+    Two-row storage holds exactly what the file holds, so the continued run
+    is bit-identical."""
     _case("qcg resume sections %s %s" % (cfg, start))
     full, cut = tmp_path / "f.qcg", tmp_path / "c.qcg"
     with _lat(**cfg) as a, _lat(**cfg) as b:
@@ -251,6 +253,24 @@ def test_qcg_resume_from_reference_sections(tmp_path, cfg, start):
         assert np.array_equal(qa[:, :2], qb[:, :2])
         assert np.array_equal(qa[3:], qb[3:]) and not np.any(qb[:3, 2:])   # P2, P4 are not in the file
         assert np.array_equal(a.wilson_measurements(raw=True), b.wilson_measurements(raw=True))
+
+
+@pytest.mark.parametrize("start", ["hot", "random"])
+def test_qcg_sections_replay_follows_the_files_start(tmp_path, start):
+    """A file whose image does not fit the lattice (another storage) resumes
+    from its sections; the image still tells how a hot (ints = 0) run began:
+    the Haar start (cold-size productions) or the random start (hot-size
+    productions), and the streams are replayed accordingly."""
+    _case("qcg sections replay %s" % start)
+    path = tmp_path / "s.qcg"
+    with _lat(prng="PM", prec="fp64", storage=18) as a, _lat(prng="PM", prec="fp64", storage=12) as b:
+        _start(a, start)
+        _cycles(a, 2, wilson=False)
+        a.save_qcg(path, start=start, iter=16, iter_counter=3)
+        b.load_qcg(path)
+        assert b.prng_counter() == a.prng_counter()
+        ia, ib = a.save_state(), b.save_state()
+        assert np.array_equal(ia[_rng_slice(a, ia)], ib[_rng_slice(b, ib)])
 
 
 def _rows(path):

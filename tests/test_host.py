@@ -143,7 +143,7 @@ def test_host_run_matches_python_driver(tmp_path, prng, ints, fkeys):
         lat.set_wilson(2, 2)
         lat.set_fmunu(g1, g2)
         lat.set_twist(*twist)
-        m = lat.simulate(nav=1, iters=4, niter=2, n_or=2, start={0: "hot", 1: "cold"}[ints], wilson=True,
+        m = lat.simulate(nav=1, iters=4, niter=2, n_or=2, start={0: "random", 1: "cold"}[ints], wilson=True,
                          slices=True, fmunu="E" if efield else "H", action_slices="ACTIONSDIFF" in fkeys)
         asl = lat.action_slices()
         wl = lat.wilson_measurements()
