@@ -570,13 +570,19 @@ __device__ __forceinline__ void update3(const UpdArgs& a, uint32_t blk, uint32_t
     if constexpr (kStream) {
         if ((w == 0 && active) || (w > 0 && a.s_quads)) rs = a.rng[i];
     }
-    // XOR128 jump table (8 KiB) staged in LDS by the whole workgroup; the
-    // barrier below publishes it.
+    // XOR128 jump table (8 KiB) staged in LDS by the whole workgroup with
+    // direct global -> LDS loads (global_load_lds_dwordx4: lane l of a
+    // wavefront writes 16 bytes at the wave-uniform LDS base + 16 l), so no
+    // register holds it and no wait for it comes before the staple gathers
+    // are issued; the vmcnt wait and the barrier below publish it.
     __shared__ uint4 jtab[RK == RK_XOR128 ? kJumpTab : 1];
     if constexpr (RK == RK_XOR128) {
+        static_assert(kJumpTab % 64 == 0, "jump table in whole 64-entry chunks");
         if (a.s_quads) {
             const uint4* src = a.jump + kJumpCols;
-            for (int k = (int)threadIdx.x; k < kJumpTab; k += kUpdThreads) jtab[k] = src[k];
+            for (int c = w; c < kJumpTab / 64; c += kUpdThreads / 64)
+                __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(src + c * 64 + lane),
+                                                 (__attribute__((address_space(3))) void*)&jtab[c * 64], 16, 0, 0);
         }
     }
     M<R, N> S;
@@ -591,6 +597,7 @@ __device__ __forceinline__ void update3(const UpdArgs& a, uint32_t blk, uint32_t
                 lds[w - 1][2 * (r * N + c) + 1][lane] = S.e[r][c].im;
             }
     }
+    if constexpr (RK == RK_XOR128) __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0): the table loads landed
     __syncthreads();
     if (w > 0) {
         if constexpr (kStream) {
