@@ -3,6 +3,12 @@
 TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and
 by bench.py's cpu_baseline leg, always as the checker (or as the timed CPU
 baseline), never on the product path.
+
+Besides the wrappers, three restatements are written here in numpy, each
+pinned by its own tests: the reference's hot start (di_pierro, random_start,
+produce_quads; tests/test_oracle_random_start.py), the twisted boundary
+(twist_matrix, twisted_view, halfstep_twisted; tests/test_oracle_tbc.py) and
+the per-slice action sums (action_slices; tests/test_oracle_action_slices.py).
 """
 import ctypes
 import os

@@ -46,9 +46,16 @@ def test_reference_init_syntax(tmp_path):
     )
     assert rc == 0
     assert (kv["L1"], kv["L2"], kv["L3"], kv["L4"]) == ("16", "12", "12", "8")
-    assert kv["GROUP"] == "2" and float(kv["BETA"]) == 2.3 and kv["INTS"] == "0"
+    assert kv["GROUP"] == "2" and float(kv["BETA"]) == 2.3 and kv["INTS"] == "0" and kv["HAAR"] == "0"
     assert kv["PRNG"] == "RANLUX2" and kv["RANDSERIES"] == "7" and kv["ITER"] == "50"
     assert kv["PRECISION"] == "SINGLE" and kv["NOR"] == "3" and kv["STORAGE"] == "4"
+
+
+def test_haar_key(tmp_path):
+    """INTS = 0 is the reference's hot start; HAAR = 1 (a key of this build)
+    asks for the Haar start instead."""
+    rc, kv = _params("INTS = 0\nHAAR = 1\n", tmp_path)
+    assert rc == 0 and kv["INTS"] == "0" and kv["HAAR"] == "1"
 
 
 @pytest.mark.parametrize(
@@ -94,7 +101,8 @@ def test_bad_precision_value(tmp_path):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("prng,ints,fkeys", [("XOR128", 1, ""), ("RANLUX3", 0, "F0MU = 1\nFMUNU1 = 1\nFMUNU7 = 1\n"),
-                                             ("PM", 0, "PHI = 3.0\nOMEGA = 0.5\nACTIONSDIFF = 1\n")])
+                                             ("PM", 0, "PHI = 3.0\nOMEGA = 0.5\nACTIONSDIFF = 1\n"),
+                                             ("XOR128", 0, "HAAR = 1\n")])
 def test_host_run_matches_python_driver(tmp_path, prng, ints, fkeys):
     """qcdgpu_hip on a small lattice writes the same plaquette, Wilson loop,
     Polyakov and F_munu series as qcdgpu_amd.Lattice.simulate with the same
@@ -143,7 +151,8 @@ def test_host_run_matches_python_driver(tmp_path, prng, ints, fkeys):
         lat.set_wilson(2, 2)
         lat.set_fmunu(g1, g2)
         lat.set_twist(*twist)
-        m = lat.simulate(nav=1, iters=4, niter=2, n_or=2, start={0: "random", 1: "cold"}[ints], wilson=True,
+        start = "hot" if "HAAR" in fkeys else {0: "random", 1: "cold"}[ints]
+        m = lat.simulate(nav=1, iters=4, niter=2, n_or=2, start=start, wilson=True,
                          slices=True, fmunu="E" if efield else "H", action_slices="ACTIONSDIFF" in fkeys)
         asl = lat.action_slices()
         wl = lat.wilson_measurements()
