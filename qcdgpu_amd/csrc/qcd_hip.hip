@@ -1151,12 +1151,14 @@ __device__ __forceinline__ void finalize_body(const FinArgs& f)
     if (f.fused || f.plq) {
         double sp = 0.0, tm = 0.0;
         if (f.fused) {
+#pragma unroll 8
             for (uint32_t j = t; j < f.n_fused; j += kBlock) {
                 sp += f.fused[j];
                 sp += f.fused[f.n_fused + j];
                 tm += f.fused[2 * f.n_fused + j];
             }
         } else {
+#pragma unroll 8
             for (uint32_t j = t; j < f.n_plq; j += kBlock) {
                 sp += f.plq[2 * j];
                 tm += f.plq[2 * j + 1];
@@ -1188,6 +1190,7 @@ __device__ __forceinline__ void finalize_body(const FinArgs& f)
     }
     if (f.poly) {
         double v[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 8
         for (uint32_t j = t; j < f.n_poly; j += kBlock)
 #pragma unroll
             for (int k = 0; k < 4; ++k) v[k] += f.poly[4 * j + k];
@@ -1426,23 +1429,28 @@ __global__ __launch_bounds__(kBlock) void k_wilson(void* table, Geom g, int wr, 
     block_partial<1>(v, partials, blockIdx.x);
 }
 
-// Polyakov-loop body for workgroup `blk` (8 lanes per spatial site).
+// Polyakov-loop body for workgroup `blk`: 8 threads per spatial site, thread
+// k taking t-slot l = k / 32 (the links t in [l c, (l+1) c), c = ceil(N4/8))
+// of site j = 32 blk + k % 32, so each wavefront's loads cover 32
+// consecutive sites (whole cache lines) at two t-slots.  The ordered product
+// over the slots is formed through LDS as ((P0 P1)(P2 P3))((P4 P5)(P6 P7)).
+constexpr int kPolySites = kBlock / 8;
 template <typename R, int N, int NREAL>
 __device__ __forceinline__ void polyakov_body(void* table, const Geom& g, double* partials, uint32_t blk,
                                               double* site_tr = nullptr)
 {
-    const uint32_t tid = blk * kBlock + threadIdx.x;
-    const uint32_t j = tid >> 3;
-    const int l8 = (int)(tid & 7u);
+    __shared__ R pm[8][2 * N * N][kPolySites];
+    const int ks = (int)(threadIdx.x % kPolySites);
+    const int l8 = (int)(threadIdx.x / kPolySites);
+    const uint32_t j = blk * kPolySites + (uint32_t)ks;
     const uint32_t vs = (uint32_t)g.n1 * g.n2 * g.n3;
     Table<R, N, NREAL> T{(typename Vec2<R>::T*)table, g.vh_pad};
     M<R, N> P;
     m_unit(P);
-    int x = 0, y = 0, z = 0;
     if (j < vs) {
-        y = (int)(j % (uint32_t)g.n2);
-        z = (int)((j / (uint32_t)g.n2) % (uint32_t)g.n3);
-        x = (int)(j / ((uint32_t)g.n2 * g.n3));
+        const int y = (int)(j % (uint32_t)g.n2);
+        const int z = (int)((j / (uint32_t)g.n2) % (uint32_t)g.n3);
+        const int x = (int)(j / ((uint32_t)g.n2 * g.n3));
         const int chunk = (g.n4 + 7) / 8;
         const int t0 = l8 * chunk;
         const int t1 = min(t0 + chunk, g.n4);
@@ -1451,10 +1459,32 @@ __device__ __forceinline__ void polyakov_body(void* table, const Geom& g, double
             P = (t == t0) ? L : m_mul(P, L);
         }
     }
+    auto put = [&](const M<R, N>& a) {
+#pragma unroll
+        for (int r = 0; r < N; ++r)
+#pragma unroll
+            for (int c = 0; c < N; ++c) {
+                pm[l8][2 * (r * N + c)][ks] = a.e[r][c].re;
+                pm[l8][2 * (r * N + c) + 1][ks] = a.e[r][c].im;
+            }
+    };
+    put(P);
+    __syncthreads();
 #pragma unroll
     for (int d = 1; d < 8; d <<= 1) {
-        const M<R, N> Q = shfl_down_m(P, d);
-        if ((l8 & (2 * d - 1)) == 0) P = m_mul(P, Q);
+        // slot l (a multiple of 2d) takes the product slot l + d formed at
+        // the previous level; no slot read here is written here
+        if ((l8 & (2 * d - 1)) == 0) {
+            M<R, N> Q;
+#pragma unroll
+            for (int r = 0; r < N; ++r)
+#pragma unroll
+                for (int c = 0; c < N; ++c)
+                    Q.e[r][c] = {pm[l8 + d][2 * (r * N + c)][ks], pm[l8 + d][2 * (r * N + c) + 1][ks]};
+            P = m_mul(P, Q);
+            if (d < 4) put(P);
+        }
+        __syncthreads();
     }
     double v[4] = {0.0, 0.0, 0.0, 0.0};
     if (j < vs && l8 == 0) {
