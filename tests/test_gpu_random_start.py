@@ -70,6 +70,8 @@ CASES = [
     (3, (6, 4, 2, 4), "fp32", 18, "XOR128", 54),
     (3, (4, 4, 4, 4), "fp32", 12, "PM", 9),
     (2, (4, 4, 4, 4), "fp32", 4, "XOR128", 3),
+    (3, (8, 8, 8, 8), "fp64", 18, "XOR128", 11),
+    (3, (6, 4, 2, 4), "fp32", 12, "RANLUX3", 21),
 ]
 IDS = ["su%d-%s-%s-%d-%s" % (c[0], "x".join(map(str, c[1])), c[2], c[3], c[4]) for c in CASES]
 
