@@ -541,7 +541,10 @@ __device__ __forceinline__ uint32_t upd_block(const UpdArgs& a)
 // Body of the three-wavefront update for 64-link unit `blk` of a lattice
 // with `nunits` units (the fused-plaquette partial of the unit goes to
 // partials[(meas_mode - 1) * nunits + blk]).
-template <typename R, int N, int NREAL, bool OVER, int RK, bool MEAS, bool TW = false>
+// MU >= 0: the direction is a compile-time constant (the neighbour steps
+// and plane bases fold away, shortening the scalar prologue before the
+// gathers); MU = -1 reads it from the arguments.
+template <typename R, int N, int NREAL, bool OVER, int RK, bool MEAS, bool TW = false, int MU = -1>
 __device__ __forceinline__ void update3(const UpdArgs& a, uint32_t blk, uint32_t nunits)
 {
     using T = R;   // compute type of the SU(2) subgroup arithmetic
@@ -554,7 +557,7 @@ __device__ __forceinline__ void update3(const UpdArgs& a, uint32_t blk, uint32_t
     const uint32_t i = blk * kLinks + lane;
     const bool active = i < a.g.vh;
     Table<R, N, NREAL> Tb{(typename Vec2<R>::T*)a.table, a.g.vh_pad};
-    const int p = a.parity, mu = a.mu;
+    const int p = a.parity, mu = (MU >= 0) ? MU : a.mu;
     const int nu = w + (w >= mu ? 1 : 0);
     const Site s = site_of(a.g, active ? i : 0u, p);
 
@@ -646,10 +649,10 @@ __device__ __forceinline__ void update3(const UpdArgs& a, uint32_t blk, uint32_t
     }
 }
 
-template <typename R, int N, int NREAL, bool OVER, int RK, bool MEAS, bool TW = false>
+template <typename R, int N, int NREAL, bool OVER, int RK, bool MEAS, bool TW = false, int MU = -1>
 __global__ __launch_bounds__(kUpdThreads) void k_update(UpdArgs a)
 {
-    update3<R, N, NREAL, OVER, RK, MEAS, TW>(a, upd_block(a), gridDim.x);
+    update3<R, N, NREAL, OVER, RK, MEAS, TW, MU>(a, upd_block(a), gridDim.x);
 }
 
 // One wavefront per 64 links: each wavefront gathers and sums the three
@@ -1770,8 +1773,18 @@ void launch_update_t(qcd_lattice* L, const UpdArgs& a, const LaunchOpt& o)
             launch(k_update1<R, N, NREAL, OVER, RK, false>, dim3(grid), dim3(kWaves1 * kLinks), L->stream, o, a, nb);
         return;
     }
-    if (o.meas_mode) launch(k_update<R, N, NREAL, OVER, RK, true>, dim3(nb), dim3(kUpdThreads), L->stream, o, a);
-    else launch(k_update<R, N, NREAL, OVER, RK, false>, dim3(nb), dim3(kUpdThreads), L->stream, o, a);
+    if (o.meas_mode) {
+        launch(k_update<R, N, NREAL, OVER, RK, true>, dim3(nb), dim3(kUpdThreads), L->stream, o, a);
+        return;
+    }
+    // the plain half-steps, most launches of a cycle: one instance per direction
+    const dim3 grid(nb), block(kUpdThreads);
+    switch (a.mu) {
+        case 0: launch(k_update<R, N, NREAL, OVER, RK, false, false, 0>, grid, block, L->stream, o, a); break;
+        case 1: launch(k_update<R, N, NREAL, OVER, RK, false, false, 1>, grid, block, L->stream, o, a); break;
+        case 2: launch(k_update<R, N, NREAL, OVER, RK, false, false, 2>, grid, block, L->stream, o, a); break;
+        default: launch(k_update<R, N, NREAL, OVER, RK, false, false, 3>, grid, block, L->stream, o, a); break;
+    }
 }
 
 template <typename R, int N, int NREAL>
