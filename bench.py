@@ -128,6 +128,8 @@ def main():
     ap.add_argument("--storage", type=int, default=18, choices=(18, 12))
     ap.add_argument("--stream", default="reference", choices=("reference", "fast"))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--xcd-map", type=int, default=None, choices=(0, 1, 2, 3),
+                    help="dispatch order of the update kernels (default: the library's)")
     ap.add_argument("--profile-steps", type=int, default=50)
     ap.add_argument(
         "--copy-calibration",
@@ -171,6 +173,8 @@ def main():
         randseries=54,
         max_measurements=args.steps + args.warmup + args.profile_steps + 8,
     )
+    if args.xcd_map is not None:
+        lat.tune(xcd_map=args.xcd_map)
     lat.init("cold")
     lat.prng_init()
     lat.capture_cycle(n_hb=1, n_or=N_OR, plaquette=True, polyakov=True)

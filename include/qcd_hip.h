@@ -110,12 +110,23 @@ int         qcd_abi_version(void);
 int  qcd_lattice_create(const qcd_params *params, qcd_lattice **out);
 int  qcd_lattice_destroy(qcd_lattice *lat);
 int  qcd_lattice_sync(qcd_lattice *lat);
-/* Launch-shape knobs of the update kernels (set by qcdgpu_amd.autotune, which
-   times candidates on the device; both only change scheduling, never results):
-   QCD_TUNE_XCD_MAP       1: each of the 8 XCDs gets one contiguous range of
-                          compact site indices, an x-slab (default);
-                          2: the same slabs, dispatched cross-section first
-                          (x, then t, then z) for L2 reuse; 0: hardware order.
+/* Launch-shape knobs of the update kernels.  They only change scheduling,
+   never results (the 64-link units of a half-step are independent):
+   QCD_TUNE_XCD_MAP       which 64-link units each of the 8 XCDs (hardware
+                          workgroup b runs on XCD b mod 8) updates:
+                          0: hardware order;
+                          1: one contiguous range of compact site indices
+                          per XCD, an x-slab;
+                          2 (default): the same x-slabs, dispatched
+                          cross-section first (x offset, then t, then z)
+                          for L2 reuse;
+                          3: each XCD owns a block of x-values and z-rows,
+                          N1/ax x N3/az with ax*az = 8, chosen for the
+                          fewest halo links (1/x-extent + 1/z-extent),
+                          dispatched cross-section first (slower than 2 on
+                          MI355X at 32^3 x 8: 0.763 vs 0.744 ms per sweep).
+                          Modes 2 and 3 need N2/2 to divide 64 and whole
+                          z-groups and units per XCD; otherwise mode 1 runs.
    QCD_TUNE_BLOCKS_PER_CU cap on resident update workgroups per CU (reserved
                           LDS), 0 = no cap (default).
    QCD_TUNE_UPDATE_SHAPE  3: three wavefronts share 64 links, one staple
@@ -123,6 +134,12 @@ int  qcd_lattice_sync(qcd_lattice *lat);
                           links, workgroups of four.                         */
 enum { QCD_TUNE_XCD_MAP = 1, QCD_TUNE_BLOCKS_PER_CU = 2, QCD_TUNE_UPDATE_SHAPE = 3 };
 int  qcd_lattice_tune(qcd_lattice *lat, int knob, int value);
+/* The value a knob takes in the update launches of this lattice: for
+   QCD_TUNE_XCD_MAP the mode that applies after the fallbacks above, and in
+   detail[0..1] (optional) the XCD block's x-values and z-rows (0 for modes
+   0 and 1).  No counterpart in the reference (its work-group order is the
+   OpenCL runtime's).                                                        */
+int  qcd_lattice_get_tune(const qcd_lattice *lat, int knob, int *value, int *detail);
 /* Bytes of device memory held by the link table. */
 int  qcd_lattice_table_bytes(const qcd_lattice *lat, size_t *bytes);
 
@@ -418,6 +435,7 @@ int  qcd_flow_evaluate(qcd_flow *flow, const float *links, int batch, double bet
         used for RANLUX).                                                     */
 int  qcd_prng_produce_host(int prng, int ranlux_luxury, unsigned n_instances, unsigned samples,
                            void *table, float *quads);
+
 
 #ifdef __cplusplus
 }

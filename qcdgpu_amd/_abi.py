@@ -84,6 +84,7 @@ _SIGNATURES = {
     "qcd_lattice_sync": [_P],
     "qcd_lattice_table_bytes": [_P, ctypes.POINTER(ctypes.c_size_t)],
     "qcd_lattice_tune": [_P, ctypes.c_int, ctypes.c_int],
+    "qcd_lattice_get_tune": [_P, ctypes.c_int, _I, _I],
     "qcd_lattice_init": [_P, ctypes.c_int],
     "qcd_prng_init": [_P],
     "qcd_lattice_import": [_P, _D],

@@ -20,7 +20,7 @@ HIPCC_FLAGS = [
 ]
 
 
-HOST_SRC = os.path.join(PKG, "host", "qcdgpu_main.cpp")
+HOST_SRC = [os.path.join(PKG, "host", n) for n in ("qcdgpu_main.cpp", "host_check.cpp")]
 HOST_BIN = os.path.join(PKG, "qcdgpu_hip")
 
 
@@ -29,7 +29,7 @@ def build_host(verbose=False):
     library it loads through an $ORIGIN run path."""
     cxx = os.environ.get("CXX", "g++")
     tmp = HOST_BIN + ".tmp"
-    cmd = [cxx, "-O2", "-std=c++17", "-Wall", "-o", tmp, HOST_SRC, "-L" + PKG, "-lqcdhip",
+    cmd = [cxx, "-O2", "-std=c++17", "-Wall", "-o", tmp] + HOST_SRC + ["-L" + PKG, "-lqcdhip",
            "-Wl,-rpath,$ORIGIN", "-L/opt/rocm/lib", "-Wl,-rpath-link,/opt/rocm/lib"]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:

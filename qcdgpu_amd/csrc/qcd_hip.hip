@@ -72,10 +72,26 @@ struct UpdArgs {
     // ReTr(U' * sum_{nu<mu} S_nu); modes 1, 2, 3 = even Y, Z, T half-steps.
     double* partials;
     int meas_mode;
-    int xcd_map;          // 0: hardware order, 1: one x-slab per XCD, 2: 1 + cross-section-first order
-    uint32_t ord_ng, ord_nt, ord_xs;   // xcd_map 2: z-groups, N4, x-values per XCD slab
+    int xcd_map;          // 0: hardware order, 1: one x-slab per XCD, >= 2: blocks of (x, z) per XCD
+    // xcd_map >= 2: z-groups of the lattice, N4, x-values per XCD block,
+    // XCD blocks along x, z-groups per XCD block
+    uint32_t ord_ng, ord_nt, ord_xs, ord_ax, ord_zr;
     Twist tw;             // twisted boundary (read only by the TW instantiations)
 };
+
+// 64-link unit handled by the l-th workgroup (or wavefront) dispatched to
+// XCD k, for xcd_map >= 2.  A unit is a group of rb = 64 / (N2 / 2) whole
+// z-rows, so unit u = zg + ng (t + N4 x).  XCD k = kx + ax kz owns the block
+// x in [kx xs, (kx+1) xs), z-groups [kz zr, (kz+1) zr); within it the units
+// go x offset fastest, then t, then z-group, so a unit's x, t and z
+// neighbours are dispatched close to it.
+__device__ __forceinline__ uint32_t block_unit(const UpdArgs& a, uint32_t k, uint32_t l)
+{
+    const uint32_t kx = k % a.ord_ax, kz = k / a.ord_ax;
+    const uint32_t c = l % a.ord_xs, r = l / a.ord_xs;
+    const uint32_t t = r % a.ord_nt, zl = r / a.ord_nt;
+    return kz * a.ord_zr + zl + a.ord_ng * (t + a.ord_nt * (kx * a.ord_xs + c));
+}
 
 // Hardware dispatches workgroup b to XCD (b mod 8).  Remap so that each XCD
 // works on one contiguous range of compact indices (an x-slab of the
@@ -418,19 +434,19 @@ __device__ __forceinline__ double wave_sum(double v)
 constexpr int kLinks = 64;
 constexpr int kUpdThreads = 3 * kLinks;
 
-// The link update of one active lane: load U, the three SU(2) subgroups
-// (heatbath or overrelaxation; SU(2): one), FP64 reunitarisation, store.
-// Returns ReTr(U' * Alow) for the fused plaquette (0 unless MEAS).  `rs` is
-// the generator state at the start of the half-step; in the fast stream
-// layout the state after consumption is written to rng_out.
+// The link update of one active lane: U (the link as loaded), the three
+// SU(2) subgroups (heatbath or overrelaxation; SU(2): one), FP64
+// reunitarisation, store.  Returns ReTr(U' * Alow) for the fused plaquette
+// (0 unless MEAS).  `rs` is the generator state at the start of the
+// half-step; in the fast stream layout the state after consumption is
+// written to rng_out.
 template <typename R, int N, int NREAL, bool OVER, int RK, bool MEAS, bool TW = false>
 __device__ __forceinline__ double update_link(const UpdArgs& a, const Table<R, N, NREAL>& Tb, uint32_t i, const Site& s,
-                                              const M<R, N>& A, const M<R, N>& Alow, uint4 rs)
+                                              const M<R, N>& A, const M<R, N>& Alow, uint4 rs, M<R, N> U)
 {
     using T = R;
     const int p = a.parity, mu = a.mu;
     double trace = 0.0;
-    M<R, N> U = Tb.load_u(p, mu, i);
     if constexpr (TW) tbc(U, a.tw, mu, i);   // stored back as updated, the twist kept (as the reference)
     SrcGid sg{};
     SrcXor sx{};
@@ -520,21 +536,15 @@ __device__ __forceinline__ double update_link(const UpdArgs& a, const Table<R, N
 
 // Workgroup -> 64-link block.  Mode 1: XCD k (hardware block b goes to XCD
 // b mod 8) gets the k-th contiguous eighth of the compact index range, an
-// x-slab of the lattice.  Mode 2: the same slabs, but within a slab blocks
-// are dispatched cross-section first (x offset fastest, then t, then the
-// group of z rows a block covers), so a block's x, t and z neighbours were
-// dispatched at most one cross-section (N4 * x-values blocks) earlier and
-// their links are still in the XCD's L2.
+// x-slab of the lattice.  Modes 2 and 3: each XCD gets one block of x-values
+// and z-groups (block_unit; mode 2 the x-slabs of mode 1, mode 3 the block
+// shape with the fewest halo links), dispatched cross-section first.
 __device__ __forceinline__ uint32_t upd_block(const UpdArgs& a)
 {
     const uint32_t b = blockIdx.x, nb = gridDim.x;
     if (a.xcd_map == 0 || (nb % 8u)) return b;
     const uint32_t per = nb / 8u, k = b % 8u, l = b / 8u;
-    if (a.xcd_map == 2) {
-        const uint32_t c = l % a.ord_xs, r = l / a.ord_xs;
-        const uint32_t t = r % a.ord_nt, zg = r / a.ord_nt;
-        return k * per + zg + a.ord_ng * (t + a.ord_nt * c);
-    }
+    if (a.xcd_map >= 2) return block_unit(a, k, l);
     return k * per + l;
 }
 
@@ -591,6 +601,11 @@ __device__ __forceinline__ void update3(const UpdArgs& a, uint32_t blk, uint32_t
     M<R, N> S;
     if (active) S = staple_nu<R, N, NREAL, TW>(Tb, a.g, s, i, p, mu, nu, a.tw);
     else m_zero(S);
+    // Wavefront 0 issues the load of the link it updates before the
+    // barrier, so its latency overlaps the wait for the other shares rather
+    // than following it.
+    M<R, N> U;
+    if (w == 0 && active) U = Tb.load_u(p, mu, i);
     if (w > 0) {
 #pragma unroll
         for (int r = 0; r < N; ++r)
@@ -640,7 +655,7 @@ __device__ __forceinline__ void update3(const UpdArgs& a, uint32_t blk, uint32_t
     m_acc(A, S2);
 
     double trace = 0.0;
-    if (active) trace = update_link<R, N, NREAL, OVER, RK, MEAS, TW>(a, Tb, i, s, A, Alow, rs);
+    if (active) trace = update_link<R, N, NREAL, OVER, RK, MEAS, TW>(a, Tb, i, s, A, Alow, rs, U);
     if constexpr (MEAS) {
         trace = wave_sum(trace);
         // slot = 64-link unit, so the final sum runs in lattice order
@@ -666,11 +681,7 @@ __device__ __forceinline__ uint32_t upd_unit1(const UpdArgs& a, int w)
     const uint32_t b = blockIdx.x, nb = gridDim.x;
     if (a.xcd_map == 0 || (nb % 8u)) return b * kWaves1 + (uint32_t)w;
     const uint32_t per = nb * kWaves1 / 8u, k = b % 8u, l = (b / 8u) * kWaves1 + (uint32_t)w;
-    if (a.xcd_map == 2) {
-        const uint32_t c = l % a.ord_xs, r = l / a.ord_xs;
-        const uint32_t t = r % a.ord_nt, zg = r / a.ord_nt;
-        return k * per + zg + a.ord_ng * (t + a.ord_nt * c);
-    }
+    if (a.xcd_map >= 2) return block_unit(a, k, l);
     return k * per + l;
 }
 
@@ -704,7 +715,7 @@ __device__ __forceinline__ void update1(const UpdArgs& a, uint32_t unit, uint32_
         }
     }
     double trace = 0.0;
-    if (active) trace = update_link<R, N, NREAL, OVER, RK, MEAS>(a, Tb, i, s, A, Alow, rs);
+    if (active) trace = update_link<R, N, NREAL, OVER, RK, MEAS>(a, Tb, i, s, A, Alow, rs, Tb.load_u(p, mu, i));
     if constexpr (kStream) {
         // Reference stream layout: every stream advances by exactly s_quads
         // quads from its start state.
@@ -1516,6 +1527,26 @@ __global__ __launch_bounds__(kBlock) void k_polyakov(void* table, Geom g, double
     polyakov_body<R, N, NREAL>(table, g, partials, blockIdx.x, site_tr);
 }
 
+// The Polyakov loop and the measurement's final sums in one launch: every
+// workgroup writes its partial, and the workgroup that finishes last (a
+// device-scope counter, which that workgroup resets for the next launch)
+// sums all partials in index order exactly as k_finalize does, so the
+// records are the same bit for bit and one launch per measurement is saved.
+template <typename R, int N, int NREAL>
+__global__ __launch_bounds__(kBlock) void k_polyakov_fin(void* table, Geom g, double* partials, FinArgs f, int* done)
+{
+    polyakov_body<R, N, NREAL>(table, g, partials, blockIdx.x, nullptr);
+    __shared__ int last;
+    __threadfence();   // release: this workgroup's partial before its count
+    __syncthreads();
+    if (threadIdx.x == 0) last = (atomicAdd(done, 1) == (int)gridDim.x - 1) ? 1 : 0;
+    __syncthreads();
+    if (!last) return;
+    __threadfence();   // acquire: every other workgroup's partial
+    finalize_body(f);
+    if (threadIdx.x == 0) *done = 0;
+}
+
 // Sums per x, y and z slice of a per-site pair of values, in a fixed order:
 // workgroup b takes slice x = b (b < N1), y = b - N1, or z = b - N1 - N2 and
 // writes record rec_index[slot] at position b.  Sites are indexed
@@ -1717,6 +1748,8 @@ namespace {
 thread_local char g_err[512] = "";
 int g_device = -1;
 
+
+
 int fail(int code, const char* fmt, ...)
 {
     va_list ap;
@@ -1755,6 +1788,58 @@ void launch(Kern k, dim3 grid, dim3 block, hipStream_t s, const LaunchOpt& o, Ar
 {
     if (o.e0) hipExtLaunchKernelGGL(k, grid, block, o.lds, s, o.e0, o.e1, 0, args...);
     else hipLaunchKernelGGL(k, grid, block, o.lds, s, args...);
+}
+
+// The unit -> XCD map the update kernels use when `want` is asked for;
+// sets a.xcd_map to the mode that applies and the block parameters.
+// Modes 2 and 3 need whole z-rows per 64-link unit (N2 / 2 divides 64),
+// whole z-groups and whole units per XCD block; mode 2 takes x-slabs of
+// N1 / 8 x-values, mode 3 the block (N1 / ax) x (N3 / az) z-rows,
+// ax az = 8, with the least halo per link: an update reads the x and z
+// neighbours of its links, so an XCD that owns a block fetches its halo
+// slices once more than the compulsory traffic, about (1 / x-extent +
+// 1 / z-extent) of the table for a block that is cut along both
+// (periodic full extents have no halo).  A mode that does not apply falls
+// back to mode 1 (and to 0 when shape1 needs whole workgroups of four
+// units per XCD and does not get them).
+void dispatch_order(const qcd_lattice* L, int want, bool shape1, UpdArgs& a)
+{
+    a.xcd_map = want;
+    a.ord_ng = a.ord_nt = a.ord_xs = a.ord_ax = a.ord_zr = 1;
+    if (want >= 2) {
+        const uint32_t h = (uint32_t)L->g.h, nb = L->g.vh / kLinks;
+        const uint32_t rb = (h <= kLinks && kLinks % h == 0) ? kLinks / h : 0;
+        bool ok = rb && (L->g.vh % kLinks == 0) && (nb % 8 == 0) && ((uint32_t)L->g.n3 % rb == 0);
+        if (ok) {
+            const uint32_t ng = (uint32_t)L->g.n3 / rb, n1 = (uint32_t)L->g.n1;
+            double best = 1e30;
+            uint32_t bax = 0;
+            for (uint32_t ax = 8; ax >= 1; ax /= 2) {
+                const uint32_t az = 8 / ax;
+                if (n1 % ax || ng % az) continue;
+                if (want == 2 && ax != 8) continue;
+                const double cost = (ax > 1 ? (double)ax / n1 : 0.0) + (az > 1 ? (double)az / L->g.n3 : 0.0);
+                if (cost < best - 1e-12) {
+                    best = cost;
+                    bax = ax;
+                }
+            }
+            ok = bax != 0;
+            if (ok) {
+                a.ord_ng = ng;
+                a.ord_nt = (uint32_t)L->g.n4;
+                a.ord_ax = bax;
+                a.ord_xs = n1 / bax;
+                a.ord_zr = ng / (8 / bax);
+            }
+        }
+        if (!ok) a.xcd_map = 1;
+    }
+    if (shape1 && a.xcd_map != 0) {
+        // four 64-link units per workgroup: the XCD blocks need whole workgroups
+        const uint32_t nb = align_up(L->g.vh, kLinks) / kLinks;
+        if (nb % (8u * kWaves1)) a.xcd_map = 0;
+    }
 }
 
 template <typename R, int N, int NREAL, bool OVER, int RK>
@@ -1807,31 +1892,8 @@ int launch_update_rn(qcd_lattice* L, int parity, int mu, int kind, int start_gid
     a.s_quads = (L->p.stream_mode == QCD_STREAM_REFERENCE) ? L->s_quads : 0u;
     a.partials = L->part_fused;
     a.meas_mode = o.meas_mode;
-    a.xcd_map = L->tune_xcd_map;
-    a.ord_ng = a.ord_nt = a.ord_xs = 1;
     a.tw = L->tw;
-    if (a.xcd_map == 2) {
-        // Cross-section order needs whole y-rows per block, whole z-groups
-        // and whole x-slabs per XCD; otherwise fall back to mode 1.
-        const uint32_t h = (uint32_t)L->g.h, nb = L->g.vh / kLinks;
-        const uint32_t rb = (h <= kLinks && kLinks % h == 0) ? kLinks / h : 0;
-        bool ok = rb && (L->g.vh % kLinks == 0) && (nb % 8 == 0) && ((uint32_t)L->g.n3 % rb == 0);
-        if (ok) {
-            const uint32_t ng = (uint32_t)L->g.n3 / rb, nt = (uint32_t)L->g.n4, per = nb / 8;
-            ok = per % (ng * nt) == 0;
-            if (ok) {
-                a.ord_ng = ng;
-                a.ord_nt = nt;
-                a.ord_xs = per / (ng * nt);
-            }
-        }
-        if (!ok) a.xcd_map = 1;
-    }
-    if (L->tune_update_shape == 1 && !a.tw.on && a.xcd_map != 0) {
-        // four 64-link units per workgroup: the XCD slabs need whole workgroups
-        const uint32_t nb = align_up(L->g.vh, kLinks) / kLinks;
-        if (nb % (8u * kWaves1)) a.xcd_map = 0;
-    }
+    dispatch_order(L, L->tune_xcd_map, L->tune_update_shape == 1 && !a.tw.on, a);
     LaunchOpt ol = o;
     if (L->tune_blocks_per_cu > 0) {
         // 160 KiB of LDS per CU on gfx950; reserve so at most k workgroups fit.
@@ -1959,7 +2021,7 @@ int launch_random(qcd_lattice* L, const float4* buf, uint32_t rowsize, int d)
 }
 
 // flags: 1 = fused plaquette partials, 2 = k_plaquette partials, 4 = Polyakov.
-int launch_finalize(qcd_lattice* L, int flags, const LaunchOpt& o)
+FinArgs fin_args(const qcd_lattice* L, int flags)
 {
     FinArgs f;
     f.fused = (flags & 1) ? L->part_fused : nullptr;
@@ -1981,9 +2043,32 @@ int launch_finalize(qcd_lattice* L, int flags, const LaunchOpt& o)
     f.a_rec = L->a_rec;
     f.a_n = L->g.n1 + L->g.n2 + L->g.n3;
     f.aslices = (flags & 64) ? 1 : 0;
-    launch(k_finalize, dim3(1), dim3(kBlock), L->stream, o, f);
+    return f;
+}
+
+int launch_finalize(qcd_lattice* L, int flags, const LaunchOpt& o)
+{
+    launch(k_finalize, dim3(1), dim3(kBlock), L->stream, o, fin_args(L, flags));
     HIPCHK(hipGetLastError());
     return QCD_OK;
+}
+
+// The Polyakov loop with the final sums of `flags` in its last workgroup.
+template <typename R, int N, int NREAL>
+int launch_polyakov_fin(qcd_lattice* L, int flags, const LaunchOpt& o)
+{
+    launch(k_polyakov_fin<R, N, NREAL>, dim3(L->poly_blocks), dim3(kBlock), L->stream, o, L->table, L->g,
+           L->part_poly, fin_args(L, flags), L->rec_index + 4);
+    HIPCHK(hipGetLastError());
+    return QCD_OK;
+}
+
+// Whether the final sums of a measurement ride in the Polyakov launch: only
+// when no kernel after it feeds them (the Wilson loop runs after it, and the
+// slice sums read its per-site traces).
+bool fin_in_polyakov(int mask)
+{
+    return (mask & QCD_MEAS_POLYAKOV) && !(mask & (QCD_MEAS_WILSON | QCD_MEAS_POLYAKOV_SLICES));
 }
 
 int ensure_device()
@@ -2190,7 +2275,7 @@ int qcd_lattice_create(const qcd_params* pp, qcd_lattice** out)
         hipMalloc((void**)&L->rng_buf[1], sizeof(uint4) * L->n_streams) != hipSuccess ||
         hipMalloc((void**)&L->jump_cols, sizeof(uint4) * (kJumpCols + kJumpTab)) != hipSuccess ||
         hipMalloc((void**)&L->partials, sizeof(double) * part_doubles) != hipSuccess ||
-        hipMalloc((void**)&L->rec_index, sizeof(int) * 4) != hipSuccess ||
+        hipMalloc((void**)&L->rec_index, sizeof(int) * 8) != hipSuccess ||
         hipMalloc((void**)&L->plq_rec, sizeof(double) * 2 * p.max_measurements) != hipSuccess ||
         hipMalloc((void**)&L->poly_rec, sizeof(double) * 4 * p.max_measurements) != hipSuccess ||
         hipMalloc((void**)&L->wil_rec, sizeof(double) * p.max_measurements) != hipSuccess ||
@@ -2216,7 +2301,7 @@ int qcd_lattice_create(const qcd_params* pp, qcd_lattice** out)
     L->part_plq = L->part_fused + (size_t)3 * L->nb_fused;
     L->part_poly = L->part_plq + (size_t)2 * 2 * L->upd_blocks;
     L->part_wil = L->part_poly + (size_t)4 * L->poly_blocks;
-    (void)hipMemset(L->rec_index, 0, sizeof(int) * 4);
+    (void)hipMemset(L->rec_index, 0, sizeof(int) * 8);   // [4]: completion counter of k_polyakov_fin
     (void)hipMemset(L->table, 0, L->table_bytes);
     {   // records start at zero, so state images of equal runs are equal byte for byte
         const size_t m = (size_t)p.max_measurements;
@@ -2279,7 +2364,7 @@ int qcd_lattice_tune(qcd_lattice* L, int knob, int value)
     if (!L) return fail(QCD_E_ARG, "null lattice");
     switch (knob) {
         case QCD_TUNE_XCD_MAP:
-            if (value < 0 || value > 2) return fail(QCD_E_ARG, "xcd map must be 0, 1 or 2");
+            if (value < 0 || value > 3) return fail(QCD_E_ARG, "xcd map must be 0, 1, 2 or 3");
             L->tune_xcd_map = value;
             break;
         case QCD_TUNE_BLOCKS_PER_CU:
@@ -2292,6 +2377,27 @@ int qcd_lattice_tune(qcd_lattice* L, int knob, int value)
             break;
         default:
             return fail(QCD_E_ARG, "unknown tuning knob %d", knob);
+    }
+    return QCD_OK;
+}
+
+int qcd_lattice_get_tune(const qcd_lattice* L, int knob, int* value, int* detail)
+{
+    if (!L || !value) return fail(QCD_E_ARG, "null argument");
+    switch (knob) {
+        case QCD_TUNE_XCD_MAP: {
+            UpdArgs a{};
+            dispatch_order(L, L->tune_xcd_map, L->tune_update_shape == 1 && !L->tw.on, a);
+            *value = a.xcd_map;
+            if (detail) {   // XCD block: x-values, z-rows (modes 2 and 3)
+                detail[0] = a.xcd_map >= 2 ? (int)a.ord_xs : 0;
+                detail[1] = a.xcd_map >= 2 ? (int)(a.ord_zr * ((uint32_t)L->g.n3 / a.ord_ng)) : 0;
+            }
+            break;
+        }
+        case QCD_TUNE_BLOCKS_PER_CU: *value = L->tune_blocks_per_cu; break;
+        case QCD_TUNE_UPDATE_SHAPE: *value = L->tw.on ? 3 : L->tune_update_shape; break;
+        default: return fail(QCD_E_ARG, "unknown tuning knob %d", knob);
     }
     return QCD_OK;
 }
@@ -2613,6 +2719,7 @@ static int measure_impl(qcd_lattice* L, int mask, bool fused_plq, const LaunchOp
         flags |= 32;
     }
     if (rc) return rc;
+    if (fin_in_polyakov(mask)) return DISPATCH(L, launch_polyakov_fin, L, flags | 4, o_poly);
     if (mask & (QCD_MEAS_POLYAKOV | QCD_MEAS_POLYAKOV_SLICES)) {
         L->want_slices = (mask & QCD_MEAS_POLYAKOV_SLICES) != 0;
         rc = DISPATCH(L, launch_simple, L, 4, o_poly);
@@ -2783,7 +2890,8 @@ static int enqueue_cycle(qcd_lattice* L, int n_hb, int n_or, int mask, std::vect
     LaunchOpt op, oq, of;
     if (ev) {
         LaunchOpt* slots[3] = {&op, &oq, &of};
-        const bool want[3] = {(mask & QCD_MEAS_PLAQUETTE) && !fuse, (mask & QCD_MEAS_POLYAKOV) != 0, true};
+        const bool want[3] = {(mask & QCD_MEAS_PLAQUETTE) && !fuse, (mask & QCD_MEAS_POLYAKOV) != 0,
+                              !fin_in_polyakov(mask)};
         for (int k = 0; k < 3; ++k)
             if (want[k]) {
                 slots[k]->e0 = (*ev)[2 * (*ev_used)];
@@ -3331,7 +3439,7 @@ UpdArgs chain_base_args(const qcd_lattice* L)
     a.s_quads = (L->p.stream_mode == QCD_STREAM_REFERENCE) ? L->s_quads : 0u;
     a.partials = L->part_fused;
     a.xcd_map = 0;
-    a.ord_ng = a.ord_nt = a.ord_xs = 1;
+    a.ord_ng = a.ord_nt = a.ord_xs = a.ord_ax = a.ord_zr = 1;
     a.jump = L->jump_cols;
     a.pm_ak = L->pm_ak;
     a.prn = L->prn_buf;

@@ -48,6 +48,7 @@
 #include <vector>
 
 #include "../../include/qcd_hip.h"
+#include "host_check.h"
 
 namespace {
 
@@ -87,6 +88,10 @@ struct Settings {
     // "Differentiated S data" block (get_actions_diff, suncl.cpp:1376-1480,
     // 1753-1790; the reference hardcodes it off, QCDGPU.cpp:218).
     bool actions_diff = false;
+    // VERIFY = 0 skips the end-of-run check of the final configuration on
+    // the host (LatticeCheckCPU, QCDGPU.cpp:254, always run by the
+    // reference outside BIGLAT); it needs 8 N^2 bytes per link of host memory.
+    bool verify = true;
 };
 
 std::string trim(const std::string& s)
@@ -174,6 +179,7 @@ bool apply(Settings& st, const std::string& key, const std::string& val)
     else if (key == "SAVEEVERY") st.save_every = fv;
     else if (key == "STOPAFTER") st.stop_after = (int)iv;
     else if (key == "ACTIONSDIFF") st.actions_diff = iv != 0;
+    else if (key == "VERIFY") st.verify = iv != 0;
     else if (key == "NOR") st.nor = (int)iv;
     else if (key == "STORAGE") st.storage = (int)iv;
     else if (key == "STREAMS") {
@@ -306,7 +312,7 @@ void print_params(const Settings& st)
            st.storage ? st.storage : (st.group == 3 ? 18 : 4), st.fast_streams ? "FAST" : "REFERENCE");
     printf("OUTPUTPATH=%s\nFINISHPATH=%s\nPLATFORM=%d\nDEVICE=%d\n", st.outputpath.c_str(), st.finishpath.c_str(),
            st.platform, st.device);
-    printf("ACTIONSDIFF=%d\n", st.actions_diff ? 1 : 0);
+    printf("ACTIONSDIFF=%d\nVERIFY=%d\n", st.actions_diff ? 1 : 0, st.verify ? 1 : 0);
     printf("INIT=%d\nSTATEFILE=%s\nSAVESTATE=%d\nSAVEEVERY=%.3f\n", st.init, st.statefile.c_str(),
            st.save_state ? 1 : 0, st.save_every);
 }
@@ -319,6 +325,62 @@ void print_params(const Settings& st)
             exit(-1); /* every device error is fatal, as in the reference */          \
         }                                                                             \
     } while (0)
+
+CheckSetup check_setup(const Settings& st)
+{
+    CheckSetup c;
+    c.group = st.group;
+    for (int k = 0; k < 4; ++k) c.n[k] = st.n[k];
+    c.beta = st.beta;
+    c.single = !st.double_precision;
+    c.pl_level = st.pl_level;
+    c.wilson = st.wilson;
+    c.wilson_r = st.wilson_r;
+    c.wilson_t = st.wilson_t;
+    c.fmunu = st.fmunu;
+    fmunu_indices(st, c.fgen[0], c.fgen[1]);
+    c.phi = st.phi;
+    c.omega = st.omega;
+    return c;
+}
+
+// The check's stdout lines (CPU_GPU_verification_*, data_analysis.cpp:76-104).
+void print_check(const CheckResult& r)
+{
+    for (const auto& name : r.failed) printf("%s test failed!\n", name.c_str());
+    printf("%s\n", r.passed ? "Verification successfully passed!" : "Verification failed!");
+}
+
+// qcdgpu_hip --verify init.dat links.f64 last.txt: the check alone, on a
+// configuration (canonical FP64 links) and device values ("name value" per
+// line) given in files; no device is used.  Exit status 0 passed, 4 failed.
+int verify_files(const Settings& st, const char* links_path, const char* last_path)
+{
+    const size_t V = (size_t)st.n[0] * st.n[1] * st.n[2] * st.n[3];
+    std::vector<double> links(V * 4 * st.group * st.group * 2);
+    FILE* f = fopen(links_path, "rb");
+    if (!f || fread(links.data(), sizeof(double), links.size(), f) != links.size()) {
+        fprintf(stderr, "cannot read %zu links from %s\n", V * 4, links_path);
+        if (f) fclose(f);
+        return 1;
+    }
+    fclose(f);
+    std::vector<std::pair<std::string, double>> last;
+    f = fopen(last_path, "r");
+    if (!f) {
+        fprintf(stderr, "cannot open %s\n", last_path);
+        return 1;
+    }
+    char name[128];
+    double v;
+    while (fscanf(f, "%127s %lf", name, &v) == 2) last.push_back({name, v});
+    fclose(f);
+    const CheckSetup cs = check_setup(st);
+    const CheckResult r = check_configuration(cs, links, last);
+    print_check(r);
+    printf("%s", check_report(cs, r).c_str());
+    return r.passed ? 0 : 4;
+}
 
 struct Series {
     std::string name;
@@ -342,11 +404,18 @@ int main(int argc, char** argv)
     Settings st;
     bool print_only = false;
     const char* file = nullptr;
+    const char* verify_args[2] = {nullptr, nullptr};
     for (int k = 1; k < argc; ++k) {
         if (!strcmp(argv[k], "--print-params")) print_only = true;
-        else file = argv[k];
+        else if (!strcmp(argv[k], "--verify") && k + 3 < argc) {
+            file = argv[k + 1];
+            verify_args[0] = argv[k + 2];
+            verify_args[1] = argv[k + 3];
+            k += 3;
+        } else file = argv[k];
     }
     if (file && !read_init_file(file, st)) return 1;
+    if (verify_args[0]) return verify_files(st, verify_args[0], verify_args[1]);
     qcd_params p, p_file;
     qcd_run_info run;
     memset(&run, 0, sizeof(run));
@@ -494,6 +563,13 @@ int main(int argc, char** argv)
     int na = 0;
     if (st.actions_diff) CHECK(qcd_read_action_slices(lat, asl.data(), &na));
     na = std::min(na, (int)cap);
+    // the final configuration, for the host check (lattice_pointer_last,
+    // suncl.cpp:5500-5502)
+    std::vector<double> final_links;
+    if (st.verify) {
+        final_links.resize((size_t)st.n[0] * st.n[1] * st.n[2] * st.n[3] * 4 * st.group * st.group * 2);
+        CHECK(qcd_lattice_export(lat, final_links.data()));
+    }
     np = std::min(np, (int)cap);
     nq = std::min(nq, (int)cap);
     nw = std::min(nw, (int)cap);
@@ -573,6 +649,24 @@ int main(int argc, char** argv)
                 fcols.push_back(se);
             }
         for (auto& c : fcols) c.analyse();
+    }
+    // the end-of-run check: every column's last device value against the
+    // host's FP64 measurement of the final configuration
+    std::string check_block;
+    bool check_passed = true;
+    if (st.verify) {
+        std::vector<std::pair<std::string, double>> last;
+        for (auto* c : cols)
+            if (!c->v.empty()) last.push_back({c->name, c->v.back()});
+        for (auto& c : fcols)
+            if (!c.v.empty()) last.push_back({c.name, c.v.back()});
+        const CheckSetup cs = check_setup(st);
+        const CheckResult r = check_configuration(cs, final_links, last);
+        print_check(r);
+        check_block = check_report(cs, r);
+        check_passed = r.passed;
+        final_links.clear();
+        final_links.shrink_to_fit();
     }
 
     // lattice_print_measurements: first five and the last measurement
@@ -685,6 +779,7 @@ int main(int argc, char** argv)
             off += st.n[d];
         }
     }
+    if (!check_block.empty()) fprintf(out, "%s", check_block.c_str());
     fprintf(out, " ***************************************************\n");
     fprintf(out, " Data fields:\n    #,  ");
     for (auto* c : cols) fprintf(out, "%-21s ", c->name.c_str());
@@ -698,5 +793,6 @@ int main(int argc, char** argv)
     }
     fclose(out);
     printf(" results: %s\n", fname.c_str());
+    (void)check_passed;   // the reference reports a failed check and still ends normally
     return 0;
 }

@@ -317,7 +317,7 @@ class Lattice:
 
     def tune(self, xcd_map=None, blocks_per_cu=None, update_shape=None):
         """Launch-shape knobs of the update kernels (scheduling only; results
-        are identical): xcd_map 0/1/2, blocks_per_cu cap (0 = none),
+        are identical): xcd_map 0/1/2/3, blocks_per_cu cap (0 = none),
         update_shape 3 (three wavefronts per 64 links) or 1."""
         if xcd_map is not None:
             check(lib().qcd_lattice_tune(self._h, _abi.TUNE_XCD_MAP, int(xcd_map)))
@@ -325,6 +325,19 @@ class Lattice:
             check(lib().qcd_lattice_tune(self._h, _abi.TUNE_BLOCKS_PER_CU, int(blocks_per_cu)))
         if update_shape is not None:
             check(lib().qcd_lattice_tune(self._h, _abi.TUNE_UPDATE_SHAPE, int(update_shape)))
+
+    def tuning(self):
+        """Knob values the update launches use: xcd_map after its fallbacks
+        (with the XCD block's x-values and z-rows), blocks_per_cu,
+        update_shape."""
+        v = ctypes.c_int()
+        det = (ctypes.c_int * 2)()
+        check(lib().qcd_lattice_get_tune(self._h, _abi.TUNE_XCD_MAP, ctypes.byref(v), det))
+        out = {"xcd_map": v.value, "xcd_block": (det[0], det[1])}
+        for name, knob in (("blocks_per_cu", _abi.TUNE_BLOCKS_PER_CU), ("update_shape", _abi.TUNE_UPDATE_SHAPE)):
+            check(lib().qcd_lattice_get_tune(self._h, knob, ctypes.byref(v), None))
+            out[name] = v.value
+        return out
 
     def table_bytes(self):
         b = ctypes.c_size_t()
@@ -465,11 +478,8 @@ def prng_produce(prng, table, samples, luxury=3):
         t = np.ascontiguousarray(table, dtype=np.uint32)
         n = t.shape[0]
     q = np.zeros((samples, n, 4), dtype=np.float32)
-    check(
-        lib().qcd_prng_produce_host(
-            code, int(luxury), n, samples, t.ctypes.data_as(ctypes.c_void_p),
-            q.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
-        )
-    )
+    fn = lib().qcd_prng_produce_host
+    check(fn(code, int(luxury), n, samples, t.ctypes.data_as(ctypes.c_void_p),
+             q.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
     table[...] = t
     return q

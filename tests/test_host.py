@@ -121,10 +121,14 @@ def test_host_run_matches_python_driver(tmp_path, prng, ints, fkeys):
     twist = (3.0, 0.5) if "PHI" in fkeys else (0.0, 0.0)
     r = subprocess.run([BIN, str(f)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
+    # the end-of-run check of the final configuration on the host
+    assert "Verification successfully passed!" in r.stdout and "test failed" not in r.stdout, r.stdout
     out = [p for p in os.listdir(tmp_path) if p.startswith("su3-") and p.endswith(".txt")]
     assert len(out) == 1
     rows, pl_rows, s_rows, section, fields = [], {}, {}, None, None
     lines = open(tmp_path / out[0]).read().splitlines()
+    assert " *** Verification successfully passed! *************" in lines
+    assert any(ln.startswith(" CPU last Wilson_loop") for ln in lines)
     for k, line in enumerate(lines):
         parts = line.split()
         if line.strip() == "Data fields:":
@@ -222,3 +226,106 @@ def test_reference_init_dat_runs(tmp_path):
     phi = [ln for ln in text.splitlines() if ln.startswith(" PHI")]
     assert len(phi) == 1 and float(phi[0].split(":")[1]) == 3.0
     assert "Differentiated Polyakov loop data" in text and "FMUNU(3, 2)" in text
+    assert " *** Verification successfully passed! *************" in text and "Verification successfully passed!" in r.stdout
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("storage", [18, 12])
+def test_host_run_single_precision_check(tmp_path, storage):
+    """A single-precision run (FP32 links, FP64 sums): the device's last
+    records agree with the host's FP64 measurement of the exported final
+    configuration within 5e-5 relative."""
+    print("[case] host run single precision storage=%d" % storage, flush=True)
+    f = tmp_path / "init.dat"
+    f.write_text("GROUP = 3\nLS = 8\nLT = 4\nITER = 3\nNITER = 2\nBETA = 5.9\nINTS = 1\nPRNG = XOR128\n"
+                 "PRECISION = SINGLE\nSTORAGE = %d\nNOR = 3\nGETWILSON = 1\nWILSONR = 3\nWILSONT = 2\n"
+                 "OUTPUTPATH = %s/\n" % (storage, tmp_path))
+    r = subprocess.run([BIN, str(f)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "Verification successfully passed!" in r.stdout and "test failed" not in r.stdout, r.stdout
+
+
+# ---------------------------------------------------------------- end-of-run check
+def _device_like_last(O, L, links, sun, dims, beta, wilson=(2, 2), efield=False, gens=(3, 8)):
+    """The last values of the results-file columns for `links`, from the
+    oracle's sums with the host program's normalisations."""
+    V = float(np.prod(dims))
+    Vs, N = V / dims[3], float(sun)
+    p = O.plaquette(L, links)
+    ps, pt = p[0] / (3 * V), p[1] / (3 * V)
+    bet = beta / N
+    pl = O.polyakov(L, links)
+    last = {"Plq_spat": ps, "Plq_temp": pt, "Plq_total": 0.5 * (ps + pt), "S_spat": bet * (N - ps),
+            "S_temp": bet * (N - pt), "S_total": 0.5 * (bet * (N - ps) + bet * (N - pt)),
+            "Polyakov_loop": pl[0] / (Vs * N), "Polyakov_loop_im": pl[1] / (Vs * N),
+            "Polyakov_loop_P2": pl[2] / (Vs * N * N), "Polyakov_loop_P4": pl[3] / (Vs * N ** 4),
+            "Wilson_loop": O.wilson(L, links, *wilson) / (3 * V)}
+    f = O.fmunu(L, links, efield=efield, gen1=gens[0], gen2=gens[1]) / V
+    names = ("xt", "yt", "zt") if efield else ("xy", "xz", "yz")
+    for g in range(2):
+        for pl_i in range(3):
+            last["Fmunu_%s_%d_re" % (names[pl_i], gens[g])] = f[g, pl_i].real
+            last["Fmunu_%s_%d_im" % (names[pl_i], gens[g])] = f[g, pl_i].imag
+        last["Fmunu_abs_%d_re" % gens[g]] = np.sqrt((f[g].real ** 2).sum())
+        last["Fmunu_abs_%d_im" % gens[g]] = np.sqrt((f[g].imag ** 2).sum())
+    return last
+
+
+def _verify(tmp_path, init, links, last):
+    (tmp_path / "init.dat").write_text(init)
+    np.ascontiguousarray(links, dtype=np.float64).tofile(tmp_path / "links.f64")
+    (tmp_path / "last.txt").write_text("".join("%s %.17g\n" % kv for kv in last.items()))
+    return subprocess.run([BIN, "--verify", str(tmp_path / "init.dat"), str(tmp_path / "links.f64"),
+                           str(tmp_path / "last.txt")], capture_output=True, text=True)
+
+
+@pytest.mark.parametrize("sun,beta,prec", [(3, 5.9, "DOUBLE"), (3, 5.9, "SINGLE"), (2, 2.3, "DOUBLE")])
+def test_end_of_run_check(O, tmp_path, sun, beta, prec):
+    """The host's own FP64 measurement of a configuration agrees with the
+    oracle's (plaquette, action, Polyakov loop, Wilson loop, F_munu): the
+    check passes and prints every CPU / GPU last pair; a record off by 1e-4
+    relative fails it at both precisions, and one off by 1e-6 only at double
+    precision (tolerances 5e-5 and 1e-11, data_analysis.cpp:40-43)."""
+    dims = (4, 4, 2, 4)
+    L = O.lattice(sun, dims, beta, 10)
+    a = O.cold(L)
+    rng = O.SeqRng("RANLUX", 5, 223)
+    for _ in range(3):
+        O.sweep(L, a, O.HB, rng)
+    init = ("GROUP = %d\nL1 = %d\nL2 = %d\nL3 = %d\nL4 = %d\nBETA = %r\nPRECISION = %s\nGETWILSON = 1\n"
+            "WILSONR = 2\nWILSONT = 2\n" % ((sun,) + dims + (beta, prec)))
+    gens = (3, 8) if sun == 3 else (3, 2)
+    last = _device_like_last(O, L, a, sun, dims, beta, gens=gens)
+    r = _verify(tmp_path, init, a, last)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "Verification successfully passed!" in r.stdout and "test failed" not in r.stdout
+    for name in ("S_total", "Plq_total", "Polyakov_loop", "Polyakov_loop_P4", "Wilson_loop", "Fmunu_xy_3_re",
+                 "Fmunu_abs_%d_re" % gens[1]):
+        assert " GPU last %-20s:" % name in r.stdout and " CPU last %-20s:" % name in r.stdout
+    assert " CPU l.varnc Fmunu_xy_3_re" in r.stdout
+    bad = dict(last, Plq_total=last["Plq_total"] * (1 + 1e-4))
+    r = _verify(tmp_path, init, a, bad)
+    assert r.returncode == 4 and "Plq_total test failed!" in r.stdout and "Verification failed!" in r.stdout
+    assert "S_total test failed" not in r.stdout
+    near = dict(last, Polyakov_loop_P2=last["Polyakov_loop_P2"] * (1 + 1e-6))
+    r = _verify(tmp_path, init, a, near)
+    assert (r.returncode == 0) == (prec == "SINGLE"), r.stdout
+
+
+def test_end_of_run_check_twisted_and_e_field(O, tmp_path):
+    """With PHI / OMEGA the host check reads the twisted Y links at
+    x = N1 - 1 as the device does (compared with the oracle's twisted view),
+    and F0MU takes the (x,t), (y,t), (z,t) planes."""
+    dims = (4, 4, 2, 4)
+    L = O.lattice(3, dims, 5.9, 10)
+    a = O.cold(L)
+    rng = O.SeqRng("RANLUX", 8, 223)
+    for _ in range(2):
+        O.sweep(L, a, O.HB, rng)
+    view = O.twisted_view(L, a, 0.7, -0.3)
+    init = ("GROUP = 3\nL1 = 4\nL2 = 4\nL3 = 2\nL4 = 4\nBETA = 5.9\nPHI = 0.7\nOMEGA = -0.3\nF0MU = 1\n"
+            "FMUNU1 = 1\nFMUNU6 = 1\nGETWILSON = 1\nWILSONR = 2\nWILSONT = 1\n")
+    last = _device_like_last(O, L, view, 3, dims, 5.9, wilson=(2, 1), efield=True, gens=(1, 6))
+    r = _verify(tmp_path, init, a, last)
+    assert r.returncode == 0, r.stdout
+    assert " GPU last Fmunu_zt_6_im" in r.stdout
