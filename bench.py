@@ -12,11 +12,13 @@ Prints one JSON line.  `roofline` is for the dominant kernel (the
 overrelaxation half-step, 32 of the 40 update launches of a sweep):
 algorithmic bytes per launch = 9 * 72 B * V/2 (each updated link reads the
 three other directions' links of both parities and its own direction's
-links of both parities once, and writes itself), divided by the mean kernel
-duration measured with HIP events on the lattice stream over `--steps`
-sweeps.  `cpu_baseline` times the reference CPU path (suncpp, built from the
-reference sources into oracle/_ref/ref_harness) on one host core for a
-bounded sample of heatbath sweeps of the same lattice.
+links of both parities once, and writes itself), divided by the kernel's
+duration: HIP events on the lattice stream around each kernel of eagerly
+launched cycles, scaled so that a cycle's kernels add up to the graph-timed
+step.  `config.step_roofline` is the whole step's algorithmic bytes over the
+timed step.  `cpu_baseline` times the reference CPU path (suncpp, built from
+the reference sources into oracle/_ref/) on the host's cores for a bounded
+sample of heatbath sweeps of the same lattice.
 """
 import argparse
 import json
@@ -36,41 +38,90 @@ HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(REPO, "profiles", "or_halfstep_su3_32c8_fp32_traffic.json")
 
 
-def cpu_baseline(sweeps=6):
-    """The reference's own CPU path on one core (kind 'reference'), or the
-    oracle restatement when the reference build is absent (kind 'port')."""
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _harness(name, args, timeout=300):
+    out = subprocess.run([os.path.join(REPO, "oracle", "_ref", name)] + args, capture_output=True, text=True,
+                         timeout=timeout, check=True).stdout.split()
+    return float(out[2])   # link updates per second
+
+
+def cpu_baseline(sweeps=4, o3_sweeps=2, agg_sweeps=2):
+    """The reference's own CPU path (suncpp, single-threaded FP64, RANLUX3,
+    heatbath only; compiled from the reference sources into oracle/_ref/):
+    one core at the reference's -O2 and at -O3 (x86-64-v3), the all-core
+    aggregate of independent processes with distinct seeds (suncpp has no
+    threads), and, labelled as an extension, the oracle's overrelaxation
+    sweep (suncpp has none).  Without the reference build the baseline is
+    the oracle port, and the line says why."""
     V = 1
     for d in DIMS:
         V *= d
-    harness = os.path.join(REPO, "oracle", "_ref", "ref_harness")
+    lat = [str(d) for d in DIMS]
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    cores = max(1, min(cores, 16))   # one process per core, at most the box's CPU share
+    info = {"nproc": os.cpu_count(), "cpu_model": _cpu_model(), "cores_available": cores}
     sample = "%d cold-start Cabibbo-Marinari heatbath sweeps of SU(3) %dx%dx%dx%d beta=%.1f, FP64, RANLUX3 (suncpp)" % (
         (sweeps,) + DIMS + (BETA,)
     )
-    if os.path.exists(harness):
-        out = subprocess.run(
-            [harness, "bench", "3"] + [str(d) for d in DIMS] + [str(BETA), "10", "RANLUX3", "54", str(sweeps)],
-            capture_output=True,
-            text=True,
-            timeout=300,
-            check=True,
-        ).stdout.split()
-        return {"value": float(out[2]), "unit": "link updates/s", "cores": 1, "kind": "reference", "sample": sample}
+    harness = os.path.join(REPO, "oracle", "_ref", "ref_harness")
+    if not os.path.exists(harness):
+        print("bench.py: WARNING: the reference CPU build oracle/_ref/ref_harness is missing (built by "
+              "oracle/build_ref.sh where the reference sources are); cpu_baseline falls back to the oracle port",
+              file=sys.stderr, flush=True)
+        sys.path.insert(0, os.path.join(REPO, "oracle"))
+        import pyoracle as O
+
+        L = O.lattice(3, DIMS, BETA, 10)
+        a = O.cold(L)
+        rng = O.SeqRng("RANLUX", 54, 223)
+        t0 = time.perf_counter()
+        O.sweep(L, a, O.HB, rng)
+        dt = time.perf_counter() - t0
+        return dict(info, value=4.0 * V / dt, unit="link updates/s", cores=1, kind="port",
+                    fallback_reason="oracle/_ref/ref_harness missing",
+                    sample="1 cold-start heatbath sweep of SU(3) 32x32x32x8, FP64 oracle restatement of suncpp")
+    base = ["bench", "3"] + lat + [str(BETA), "10", "RANLUX3"]
+    o2 = _harness("ref_harness", base + ["54", str(sweeps)])
+    rows = {"O2_1core": o2}
+    o3_path = os.path.join(REPO, "oracle", "_ref", "ref_harness_o3")
+    if os.path.exists(o3_path):
+        rows["O3_x86-64-v3_1core"] = _harness("ref_harness_o3", base + ["54", str(o3_sweeps)])
+    # all-core aggregate: one process per core, distinct random series
+    procs = [subprocess.Popen([harness] + base + [str(54 + k), str(agg_sweeps)], stdout=subprocess.PIPE, text=True)
+             for k in range(cores)]
+    agg = 0.0
+    for pr in procs:
+        out, _ = pr.communicate(timeout=600)
+        if pr.returncode != 0:
+            raise RuntimeError("ref_harness failed in the all-core run")
+        agg += float(out.split()[2])
+    rows["O2_all_cores"] = agg
+    # extension: overrelaxation (not in suncpp), the oracle's FP64 C restatement
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import pyoracle as O
 
     L = O.lattice(3, DIMS, BETA, 10)
     a = O.cold(L)
-    rng = O.SeqRng("RANLUX", 54, 223)
     t0 = time.perf_counter()
-    O.sweep(L, a, O.HB, rng)
-    dt = time.perf_counter() - t0
-    return {
-        "value": 4.0 * V / dt,
-        "unit": "link updates/s",
-        "cores": 1,
-        "kind": "port",
-        "sample": "1 cold-start heatbath sweep of SU(3) 32x32x32x8, FP64 oracle restatement of suncpp",
-    }
+    O.sweep(L, a, O.OR, O.GidRng())
+    rows["extension_overrelaxation_port_1core"] = 4.0 * V / (time.perf_counter() - t0)
+    return dict(info, value=o2, unit="link updates/s", cores=1, kind="reference", sample=sample,
+                rows=rows, aggregate_cores=cores,
+                rows_sample="O3: %d sweeps; all cores: %d sweeps per process, random series 54 + k; "
+                            "overrelaxation: 1 sweep of the oracle (FP64 C)" % (o3_sweeps, agg_sweeps))
 
 
 def copy_calibration(mib=256, repeats=20):
@@ -187,12 +238,27 @@ def main():
     value = updates / (ms * 1e-3)
 
     prof = lat.profile_cycle(max(1, min(args.profile_steps, args.steps)))
-    bytes_per_launch = 9.0 * args.storage * 4 * (V // 2)
-    achieved = bytes_per_launch / (prof["or_us"] * 1e-6) / 1e9
+    link_bytes = args.storage * 4
+    bytes_per_launch = 9.0 * link_bytes * (V // 2)
     # heatbath half-step: the same links plus a 16-byte stream state read and
     # written per updated link
     hb_bytes = bytes_per_launch + 2.0 * 16 * (V // 2)
-    hb_achieved = hb_bytes / (prof["hb_us"] * 1e-6) / 1e9
+    # Kernel durations: HIP events around each kernel of an eagerly launched
+    # cycle exclude the gaps between the graph's kernels; scaled so that one
+    # cycle's kernels sum to the graph-timed step, they are the per-kernel
+    # share of the timed region (what a kernel trace of the graph replays
+    # reports, which counts each gap into the kernel before it).
+    ms_step = ms / args.steps
+    n_hb_launch, n_or_launch = 8, 8 * N_OR - 3   # three OR launches of the last sweep carry the plaquette
+    event_sum_us = n_hb_launch * prof["hb_us"] + n_or_launch * prof["or_us"] + prof["meas_us"]
+    gap_scale = 1000.0 * ms_step / event_sum_us
+    or_us, hb_us = prof["or_us"] * gap_scale, prof["hb_us"] * gap_scale
+    achieved = bytes_per_launch / (or_us * 1e-6) / 1e9
+    hb_achieved = hb_bytes / (hb_us * 1e-6) / 1e9
+    # whole step: 4V link updates of each sweep at 9 s (+ 2 x 16 B of stream
+    # state for the heatbath), plaquette 4V s and Polyakov V s
+    step_bytes = 4.0 * V * ((9 * link_bytes + 32) + N_OR * 9 * link_bytes) + 5.0 * V * link_bytes
+    step_achieved = step_bytes / (ms_step * 1e-3) / 1e9
     traffic = None
     if os.path.exists(TRAFFIC_FILE) and args.storage == 18:
         try:
@@ -225,13 +291,22 @@ def main():
             "nhit": 10,
             "start": "cold",
             "updates_per_step": int(5 * 4 * V),
-            "kernel_us": {"heatbath_halfstep": prof["hb_us"], "overrelax_halfstep": prof["or_us"], "measure": prof["meas_us"]},
+            "kernel_us": {"heatbath_halfstep": hb_us, "overrelax_halfstep": or_us,
+                          "measure_per_step": prof["meas_us"] * gap_scale},
+            "kernel_us_events": {"heatbath_halfstep": prof["hb_us"], "overrelax_halfstep": prof["or_us"],
+                                 "measure_per_step": prof["meas_us"], "gap_scale": gap_scale},
             "sweeps_per_s": args.steps / (ms * 1e-3),
             "heatbath_roofline": {
                 "achieved": hb_achieved,
                 "unit": "GB/s",
                 "frac": hb_achieved / HBM_PEAK_GBS,
                 "algorithmic_bytes_per_launch": hb_bytes,
+            },
+            "step_roofline": {
+                "achieved": step_achieved,
+                "unit": "GB/s",
+                "frac": step_achieved / HBM_PEAK_GBS,
+                "algorithmic_bytes_per_step": step_bytes,
             },
             "plaquette_last": float(meas["plq_total"][-1] / 3.0) if "plq_total" in meas else None,
         },
@@ -243,7 +318,11 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
+            "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this kernel, scaled by a plain copy "
+                              "(profiles/or_halfstep_su3_32c8_fp32_traffic.txt); a profile constant, not measured "
+                              "in this run" if traffic else None,
             "algorithmic_bytes_per_launch": bytes_per_launch,
+            "duration_us": or_us,
         },
         "cpu_baseline": None if args.no_cpu else cpu_baseline(),
     }

@@ -93,6 +93,19 @@ typedef struct {
     int      stream_mode;    /* QCD_STREAM_*                                    */
     unsigned randseries;     /* RANDSERIES (0 = time based, as the reference)   */
     int      max_measurements; /* capacity of the on-device measurement arrays  */
+    unsigned prng_instances; /* PRNG_instances (random.cpp:217-223): 0 = PRNGSTEP
+                                = align64(V/2), one fused per-site stream per
+                                link of a half-step; otherwise a multiple of 64
+                                (the reference's automatic choice is the
+                                device's CL_DEVICE_IMAGE3D_MAX_WIDTH,
+                                clinterface.cpp:419): every heatbath half-step
+                                then runs the reference's produce() of
+                                1 + (align64(3*PRNGSTEP*(NHIT+1)) - 1)/instances
+                                quads per instance into a PRN buffer (quad
+                                sample*instances + instance) and link i reads
+                                quad i + k*PRNGSTEP, for XOR128, PM and RANLUX
+                                alike.  Needs QCD_STREAM_REFERENCE; no Haar
+                                start and no batch with it.               */
 } qcd_params;
 
 typedef struct qcd_lattice qcd_lattice;
@@ -389,7 +402,11 @@ int  qcd_batch_sync(qcd_batch *batch);
    (heatbath, overrelaxation).  Capture the cycle after tuning.            */
 int  qcd_batch_autotune(qcd_batch *batch, int repeats, int *shapes_out);
 /* Chain `index` as a lattice handle (export, records, measure); the batch's
-   work is complete when this returns.  Do not destroy it.                 */
+   work is complete when this returns.  The batch owns it: destroying it,
+   re-sizing its streams (a reference hot start) or resuming a hot run on it
+   fail with QCD_E_STATE, and so does the next batch sweep or cycle after
+   the chain was swept on its own (its stream buffers would no longer be
+   the batch's).                                                           */
 int  qcd_batch_chain(qcd_batch *batch, int index, qcd_lattice **lat);
 
 /* ---- learned global proposal for SU(2) (new; no reference counterpart): a

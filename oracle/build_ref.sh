@@ -30,3 +30,10 @@ SRCS=(
 g++ -D CPU_RUN -D CL_TARGET_OPENCL_VERSION=120 -g -O2 -w \
     -I "$REF" "${SRCS[@]}" "$HERE/ref_harness.cpp" -o "$OUT/ref_harness"
 echo "build_ref: built $OUT/ref_harness"
+# Second CPU-baseline row: the same sources at -O3 with AVX2/FMA code
+# (x86-64-v3).  -march=native would target the CPU of the machine that
+# builds it, which need not be the one that runs the benchmark; x86-64-v3
+# runs on every AVX2 host.
+g++ -D CPU_RUN -D CL_TARGET_OPENCL_VERSION=120 -O3 -march=x86-64-v3 -w \
+    -I "$REF" "${SRCS[@]}" "$HERE/ref_harness.cpp" -o "$OUT/ref_harness_o3"
+echo "build_ref: built $OUT/ref_harness_o3"

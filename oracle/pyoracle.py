@@ -180,16 +180,18 @@ def random_start(L, bufs, rowsize):
     return np.ascontiguousarray(np.stack([U.real, U.imag], axis=-1))
 
 
-def produce_quads(L, hot_run):
-    """Quads per instance of one produce() (suncl.cpp:2302-2307) with
-    PRNG_instances = PRNGSTEP = align64(V/2); a hot run adds ROWSIZE * ND
-    PRNs, ROWSIZE = align64(V)."""
+def produce_quads(L, hot_run, instances=None):
+    """Quads per instance of one produce() (suncl.cpp:2302-2307,
+    random.cpp:219-223): align64 of 3 PRNGSTEP (NHIT + 1) quads, a hot run
+    adding ROWSIZE * ND, ROWSIZE = align64(V), over PRNG_instances
+    (default PRNGSTEP = align64(V/2)): PRNG_samples = 1 + (total - 1) /
+    instances."""
     V = int(np.prod(list(L.n)))
     prngstep = -(-(V // 2) // 64) * 64
     rowsize = -(-V // 64) * 64
     total = 3 * prngstep * (L.nhit + 1) + (4 * rowsize if hot_run else 0)
     total = -(-total // 64) * 64
-    return 1 + (total - 1) // prngstep
+    return 1 + (total - 1) // (instances or prngstep)
 
 
 def rand_fill(seed, count):

@@ -37,6 +37,7 @@ class Params(ctypes.Structure):
         ("stream_mode", ctypes.c_int),
         ("randseries", ctypes.c_uint),
         ("max_measurements", ctypes.c_int),
+        ("prng_instances", ctypes.c_uint),
     ]
 
 

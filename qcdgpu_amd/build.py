@@ -17,6 +17,11 @@ HIPCC_FLAGS = [
     "-target-feature",
     "-Xclang",
     "-packed-fp32-ops",
+    # a * b + c is fused into one FMA only where the source writes it as one
+    # expression, never across statements by the back end: the rounding then
+    # follows the source, so every kernel shape, direction instance and
+    # dispatch order that runs the same arithmetic gives the same bits.
+    "-ffp-contract=on",
 ]
 
 

@@ -1527,26 +1527,6 @@ __global__ __launch_bounds__(kBlock) void k_polyakov(void* table, Geom g, double
     polyakov_body<R, N, NREAL>(table, g, partials, blockIdx.x, site_tr);
 }
 
-// The Polyakov loop and the measurement's final sums in one launch: every
-// workgroup writes its partial, and the workgroup that finishes last (a
-// device-scope counter, which that workgroup resets for the next launch)
-// sums all partials in index order exactly as k_finalize does, so the
-// records are the same bit for bit and one launch per measurement is saved.
-template <typename R, int N, int NREAL>
-__global__ __launch_bounds__(kBlock) void k_polyakov_fin(void* table, Geom g, double* partials, FinArgs f, int* done)
-{
-    polyakov_body<R, N, NREAL>(table, g, partials, blockIdx.x, nullptr);
-    __shared__ int last;
-    __threadfence();   // release: this workgroup's partial before its count
-    __syncthreads();
-    if (threadIdx.x == 0) last = (atomicAdd(done, 1) == (int)gridDim.x - 1) ? 1 : 0;
-    __syncthreads();
-    if (!last) return;
-    __threadfence();   // acquire: every other workgroup's partial
-    finalize_body(f);
-    if (threadIdx.x == 0) *done = 0;
-}
-
 // Sums per x, y and z slice of a per-site pair of values, in a fixed order:
 // workgroup b takes slice x = b (b < N1), y = b - N1, or z = b - N1 - N2 and
 // writes record rec_index[slot] at position b.  Sites are indexed
@@ -1698,7 +1678,14 @@ struct qcd_lattice {
     uint32_t prn_quads = 0;
     uint32_t pm_ak = 1;                       // Park-Miller jump multiplier
     int rng_cur = 0;
-    uint32_t n_streams = 0;
+    uint32_t n_streams = 0;   // PRNGSTEP = align64(V/2): links of a half-step's grid, the PRN buffer's stride
+    // Generator instances (PRNG_instances): n_streams, or the count the
+    // parameters give.  With quad_buffer, every heatbath half-step is
+    // preceded by a produce() of s_quads quads per instance into prn_buf
+    // (quad q = sample * n_inst + instance) and link i reads quad
+    // i + k * n_streams, the reference's PRN buffer itself.
+    uint32_t n_inst = 0;
+    bool quad_buffer = false;
     uint32_t s_quads = 0;   // S
     double* partials = nullptr;     // one allocation, three regions:
     double* part_fused = nullptr;   //   3 x nb_fused  (fused plaquette)
@@ -1741,14 +1728,16 @@ struct qcd_lattice {
     int tune_xcd_map = 2;       // QCD_TUNE_XCD_MAP
     int tune_blocks_per_cu = 0; // QCD_TUNE_BLOCKS_PER_CU (0 = no cap)
     int tune_update_shape = 3;  // QCD_TUNE_UPDATE_SHAPE: wavefronts per 64 links (3 or 1)
+    // A chain of a qcd_batch: the batch's device descriptors hold this
+    // lattice's buffers and stream sizing, so neither may change (no
+    // re-sizing start, no destroy except through the batch).
+    bool batch_owned = false;
 };
 
 namespace {
 
 thread_local char g_err[512] = "";
 int g_device = -1;
-
-
 
 int fail(int code, const char* fmt, ...)
 {
@@ -1872,6 +1861,29 @@ void launch_update_t(qcd_lattice* L, const UpdArgs& a, const LaunchOpt& o)
     }
 }
 
+// One produce() of `samples` quads per generator instance into `out` (quad
+// q = sample * n_inst + instance), or a skip of as many when out is null;
+// XOR128 / PM states advance in place in the current state buffer.
+void produce_quads_into(qcd_lattice* L, uint32_t samples, float4* out)
+{
+    const uint32_t n = L->n_inst, nb = align_up(n, kBlock) / kBlock;
+    if (L->p.prng == QCD_PRNG_RANLUX) {
+        launch_ranlux_produce(L->p.ranlux_luxury, L->rl_state, n, samples, out, L->stream);
+    } else if (L->p.prng == QCD_PRNG_XOR128) {
+        if (out)
+            hipLaunchKernelGGL(k_xor128_produce, dim3(nb), dim3(kBlock), 0, L->stream, L->rng_buf[L->rng_cur], n,
+                               samples, out);
+        else
+            hipLaunchKernelGGL(k_xor128_skip, dim3(nb), dim3(kBlock), 0, L->stream, L->rng_buf[L->rng_cur], n, samples);
+    } else {
+        if (out)
+            hipLaunchKernelGGL(k_pm_produce, dim3(nb), dim3(kBlock), 0, L->stream, L->rng_buf[L->rng_cur], n, samples,
+                               out);
+        else
+            hipLaunchKernelGGL(k_pm_skip, dim3(nb), dim3(kBlock), 0, L->stream, L->rng_buf[L->rng_cur], n, samples);
+    }
+}
+
 template <typename R, int N, int NREAL>
 int launch_update_rn(qcd_lattice* L, int parity, int mu, int kind, int start_gid, const LaunchOpt& o)
 {
@@ -1907,8 +1919,10 @@ int launch_update_rn(qcd_lattice* L, int parity, int mu, int kind, int start_gid
         launch_update_t<R, N, NREAL, false, RK_GID>(L, a, ol);
     } else {
         if (!L->rng_ready) return fail(QCD_E_STATE, "heatbath needs qcd_prng_init first");
-        if (L->p.prng == QCD_PRNG_RANLUX) {
-            launch_ranlux_produce(L->p.ranlux_luxury, L->rl_state, L->n_streams, L->s_quads, L->prn_buf, L->stream);
+        if (L->quad_buffer) {
+            // the reference's produce() into the PRN buffer, then the update
+            // reads quad k of link i at k * PRNGSTEP + i
+            produce_quads_into(L, L->s_quads, L->prn_buf);
             a.prn = L->prn_buf;
             a.prn_stride = L->n_streams;
             launch_update_t<R, N, NREAL, false, RK_BUF>(L, a, ol);
@@ -2053,24 +2067,6 @@ int launch_finalize(qcd_lattice* L, int flags, const LaunchOpt& o)
     return QCD_OK;
 }
 
-// The Polyakov loop with the final sums of `flags` in its last workgroup.
-template <typename R, int N, int NREAL>
-int launch_polyakov_fin(qcd_lattice* L, int flags, const LaunchOpt& o)
-{
-    launch(k_polyakov_fin<R, N, NREAL>, dim3(L->poly_blocks), dim3(kBlock), L->stream, o, L->table, L->g,
-           L->part_poly, fin_args(L, flags), L->rec_index + 4);
-    HIPCHK(hipGetLastError());
-    return QCD_OK;
-}
-
-// Whether the final sums of a measurement ride in the Polyakov launch: only
-// when no kernel after it feeds them (the Wilson loop runs after it, and the
-// slice sums read its per-site traces).
-bool fin_in_polyakov(int mask)
-{
-    return (mask & QCD_MEAS_POLYAKOV) && !(mask & (QCD_MEAS_WILSON | QCD_MEAS_POLYAKOV_SLICES));
-}
-
 int ensure_device()
 {
     if (g_device < 0) {
@@ -2133,19 +2129,21 @@ int qcd_device_info(char* name, int name_len, int* compute_units, size_t* global
     return QCD_OK;
 }
 
-// Quads per stream of one produce() (suncl.cpp:2302-2307, random.cpp:219-223)
-// with PRNG_instances = PRNGSTEP = align64(V/2): align64(3 PRNGSTEP (NHIT+1)
-// NHITPar) PRNs, or, in a run that starts hot (model_start_hot),
-// align64(ROWSIZE ND + 3 PRNGSTEP (NHIT+1)) with ROWSIZE = align64(V), over
-// PRNGSTEP instances, rounded up.
+// Quads per instance of one produce() (suncl.cpp:2302-2307,
+// random.cpp:219-223): align64(3 PRNGSTEP (NHIT+1) NHITPar) quads in all,
+// or, in a run that starts hot (model_start_hot), align64(ROWSIZE ND +
+// 3 PRNGSTEP (NHIT+1)) with ROWSIZE = align64(V), over PRNG_instances
+// (PRNGSTEP = align64(V/2) unless the parameters set another count),
+// rounded up: PRNG_samples = 1 + (total - 1) / PRNG_instances.
 static uint32_t produce_quads(const qcd_params& p, uint32_t vh, bool hot_run)
 {
     auto align64 = [](uint64_t x) { return (x + 63u) / 64u * 64u; };
     const uint64_t prngstep = align64(vh);
+    const uint64_t inst = p.prng_instances ? (uint64_t)p.prng_instances : prngstep;
     const uint64_t rowsize = align64(2ull * vh);
     const uint64_t upd = 3ull * prngstep * (uint64_t)(p.nhit + 1);
     const uint64_t total = align64(hot_run ? rowsize * 4u + upd : (uint64_t)p.nhitpar * upd);
-    return (uint32_t)(1 + (total - 1) / prngstep);
+    return (uint32_t)(1 + (total - 1) / inst);
 }
 
 // Reference-stream skip of s_quads quads per produce(): XOR128 jump matrix
@@ -2194,6 +2192,9 @@ static int set_sizing(qcd_lattice* L, bool hot_run, bool* changed)
     *changed = false;
     const uint32_t S = produce_quads(L->p, L->g.vh, hot_run);
     if (S == L->s_quads && hot_run == L->hot_run) return QCD_OK;
+    if (L->batch_owned)
+        return fail(QCD_E_STATE, "a batched chain keeps its stream sizing (its batch holds the buffers): "
+                                 "no reference hot start or resume of a hot run on it");
     HIPCHK(hipStreamSynchronize(L->stream));
     *changed = true;
     L->hot_run = hot_run;
@@ -2206,12 +2207,12 @@ static int set_sizing(qcd_lattice* L, bool hot_run, bool* changed)
         (void)hipGraphDestroy(L->graph);
         L->graph = nullptr;
     }
-    if (L->p.prng == QCD_PRNG_RANLUX && L->prn_quads < S) {
+    if (L->quad_buffer && L->prn_quads < S) {
         (void)hipFree(L->prn_buf);
         L->prn_buf = nullptr;
         L->prn_quads = S;
-        if (hipMalloc((void**)&L->prn_buf, sizeof(float4) * (size_t)S * L->n_streams) != hipSuccess)
-            return fail(QCD_E_NOMEM, "RANLUX quad buffer of %u quads per stream", S);
+        if (hipMalloc((void**)&L->prn_buf, sizeof(float4) * (size_t)S * L->n_inst) != hipSuccess)
+            return fail(QCD_E_NOMEM, "PRN quad buffer of %u quads per instance", S);
     }
     return build_jumps(L);
 }
@@ -2230,6 +2231,10 @@ int qcd_lattice_create(const qcd_params* pp, qcd_lattice** out)
     if (p.nhit < 1) return fail(QCD_E_ARG, "nhit must be >= 1");
     if (p.nhitpar != 1) return fail(QCD_E_UNSUPPORTED, "NHITPar other than 1 is not implemented");
     if (p.max_measurements < 1) return fail(QCD_E_ARG, "max_measurements must be >= 1");
+    if (p.prng_instances % 64u)
+        return fail(QCD_E_ARG, "prng_instances = %u: 0 or a multiple of 64", p.prng_instances);
+    if (p.prng_instances && p.stream_mode != QCD_STREAM_REFERENCE)
+        return fail(QCD_E_ARG, "prng_instances needs the reference stream layout");
     uint64_t V = (uint64_t)p.n[0] * p.n[1] * p.n[2] * p.n[3];
     if (V / 2 > 0x7fffffffull) return fail(QCD_E_ARG, "lattice too large");
     int rc = ensure_device();
@@ -2259,6 +2264,8 @@ int qcd_lattice_create(const qcd_params* pp, qcd_lattice** out)
     uint32_t vs = (uint32_t)p.n[0] * p.n[1] * p.n[2];
     L->poly_blocks = align_up(8u * vs, kBlock) / kBlock;   // 8 lanes per spatial site
     L->n_streams = align_up(L->g.vh, 64);
+    L->n_inst = p.prng_instances ? p.prng_instances : L->n_streams;
+    L->quad_buffer = (p.prng == QCD_PRNG_RANLUX) || p.prng_instances != 0;
     L->s_quads = produce_quads(p, L->g.vh, false);
 
     hipError_t e;
@@ -2271,8 +2278,8 @@ int qcd_lattice_create(const qcd_params* pp, qcd_lattice** out)
     size_t part_doubles = (size_t)3 * L->nb_fused + (size_t)2 * 2 * L->upd_blocks + (size_t)4 * L->poly_blocks +
                           (size_t)L->wil_blocks;
     if (hipMalloc(&L->table, L->table_bytes) != hipSuccess ||
-        hipMalloc((void**)&L->rng_buf[0], sizeof(uint4) * L->n_streams) != hipSuccess ||
-        hipMalloc((void**)&L->rng_buf[1], sizeof(uint4) * L->n_streams) != hipSuccess ||
+        hipMalloc((void**)&L->rng_buf[0], sizeof(uint4) * L->n_inst) != hipSuccess ||
+        hipMalloc((void**)&L->rng_buf[1], sizeof(uint4) * L->n_inst) != hipSuccess ||
         hipMalloc((void**)&L->jump_cols, sizeof(uint4) * (kJumpCols + kJumpTab)) != hipSuccess ||
         hipMalloc((void**)&L->partials, sizeof(double) * part_doubles) != hipSuccess ||
         hipMalloc((void**)&L->rec_index, sizeof(int) * 8) != hipSuccess ||
@@ -2287,21 +2294,22 @@ int qcd_lattice_create(const qcd_params* pp, qcd_lattice** out)
         qcd_lattice_destroy(L);
         return fail(QCD_E_NOMEM, "device allocation failed (table %zu bytes)", L->table_bytes);
     }
-    if (p.prng == QCD_PRNG_RANLUX) {
-        // One produce() per heatbath half-step (s_quads per stream) or per hot
-        // start (8 links x 3 quads for SU(3)).
+    if (L->quad_buffer) {
+        // One produce() per heatbath half-step (s_quads per instance) or, with
+        // PRNGSTEP instances, per Haar start (8 links x 3 quads for SU(3)).
         L->prn_quads = L->s_quads > 24u ? L->s_quads : 24u;
-        if (hipMalloc((void**)&L->rl_state, sizeof(float4) * 7 * L->n_streams) != hipSuccess ||
-            hipMalloc((void**)&L->prn_buf, sizeof(float4) * (size_t)L->prn_quads * L->n_streams) != hipSuccess) {
+        if ((p.prng == QCD_PRNG_RANLUX &&
+             hipMalloc((void**)&L->rl_state, sizeof(float4) * 7 * L->n_inst) != hipSuccess) ||
+            hipMalloc((void**)&L->prn_buf, sizeof(float4) * (size_t)L->prn_quads * L->n_inst) != hipSuccess) {
             qcd_lattice_destroy(L);
-            return fail(QCD_E_NOMEM, "RANLUX buffers");
+            return fail(QCD_E_NOMEM, "PRN quad buffer");
         }
     }
     L->part_fused = L->partials;
     L->part_plq = L->part_fused + (size_t)3 * L->nb_fused;
     L->part_poly = L->part_plq + (size_t)2 * 2 * L->upd_blocks;
     L->part_wil = L->part_poly + (size_t)4 * L->poly_blocks;
-    (void)hipMemset(L->rec_index, 0, sizeof(int) * 8);   // [4]: completion counter of k_polyakov_fin
+    (void)hipMemset(L->rec_index, 0, sizeof(int) * 8);
     (void)hipMemset(L->table, 0, L->table_bytes);
     {   // records start at zero, so state images of equal runs are equal byte for byte
         const size_t m = (size_t)p.max_measurements;
@@ -2327,6 +2335,7 @@ int qcd_lattice_create(const qcd_params* pp, qcd_lattice** out)
 int qcd_lattice_destroy(qcd_lattice* L)
 {
     if (!L) return QCD_OK;
+    if (L->batch_owned) return fail(QCD_E_STATE, "a batched chain is destroyed with its batch (qcd_batch_destroy)");
     if (L->stream) (void)hipStreamSynchronize(L->stream);
     if (L->exec) (void)hipGraphExecDestroy(L->exec);
     if (L->graph) (void)hipGraphDestroy(L->graph);
@@ -2424,15 +2433,14 @@ static int random_start(qcd_lattice* L)
     if (rc) return rc;
     if ((rc = qcd_prng_init(L))) return rc;   // seeding and the initial produce()
     L->gid_updates = false;
-    const uint32_t n = L->n_streams, S = L->s_quads;
+    const uint32_t n = L->n_inst, S = L->s_quads;
     const uint64_t V = 2ull * L->g.vh;
     const uint64_t rowsize = (V + 63u) / 64u * 64u;
     const uint64_t used = (L->p.sun == 3) ? V + 2u * rowsize : V;   // quads one direction reads
     const uint32_t need = (uint32_t)((used + n - 1) / n);
     if (need > S) return fail(QCD_E_STATE, "internal: random start reads %u of %u quads", need, S);
-    float4* buf = L->prn_buf;   // RANLUX: prn_quads >= S quads per stream
-    if (L->p.prng != QCD_PRNG_RANLUX)
-        HIPCHK(hipMalloc((void**)&buf, sizeof(float4) * (size_t)need * n));
+    float4* buf = L->prn_buf;   // the quad buffer: prn_quads >= S quads per instance
+    if (!L->quad_buffer) HIPCHK(hipMalloc((void**)&buf, sizeof(float4) * (size_t)need * n));
     const uint32_t nb = align_up(n, kBlock) / kBlock;
     for (int d = 0; d < 4 && rc == QCD_OK; ++d) {
         if (L->p.prng == QCD_PRNG_RANLUX) {
@@ -2452,7 +2460,7 @@ static int random_start(qcd_lattice* L)
         if (rc == QCD_OK) ++L->prng_counter;
     }
     const hipError_t e = hipStreamSynchronize(L->stream);
-    if (L->p.prng != QCD_PRNG_RANLUX) (void)hipFree(buf);
+    if (!L->quad_buffer) (void)hipFree(buf);
     if (rc == QCD_OK && e != hipSuccess) rc = fail(QCD_E_HIP, "random start: %s", hipGetErrorString(e));
     return rc;
 }
@@ -2482,6 +2490,9 @@ int qcd_lattice_init(qcd_lattice* L, int start)
     }
     if (start == QCD_START_HOT) {
         if (!L->rng_ready) return fail(QCD_E_STATE, "the hot start draws from the streams: call qcd_prng_init first");
+        if (L->p.prng_instances)
+            return fail(QCD_E_UNSUPPORTED, "the Haar start draws one stream per site pair: not with prng_instances set "
+                                           "(cold, GID or the reference's random start)");
         L->gid_updates = false;
         const int rc = DISPATCH(L, launch_simple, L, 5, LaunchOpt{});
         // the reference produces before each of its four init kernels
@@ -2504,9 +2515,9 @@ int qcd_prng_init(qcd_lattice* L)
         // fills the quad buffer before every heatbath half-step.
         std::vector<float> tab;
         srand(seed);
-        ranlux_seed_table(L->n_streams, tab);
+        ranlux_seed_table(L->n_inst, tab);
         HIPCHK(hipMemcpyAsync(L->rl_state, tab.data(), sizeof(float) * tab.size(), hipMemcpyHostToDevice, L->stream));
-        launch_ranlux_produce(L->p.ranlux_luxury, L->rl_state, L->n_streams, L->s_quads, nullptr, L->stream);
+        launch_ranlux_produce(L->p.ranlux_luxury, L->rl_state, L->n_inst, L->s_quads, nullptr, L->stream);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(L->stream));
         L->rng_ready = true;
@@ -2515,7 +2526,7 @@ int qcd_prng_init(qcd_lattice* L)
     }
     if (L->p.prng != QCD_PRNG_XOR128 && L->p.prng != QCD_PRNG_PM)
         return fail(QCD_E_UNSUPPORTED, "generator %d is not available", L->p.prng);
-    std::vector<uint32_t> tab((size_t)4 * L->n_streams);
+    std::vector<uint32_t> tab((size_t)4 * L->n_inst);
     srand(seed);
     // XOR128: instance 0 = XOR128_initialize_CPU (4 x rand(), random.cpp:675-681),
     // others 4 x rand() (random.cpp:653-659).  PM: every entry rand() % (2^31-1)
@@ -2526,12 +2537,12 @@ int qcd_prng_init(qcd_lattice* L)
     HIPCHK(hipMemcpyAsync(L->rng_buf[0], tab.data(), sizeof(uint32_t) * tab.size(), hipMemcpyHostToDevice, L->stream));
     // The initial produce() of PRNG::initialize (random.cpp:276) consumes S
     // quads of every instance before the first half-step.
-    uint32_t nb = align_up(L->n_streams, kBlock) / kBlock;
+    uint32_t nb = align_up(L->n_inst, kBlock) / kBlock;
     if (L->p.stream_mode == QCD_STREAM_REFERENCE) {
         if (L->p.prng == QCD_PRNG_XOR128)
-            hipLaunchKernelGGL(k_xor128_skip, dim3(nb), dim3(kBlock), 0, L->stream, L->rng_buf[0], L->n_streams, L->s_quads);
+            hipLaunchKernelGGL(k_xor128_skip, dim3(nb), dim3(kBlock), 0, L->stream, L->rng_buf[0], L->n_inst, L->s_quads);
         else
-            hipLaunchKernelGGL(k_pm_skip, dim3(nb), dim3(kBlock), 0, L->stream, L->rng_buf[0], L->n_streams, L->s_quads);
+            hipLaunchKernelGGL(k_pm_skip, dim3(nb), dim3(kBlock), 0, L->stream, L->rng_buf[0], L->n_inst, L->s_quads);
         HIPCHK(hipGetLastError());
     }
     HIPCHK(hipStreamSynchronize(L->stream));
@@ -2719,7 +2730,6 @@ static int measure_impl(qcd_lattice* L, int mask, bool fused_plq, const LaunchOp
         flags |= 32;
     }
     if (rc) return rc;
-    if (fin_in_polyakov(mask)) return DISPATCH(L, launch_polyakov_fin, L, flags | 4, o_poly);
     if (mask & (QCD_MEAS_POLYAKOV | QCD_MEAS_POLYAKOV_SLICES)) {
         L->want_slices = (mask & QCD_MEAS_POLYAKOV_SLICES) != 0;
         rc = DISPATCH(L, launch_simple, L, 4, o_poly);
@@ -2890,8 +2900,7 @@ static int enqueue_cycle(qcd_lattice* L, int n_hb, int n_or, int mask, std::vect
     LaunchOpt op, oq, of;
     if (ev) {
         LaunchOpt* slots[3] = {&op, &oq, &of};
-        const bool want[3] = {(mask & QCD_MEAS_PLAQUETTE) && !fuse, (mask & QCD_MEAS_POLYAKOV) != 0,
-                              !fin_in_polyakov(mask)};
+        const bool want[3] = {(mask & QCD_MEAS_PLAQUETTE) && !fuse, (mask & QCD_MEAS_POLYAKOV) != 0, true};
         for (int k = 0; k < 3; ++k)
             if (want[k]) {
                 slots[k]->e0 = (*ev)[2 * (*ev_used)];
@@ -3096,8 +3105,8 @@ int qcd_prng_produce_host(int prng, int ranlux_luxury, unsigned n, unsigned samp
 
 namespace {
 
-constexpr char kStateMagic[8] = {'Q', 'C', 'D', 'S', 'T', 'A', 'T', '1'};
-constexpr size_t kStateHead = 128;   // bytes of the head qcd_lattice_state_save writes
+constexpr char kStateMagic[8] = {'Q', 'C', 'D', 'S', 'T', 'A', 'T', '2'};
+constexpr size_t kStateHead = 136;   // bytes of the head qcd_lattice_state_save writes
 
 struct ByteWriter {
     unsigned char* p;
@@ -3113,7 +3122,7 @@ struct ByteReader {
 
 size_t rng_state_bytes(const qcd_lattice* L)
 {
-    return (L->p.prng == QCD_PRNG_RANLUX ? 7u : 1u) * sizeof(uint4) * (size_t)L->n_streams;
+    return (L->p.prng == QCD_PRNG_RANLUX ? 7u : 1u) * sizeof(uint4) * (size_t)L->n_inst;
 }
 
 size_t state_image_bytes(const qcd_lattice* L)
@@ -3224,6 +3233,8 @@ int qcd_lattice_state_save(qcd_lattice* L, void* buf, size_t bytes)
     for (int k = 0; k < 4; ++k) w.u32((uint32_t)idx[k]);
     w.u64(L->table_bytes);
     w.u64(rng_state_bytes(L));
+    w.u32(L->p.prng_instances);
+    w.u32(0u);
     if ((size_t)(w.p - (unsigned char*)buf) != kStateHead) return fail(QCD_E_STATE, "internal: state head size");
     HIPCHK(hipMemcpy(w.p, L->table, L->table_bytes, hipMemcpyDeviceToHost));
     w.p += L->table_bytes;
@@ -3266,6 +3277,8 @@ int qcd_lattice_state_load(qcd_lattice* L, const void* buf, size_t bytes)
     for (int k = 0; k < 4; ++k) idx[k] = (int)r.u32();
     if (r.u64() != L->table_bytes || r.u64() != rng_state_bytes(L))
         return fail(QCD_E_ARG, "state image sizes do not match this lattice");
+    if (r.u32() != L->p.prng_instances) return fail(QCD_E_ARG, "state image has another prng_instances than this lattice");
+    r.p += 4;
     if (ready & ~3u) return fail(QCD_E_ARG, "state image with unknown start flags %#x", ready);
     bool changed = false;
     int rc = set_sizing(L, (ready & 2u) != 0, &changed);   // the run's produce() size
@@ -3324,6 +3337,8 @@ int qcd_prng_replay(qcd_lattice* L, unsigned counter, int start)
         return fail(QCD_E_ARG, "bad start type %d", start);
     if (start == QCD_START_RANDOM && L->p.prng != QCD_PRNG_RANLUX && L->p.stream_mode != QCD_STREAM_REFERENCE)
         return fail(QCD_E_UNSUPPORTED, "the random start reads produce() buffers: it needs QCD_STREAM_REFERENCE");
+    if (start == QCD_START_HOT && L->p.prng_instances)
+        return fail(QCD_E_UNSUPPORTED, "no Haar start with prng_instances set");
     bool changed = false;
     int rc = set_sizing(L, start == QCD_START_RANDOM, &changed);
     if (rc) return rc;
@@ -3339,13 +3354,13 @@ int qcd_prng_replay(qcd_lattice* L, unsigned counter, int start)
     if (start == QCD_START_RANDOM && left > 0 && left < 4)
         return fail(QCD_E_ARG, "counter %u cannot follow a hot start (it takes 4 productions)", counter);
     const bool ranlux = (L->p.prng == QCD_PRNG_RANLUX);
-    const uint32_t nb = align_up(L->n_streams, kBlock) / kBlock;
+    const uint32_t nb = align_up(L->n_inst, kBlock) / kBlock;
     if (start == QCD_START_HOT && left > 0) {
         // the hot start's draw: 8 links x (3 or 1) quads per stream
         if (left < 4) return fail(QCD_E_ARG, "counter %u cannot follow a hot start (it takes 4 productions)", counter);
         const uint32_t q = (L->p.sun == 3) ? 24u : 8u;
         if (ranlux) {
-            launch_ranlux_produce(L->p.ranlux_luxury, L->rl_state, L->n_streams, q, nullptr, L->stream);
+            launch_ranlux_produce(L->p.ranlux_luxury, L->rl_state, L->n_inst, q, nullptr, L->stream);
         } else {
             const uint32_t nv = L->g.vh, nbv = align_up(nv, kBlock) / kBlock;
             if (L->p.prng == QCD_PRNG_XOR128)
@@ -3359,7 +3374,7 @@ int qcd_prng_replay(qcd_lattice* L, unsigned counter, int start)
     if (left > 0) {
         if (ranlux) {
             for (uint32_t k = 0; k < left; ++k)
-                launch_ranlux_produce(L->p.ranlux_luxury, L->rl_state, L->n_streams, L->s_quads, nullptr, L->stream);
+                launch_ranlux_produce(L->p.ranlux_luxury, L->rl_state, L->n_inst, L->s_quads, nullptr, L->stream);
             HIPCHK(hipGetLastError());
         } else if (L->p.stream_mode != QCD_STREAM_REFERENCE) {
             return fail(QCD_E_UNSUPPORTED,
@@ -3375,7 +3390,7 @@ int qcd_prng_replay(qcd_lattice* L, unsigned counter, int start)
                 return fail(QCD_E_HIP, "jump matrix upload failed");
             }
             hipLaunchKernelGGL(k_xor128_jump_apply, dim3(nb), dim3(kBlock), 0, L->stream, L->rng_buf[L->rng_cur],
-                               L->n_streams, (const uint4*)d);
+                               L->n_inst, (const uint4*)d);
             const hipError_t e1 = hipGetLastError(), e2 = hipStreamSynchronize(L->stream);
             (void)hipFree(d);
             if (e1 != hipSuccess || e2 != hipSuccess)
@@ -3387,7 +3402,7 @@ int qcd_prng_replay(qcd_lattice* L, unsigned counter, int start)
                 b = pm_mulmod(b, b);
             }
             hipLaunchKernelGGL(k_pm_jump_apply, dim3(nb), dim3(kBlock), 0, L->stream, L->rng_buf[L->rng_cur],
-                               L->n_streams, ak);
+                               L->n_inst, ak);
             HIPCHK(hipGetLastError());
         }
     }
@@ -3575,8 +3590,20 @@ int batch_halfstep_rn(qcd_batch* B, int parity, int mu, int kind, int meas_mode,
     return QCD_OK;
 }
 
+// The batch kernels take one stream-buffer index for every chain: a chain
+// swept on its own through its lattice handle would have flipped only its
+// own, so the batch refuses to continue from such a state.
+int batch_streams_in_step(const qcd_batch* B)
+{
+    for (const auto* L : B->chains)
+        if (L->rng_cur != B->rng_cur || L->s_quads != B->chains[0]->s_quads)
+            return fail(QCD_E_STATE, "a batched chain was stepped or re-sized on its own");
+    return QCD_OK;
+}
+
 int batch_sweep(qcd_batch* B, int kind, bool fuse_plaquette)
 {
+    if (const int rc = batch_streams_in_step(B)) return rc;
     for (int parity = 1; parity >= 0; --parity)
         for (int mu = 0; mu < 4; ++mu) {
             const int meas = (fuse_plaquette && parity == 0 && mu >= 1) ? mu : 0;
@@ -3599,6 +3626,7 @@ int batch_polyakov_rn(qcd_batch* B)
 // plaquette (fused into the last sweep) and Polyakov records of every chain.
 int batch_enqueue_cycle(qcd_batch* B, int n_hb, int n_or, int mask)
 {
+    if (const int rc = batch_streams_in_step(B)) return rc;
     const int total = n_hb + n_or;
     const bool fuse = (mask & QCD_MEAS_PLAQUETTE) && total > 0;
     for (int k = 0; k < total; ++k) {
@@ -3628,6 +3656,8 @@ int qcd_batch_create(const qcd_params* common, int nchains, const qcd_chain_desc
     if (!common || !chains || !out || nchains < 1) return fail(QCD_E_ARG, "null argument or no chains");
     if (common->prng != QCD_PRNG_XOR128 && common->prng != QCD_PRNG_PM && common->prng != QCD_PRNG_RANLUX)
         return fail(QCD_E_UNSUPPORTED, "batched chains need per-site streams (XOR128, PM or RANLUX)");
+    if (common->prng_instances)
+        return fail(QCD_E_UNSUPPORTED, "batched chains run PRNGSTEP generator instances (prng_instances 0)");
     *out = nullptr;
     qcd_batch* B = new qcd_batch;
     B->p = *common;
@@ -3643,6 +3673,11 @@ int qcd_batch_create(const qcd_params* common, int nchains, const qcd_chain_desc
             return rc;
         }
         B->chains.push_back(L);
+        L->batch_owned = true;
+        if (L->s_quads != B->chains[0]->s_quads) {   // one produce() size for the batched RANLUX launch
+            qcd_batch_destroy(B);
+            return fail(QCD_E_UNSUPPORTED, "batched chains need one stream production size");
+        }
     }
     // buckets: chains of equal spatial extent
     for (int c = 0; c < nchains; ++c) {
@@ -3690,7 +3725,10 @@ int qcd_batch_destroy(qcd_batch* B)
     if (B->stream) (void)hipStreamSynchronize(B->stream);
     if (B->exec) (void)hipGraphExecDestroy(B->exec);
     if (B->graph) (void)hipGraphDestroy(B->graph);
-    for (auto* L : B->chains) qcd_lattice_destroy(L);
+    for (auto* L : B->chains) {
+        L->batch_owned = false;
+        qcd_lattice_destroy(L);
+    }
     (void)hipFree(B->d_desc);
     (void)hipFree(B->d_map);
     if (B->stream) (void)hipStreamDestroy(B->stream);

@@ -19,6 +19,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <exception>
 #include <string>
 #include <vector>
 
@@ -265,10 +266,13 @@ int apply_settings(qcd_lattice* lat, const qcd_run_info& ri)
 }
 
 // Parameter fields of an exact image head (see qcd_lattice_state_save):
-// magic[8], head bytes, 14 uint32 fields, beta bits.
+// magic[8], head bytes, 14 uint32 fields, beta bits; version 2 holds
+// prng_instances at byte 128 (version 1 had none: 0).
 bool image_params(const unsigned char* head, size_t n, qcd_params* q)
 {
-    if (n < 8 + 4 + 14 * 4 + 8 || memcmp(head, "QCDSTAT1", 8) != 0) return false;
+    const bool v2 = n >= 8 && memcmp(head, "QCDSTAT2", 8) == 0;
+    if (n < 8 + 4 + 14 * 4 + 8 || (!v2 && memcmp(head, "QCDSTAT1", 8) != 0)) return false;
+    if (v2 && n < 132) return false;
     uint32_t f[14];
     memcpy(f, head + 12, sizeof(f));
     q->sun = (int)f[0];
@@ -283,6 +287,8 @@ bool image_params(const unsigned char* head, size_t n, qcd_params* q)
     q->randseries = f[12];
     q->max_measurements = (int)f[13];
     memcpy(&q->beta, head + 12 + 56, 8);
+    q->prng_instances = 0;
+    if (v2) memcpy(&q->prng_instances, head + 128, 4);
     return true;
 }
 
@@ -292,7 +298,7 @@ bool image_params(const unsigned char* head, size_t n, qcd_params* q)
 bool image_start_flags(const unsigned char* head, size_t n, uint32_t* flags)
 {
     constexpr size_t at = 8 + 4 + 14 * 4 + 8 + 4;
-    if (n < at + 4 || memcmp(head, "QCDSTAT1", 8) != 0) return false;
+    if (n < at + 4 || (memcmp(head, "QCDSTAT1", 8) != 0 && memcmp(head, "QCDSTAT2", 8) != 0)) return false;
     memcpy(flags, head + at, 4);
     return true;
 }
@@ -302,7 +308,7 @@ bool same_params(const qcd_params& a, const qcd_params& b)
     bool s = a.sun == b.sun && a.precision == b.precision && a.storage_reals == b.storage_reals &&
              a.nhit == b.nhit && a.nhitpar == b.nhitpar && a.prng == b.prng && a.stream_mode == b.stream_mode &&
              a.randseries == b.randseries && a.max_measurements == b.max_measurements &&
-             memcmp(&a.beta, &b.beta, 8) == 0;
+             a.prng_instances == b.prng_instances && memcmp(&a.beta, &b.beta, 8) == 0;
     if (a.prng == QCD_PRNG_RANLUX) s = s && a.ranlux_luxury == b.ranlux_luxury;
     for (int k = 0; k < 4; ++k) s = s && a.n[k] == b.n[k];
     return s;
@@ -330,16 +336,16 @@ int open_state(const char* path, File& fh, Header& h, qcd_params& q, qcd_run_inf
         uint64_t n = 0;
         if (fseeko(fh.f, (off_t)end, SEEK_SET) != 0 || !get(fh.f, magic, 8) || !get(fh.f, &n, 8))
             return err(QCD_E_ARG, "cannot read state file extension");
-        if (memcmp(magic, kExtMagic, 8) == 0 && size >= end + 16 + n) ext_bytes = n;
+        // n comes from the file: compare against what is left, so a huge
+        // value cannot wrap the bound; a bad extension is treated as absent.
+        if (memcmp(magic, kExtMagic, 8) == 0 && n <= size - end - 16) ext_bytes = n;
     }
     return QCD_OK;
 }
 
 }  // namespace
 
-extern "C" {
-
-int qcd_qcg_write(qcd_lattice* lat, const char* path, const qcd_run_info* info)
+static int qcg_write(qcd_lattice* lat, const char* path, const qcd_run_info* info)
 {
     if (!lat || !path || !info) return err(QCD_E_ARG, "null argument");
     qcd_params p;
@@ -503,7 +509,7 @@ int qcd_qcg_write(qcd_lattice* lat, const char* path, const qcd_run_info* info)
     return QCD_OK;
 }
 
-int qcd_qcg_read_header(const char* path, qcd_params* params, qcd_run_info* info, int* has_exact)
+static int qcg_read_header(const char* path, qcd_params* params, qcd_run_info* info, int* has_exact)
 {
     File fh;
     Header h;
@@ -524,7 +530,7 @@ int qcd_qcg_read_header(const char* path, qcd_params* params, qcd_run_info* info
     return QCD_OK;
 }
 
-int qcd_qcg_read(qcd_lattice* lat, const char* path, qcd_run_info* info)
+static int qcg_read(qcd_lattice* lat, const char* path, qcd_run_info* info)
 {
     if (!lat) return err(QCD_E_ARG, "null lattice");
     File fh;
@@ -630,6 +636,37 @@ int qcd_qcg_read(qcd_lattice* lat, const char* path, qcd_run_info* info)
     if (start != QCD_START_GID && (rc = qcd_prng_replay(lat, ri.prng_counter, start))) return rc;
     if (info) *info = ri;
     return QCD_OK;
+}
+
+// The C ABI: no C++ exception (a failed allocation for a corrupt size, a
+// stream error) leaves these entry points; it becomes an error code.
+extern "C" {
+
+int qcd_qcg_write(qcd_lattice* lat, const char* path, const qcd_run_info* info)
+{
+    try {
+        return qcg_write(lat, path, info);
+    } catch (const std::exception& e) {
+        return err(QCD_E_NOMEM, std::string("qcd_qcg_write: ") + e.what());
+    }
+}
+
+int qcd_qcg_read_header(const char* path, qcd_params* params, qcd_run_info* info, int* has_exact)
+{
+    try {
+        return qcg_read_header(path, params, info, has_exact);
+    } catch (const std::exception& e) {
+        return err(QCD_E_ARG, std::string("qcd_qcg_read_header: ") + e.what());
+    }
+}
+
+int qcd_qcg_read(qcd_lattice* lat, const char* path, qcd_run_info* info)
+{
+    try {
+        return qcg_read(lat, path, info);
+    } catch (const std::exception& e) {
+        return err(QCD_E_ARG, std::string("qcd_qcg_read: ") + e.what());
+    }
 }
 
 }  // extern "C"

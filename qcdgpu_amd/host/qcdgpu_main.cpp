@@ -21,7 +21,10 @@
 // Keys added by this program (absent from the reference): PRECISION
 // (SINGLE/DOUBLE, the reference sets it in QCDGPU.cpp), NOR (overrelaxation
 // sweeps after every heatbath sweep, default 0), STORAGE (18 or 12 reals per
-// SU(3) link), STREAMS (REFERENCE or FAST generator layout), LUXURY is
+// SU(3) link), STREAMS (REFERENCE or FAST generator layout),
+// PRNG_INSTANCES (the generator instance count the reference takes from the
+// device, CL_DEVICE_IMAGE3D_MAX_WIDTH; 0, the default, means one instance
+// per link of a half-step), LUXURY is
 // carried in the PRNG name (RANLUX0..RANLUX4) as in the reference; INIT,
 // STATEFILE, SAVESTATE and SAVEEVERY set what QCDGPU.cpp hardcodes for
 // checkpoints (INIT, fload, turnoff_config_save,
@@ -71,6 +74,7 @@ struct Settings {
     bool double_precision = true;    // QCDGPU.cpp: model_precision_double
     int nor = 0, storage = 0;        // storage 0: 18 reals (SU(3)) / 4 (SU(2))
     bool fast_streams = false;
+    unsigned prng_instances = 0;
     // checkpoints (QCDGPU.cpp:46,180,203,215): INIT 0 continues the run in
     // STATEFILE (relative to OUTPUTPATH unless absolute); SAVESTATE writes
     // .qcg files after the start measurement, every SAVEEVERY seconds (checked
@@ -182,6 +186,7 @@ bool apply(Settings& st, const std::string& key, const std::string& val)
     else if (key == "VERIFY") st.verify = iv != 0;
     else if (key == "NOR") st.nor = (int)iv;
     else if (key == "STORAGE") st.storage = (int)iv;
+    else if (key == "PRNG_INSTANCES") st.prng_instances = (unsigned)iv;
     else if (key == "STREAMS") {
         const std::string u = upper(val);
         if (u == "FAST") st.fast_streams = true;
@@ -248,6 +253,7 @@ bool to_params(const Settings& st, qcd_params& p, std::string& why)
     p.stream_mode = st.fast_streams ? QCD_STREAM_FAST : QCD_STREAM_REFERENCE;
     p.randseries = st.randseries;
     p.max_measurements = st.iter;
+    p.prng_instances = st.prng_instances;
     return true;
 }
 
@@ -273,6 +279,7 @@ void from_state(Settings& st, const qcd_params& p, const qcd_run_info& r)
                                           : "RANLUX" + std::to_string(p.ranlux_luxury);
     st.randseries = p.randseries;
     st.fast_streams = (p.stream_mode == QCD_STREAM_FAST);
+    st.prng_instances = p.prng_instances;
     st.ints = (int)r.start;
     st.nav = (int)r.nav;
     st.niter = (int)r.niter;
@@ -312,7 +319,8 @@ void print_params(const Settings& st)
            st.storage ? st.storage : (st.group == 3 ? 18 : 4), st.fast_streams ? "FAST" : "REFERENCE");
     printf("OUTPUTPATH=%s\nFINISHPATH=%s\nPLATFORM=%d\nDEVICE=%d\n", st.outputpath.c_str(), st.finishpath.c_str(),
            st.platform, st.device);
-    printf("ACTIONSDIFF=%d\nVERIFY=%d\n", st.actions_diff ? 1 : 0, st.verify ? 1 : 0);
+    printf("ACTIONSDIFF=%d\nVERIFY=%d\nPRNG_INSTANCES=%u\n", st.actions_diff ? 1 : 0, st.verify ? 1 : 0,
+           st.prng_instances);
     printf("INIT=%d\nSTATEFILE=%s\nSAVESTATE=%d\nSAVEEVERY=%.3f\n", st.init, st.statefile.c_str(),
            st.save_state ? 1 : 0, st.save_every);
 }

@@ -32,7 +32,12 @@ class Lattice:
         stream="reference",
         randseries=54,
         max_measurements=1024,
+        prng_instances=0,
     ):
+        """prng_instances: the reference's PRNG_instances (0: one instance
+        per link of a half-step, the fused per-site streams; otherwise a
+        multiple of 64 and the reference's PRN buffer is produced before
+        every heatbath half-step)."""
         p = _abi.Params()
         p.sun = sun
         for i in range(4):
@@ -47,6 +52,7 @@ class Lattice:
         p.stream_mode = _abi.STREAM_REFERENCE if stream == "reference" else _abi.STREAM_FAST
         p.randseries = int(randseries)
         p.max_measurements = int(max_measurements)
+        p.prng_instances = int(prng_instances)
         self.params = p
         self.sun = sun
         self.dims = tuple(int(d) for d in dims)

@@ -59,3 +59,24 @@ def test_missing_library_raises(monkeypatch):
     monkeypatch.setattr(_abi, "LIB_PATH", "/nonexistent/libqcdhip.so")
     with pytest.raises(qcdgpu_amd.QCDError):
         _abi.lib()
+
+
+@pytest.mark.parametrize("inst,stream,message", [(100, 0, b"multiple of 64"), (64, 1, b"reference stream layout")])
+def test_prng_instances_validated(inst, stream, message):
+    """PRNG_instances: 0 or a multiple of 64, reference stream layout only
+    (checked before any device call)."""
+    lib = _abi.lib()
+    p = _abi.Params()
+    p.sun = 3
+    for i, v in enumerate((4, 4, 4, 4)):
+        p.n[i] = v
+    p.storage_reals = 18
+    p.nhit = 10
+    p.nhitpar = 1
+    p.max_measurements = 4
+    p.prng = _abi.PRNG_XOR128
+    p.stream_mode = stream
+    p.prng_instances = inst
+    h = ctypes.c_void_p()
+    assert lib.qcd_lattice_create(ctypes.byref(p), ctypes.byref(h)) == 1
+    assert message in lib.qcd_last_error()

@@ -59,19 +59,22 @@ CASES = [
     ((8, 16, 8, 4), 3, (3, 1, 8)),    # one z-group: mode 3 keeps the slabs
     ((8, 16, 16, 4), 3, (3, 2, 8)),   # blocks of 2 x-values x 8 z-rows
     ((16, 16, 16, 4), 3, (3, 4, 8)),  # blocks of 4 x-values x 8 z-rows
+    ((8, 32, 8, 4), 2, (2, 1, 8)),    # N2 = 32 as in the benchmark: 4 z-rows per unit
+    ((8, 32, 8, 4), 3, (3, 2, 4)),
 ]
 STORAGE = [("fp64", 18), ("fp32", 18), ("fp32", 12)]
 
 
 @pytest.mark.parametrize("prec,storage", STORAGE, ids=["%s-%d" % s for s in STORAGE])
 @pytest.mark.parametrize("dims,mode,expect", CASES, ids=["%s-m%d" % ("x".join(map(str, c[0])), c[1]) for c in CASES])
-def test_dispatch_order_vs_oracle(O, dims, mode, expect, prec, storage):
-    _case("dispatch %s mode %d %s-%d" % (dims, mode, prec, storage))
+@pytest.mark.parametrize("shape", [3, 1])
+def test_dispatch_order_vs_oracle(O, dims, mode, expect, prec, storage, shape):
+    _case("dispatch %s mode %d %s-%d shape %d" % (dims, mode, prec, storage, shape))
     seed = 11
     L, ref = _oracle_run(O, dims, seed, 2, 1)
     with Q.Lattice(sun=3, dims=dims, beta=5.9, precision=prec, storage_reals=storage, randseries=seed,
                    max_measurements=4) as lat:
-        lat.tune(xcd_map=mode)
+        lat.tune(xcd_map=mode, update_shape=shape)
         t = lat.tuning()
         assert (t["xcd_map"],) + tuple(t["xcd_block"]) == expect, t
         lat.init("cold")
@@ -91,35 +94,52 @@ def test_dispatch_order_vs_oracle(O, dims, mode, expect, prec, storage):
         assert per_link.max() <= 1e-5
 
 
+def _c1_run(mode, shape, n_cycles=2):
+    dims = (32, 32, 32, 8)
+    with Q.Lattice(sun=3, dims=dims, beta=5.9, precision="fp32", randseries=54, max_measurements=8) as lat:
+        lat.tune(xcd_map=mode, update_shape=shape)
+        t = lat.tuning()
+        assert t["xcd_map"] == mode, t
+        if mode == 2:
+            assert t["xcd_block"] == (4, 32), t
+        if mode == 3:
+            assert t["xcd_block"] == (8, 16), t
+        lat.init("cold")
+        lat.prng_init()
+        lat.capture_cycle(n_hb=1, n_or=4, plaquette=True, polyakov=True)
+        lat.launch_cycle(n_cycles)
+        lat.sync()
+        plq, poly = lat.raw_measurements()
+        assert len(plq) == n_cycles and len(poly) == n_cycles
+        return lat.save_state(), lat.export_links()
+
+
 def test_dispatch_orders_bit_identical_c1():
     """SU(3) 32^3 x 8 FP32 18-real (the benchmarked workload): two captured
     cycles of 1 heatbath + 4 overrelaxation sweeps with plaquette and
-    Polyakov records under every dispatch order and both kernel shapes give
-    byte-identical state images (links as stored, generator states, records
-    and their counts)."""
-    dims = (32, 32, 32, 8)
+    Polyakov records under every dispatch order give byte-identical state
+    images (links as stored, generator states, records and their counts)."""
     images = {}
-    for mode, shape in ((0, 3), (1, 3), (2, 3), (3, 3), (3, 1)):
-        _case("C1 cycles xcd_map=%d shape=%d" % (mode, shape))
-        with Q.Lattice(sun=3, dims=dims, beta=5.9, precision="fp32", randseries=54, max_measurements=8) as lat:
-            lat.tune(xcd_map=mode, update_shape=shape)
-            t = lat.tuning()
-            assert t["xcd_map"] == mode, t
-            if mode == 2:
-                assert t["xcd_block"] == (4, 32), t
-            if mode == 3:
-                assert t["xcd_block"] == (8, 16), t
-            lat.init("cold")
-            lat.prng_init()
-            lat.capture_cycle(n_hb=1, n_or=4, plaquette=True, polyakov=True)
-            lat.launch_cycle(2)
-            lat.sync()
-            plq, poly = lat.raw_measurements()
-            assert len(plq) == 2 and len(poly) == 2
-            images[(mode, shape)] = lat.save_state()
-    base = images[(0, 3)]
+    for mode in (0, 1, 2, 3):
+        _case("C1 cycles xcd_map=%d shape=3" % mode)
+        images[mode] = _c1_run(mode, 3)[0]
+    base = images[0]
     for k, img in images.items():
         assert img.size == base.size
         diff = int(np.count_nonzero(img != base))
-        print("  %s: %d differing bytes" % (k, diff))
+        print("  mode %d: %d differing bytes" % (k, diff))
         assert diff == 0
+
+
+@pytest.mark.parametrize("n_cycles", [1, 2])
+def test_update_shapes_identical_c1(n_cycles):
+    """The one-wavefront kernel shape against the three-wavefront one at the
+    benchmarked workload (both in dispatch order 3): state images equal byte
+    for byte."""
+    _case("C1 shapes, %d cycles" % n_cycles)
+    img3, u3 = _c1_run(3, 3, n_cycles)
+    img1, u1 = _c1_run(3, 1, n_cycles)
+    d = np.abs(u1 - u3).reshape(u1.shape[0], 4, -1).max(axis=-1)
+    nb = int(np.count_nonzero(img1 != img3))
+    print("  differing bytes %d; links differing %d of %d, max|dU| %.3e" % (nb, int((d > 0).sum()), d.size, d.max()))
+    assert nb == 0

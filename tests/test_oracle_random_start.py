@@ -112,3 +112,7 @@ def test_produce_size(O):
     assert O.produce_quads(su2, True) == 41
     odd = O.lattice(3, (6, 6, 6, 4), 5.9, 10)       # V/2 = 432: PRNGSTEP 448, ROWSIZE 896
     assert O.produce_quads(odd, False) == 33 and O.produce_quads(odd, True) == 1 + (4 * 896 + 33 * 448 - 1) // 448
+    # other PRNG_instances (random.cpp:219-223: PRNG_samples = 1 + (samples - 1) / instances):
+    # 4,325,376 quads in all for 32^3 x 8
+    assert O.produce_quads(cold, False, 2048) == 2112 and O.produce_quads(cold, False, 16384) == 264
+    assert O.produce_quads(cold, True, 2048) == 1 + (4 * 262144 + 4325376 - 1) // 2048

@@ -160,3 +160,19 @@ def test_library_rejects_truncated(tmp_path):
         read_qcg_header(path)
     with pytest.raises(QCDError, match="cannot open"):
         read_qcg_header(tmp_path / "missing.qcg")
+
+
+@pytest.mark.parametrize("n", [2**64 - 1, 2**63, 1 << 40, 17])
+def test_library_ignores_bad_extension_length(tmp_path, n):
+    """An extension block whose byte count does not fit in what follows it
+    (a huge count would wrap an unsigned bound test) is treated as absent:
+    the file reads as a reference file, no state image."""
+    import struct
+
+    rng = np.random.default_rng(4)
+    h = qcg.make_header(3, (4, 4, 4, 2), 5.9, "single", "XOR128", iter=4)
+    data = qcg.encode(h, haar(rng, 128, 3)) + b"QCDGPUX1" + struct.pack("<Q", n) + b"\0" * 8
+    path = tmp_path / "e.qcg"
+    path.write_bytes(data)
+    p, _, exact = read_qcg_header(path)
+    assert not exact and p.sun == 3
