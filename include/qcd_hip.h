@@ -204,6 +204,15 @@ int  qcd_reunitarize(qcd_lattice *lat);
           plaquette: (sum ReTr spatial, sum ReTr temporal)
           polyakov : (sum ReTr P, sum ImTr P, sum |Tr P|^2, sum |Tr P|^4)   */
 int  qcd_measure(qcd_lattice *lat, int mask);
+/* qcd_measure without allocating: no reference counterpart.  For callers
+   that cannot allocate device memory (stream capture, serving loops): the
+   partial sums and per-site buffers of every measurement kernel live in one
+   fixed 16 MiB device array per GPU instead of the lattice's buffers or the
+   action slices' buffer allocated on first use.  Lattices whose buffers for
+   `mask` exceed it fail with QCD_E_ARG before any launch (SU(3) 32^3 x 8
+   with every measurement fits).  Records are those qcd_measure appends, bit
+   for bit.  Calls that overlap on one GPU share the array and race.      */
+int  qcd_measure_noalloc(qcd_lattice *lat, int mask);
 /* Copies the records so far (synchronous).  plq: 2*count, poly: 4*count.  */
 int  qcd_read_measurements(qcd_lattice *lat, double *plq, double *poly, int *count_plq, int *count_poly);
 /* ---- Wilson loop: lattice_measurement_wilson + reduce (wilson_loop.cl:55-465;

@@ -94,6 +94,7 @@ _SIGNATURES = {
     "qcd_sweep": [_P, ctypes.c_int],
     "qcd_reunitarize": [_P],
     "qcd_measure": [_P, ctypes.c_int],
+    "qcd_measure_noalloc": [_P, ctypes.c_int],
     "qcd_read_measurements": [_P, _D, _D, _I, _I],
     "qcd_reset_measurements": [_P],
     "qcd_lattice_set_wilson": [_P, ctypes.c_int, ctypes.c_int],

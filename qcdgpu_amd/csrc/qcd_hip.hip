@@ -46,6 +46,7 @@ using namespace qcd;
 namespace {
 
 constexpr int kBlock = 256;
+constexpr size_t kScratchBytesDev = (size_t)16 << 20;   // the allocation-free measurement's device array
 constexpr double kPi2 = 6.2831853071795864769252867665590;
 
 // RK_BUF: quads read from a buffer filled by a produce kernel (RANLUX),
@@ -1664,6 +1665,8 @@ __global__ __launch_bounds__(kBlock) void k_finalize_batch(const ChainDesc* cd, 
 // ============================================================================
 // Host side
 // ============================================================================
+__device__ __align__(256) double scratch[kScratchBytesDev / sizeof(double)];
+
 struct qcd_lattice {
     qcd_params p;
     Geom g;
@@ -1738,6 +1741,16 @@ namespace {
 
 thread_local char g_err[512] = "";
 int g_device = -1;
+
+// Device memory of the allocation-free measurement (qcd_measure_noalloc):
+// its partial sums and per-site buffers live in this file-scope array,
+// which every GPU holds once, at its own address, until the process ends.
+// Host code never uses the name `scratch` as an address: qcd_device_open
+// looks up each device's copy with hipGetSymbolAddress and keeps it in
+// g_scratch[device].
+constexpr size_t kScratchBytes = kScratchBytesDev;
+constexpr int kMaxDevices = 64;
+double* g_scratch[kMaxDevices] = {};
 
 int fail(int code, const char* fmt, ...)
 {
@@ -2106,6 +2119,14 @@ int qcd_device_open(int platform, int device)
     HIPCHK(hipGetDeviceCount(&count));
     if (device < 0 || device >= count) return fail(QCD_E_ARG, "device %d not present (%d devices)", device, count);
     HIPCHK(hipSetDevice(device));
+    if (device < kMaxDevices && !g_scratch[device]) {
+        // this device's copy of the allocation-free measurement's array
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(QCD_E_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+        g_scratch[device] = (double*)ptr;
+    }
     g_device = device;
     return QCD_OK;
 }
@@ -2750,6 +2771,47 @@ int qcd_measure(qcd_lattice* L, int mask)
 {
     if (!L) return fail(QCD_E_ARG, "null lattice");
     return measure_impl(L, mask, false, LaunchOpt{}, LaunchOpt{}, LaunchOpt{});
+}
+
+// qcd_measure with every partial sum and per-site buffer in the device's
+// `scratch` array instead of the lattice's own (or lazily allocated)
+// buffers: no allocation, so it can run under stream capture or in loops
+// that must not allocate.  Each kernel writes every slot it later reads, so
+// nothing in the array needs clearing.  Calls that overlap on one GPU share
+// the array and race.
+int qcd_measure_noalloc(qcd_lattice* L, int mask)
+{
+    if (!L) return fail(QCD_E_ARG, "null lattice");
+    if (g_device < 0 || g_device >= kMaxDevices || !g_scratch[g_device])
+        return fail(QCD_E_STATE, "no device scratch (qcd_device_open failed or was not called)");
+    const uint32_t vs = L->g.vh * 2u / (uint32_t)L->g.n4;
+    const size_t n_plq = (size_t)4 * L->upd_blocks, n_poly = (size_t)4 * L->poly_blocks;
+    const size_t n_wil = L->wil_blocks, n_f = (size_t)24 * L->upd_blocks;
+    const size_t n_site_pl = (mask & QCD_MEAS_ACTION_SLICES) ? (size_t)4 * L->g.vh : 0;
+    const size_t n_site_tr = (mask & QCD_MEAS_POLYAKOV_SLICES) ? (size_t)2 * vs : 0;
+    // every region starts on a 256-byte boundary
+    auto up = [](size_t n) { return (n + 31) & ~(size_t)31; };
+    const size_t need = up(n_plq) + up(n_poly) + up(n_wil) + up(n_f) + up(n_site_pl) + up(n_site_tr);
+    if (need * sizeof(double) > kScratchBytes)
+        return fail(QCD_E_ARG, "lattice too large for the allocation-free measurement: %zu bytes of scratch needed, "
+                               "%zu available", need * sizeof(double), kScratchBytes);
+    double* base = g_scratch[g_device];
+    double* const saved[6] = {L->part_plq, L->part_poly, L->part_wil, L->part_f, L->site_pl, L->site_tr};
+    L->part_plq = base;
+    L->part_poly = L->part_plq + up(n_plq);
+    L->part_wil = L->part_poly + up(n_poly);
+    L->part_f = L->part_wil + up(n_wil);
+    double* const pl = L->part_f + up(n_f);
+    if (n_site_pl) L->site_pl = pl;
+    if (n_site_tr) L->site_tr = pl + up(n_site_pl);
+    const int rc = measure_impl(L, mask, false, LaunchOpt{}, LaunchOpt{}, LaunchOpt{});
+    L->part_plq = saved[0];
+    L->part_poly = saved[1];
+    L->part_wil = saved[2];
+    L->part_f = saved[3];
+    L->site_pl = saved[4];
+    L->site_tr = saved[5];
+    return rc;
 }
 
 int qcd_read_measurements(qcd_lattice* L, double* plq, double* poly, int* count_plq, int* count_poly)
