@@ -171,6 +171,53 @@ def c4(args):
     print(last, flush=True)
 
 
+def c3(args):
+    """The batched-ensemble workload: 96 SU(3) chains, (N_s, N_t) drawn from
+    {12,16,20,24} x {4,6,8} with seed 54, beta_i = 5.5 + 0.7 i / 95, FP32
+    18-real, RANLUX3 per-chain streams, one step = 1 heatbath + 4
+    overrelaxation sweeps of every chain with plaquette and Polyakov
+    records, captured once as a graph after the autotuner picked each size
+    bucket's kernel shape and unit placement."""
+    import numpy as np
+
+    import qcdgpu_amd as Q
+
+    rng = np.random.default_rng(54)
+    ns = rng.choice([12, 16, 20, 24], size=96)
+    nt = rng.choice([4, 6, 8], size=96)
+    chains = [((int(ns[i]),) * 3 + (int(nt[i]),), 5.5 + 0.7 * i / 95, 1000 + i) for i in range(96)]
+    V = sum(int(np.prod(c[0])) for c in chains)
+    with Q.Batch(chains, precision="fp32", prng="RANLUX", max_measurements=args.steps + args.warmup + 4) as b:
+        b.init("hot")
+        tuning = b.autotune(repeats=4)
+        b.capture_cycle(n_hb=1, n_or=N_OR)
+        b.launch_cycle(args.warmup)
+        b.sync()
+        ms = b.time_cycle(args.steps)
+        b.sync()
+    ms_step = ms / args.steps
+    # 4V link updates per sweep at 9 x 72 B (+ 2 x 112 B of RANLUX state in
+    # the heatbath), plaquette and Polyakov 5V x 72 B
+    step_bytes = 4.0 * V * ((9 * 72 + 2 * 112) + N_OR * 9 * 72) + 5.0 * V * 72
+    gbs = step_bytes / (ms_step * 1e-3) / 1e9
+    print(json.dumps({
+        "metric": "SU(3) link updates/sec, batched ensemble",
+        "value": 5.0 * 4.0 * V / (ms_step * 1e-3),
+        "unit": "link updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": "96 SU(3) chains N_s in {12,16,20,24}, N_t in {4,6,8}, beta 5.5-6.2, RANLUX3; "
+                               "1 CM heatbath + 4 overrelaxation sweeps + plaquette + Polyakov per step",
+                   "sites": V, "tuning": tuning},
+        "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+                     "algorithmic_bytes_per_step": step_bytes},
+    }), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -193,12 +240,16 @@ def main():
     ap.add_argument("--c4-train", type=int, default=20000, help="heatbath training configurations")
     ap.add_argument("--c4-train-steps", type=int, default=1000)
     ap.add_argument("--c4-steps", type=int, default=4000, help="steps of each chain")
+    ap.add_argument("--c3", action="store_true", help="run the batched-ensemble workload (96 chains) instead")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
         return
     if args.c4:
         c4(args)
+        return
+    if args.c3:
+        c3(args)
         return
     if args.gpus != 1:
         print(

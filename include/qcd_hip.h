@@ -147,6 +147,38 @@ int  qcd_lattice_sync(qcd_lattice *lat);
                           links, workgroups of four.                         */
 enum { QCD_TUNE_XCD_MAP = 1, QCD_TUNE_BLOCKS_PER_CU = 2, QCD_TUNE_UPDATE_SHAPE = 3 };
 int  qcd_lattice_tune(qcd_lattice *lat, int knob, int value);
+/* ---- kernel timing (GPU::print_time_detailed, clinterface.cpp:1626-1666,
+        shown when GPU_debug.profiling is set, suncl.cpp:265).  With it on,
+        every kernel family the lattice launches eagerly (update half-steps,
+        PRN production, starts, measurement kernels, final sums) is timed
+        with events on the lattice stream and waited for; a captured cycle
+        is refused while it is on.  Turning it on clears the table.         */
+int  qcd_lattice_profile(qcd_lattice *lat, int on);
+int  qcd_profile_count(const qcd_lattice *lat, int *n);
+/* Entry i: the family's name, launches, mean and standard deviation in ms. */
+int  qcd_profile_entry(const qcd_lattice *lat, int i, char *name, int name_len, unsigned *starts, double *mean_ms,
+                       double *dev_ms);
+/* ---- x-slab: one GPU's share of a lattice cut into slabs along x, as the
+        reference's multi-device BIGLAT path holds it (suncl.cpp:2422: every
+        device keeps Nx + 2 x-slices; UpdateEdges, suncl.cpp:4005-4062,
+        refreshes the two edge slices).  The lattice's N1 counts the two halo
+        slices x = 0 and x = N1 - 1: with the slab on, half-steps update the
+        interior slices 1 .. N1 - 2 only, reading the halos as they stand (no
+        periodic wrap along x is ever taken); measurements still cover the
+        whole table, and cycles measure the plaquette with its own kernel.
+        Needs N1 >= 4 and N2 N3 N4 / 2 divisible by 64; no twisted
+        boundary.  Parities are the slab's own (x counted from its first
+        halo slice).                                                        */
+int  qcd_lattice_set_slab(qcd_lattice *lat, int on);
+/* Bytes of one face: the links of one parity and direction in an x-slice. */
+int  qcd_slab_face_bytes(const qcd_lattice *lat, size_t *bytes);
+/* Device-to-device copy, on the lattice's stream, of the (parity, mu) links
+   of one face: unpack = 0 copies the first (side 0) or last (side 1)
+   interior slice into dev_buf, unpack = 1 copies dev_buf into the halo on
+   that side.  After each half-step (parity, mu), a slab sends both of its
+   boundary faces of that group and receives both halos -- what a multi-GPU
+   run moves with RCCL send / recv between x-neighbours.                    */
+int  qcd_slab_face(qcd_lattice *lat, int side, int parity, int mu, void *dev_buf, int unpack);
 /* The value a knob takes in the update launches of this lattice: for
    QCD_TUNE_XCD_MAP the mode that applies after the fallbacks above, and in
    detail[0..1] (optional) the XCD block's x-values and z-rows (0 for modes
@@ -405,11 +437,21 @@ int  qcd_batch_cycle_launch(qcd_batch *batch, int repeats);
 int  qcd_batch_cycle_time(qcd_batch *batch, int repeats, double *ms);
 int  qcd_batch_sync(qcd_batch *batch);
 /* Autotuner: times every bucket's half-steps with each update-kernel shape
-   (three wavefronts per 64 links or one) and keeps the faster one per bucket
-   and update kind; chains are saved and restored around the timing, so
-   results do not depend on it.  shapes_out (optional): 2 ints per bucket
-   (heatbath, overrelaxation).  Capture the cycle after tuning.            */
+   (three wavefronts per 64 links or one) and each placement of the 64-link
+   units over the 8 XCDs (0: every chain whole on one XCD, units in index
+   order; 1: the same in cross-section order, x fastest, then t, then z;
+   2: every chain cut into eight slabs, one per XCD) and keeps the fastest
+   per bucket and update kind; chains are saved and restored around the
+   timing, so results do not depend on it.  shapes_out (optional): 2 ints
+   per bucket (heatbath, overrelaxation shape).  Capture the cycle after
+   tuning.                                                                 */
 int  qcd_batch_autotune(qcd_batch *batch, int repeats, int *shapes_out);
+/* The choices for bucket `bucket`: out[0..5] = heatbath shape,
+   overrelaxation shape, heatbath placement, overrelaxation placement, the
+   bucket's N1 (spatial extent), chains in the bucket.                      */
+int  qcd_batch_get_tuning(qcd_batch *batch, int bucket, int *out);
+/* Sets bucket `bucket`'s choices by hand (shapes 1 or 3, placements 0..2). */
+int  qcd_batch_set_tuning(qcd_batch *batch, int bucket, int shape_hb, int shape_or, int place_hb, int place_or);
 /* Chain `index` as a lattice handle (export, records, measure); the batch's
    work is complete when this returns.  The batch owns it: destroying it,
    re-sizing its streams (a reference hot start) or resuming a hot run on it
@@ -448,6 +490,32 @@ int  qcd_flow_sample(qcd_flow *flow, unsigned long long seed, int batch, double 
                      double *logq, double *action);
 /* log q and the action of given configurations (the inverse flow).         */
 int  qcd_flow_evaluate(qcd_flow *flow, const float *links, int batch, double beta, double *logq, double *action);
+/* One learned Metropolis sweep of an SU(2) FP32 lattice of the flow's
+   extents, on the lattice's stream: for every coupling layer in turn, its
+   spline parameters from the current (frozen) links, then every active link
+   independently replaced by the layer's map of a Haar draw with probability
+   min(1, exp(-dS) q(U)/q(U')) -- exact Metropolis-within-Gibbs, since the
+   layer's conditioners see frozen links only and no plaquette holds two
+   active links.  Eight layers visit every link once.  Asynchronous.        */
+int  qcd_flow_imh_sweep(qcd_flow *flow, qcd_lattice *lat, unsigned long long seed, double beta);
+/* Training by hand-written kernels (BF16 MFMA, FP32 accumulation): the
+   layer-local forward KL of `batch` heatbath configurations (host links in
+   the sample / evaluate layout) -- for every coupling layer, the sum over
+   its active links of -log q_a(U_a | frozen links) on the data itself, the
+   density the learned Metropolis sweep proposes with -- and its gradient:
+   the spline part by central differences in double precision, then conv2,
+   tanh and conv1 backwards (weight gradients as MFMA GEMMs over the sites,
+   the hidden gradient as the transposed stencil convolution).
+   qcd_flow_train_grad returns the loss per configuration and the gradient
+   summed over the batch (grad: param_count floats, optional);
+   qcd_flow_train_step takes one Adam step (beta 0.9 / 0.999) on the batch
+   mean and updates the flow's parameters; qcd_flow_get_params reads them. */
+int  qcd_flow_train_grad(qcd_flow *flow, const float *links, int batch, double *loss, float *grad);
+int  qcd_flow_train_step(qcd_flow *flow, const float *links, int batch, float lr, double *loss);
+int  qcd_flow_get_params(qcd_flow *flow, float *params, size_t count);
+/* Links accepted and proposed by qcd_flow_imh_sweep so far (synchronous);
+   reset != 0 starts the counts again.                                       */
+int  qcd_flow_imh_counts(qcd_flow *flow, unsigned long long *accepted, unsigned long long *proposed, int reset);
 
 /* ---- reference PRN buffer semantics (PRNG::produce, random.cpp:278-284;
         kernels xor128 prngcl_xor128.cl:100, pm prngcl_pm.cl:86,

@@ -85,6 +85,12 @@ _SIGNATURES = {
     "qcd_lattice_sync": [_P],
     "qcd_lattice_table_bytes": [_P, ctypes.POINTER(ctypes.c_size_t)],
     "qcd_lattice_tune": [_P, ctypes.c_int, ctypes.c_int],
+    "qcd_lattice_profile": [_P, ctypes.c_int],
+    "qcd_profile_count": [_P, ctypes.POINTER(ctypes.c_int)],
+    "qcd_profile_entry": [_P, ctypes.c_int, ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_uint), _D, _D],
+    "qcd_lattice_set_slab": [_P, ctypes.c_int],
+    "qcd_slab_face_bytes": [_P, ctypes.POINTER(ctypes.c_size_t)],
+    "qcd_slab_face": [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int],
     "qcd_lattice_get_tune": [_P, ctypes.c_int, _I, _I],
     "qcd_lattice_init": [_P, ctypes.c_int],
     "qcd_prng_init": [_P],
@@ -121,6 +127,8 @@ _SIGNATURES = {
     "qcd_batch_cycle_time": [_P, ctypes.c_int, _D],
     "qcd_batch_sync": [_P],
     "qcd_batch_autotune": [_P, ctypes.c_int, _I],
+    "qcd_batch_get_tuning": [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_int)],
+    "qcd_batch_set_tuning": [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int],
     "qcd_batch_chain": [_P, ctypes.c_int, ctypes.POINTER(_P)],
     "qcd_lattice_get_params": [_P, ctypes.POINTER(Params)],
     "qcd_prng_counter": [_P, ctypes.POINTER(ctypes.c_uint)],
@@ -138,6 +146,11 @@ _SIGNATURES = {
     "qcd_flow_set_params": [_P, ctypes.POINTER(ctypes.c_float), ctypes.c_size_t],
     "qcd_flow_sample": [_P, ctypes.c_ulonglong, ctypes.c_int, ctypes.c_double, ctypes.POINTER(ctypes.c_float), _D, _D],
     "qcd_flow_evaluate": [_P, ctypes.POINTER(ctypes.c_float), ctypes.c_int, ctypes.c_double, _D, _D],
+    "qcd_flow_imh_sweep": [_P, _P, ctypes.c_ulonglong, ctypes.c_double],
+    "qcd_flow_train_grad": [_P, ctypes.POINTER(ctypes.c_float), ctypes.c_int, _D, ctypes.POINTER(ctypes.c_float)],
+    "qcd_flow_train_step": [_P, ctypes.POINTER(ctypes.c_float), ctypes.c_int, ctypes.c_float, _D],
+    "qcd_flow_get_params": [_P, ctypes.POINTER(ctypes.c_float), ctypes.c_size_t],
+    "qcd_flow_imh_counts": [_P, ctypes.POINTER(ctypes.c_ulonglong), ctypes.POINTER(ctypes.c_ulonglong), ctypes.c_int],
     "qcd_prng_produce_host": [
         ctypes.c_int,
         ctypes.c_int,

@@ -101,12 +101,26 @@ class Batch:
         check(lib().qcd_batch_sync(self._h))
 
     def autotune(self, repeats=4):
-        """Picks the update-kernel shape per bucket and update kind; returns
-        [(heatbath shape, overrelaxation shape)] per bucket."""
+        """Picks the update-kernel shape and the unit placement over the
+        XCDs per bucket and update kind; returns tuning()."""
         nb = self.info()["buckets"]
         out = (ctypes.c_int * (2 * nb))()
         check(lib().qcd_batch_autotune(self._h, int(repeats), out))
-        return [(out[2 * k], out[2 * k + 1]) for k in range(nb)]
+        return self.tuning()
+
+    def set_tuning(self, bucket, shape=(3, 3), placement=(0, 0)):
+        check(lib().qcd_batch_set_tuning(self._h, int(bucket), int(shape[0]), int(shape[1]), int(placement[0]),
+                                         int(placement[1])))
+
+    def tuning(self):
+        """Per bucket: {"n_s", "chains", "shape": (heatbath, overrelaxation),
+        "placement": (heatbath, overrelaxation)}."""
+        res = []
+        for k in range(self.info()["buckets"]):
+            o = (ctypes.c_int * 6)()
+            check(lib().qcd_batch_get_tuning(self._h, k, o))
+            res.append({"n_s": o[4], "chains": o[5], "shape": (o[0], o[1]), "placement": (o[2], o[3])})
+        return res
 
     def chain(self, index):
         """Chain `index` as a non-owning Lattice view (export, records)."""

@@ -10,13 +10,17 @@ GPU, the others libqcdhip.so; no process loads both):
 `gen` runs the plain heatbath of the C0 form (Haar hot start, 1 Kennedy-
 Pendleton + 3 overrelaxation sweeps per configuration) and keeps every
 configuration after thermalisation.  `train` fits the flow by forward KL.
-`chain` runs two chains from the same start for the same number of steps:
+`chain` runs chains from the same start for the same number of steps:
 plain = 1 heatbath sweep per step; learned = 1 heatbath sweep and one
-independence-Metropolis step with the flow per step; both measure the
-plaquette, the Polyakov loop and the 2x2 Wilson loop every step.  It prints
-one JSON line: observables with binned errors and their difference in
-standard deviations, the Polyakov loop's tau_int, steps per second and
-independent configurations per second of each chain, and the acceptance.
+learned Metropolis sweep per step (every coupling layer proposes its active
+links from its map of Haar draws, each accepted with the exact
+Wilson-action Metropolis probability, all on the device); with --global
+also the whole-lattice independence-Metropolis step with the flow.  Every
+arm measures the plaquette, the Polyakov loop and the 2x2 Wilson loop every
+step.  It prints one JSON line: observables with binned errors and their
+difference from the plain arm in standard deviations, the Polyakov loop's
+tau_int, steps per second and independent configurations per second of
+each arm, and the acceptances.
 """
 import argparse
 import json
@@ -85,8 +89,58 @@ def _stats(series, sweeps_per_s):
     return m, e, tau, sweeps_per_s / (2.0 * tau)
 
 
-def cmd_chain(a):
+def run_arm(arm, a, flow):
+    """One chain: "plain" (heatbath), "learned" (heatbath + learned
+    Metropolis sweep) or "global" (heatbath + whole-lattice independence
+    Metropolis).  Returns the arm's result dict."""
     import qcdgpu_amd as Q
+    from qcdgpu_amd import flow as QF
+
+    dims = tuple(a.dims)
+    seed = a.seed + {"plain": 0, "learned": 1000, "global": 2000}[arm]
+    with Q.Lattice(sun=2, dims=dims, beta=a.beta, precision="fp32", prng="XOR128", randseries=seed,
+                   max_measurements=a.steps + 2) as lat:
+        lat.set_wilson(2, 2)
+        lat.prng_init()
+        lat.init("hot")
+        for _ in range(a.therm):
+            lat.sweep("hb")
+        lat.reset_measurements()
+        lw = {}
+        if arm == "learned":
+            flow.imh_counts(reset=True)
+            lat.sync()
+            t0 = time.perf_counter()
+            for step in range(a.steps):
+                lat.sweep("hb")
+                flow.imh_sweep(lat, seed * 7919 + step, a.beta)
+                lat.measure(wilson=True)
+            lat.sync()
+            secs = time.perf_counter() - t0
+            acc_n, prop_n = flow.imh_counts()
+            acc = acc_n / max(prop_n, 1)
+        else:
+            acc, secs = QF.imh_chain(lat, flow if arm == "global" else None, a.beta, a.steps, hb_per_step=1,
+                                     seed=seed + 17, wilson=True, log_weights=lw)
+        m = lat.measurements()
+        plq = m["plq_total"] / 2.0
+        pol = m["polyakov_re"]
+        wil = lat.wilson_measurements() / 2.0
+        rate = a.steps / secs
+        out = {"steps_per_s": round(rate, 1), "acceptance": round(acc, 4)}
+        for name, s in (("plaquette", plq), ("polyakov", pol), ("wilson_2x2", wil)):
+            mu, err, tau, ind = _stats(s, rate)
+            out[name] = {"mean": mu, "err": err, "tau_int": round(tau, 2)}
+            if name == "polyakov":
+                out["independent_configs_per_s"] = round(ind, 2)
+        out["tau_int_abs_polyakov"] = round(_stats(np.abs(pol), rate)[2], 2)
+        for k, v in lw.items():   # log w = -S - log q: the acceptance compares proposal with current
+            v = np.asarray(v)
+            out["log_w_" + k] = {"mean": round(float(v.mean()), 2), "std": round(float(v.std()), 2)}
+        return out
+
+
+def cmd_chain(a):
     from qcdgpu_amd import flow as QF
 
     dims = tuple(a.dims)
@@ -94,49 +148,26 @@ def cmd_chain(a):
     flow = None
     if a.params:
         flow = QF.Flow(dims=dims, layers=a.layers, bins=a.bins, max_batch=a.batch)
-        flow.set_params(np.load(a.params))
-    for arm in ("plain", "learned"):
-        with Q.Lattice(sun=2, dims=dims, beta=a.beta, precision="fp32", prng="XOR128", randseries=a.seed,
-                       max_measurements=a.steps + 2) as lat:
-            lat.set_wilson(2, 2)
-            lat.prng_init()
-            lat.init("hot")
-            for _ in range(a.therm):
-                lat.sweep("hb")
-            lat.reset_measurements()
-            use = flow if arm == "learned" else None
-            lw = {}
-            acc, secs = QF.imh_chain(lat, use, a.beta, a.steps, hb_per_step=1, seed=a.seed + 17, wilson=True,
-                                     log_weights=lw)
-            m = lat.measurements()
-            plq = m["plq_total"] / 2.0
-            pol = m["polyakov_re"]
-            wil = lat.wilson_measurements() / 2.0
-            rate = a.steps / secs
-            out = {"steps_per_s": round(rate, 1), "acceptance": round(acc, 4)}
-            for name, s in (("plaquette", plq), ("polyakov", pol), ("wilson_2x2", wil)):
-                mu, err, tau, ind = _stats(s, rate)
-                out[name] = {"mean": mu, "err": err, "tau_int": round(tau, 2)}
-                if name == "polyakov":
-                    out["independent_configs_per_s"] = round(ind, 2)
-            out["tau_int_abs_polyakov"] = round(_stats(np.abs(pol), rate)[2], 2)
-            for k, v in lw.items():   # log w = -S - log q: the acceptance compares proposal with current
-                v = np.asarray(v)
-                out["log_w_" + k] = {"mean": round(float(v.mean()), 2), "std": round(float(v.std()), 2)}
-            res[arm] = out
-            print("  %s: %s" % (arm, json.dumps(out)), flush=True)
+        flow.set_params(np.load(a.params).astype(np.float32))
+    arms = ["plain"] + (["learned"] + (["global"] if a.global_imh else []) if flow is not None else [])
+    for arm in arms:
+        res[arm] = run_arm(arm, a, flow)
+        print("  %s: %s" % (arm, json.dumps(res[arm])), flush=True)
     z = {}
-    for name in ("plaquette", "polyakov", "wilson_2x2"):
-        p, l = res["plain"][name], res["learned"][name]
-        z[name] = round((l["mean"] - p["mean"]) / np.hypot(p["err"], l["err"]), 2)
+    for arm in arms[1:]:
+        z[arm] = {}
+        for name in ("plaquette", "polyakov", "wilson_2x2"):
+            p, l = res["plain"][name], res[arm][name]
+            z[arm][name] = round((l["mean"] - p["mean"]) / np.hypot(p["err"], l["err"]), 2)
+    best = "learned" if "learned" in res else "plain"
     line = {
         "metric": "independent configurations/s (Polyakov loop tau_int), learned + heatbath vs plain heatbath",
-        "value": res["learned"]["independent_configs_per_s"],
+        "value": res[best]["independent_configs_per_s"],
         "unit": "configurations/s",
         "plain": res["plain"]["independent_configs_per_s"],
-        "ratio": round(res["learned"]["independent_configs_per_s"] / res["plain"]["independent_configs_per_s"], 3),
-        "acceptance": res["learned"]["acceptance"],
-        "z_learned_minus_plain": z,
+        "ratio": round(res[best]["independent_configs_per_s"] / res["plain"]["independent_configs_per_s"], 3),
+        "acceptance": {arm: res[arm]["acceptance"] for arm in arms[1:]},
+        "z_vs_plain": z,
         "config": {"workload": "SU(2) %dx%dx%dx%d beta=%.2f, FP32 links" % (dims + (a.beta,)), "steps": a.steps,
                    "thermalisation": a.therm, "flow": {"layers": a.layers, "bins": a.bins, "hidden": 32}},
         "arms": res,
@@ -167,6 +198,8 @@ def main(argv=None):
     c.add_argument("--steps", type=int, default=4000)
     c.add_argument("--therm", type=int, default=200)
     c.add_argument("--batch", type=int, default=64)
+    c.add_argument("--global", dest="global_imh", action="store_true",
+                   help="also run the whole-lattice independence-Metropolis arm")
     for q in (g, t, c):
         q.add_argument("--dims", type=int, nargs=4, default=list(DIMS))
         q.add_argument("--beta", type=float, default=BETA)

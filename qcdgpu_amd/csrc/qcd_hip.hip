@@ -35,6 +35,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <ctime>
+#include <string>
 #include <vector>
 
 #include "../../include/qcd_hip.h"
@@ -77,6 +78,7 @@ struct UpdArgs {
     // xcd_map >= 2: z-groups of the lattice, N4, x-values per XCD block,
     // XCD blocks along x, z-groups per XCD block
     uint32_t ord_ng, ord_nt, ord_xs, ord_ax, ord_zr;
+    uint32_t unit0;       // first 64-link unit of the launch (an x-slab's first interior slice; else 0)
     Twist tw;             // twisted boundary (read only by the TW instantiations)
 };
 
@@ -565,7 +567,7 @@ __device__ __forceinline__ void update3(const UpdArgs& a, uint32_t blk, uint32_t
     // derived from it live in scalar registers.
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int lane = threadIdx.x & 63;
-    const uint32_t i = blk * kLinks + lane;
+    const uint32_t i = (a.unit0 + blk) * kLinks + lane;
     const bool active = i < a.g.vh;
     Table<R, N, NREAL> Tb{(typename Vec2<R>::T*)a.table, a.g.vh_pad};
     const int p = a.parity, mu = (MU >= 0) ? MU : a.mu;
@@ -691,7 +693,7 @@ template <typename R, int N, int NREAL, bool OVER, int RK, bool MEAS>
 __device__ __forceinline__ void update1(const UpdArgs& a, uint32_t unit, uint32_t units)
 {
     const int lane = threadIdx.x & 63;
-    const uint32_t i = unit * kLinks + lane;   // < units * 64 = number of streams
+    const uint32_t i = (a.unit0 + unit) * kLinks + lane;   // < number of streams
     const bool active = i < a.g.vh;
     Table<R, N, NREAL> Tb{(typename Vec2<R>::T*)a.table, a.g.vh_pad};
     const int p = a.parity, mu = a.mu;
@@ -1528,6 +1530,41 @@ __global__ __launch_bounds__(kBlock) void k_polyakov(void* table, Geom g, double
     polyakov_body<R, N, NREAL>(table, g, partials, blockIdx.x, site_tr);
 }
 
+// SU(2) FP32 links between the lattice table and unit quaternions
+// q[mu * V + site] = (a0, a1, a2, a3), U = [[a0 + i a3, a2 + i a1],
+// [-a2 + i a1, a0 - i a3]], site in the canonical order of the C ABI.
+// blockIdx.y = parity * 4 + mu, so parity and direction are uniform.
+__global__ __launch_bounds__(kBlock) void k_su2_to_quats(void* table, Geom g, float4* q)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= g.vh) return;
+    const int p = (int)blockIdx.y >> 2, mu = (int)blockIdx.y & 3;
+    Table<float, 2, 4> T{(typename Vec2<float>::T*)table, g.vh_pad};
+    const Site s = site_of(g, i, p);
+    const M<float, 2> U = T.load_u(p, mu, i);
+    const uint32_t site = (uint32_t)s.y + (uint32_t)g.n2 * ((uint32_t)s.z + (uint32_t)g.n3 *
+                                                              ((uint32_t)s.t + (uint32_t)g.n4 * (uint32_t)s.x));
+    q[(size_t)mu * 2u * g.vh + site] = make_float4(U.e[0][0].re, U.e[0][1].im, U.e[0][1].re, U.e[0][0].im);
+}
+
+__global__ __launch_bounds__(kBlock) void k_quats_to_su2(void* table, Geom g, const float4* q)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= g.vh) return;
+    const int p = (int)blockIdx.y >> 2, mu = (int)blockIdx.y & 3;
+    Table<float, 2, 4> T{(typename Vec2<float>::T*)table, g.vh_pad};
+    const Site s = site_of(g, i, p);
+    const uint32_t site = (uint32_t)s.y + (uint32_t)g.n2 * ((uint32_t)s.z + (uint32_t)g.n3 *
+                                                              ((uint32_t)s.t + (uint32_t)g.n4 * (uint32_t)s.x));
+    const float4 a = q[(size_t)mu * 2u * g.vh + site];
+    M<float, 2> U;
+    U.e[0][0] = {a.x, a.w};
+    U.e[0][1] = {a.z, a.y};
+    U.e[1][0] = {-a.z, a.y};
+    U.e[1][1] = {a.x, -a.w};
+    T.store_u(p, mu, i, U);
+}
+
 // Sums per x, y and z slice of a per-site pair of values, in a fixed order:
 // workgroup b takes slice x = b (b < N1), y = b - N1, or z = b - N1 - N2 and
 // writes record rec_index[slot] at position b.  Sites are indexed
@@ -1735,6 +1772,22 @@ struct qcd_lattice {
     // lattice's buffers and stream sizing, so neither may change (no
     // re-sizing start, no destroy except through the batch).
     bool batch_owned = false;
+    // x-slab (one GPU's share of a lattice cut along x): the slices x = 0 and
+    // x = N1 - 1 are halos that the updates read and never write; they are
+    // refreshed through qcd_slab_face.
+    bool slab = false;
+    // Kernel timing (GPU::print_time_detailed, clinterface.cpp:1626-1666):
+    // with profiling on, every kernel family's launches are timed with a
+    // pair of events on the lattice stream and waited for, as the
+    // reference's profiling queue serialises them.
+    struct ProfStat {
+        std::string name;
+        unsigned n = 0;
+        double sum = 0.0, sumsq = 0.0;   // ms
+    };
+    bool profiling = false;
+    std::vector<ProfStat> prof;
+    hipEvent_t prof_ev[2] = {nullptr, nullptr};
 };
 
 namespace {
@@ -1792,6 +1845,43 @@ void launch(Kern k, dim3 grid, dim3 block, hipStream_t s, const LaunchOpt& o, Ar
     else hipLaunchKernelGGL(k, grid, block, o.lds, s, args...);
 }
 
+// Times what is enqueued on the lattice stream during its lifetime under
+// `name` when profiling is on (events before and after, then a wait).
+struct ProfScope {
+    qcd_lattice* L;
+    const char* name;
+    explicit ProfScope(qcd_lattice* l, const char* n) : L(l), name(n)
+    {
+        if (L->profiling) (void)hipEventRecord(L->prof_ev[0], L->stream);
+    }
+    ~ProfScope()
+    {
+        if (!L->profiling) return;
+        float ms = 0.f;
+        if (hipEventRecord(L->prof_ev[1], L->stream) != hipSuccess || hipEventSynchronize(L->prof_ev[1]) != hipSuccess ||
+            hipEventElapsedTime(&ms, L->prof_ev[0], L->prof_ev[1]) != hipSuccess)
+            return;
+        for (auto& e : L->prof)
+            if (e.name == name) {
+                ++e.n;
+                e.sum += ms;
+                e.sumsq += (double)ms * ms;
+                return;
+            }
+        L->prof.push_back({name, 1u, (double)ms, (double)ms * ms});
+    }
+};
+
+// 64-link units one update launch covers, and the first of them: the whole
+// lattice, or an x-slab's interior slices 1 .. N1 - 2 (x is the slowest
+// coordinate of the compact index, so they are one contiguous range).
+uint32_t update_units(const qcd_lattice* L)
+{
+    if (L->slab) return (L->g.vh / (uint32_t)L->g.n1) * (uint32_t)(L->g.n1 - 2) / kLinks;
+    return align_up(L->g.vh, kLinks) / kLinks;
+}
+uint32_t update_unit0(const qcd_lattice* L) { return L->slab ? L->g.vh / (uint32_t)L->g.n1 / kLinks : 0u; }
+
 // The unit -> XCD map the update kernels use when `want` is asked for;
 // sets a.xcd_map to the mode that applies and the block parameters.
 // Modes 2 and 3 need whole z-rows per 64-link unit (N2 / 2 divides 64),
@@ -1806,6 +1896,7 @@ void launch(Kern k, dim3 grid, dim3 block, hipStream_t s, const LaunchOpt& o, Ar
 // units per XCD and does not get them).
 void dispatch_order(const qcd_lattice* L, int want, bool shape1, UpdArgs& a)
 {
+    if (L->slab && want > 1) want = 1;   // the block orders cover the whole lattice
     a.xcd_map = want;
     a.ord_ng = a.ord_nt = a.ord_xs = a.ord_ax = a.ord_zr = 1;
     if (want >= 2) {
@@ -1839,15 +1930,14 @@ void dispatch_order(const qcd_lattice* L, int want, bool shape1, UpdArgs& a)
     }
     if (shape1 && a.xcd_map != 0) {
         // four 64-link units per workgroup: the XCD blocks need whole workgroups
-        const uint32_t nb = align_up(L->g.vh, kLinks) / kLinks;
-        if (nb % (8u * kWaves1)) a.xcd_map = 0;
+        if (update_units(L) % (8u * kWaves1)) a.xcd_map = 0;
     }
 }
 
 template <typename R, int N, int NREAL, bool OVER, int RK>
 void launch_update_t(qcd_lattice* L, const UpdArgs& a, const LaunchOpt& o)
 {
-    const uint32_t nb = align_up(L->g.vh, kLinks) / kLinks;   // 64-link units
+    const uint32_t nb = update_units(L);   // 64-link units
     if (a.tw.on) {   // twisted boundary: three-wavefront shape, plaquette never fused
         launch(k_update<R, N, NREAL, OVER, RK, false, true>, dim3(nb), dim3(kUpdThreads), L->stream, o, a);
         return;
@@ -1917,6 +2007,7 @@ int launch_update_rn(qcd_lattice* L, int parity, int mu, int kind, int start_gid
     a.s_quads = (L->p.stream_mode == QCD_STREAM_REFERENCE) ? L->s_quads : 0u;
     a.partials = L->part_fused;
     a.meas_mode = o.meas_mode;
+    a.unit0 = update_unit0(L);
     a.tw = L->tw;
     dispatch_order(L, L->tune_xcd_map, L->tune_update_shape == 1 && !a.tw.on, a);
     LaunchOpt ol = o;
@@ -1927,19 +2018,26 @@ int launch_update_rn(qcd_lattice* L, int parity, int mu, int kind, int start_gid
         ol.lds = per > stat + 256 ? per - stat - 256 : 0;
     }
     if (kind == QCD_OVERRELAX) {
+        ProfScope ps(L, o.meas_mode ? "overrelaxation half-step + plaquette" : "overrelaxation half-step");
         launch_update_t<R, N, NREAL, true, RK_NONE>(L, a, ol);
     } else if (start_gid) {
+        ProfScope ps(L, "heatbath half-step (GID numbers)");
         launch_update_t<R, N, NREAL, false, RK_GID>(L, a, ol);
     } else {
         if (!L->rng_ready) return fail(QCD_E_STATE, "heatbath needs qcd_prng_init first");
         if (L->quad_buffer) {
             // the reference's produce() into the PRN buffer, then the update
             // reads quad k of link i at k * PRNGSTEP + i
-            produce_quads_into(L, L->s_quads, L->prn_buf);
+            {
+                ProfScope ps(L, "PRNG produce");
+                produce_quads_into(L, L->s_quads, L->prn_buf);
+            }
             a.prn = L->prn_buf;
             a.prn_stride = L->n_streams;
+            ProfScope ps(L, o.meas_mode ? "heatbath half-step + plaquette" : "heatbath half-step");
             launch_update_t<R, N, NREAL, false, RK_BUF>(L, a, ol);
         } else {
+            ProfScope ps(L, o.meas_mode ? "heatbath half-step + plaquette" : "heatbath half-step");
             if (L->p.prng == QCD_PRNG_XOR128) launch_update_t<R, N, NREAL, false, RK_XOR128>(L, a, ol);
             else if (L->p.prng == QCD_PRNG_PM) launch_update_t<R, N, NREAL, false, RK_PM>(L, a, ol);
             else return fail(QCD_E_UNSUPPORTED, "generator %d not available for fused updates", L->p.prng);
@@ -2103,6 +2201,26 @@ int qcdhip::write_fmunu(qcd_lattice* L, const double* f, int count)
     const int n = count < m ? count : m;
     HIPCHK(hipStreamSynchronize(L->stream));
     if (n) HIPCHK(hipMemcpy(L->f_rec, f, sizeof(double) * 12 * n, hipMemcpyHostToDevice));
+    return QCD_OK;
+}
+
+int qcdhip::su2_quats(qcd_lattice* L, float4* q, bool to_quats, hipStream_t* stream)
+{
+    if (!L || !q) return fail(QCD_E_ARG, "null argument");
+    if (L->p.sun != 2 || L->p.precision != QCD_FP32)
+        return fail(QCD_E_UNSUPPORTED, "quaternion links: SU(2) lattices in FP32 only");
+    const dim3 grid(align_up(L->g.vh, kBlock) / kBlock, 8);
+    if (to_quats) hipLaunchKernelGGL(k_su2_to_quats, grid, dim3(kBlock), 0, L->stream, L->table, L->g, q);
+    else hipLaunchKernelGGL(k_quats_to_su2, grid, dim3(kBlock), 0, L->stream, L->table, L->g, (const float4*)q);
+    HIPCHK(hipGetLastError());
+    if (stream) *stream = L->stream;
+    return QCD_OK;
+}
+
+int qcdhip::lattice_dims(const qcd_lattice* L, int n[4])
+{
+    if (!L) return fail(QCD_E_ARG, "null lattice");
+    for (int k = 0; k < 4; ++k) n[k] = L->p.n[k];
     return QCD_OK;
 }
 
@@ -2360,6 +2478,8 @@ int qcd_lattice_destroy(qcd_lattice* L)
     if (L->stream) (void)hipStreamSynchronize(L->stream);
     if (L->exec) (void)hipGraphExecDestroy(L->exec);
     if (L->graph) (void)hipGraphDestroy(L->graph);
+    for (auto& e : L->prof_ev)
+        if (e) (void)hipEventDestroy(e);
     (void)hipFree(L->table);
     (void)hipFree(L->rng_buf[0]);
     (void)hipFree(L->rng_buf[1]);
@@ -2386,6 +2506,91 @@ int qcd_lattice_sync(qcd_lattice* L)
 {
     if (!L) return fail(QCD_E_ARG, "null lattice");
     HIPCHK(hipStreamSynchronize(L->stream));
+    return QCD_OK;
+}
+
+// x-slab: the one-GPU share of a lattice cut into slabs along x (BIGLAT,
+// suncl.cpp:2422 N1 = Nx + 2, UpdateEdges :4005-4062).  Half-steps update the
+// interior slices only; after each one the slab's first and last interior
+// slices of the updated (parity, mu) group go to the neighbours' halos.
+int qcd_lattice_set_slab(qcd_lattice* L, int on)
+{
+    if (!L) return fail(QCD_E_ARG, "null lattice");
+    if (on) {
+        const uint32_t hn = L->g.vh / (uint32_t)L->g.n1;   // links of one parity and direction in an x-slice
+        if (L->g.n1 < 4) return fail(QCD_E_ARG, "an x-slab needs N1 >= 4 (two halo slices and an interior)");
+        if (hn % kLinks) return fail(QCD_E_ARG, "an x-slab needs N2 N3 N4 / 2 divisible by 64 (got %u)", hn);
+        if (L->tw.on) return fail(QCD_E_UNSUPPORTED, "no twisted boundary on an x-slab");
+    }
+    HIPCHK(hipStreamSynchronize(L->stream));
+    if (L->slab != (on != 0)) {   // a captured cycle holds the other launch shape
+        if (L->exec) (void)hipGraphExecDestroy(L->exec);
+        if (L->graph) (void)hipGraphDestroy(L->graph);
+        L->exec = nullptr;
+        L->graph = nullptr;
+    }
+    L->slab = on != 0;
+    return QCD_OK;
+}
+
+int qcd_lattice_profile(qcd_lattice* L, int on)
+{
+    if (!L) return fail(QCD_E_ARG, "null lattice");
+    if (on && !L->prof_ev[0]) {
+        HIPCHK(hipEventCreate(&L->prof_ev[0]));
+        HIPCHK(hipEventCreate(&L->prof_ev[1]));
+    }
+    HIPCHK(hipStreamSynchronize(L->stream));
+    L->profiling = on != 0;
+    if (on) L->prof.clear();
+    return QCD_OK;
+}
+
+int qcd_profile_count(const qcd_lattice* L, int* n)
+{
+    if (!L || !n) return fail(QCD_E_ARG, "null argument");
+    *n = (int)L->prof.size();
+    return QCD_OK;
+}
+
+int qcd_profile_entry(const qcd_lattice* L, int i, char* name, int name_len, unsigned* starts, double* mean_ms,
+                      double* dev_ms)
+{
+    if (!L) return fail(QCD_E_ARG, "null lattice");
+    if (i < 0 || i >= (int)L->prof.size()) return fail(QCD_E_ARG, "profile entry %d out of range", i);
+    const auto& e = L->prof[i];
+    if (name && name_len > 0) snprintf(name, (size_t)name_len, "%s", e.name.c_str());
+    const double m = e.sum / e.n, var = e.sumsq / e.n - m * m;
+    if (starts) *starts = e.n;
+    if (mean_ms) *mean_ms = m;
+    if (dev_ms) *dev_ms = var > 0.0 ? sqrt(var) : 0.0;
+    return QCD_OK;
+}
+
+int qcd_slab_face_bytes(const qcd_lattice* L, size_t* bytes)
+{
+    if (!L || !bytes) return fail(QCD_E_ARG, "null argument");
+    *bytes = (size_t)L->np * (L->g.vh / (uint32_t)L->g.n1) * L->elem;
+    return QCD_OK;
+}
+
+int qcd_slab_face(qcd_lattice* L, int side, int parity, int mu, void* buf, int unpack)
+{
+    if (!L || !buf) return fail(QCD_E_ARG, "null argument");
+    if (!L->slab) return fail(QCD_E_STATE, "not an x-slab (qcd_lattice_set_slab)");
+    if ((side != 0 && side != 1) || (parity != 0 && parity != 1) || mu < 0 || mu > 3)
+        return fail(QCD_E_ARG, "side, parity 0 or 1 and mu 0..3");
+    const size_t hn = L->g.vh / (uint32_t)L->g.n1;
+    // pack: first (side 0) or last (side 1) interior slice; unpack: the halo
+    // on that side
+    const size_t x = unpack ? (side ? (size_t)L->g.n1 - 1 : 0) : (side ? (size_t)L->g.n1 - 2 : 1);
+    char* tab = (char*)L->table;
+    for (int k = 0; k < L->np; ++k) {
+        char* plane = tab + ((size_t)((parity * 4 + mu) * L->np + k) * L->g.vh_pad + x * hn) * L->elem;
+        char* b = (char*)buf + (size_t)k * hn * L->elem;
+        HIPCHK(hipMemcpyAsync(unpack ? plane : b, unpack ? b : plane, hn * L->elem, hipMemcpyDeviceToDevice,
+                              L->stream));
+    }
     return QCD_OK;
 }
 
@@ -2503,10 +2708,12 @@ int qcd_lattice_init(qcd_lattice* L, int start)
     // GID numbers, as -D GID_UPD does in the reference (suncl.cpp:3278-3279).
     if (start == QCD_START_COLD) {
         L->gid_updates = false;
+        ProfScope ps(L, "cold start (lattice_init_cold)");
         return DISPATCH(L, launch_simple, L, 0, LaunchOpt{});
     }
     if (start == QCD_START_GID) {
         L->gid_updates = true;
+        ProfScope ps(L, "GID start (lattice_init_gid)");
         return DISPATCH(L, launch_simple, L, 1, LaunchOpt{});
     }
     if (start == QCD_START_HOT) {
@@ -2515,6 +2722,7 @@ int qcd_lattice_init(qcd_lattice* L, int start)
             return fail(QCD_E_UNSUPPORTED, "the Haar start draws one stream per site pair: not with prng_instances set "
                                            "(cold, GID or the reference's random start)");
         L->gid_updates = false;
+        ProfScope ps(L, "Haar start");
         const int rc = DISPATCH(L, launch_simple, L, 5, LaunchOpt{});
         // the reference produces before each of its four init kernels
         // (suncl.cpp:5236-5248)
@@ -2739,6 +2947,7 @@ static int measure_impl(qcd_lattice* L, int mask, bool fused_plq, const LaunchOp
     if (fused_plq) flags |= 1;
     if ((mask & QCD_MEAS_PLAQUETTE) && !fused_plq) {
         L->want_aslices = (mask & QCD_MEAS_ACTION_SLICES) != 0;
+        ProfScope ps(L, "plaquette (lattice_measurement_plq)");
         rc = DISPATCH(L, launch_simple, L, 3, o_plq);
         L->want_aslices = false;
         flags |= 2;
@@ -2747,12 +2956,14 @@ static int measure_impl(qcd_lattice* L, int mask, bool fused_plq, const LaunchOp
     if (rc) return rc;
     if (fmask) {
         L->f_efield = (fmask == QCD_MEAS_F0MU) ? 1 : 0;
+        ProfScope ps(L, "F_munu (lattice_measurement)");
         rc = DISPATCH(L, launch_simple, L, 7, LaunchOpt{});
         flags |= 32;
     }
     if (rc) return rc;
     if (mask & (QCD_MEAS_POLYAKOV | QCD_MEAS_POLYAKOV_SLICES)) {
         L->want_slices = (mask & QCD_MEAS_POLYAKOV_SLICES) != 0;
+        ProfScope ps(L, "Polyakov loop (lattice_polyakov)");
         rc = DISPATCH(L, launch_simple, L, 4, o_poly);
         L->want_slices = false;
         flags |= 4;
@@ -2760,11 +2971,13 @@ static int measure_impl(qcd_lattice* L, int mask, bool fused_plq, const LaunchOp
     }
     if (rc) return rc;
     if (mask & QCD_MEAS_WILSON) {
+        ProfScope ps(L, "Wilson loop (lattice_measurement_wilson)");
         rc = DISPATCH(L, launch_simple, L, 6, LaunchOpt{});
         flags |= 8;
     }
-    if (rc) return rc;
-    return flags ? launch_finalize(L, flags, o_fin) : QCD_OK;
+    if (rc || !flags) return rc;
+    ProfScope ps(L, "final sums (reduce)");
+    return launch_finalize(L, flags, o_fin);
 }
 
 int qcd_measure(qcd_lattice* L, int mask)
@@ -2844,6 +3057,7 @@ int qcd_lattice_set_fmunu(qcd_lattice* L, int gen1, int gen2)
 int qcd_lattice_set_twist(qcd_lattice* L, double phi, double omega)
 {
     if (!L) return fail(QCD_E_ARG, "null lattice");
+    if (L->slab && (phi != 0.0 || omega != 0.0)) return fail(QCD_E_UNSUPPORTED, "no twisted boundary on an x-slab");
     if (!std::isfinite(phi) || !std::isfinite(omega)) return fail(QCD_E_ARG, "PHI and OMEGA must be finite");
     L->tw_phi = phi;
     L->tw_omega = omega;
@@ -2944,7 +3158,7 @@ int qcd_reset_measurements(qcd_lattice* L)
 static bool cycle_fuses(const qcd_lattice* L, int mask, int total)
 {
     return (mask & QCD_MEAS_PLAQUETTE) && total > 0 &&
-           !(mask & (QCD_MEAS_FMUNU | QCD_MEAS_F0MU | QCD_MEAS_ACTION_SLICES)) && !L->tw.on;
+           !(mask & (QCD_MEAS_FMUNU | QCD_MEAS_F0MU | QCD_MEAS_ACTION_SLICES)) && !L->tw.on && !L->slab;
 }
 
 // A measurement cycle: n_hb heatbath sweeps, n_or overrelaxation sweeps,
@@ -2976,6 +3190,7 @@ static int enqueue_cycle(qcd_lattice* L, int n_hb, int n_or, int mask, std::vect
 int qcd_cycle_capture(qcd_lattice* L, int n_hb, int n_or, int mask)
 {
     if (!L) return fail(QCD_E_ARG, "null lattice");
+    if (L->profiling) return fail(QCD_E_STATE, "kernel timing waits for every kernel: run cycles eagerly while it is on");
     if (n_hb < 0 || n_or < 0) return fail(QCD_E_ARG, "negative sweep count");
     if (L->exec) {
         (void)hipGraphExecDestroy(L->exec);
@@ -3489,11 +3704,18 @@ struct qcd_batch {
     hipStream_t stream = nullptr;
     ChainDesc* d_desc = nullptr;
     uint2* d_map = nullptr;                // all work maps, concatenated
+    // Unit placements over the 8 XCDs (hardware workgroup b runs on XCD
+    // b mod 8): 0 each chain whole on one XCD, its units in index order;
+    // 1 the same with each chain's units in cross-section order (x fastest,
+    // then t, then z), so a unit's x and t neighbours go out close to it;
+    // 2 every chain cut into 8 contiguous slabs of units, one per XCD.
+    static constexpr int kPlacements = 3;
     struct Bucket {
         std::vector<int> members;
-        uint32_t off3 = 0, len3 = 0;       // three-wavefront map: one entry per workgroup
-        uint32_t off1 = 0, len1 = 0;       // one-wavefront map: one entry per wavefront
+        uint32_t off3[kPlacements] = {}, len3[kPlacements] = {};   // three-wavefront maps: one entry per workgroup
+        uint32_t off1[kPlacements] = {}, len1[kPlacements] = {};   // one-wavefront maps: one entry per wavefront
         int shape[2] = {3, 3};             // [heatbath, overrelaxation]
+        int place[2] = {0, 0};
     };
     std::vector<Bucket> buckets;
     uint32_t off_units = 0, len_units = 0; // every (chain, unit): RANLUX produce
@@ -3546,9 +3768,26 @@ FinArgs chain_fin_args(const qcd_lattice* L)
     return f;
 }
 
-// Work maps.  Chains of a bucket go to the XCD with the least work so far
-// (largest first); each XCD's list holds its chains' units in order.
-// Hardware workgroup b runs on XCD b mod 8.
+// The 64-link units of chain L in cross-section order: sorted by the z,
+// t and x coordinates of their first link (z slowest, x fastest).
+std::vector<uint32_t> cross_section_units(const qcd_lattice* L)
+{
+    const uint32_t nu = L->nb_fused;
+    std::vector<uint64_t> key(nu);
+    for (uint32_t u = 0; u < nu; ++u) {
+        const Site st = site_of(L->g, u * kLinks, 0);
+        key[u] = (((uint64_t)st.z * (uint64_t)L->g.n4 + (uint64_t)st.t) * (uint64_t)L->g.n1 + (uint64_t)st.x) << 32 | u;
+    }
+    std::sort(key.begin(), key.end());
+    std::vector<uint32_t> out(nu);
+    for (uint32_t u = 0; u < nu; ++u) out[u] = (uint32_t)(key[u] & 0xffffffffu);
+    return out;
+}
+
+// Work maps of every bucket and placement.  Placements 0 and 1: chains of a
+// bucket go to the XCD with the least work so far (largest first); each
+// XCD's list holds its chains' units.  Placement 2: XCD k takes the k-th
+// eighth of every chain's units.
 void build_maps(qcd_batch* B, std::vector<uint2>& map)
 {
     const uint2 idle = make_uint2(~0u, 0u);
@@ -3557,31 +3796,43 @@ void build_maps(qcd_batch* B, std::vector<uint2>& map)
         std::sort(order.begin(), order.end(), [&](int a, int b) {
             return B->chains[a]->nb_fused > B->chains[b]->nb_fused;
         });
-        std::vector<std::vector<uint2>> xcd(8);
-        std::vector<uint64_t> load(8, 0);
-        for (int c : order) {
-            const int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-            const uint32_t nu = B->chains[c]->nb_fused;
-            for (uint32_t u = 0; u < nu; ++u) xcd[k].push_back(make_uint2((uint32_t)c, u));
-            load[k] += nu;
-        }
-        size_t maxlen = 0;
-        for (auto& l : xcd) maxlen = std::max(maxlen, l.size());
-        bk.off3 = (uint32_t)map.size();
-        bk.len3 = (uint32_t)(8 * maxlen);
-        for (uint32_t b = 0; b < bk.len3; ++b) {
-            const auto& l = xcd[b % 8];
-            map.push_back(b / 8 < l.size() ? l[b / 8] : idle);
-        }
-        const size_t rows = (maxlen + kWaves1 - 1) / kWaves1;
-        bk.off1 = (uint32_t)map.size();
-        bk.len1 = (uint32_t)(8 * rows * kWaves1);
-        for (uint32_t b = 0; b < 8 * rows; ++b)
-            for (int w = 0; w < kWaves1; ++w) {
-                const auto& l = xcd[b % 8];
-                const size_t e = (b / 8) * kWaves1 + w;
-                map.push_back(e < l.size() ? l[e] : idle);
+        for (int pl = 0; pl < qcd_batch::kPlacements; ++pl) {
+            std::vector<std::vector<uint2>> xcd(8);
+            std::vector<uint64_t> load(8, 0);
+            for (int c : order) {
+                const uint32_t nu = B->chains[c]->nb_fused;
+                if (pl == 2) {
+                    for (uint32_t k = 0; k < 8; ++k)
+                        for (uint32_t u = (uint32_t)((uint64_t)nu * k / 8); u < (uint32_t)((uint64_t)nu * (k + 1) / 8); ++u)
+                            xcd[k].push_back(make_uint2((uint32_t)c, u));
+                    continue;
+                }
+                const int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+                if (pl == 1) {
+                    for (uint32_t u : cross_section_units(B->chains[c])) xcd[k].push_back(make_uint2((uint32_t)c, u));
+                } else {
+                    for (uint32_t u = 0; u < nu; ++u) xcd[k].push_back(make_uint2((uint32_t)c, u));
+                }
+                load[k] += nu;
             }
+            size_t maxlen = 0;
+            for (auto& l : xcd) maxlen = std::max(maxlen, l.size());
+            bk.off3[pl] = (uint32_t)map.size();
+            bk.len3[pl] = (uint32_t)(8 * maxlen);
+            for (uint32_t b = 0; b < bk.len3[pl]; ++b) {
+                const auto& l = xcd[b % 8];
+                map.push_back(b / 8 < l.size() ? l[b / 8] : idle);
+            }
+            const size_t rows = (maxlen + kWaves1 - 1) / kWaves1;
+            bk.off1[pl] = (uint32_t)map.size();
+            bk.len1[pl] = (uint32_t)(8 * rows * kWaves1);
+            for (uint32_t b = 0; b < 8 * rows; ++b)
+                for (int w = 0; w < kWaves1; ++w) {
+                    const auto& l = xcd[b % 8];
+                    const size_t e = (b / 8) * kWaves1 + w;
+                    map.push_back(e < l.size() ? l[e] : idle);
+                }
+        }
     }
     B->off_units = (uint32_t)map.size();
     for (size_t c = 0; c < B->chains.size(); ++c)
@@ -3596,10 +3847,10 @@ void build_maps(qcd_batch* B, std::vector<uint2>& map)
 template <typename R, int N, int NREAL, bool OVER, int RK>
 void launch_batch_update(qcd_batch* B, const qcd_batch::Bucket& bk, int parity, int mu, int meas_mode, const LaunchOpt& o)
 {
-    const int shape = bk.shape[OVER ? 1 : 0];
+    const int shape = bk.shape[OVER ? 1 : 0], pl = bk.place[OVER ? 1 : 0];
     if (shape == 1) {
-        const dim3 grid(bk.len1 / kWaves1), block(kWaves1 * kLinks);
-        const uint2* map = B->d_map + bk.off1;
+        const dim3 grid(bk.len1[pl] / kWaves1), block(kWaves1 * kLinks);
+        const uint2* map = B->d_map + bk.off1[pl];
         if (meas_mode)
             launch(k_update_batch1<R, N, NREAL, OVER, RK, true>, grid, block, B->stream, o, (const ChainDesc*)B->d_desc,
                    map, parity, mu, meas_mode, B->rng_cur);
@@ -3607,8 +3858,8 @@ void launch_batch_update(qcd_batch* B, const qcd_batch::Bucket& bk, int parity, 
             launch(k_update_batch1<R, N, NREAL, OVER, RK, false>, grid, block, B->stream, o, (const ChainDesc*)B->d_desc,
                    map, parity, mu, meas_mode, B->rng_cur);
     } else {
-        const dim3 grid(bk.len3), block(kUpdThreads);
-        const uint2* map = B->d_map + bk.off3;
+        const dim3 grid(bk.len3[pl]), block(kUpdThreads);
+        const uint2* map = B->d_map + bk.off3[pl];
         if (meas_mode)
             launch(k_update_batch3<R, N, NREAL, OVER, RK, true>, grid, block, B->stream, o, (const ChainDesc*)B->d_desc,
                    map, parity, mu, meas_mode, B->rng_cur);
@@ -3930,9 +4181,11 @@ int qcd_batch_autotune(qcd_batch* B, int repeats, int* shapes_out)
         for (size_t k = 0; k < B->buckets.size() && !rc; ++k)
             for (int kind = 0; kind < 2 && !rc; ++kind) {
                 float best = 1e30f;
-                int best_shape = 3;
-                for (int shape : {3, 1}) {
+                int best_shape = 3, best_place = 0;
+                for (int cand = 0; cand < 2 * qcd_batch::kPlacements && !rc; ++cand) {
+                    const int shape = (cand % 2) ? 1 : 3, place = cand / 2;
                     B->buckets[k].shape[kind] = shape;
+                    B->buckets[k].place[kind] = place;
                     // one untimed launch, then `repeats` timed ones
                     rc = DISPATCH(B, batch_halfstep_rn, B, 1, 0, kind == 0 ? QCD_HEATBATH : QCD_OVERRELAX, 0, (int)k,
                                   LaunchOpt{});
@@ -3947,9 +4200,11 @@ int qcd_batch_autotune(qcd_batch* B, int repeats, int* shapes_out)
                     if (t < best) {
                         best = t;
                         best_shape = shape;
+                        best_place = place;
                     }
                 }
                 B->buckets[k].shape[kind] = best_shape;
+                B->buckets[k].place[kind] = best_place;
             }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
@@ -3977,6 +4232,37 @@ int qcd_batch_autotune(qcd_batch* B, int repeats, int* shapes_out)
             shapes_out[2 * k + 1] = B->buckets[k].shape[1];
         }
     // a captured cycle keeps the shapes it was captured with
+    return QCD_OK;
+}
+
+int qcd_batch_set_tuning(qcd_batch* B, int bucket, int shape_hb, int shape_or, int place_hb, int place_or)
+{
+    if (!B) return fail(QCD_E_ARG, "null batch");
+    if (bucket < 0 || bucket >= (int)B->buckets.size()) return fail(QCD_E_ARG, "bucket %d out of range", bucket);
+    for (int v : {shape_hb, shape_or})
+        if (v != 1 && v != 3) return fail(QCD_E_ARG, "kernel shape %d: 1 or 3", v);
+    for (int v : {place_hb, place_or})
+        if (v < 0 || v >= qcd_batch::kPlacements) return fail(QCD_E_ARG, "placement %d: 0..%d", v, qcd_batch::kPlacements - 1);
+    auto& bk = B->buckets[bucket];
+    bk.shape[0] = shape_hb;
+    bk.shape[1] = shape_or;
+    bk.place[0] = place_hb;
+    bk.place[1] = place_or;
+    return QCD_OK;   // a captured cycle keeps the choices it was captured with
+}
+
+int qcd_batch_get_tuning(qcd_batch* B, int bucket, int* out)
+{
+    if (!B || !out) return fail(QCD_E_ARG, "null argument");
+    if (bucket < 0 || bucket >= (int)B->buckets.size()) return fail(QCD_E_ARG, "bucket %d out of range", bucket);
+    const auto& bk = B->buckets[bucket];
+    out[0] = bk.shape[0];
+    out[1] = bk.shape[1];
+    out[2] = bk.place[0];
+    out[3] = bk.place[1];
+    const qcd_lattice* L = B->chains[bk.members[0]];
+    out[4] = L->p.n[0];
+    out[5] = (int)bk.members.size();
     return QCD_OK;
 }
 

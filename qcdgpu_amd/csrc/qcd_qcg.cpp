@@ -519,7 +519,7 @@ static int qcg_read_header(const char* path, qcd_params* params, qcd_run_info* i
     int rc = open_state(path, fh, h, q, info, lay, ext);
     if (rc) return rc;
     if (ext) {
-        unsigned char head[80];
+        unsigned char head[136];   // the image head, parameter fields included
         qcd_params e = q;
         const size_t n = ext < sizeof(head) ? (size_t)ext : sizeof(head);
         if (!get(fh.f, head, n) || !image_params(head, n, &e)) return err(QCD_E_ARG, "bad state image head");

@@ -95,6 +95,44 @@ class Flow:
                                     S.ctypes.data_as(D)))
         return links, lq, S
 
+    def train_grad(self, links):
+        """Layer-local forward-KL loss per configuration and its gradient
+        (summed over the batch) by the device's training kernels."""
+        a = np.ascontiguousarray(links, dtype=np.float32).reshape(-1, 4, self.V, 4)
+        loss = ctypes.c_double()
+        g = np.empty(self.param_count(), dtype=np.float32)
+        F = ctypes.POINTER(ctypes.c_float)
+        check(lib().qcd_flow_train_grad(self._h, a.ctypes.data_as(F), a.shape[0], ctypes.byref(loss),
+                                        g.ctypes.data_as(F)))
+        return loss.value, g
+
+    def train_step(self, links, lr):
+        """One Adam step of the layer-local forward KL on `links`; returns
+        the loss per configuration before the step."""
+        a = np.ascontiguousarray(links, dtype=np.float32).reshape(-1, 4, self.V, 4)
+        loss = ctypes.c_double()
+        check(lib().qcd_flow_train_step(self._h, a.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), a.shape[0],
+                                        float(lr), ctypes.byref(loss)))
+        return loss.value
+
+    def get_params(self):
+        p = np.empty(self.param_count(), dtype=np.float32)
+        check(lib().qcd_flow_get_params(self._h, p.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), p.size))
+        return p
+
+    def imh_sweep(self, lat, seed, beta):
+        """One learned Metropolis sweep of `lat` (SU(2) FP32, the flow's
+        extents) on the device: every coupling layer proposes its active
+        links from its map of Haar draws and each is accepted with the
+        exact Wilson-action Metropolis probability."""
+        check(lib().qcd_flow_imh_sweep(self._h, lat._h, int(seed), float(beta)))
+
+    def imh_counts(self, reset=False):
+        """(accepted, proposed) links of imh_sweep so far."""
+        a, p = ctypes.c_ulonglong(), ctypes.c_ulonglong()
+        check(lib().qcd_flow_imh_counts(self._h, ctypes.byref(a), ctypes.byref(p), 1 if reset else 0))
+        return a.value, p.value
+
     def evaluate(self, links, beta):
         a = np.ascontiguousarray(links, dtype=np.float32).reshape(-1, 4, self.V, 4)
         lq, S = np.empty(a.shape[0]), np.empty(a.shape[0])
@@ -299,6 +337,17 @@ class FlowModel:
         L = torch.cat([L[:, :mu], Un.unsqueeze(1), L[:, mu + 1:]], dim=1)
         return L, lj
 
+    def local_nll(self, U):
+        """The layer-local forward-KL loss per configuration: for every layer
+        l, the sum over its active links of -log q_a(U_a | frozen links) with
+        the layer applied to U itself (what the device's training kernels
+        minimise)."""
+        tot = 0
+        for l in range(self.layers):
+            _, lj = self.layer(U, l, True)
+            tot = tot + lj
+        return tot
+
     def forward(self, z):
         """Prior configurations -> (flow configurations, log q)."""
         logj = 0
@@ -366,7 +415,11 @@ def imh_chain(lat, flow, beta, steps, hb_per_step=1, batch=None, seed=1, rng=Non
             cur = to_quaternions(lat.export_links(), dims)
             lq_c, S_c = flow.evaluate(cur[None], beta)
             if not pool:
-                links, lq, S = flow.sample(seed * 1000003 + drawn, batch, beta)
+                links, _, S = flow.sample(seed * 1000003 + drawn, batch, beta)
+                # log q of each proposal from the inverse pass, the same
+                # density as the current state's, so that the acceptance is
+                # one function of the two states
+                lq, _ = flow.evaluate(links, beta)
                 drawn += 1
                 pool = [(links[b], lq[b], S[b]) for b in range(batch)]
             prop, lq_p, S_p = pool.pop()

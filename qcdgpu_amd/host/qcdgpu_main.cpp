@@ -96,6 +96,11 @@ struct Settings {
     // the host (LatticeCheckCPU, QCDGPU.cpp:254, always run by the
     // reference outside BIGLAT); it needs 8 N^2 bytes per link of host memory.
     bool verify = true;
+    // PROFILE = 1 times every kernel family and prints the table of
+    // GPU::print_time_detailed (clinterface.cpp:1626-1666) at the end, as
+    // the reference does with GPU_debug.profiling (hardcoded false,
+    // QCDGPU.cpp:88; suncl.cpp:265).  Kernels are then waited for one by one.
+    bool profile = false;
 };
 
 std::string trim(const std::string& s)
@@ -184,6 +189,7 @@ bool apply(Settings& st, const std::string& key, const std::string& val)
     else if (key == "STOPAFTER") st.stop_after = (int)iv;
     else if (key == "ACTIONSDIFF") st.actions_diff = iv != 0;
     else if (key == "VERIFY") st.verify = iv != 0;
+    else if (key == "PROFILE") st.profile = iv != 0;
     else if (key == "NOR") st.nor = (int)iv;
     else if (key == "STORAGE") st.storage = (int)iv;
     else if (key == "PRNG_INSTANCES") st.prng_instances = (unsigned)iv;
@@ -319,8 +325,8 @@ void print_params(const Settings& st)
            st.storage ? st.storage : (st.group == 3 ? 18 : 4), st.fast_streams ? "FAST" : "REFERENCE");
     printf("OUTPUTPATH=%s\nFINISHPATH=%s\nPLATFORM=%d\nDEVICE=%d\n", st.outputpath.c_str(), st.finishpath.c_str(),
            st.platform, st.device);
-    printf("ACTIONSDIFF=%d\nVERIFY=%d\nPRNG_INSTANCES=%u\n", st.actions_diff ? 1 : 0, st.verify ? 1 : 0,
-           st.prng_instances);
+    printf("ACTIONSDIFF=%d\nVERIFY=%d\nPRNG_INSTANCES=%u\nPROFILE=%d\n", st.actions_diff ? 1 : 0, st.verify ? 1 : 0,
+           st.prng_instances, st.profile ? 1 : 0);
     printf("INIT=%d\nSTATEFILE=%s\nSAVESTATE=%d\nSAVEEVERY=%.3f\n", st.init, st.statefile.c_str(),
            st.save_state ? 1 : 0, st.save_every);
 }
@@ -465,6 +471,7 @@ int main(int argc, char** argv)
 
     qcd_lattice* lat = nullptr;
     CHECK(qcd_lattice_create(&p, &lat));
+    if (st.profile) CHECK(qcd_lattice_profile(lat, 1));
     const int start = st.ints == 1 ? QCD_START_COLD
                                    : (st.ints == 2 ? QCD_START_GID : (st.haar ? QCD_START_HOT : QCD_START_RANDOM));
     const int mask = QCD_MEAS_PLAQUETTE | (st.pl_level > 0 ? QCD_MEAS_POLYAKOV : 0) | (st.wilson ? QCD_MEAS_WILSON : 0) |
@@ -582,6 +589,20 @@ int main(int argc, char** argv)
     nq = std::min(nq, (int)cap);
     nw = std::min(nw, (int)cap);
     ns = std::min(ns, (int)cap);
+    if (st.profile) {   // GPU::print_time_detailed's kernel block, full form
+        int nk = 0;
+        CHECK(qcd_profile_count(lat, &nk));
+        printf("--------------------------------------------------------\n");
+        for (int k = 0; k < nk; ++k) {
+            char name[128];
+            unsigned starts = 0;
+            double mean = 0.0, dev = 0.0;
+            CHECK(qcd_profile_entry(lat, k, name, (int)sizeof(name), &starts, &mean, &dev));
+            printf("[%2d] kernel \"%s\" executed %u times:\n\t execution time: %f (+/-%f) ms\n", k + 1, name, starts,
+                   mean, dev);
+        }
+        printf("--------------------------------------------------------\n");
+    }
     CHECK(qcd_lattice_destroy(lat));
     CHECK(qcd_device_close());
 

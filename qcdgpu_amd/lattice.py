@@ -332,6 +332,23 @@ class Lattice:
         if update_shape is not None:
             check(lib().qcd_lattice_tune(self._h, _abi.TUNE_UPDATE_SHAPE, int(update_shape)))
 
+    def set_slab(self, on=True):
+        """x-slab mode: x = 0 and x = N1 - 1 are halo slices that half-steps
+        read and never update (one GPU's share of a lattice cut along x)."""
+        check(lib().qcd_lattice_set_slab(self._h, 1 if on else 0))
+
+    def slab_face_bytes(self):
+        b = ctypes.c_size_t()
+        check(lib().qcd_slab_face_bytes(self._h, ctypes.byref(b)))
+        return b.value
+
+    def slab_face(self, side, parity, mu, dev_ptr, unpack):
+        """Copy the (parity, mu) links of the first (side 0) or last (side 1)
+        interior slice into device memory at dev_ptr (unpack False), or
+        device memory into the halo on that side (unpack True)."""
+        check(lib().qcd_slab_face(self._h, int(side), int(parity), int(mu), ctypes.c_void_p(dev_ptr),
+                                  1 if unpack else 0))
+
     def tuning(self):
         """Knob values the update launches use: xcd_map after its fallbacks
         (with the XCD block's x-values and z-rows), blocks_per_cu,

@@ -1,0 +1,125 @@
+"""The learned proposal at the C4 workload (SU(2) 8^3 x 4, beta 2.2, eight
+coupling layers of the trained flow in tests/golden/c4_flow_params.npy:
+400 Adam steps of forward KL on 5000 heatbath configurations of that
+lattice, stored as float16).
+
+- the flow's forward pass (the density of a drawn proposal) against its
+  inverse pass (the density of a given configuration): within 0.1 nats on
+  |log q| ~ 4800 (measured 0.023; the two passes round differently in FP32
+  and BF16, a wrong Jacobian term or spline branch is off by O(1) per link);
+- the learned Metropolis sweep alone samples the Wilson action: plaquette of
+  learned sweeps only against heatbath sweeps only on 4^4 at 2 sigma (binned
+  errors), acceptance strictly between 0 and 1 (an error in q or in the
+  local action shows at many sigma: neighbouring sweeps that shared random
+  numbers gave 7 sigma);
+- the C4 chains: 1 heatbath sweep per step against 1 heatbath sweep plus one
+  learned sweep per step, plaquette, Polyakov loop and 2x2 Wilson loop at
+  2 sigma, on fixed seeds."""
+import os
+
+import numpy as np
+import pytest
+
+import qcdgpu_amd as Q
+from qcdgpu_amd import analysis as A
+from qcdgpu_amd import flow as QF
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+DIMS, BETA = (8, 8, 8, 4), 2.2
+
+
+def _case(msg):
+    print("[case]", msg, flush=True)
+
+
+def _params():
+    return np.load(os.path.join(HERE, "golden", "c4_flow_params.npy")).astype(np.float32)
+
+
+def _flow(dims, batch=8):
+    f = QF.Flow(dims=dims, layers=8, bins=4, max_batch=batch)
+    f.set_params(_params())
+    return f
+
+
+def test_c4_sample_evaluate_consistency():
+    _case("c4 sample vs evaluate")
+    with _flow(DIMS) as f:
+        links, lq_fwd, S = f.sample(123, 8, BETA)
+        lq_inv, S_inv = f.evaluate(links, BETA)
+    d = np.abs(lq_fwd - lq_inv)
+    print("  |log q| %.1f, max |fwd - inv| %.4f" % (float(np.abs(lq_fwd).mean()), float(d.max())))
+    assert np.abs(lq_fwd).min() > 1000.0   # a trained, steep map (the identity gives 0)
+    assert d.max() <= 0.1
+    assert np.allclose(S, S_inv, rtol=1e-12)
+
+
+def _plaquette_chain(dims, seed, sweeps, therm, flow=None, hb=True):
+    with Q.Lattice(sun=2, dims=dims, beta=BETA, precision="fp32", prng="XOR128", randseries=seed,
+                   max_measurements=sweeps + 2) as lat:
+        lat.prng_init()
+        lat.init("hot")
+        for i in range(therm + sweeps):
+            if hb:
+                lat.sweep("hb")
+            if flow is not None:
+                flow.imh_sweep(lat, 977 * seed + i, BETA)
+            if i >= therm:
+                lat.measure(polyakov=False)
+        plq = lat.measurements()["plq_total"] / 2.0
+    return plq
+
+
+def test_c4_learned_sweep_alone_is_exact():
+    _case("learned sweeps alone vs heatbath, 4^4")
+    dims, n = (4, 4, 4, 4), 20000
+    p_hb = _plaquette_chain(dims, 77, n, 200)
+    with _flow(dims, batch=1) as f:
+        f.imh_counts(reset=True)
+        p_fl = _plaquette_chain(dims, 78, n, 200, flow=f, hb=False)
+        acc, prop = f.imh_counts()
+    m1, e1 = A.binned_error(p_hb, nbins=20)
+    m2, e2 = A.binned_error(p_fl, nbins=20)
+    z = (m2 - m1) / np.hypot(e1, e2)
+    print("  heatbath %.6f(%.6f) learned %.6f(%.6f) z %.2f acceptance %.3f" % (m1, e1, m2, e2, z, acc / prop))
+    assert prop == (n + 200) * 4 * 4 ** 4
+    assert 0.05 < acc / prop < 0.95
+    assert abs(z) < 2.0
+
+
+def test_c4_chains_match_plain_heatbath():
+    _case("c4 chains")
+    steps = 4000
+    res = {}
+    with _flow(DIMS, batch=1) as f:
+        for arm in ("plain", "learned"):
+            seed = 54 if arm == "plain" else 1054
+            with Q.Lattice(sun=2, dims=DIMS, beta=BETA, precision="fp32", prng="XOR128", randseries=seed,
+                           max_measurements=steps + 2) as lat:
+                lat.set_wilson(2, 2)
+                lat.prng_init()
+                lat.init("hot")
+                for _ in range(200):
+                    lat.sweep("hb")
+                lat.reset_measurements()
+                f.imh_counts(reset=True)
+                for i in range(steps):
+                    lat.sweep("hb")
+                    if arm == "learned":
+                        f.imh_sweep(lat, seed * 7919 + i, BETA)
+                    lat.measure(wilson=True)
+                m = lat.measurements()
+                res[arm] = {"plaquette": m["plq_total"] / 2.0, "polyakov": m["polyakov_re"],
+                            "wilson": lat.wilson_measurements() / 2.0}
+                if arm == "learned":
+                    acc, prop = f.imh_counts()
+                    print("  acceptance %.3f" % (acc / prop))
+                    assert 0.05 < acc / prop < 0.95
+    for name in ("plaquette", "polyakov", "wilson"):
+        m1, e1 = A.binned_error(res["plain"][name], nbins=20)
+        m2, e2 = A.binned_error(res["learned"][name], nbins=20)
+        z = (m2 - m1) / np.hypot(e1, e2)
+        print("  %s plain %.6f(%.6f) learned %.6f(%.6f) z %.2f" % (name, m1, e1, m2, e2, z))
+        assert abs(z) < 2.0

@@ -3,13 +3,13 @@ against the FP32 training model on the same parameters, forward/inverse
 consistency, the Haar prior, and exactness of the independence-Metropolis
 chain against the plain heatbath on a small lattice.
 
-Bars: log q from the device within 0.5 + 2 % of |log q| of the FP32 model
-(BF16 activations and weights in the two convolutions; a wrong MFMA operand
-layout gives O(|log q|) errors); action within 1e-6 relative (FP32 plaquette
-products summed in FP64, against float64 products); device sample -> device
-evaluate equal to FP32
-rounding; Haar moments of a0 at 5 sigma; IMH-only and heatbath-only chains
-agree on the mean plaquette within 4 binned standard errors."""
+Bars: log q from the device within 0.5 + 0.3 % of |log q| of the FP32 model
+(BF16 activations and weights in the two convolutions, measured 0.06 %; a
+wrong MFMA operand layout gives O(|log q|) errors); action within 1e-6
+relative (FP32 plaquette products summed in FP64, against float64
+products); device sample -> device evaluate equal to FP32 rounding; Haar
+moments of a0 at 5 sigma; IMH-only and heatbath-only chains agree on the
+mean plaquette within 2 binned standard errors."""
 import numpy as np
 import pytest
 
@@ -67,7 +67,7 @@ def test_device_flow_matches_fp32_model():
             S_t = m.action(Ut.double(), 2.2).numpy()
         print("  log q device", np.round(lq, 3), "FP32 model", np.round(lq_t, 3))
         assert np.all(np.abs(lq) > 1.0)   # a non-trivial map
-        assert np.all(np.abs(lq - lq_t) < 0.5 + 0.02 * np.abs(lq_t))
+        assert np.all(np.abs(lq - lq_t) < 0.5 + 0.003 * np.abs(lq_t))
         assert np.allclose(S, S_t, rtol=1e-6, atol=1e-6)   # FP32 plaquette products, FP64 sums
         # sample -> evaluate round trip.  Inverting a steep spline amplifies
         # rounding by 1/f' (down to 1e-3 here), which moves BF16-quantised
@@ -91,7 +91,7 @@ def test_device_flow_matches_fp32_model():
         print("  mild map: sample log q", np.round(lq_s[:4], 4), "re-evaluated", np.round(lq_e[:4], 4),
               "FP32 model", np.round(lq_m[:4], 4), "max |d| %.2e" % np.max(np.abs(lq_e - lq_s)))
         assert np.all(np.abs(lq) > 1.0) and np.all(np.abs(lq_e - lq_s) < 0.05 + 1e-3 * np.abs(lq_s))
-        assert np.all(np.abs(lq_s - lq_m) < 0.5 + 0.02 * np.abs(lq_m))
+        assert np.all(np.abs(lq_s - lq_m) < 0.5 + 0.003 * np.abs(lq_m))
 
 
 def test_identity_flow_samples_haar():
@@ -135,4 +135,4 @@ def test_imh_chain_is_exact():
         m2, e2 = A.binned_error(p_im)
         print("  acceptance %.3f  <plaq> heatbath %.5f +- %.5f  IMH %.5f +- %.5f" % (acc, m1, e1, m2, e2))
         assert acc > 0.05
-        assert abs(m1 - m2) < 4 * np.hypot(e1, e2)
+        assert abs(m1 - m2) < 2 * np.hypot(e1, e2)

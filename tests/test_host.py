@@ -329,3 +329,29 @@ def test_end_of_run_check_twisted_and_e_field(O, tmp_path):
     r = _verify(tmp_path, init, a, last)
     assert r.returncode == 0, r.stdout
     assert " GPU last Fmunu_zt_6_im" in r.stdout
+
+
+@pytest.mark.gpu
+def test_host_profile_table(tmp_path):
+    """PROFILE = 1 prints GPU::print_time_detailed's kernel table
+    (clinterface.cpp:1626-1666): one entry per kernel family with its launch
+    count -- 8 heatbath and 8 NOR overrelaxation half-steps per sweep, one
+    plaquette, Polyakov and final-sum launch per measurement -- and a mean
+    time."""
+    print("[case] host profile table", flush=True)
+    f = tmp_path / "init.dat"
+    f.write_text("GROUP = 3\nLS = 8\nLT = 4\nITER = 3\nNITER = 2\nNAV = 1\nBETA = 5.9\nINTS = 1\nPRNG = XOR128\n"
+                 "PRECISION = SINGLE\nNOR = 1\nPL_LEVEL = 1\nTURNOFFFMUNU = 1\nPROFILE = 1\nOUTPUTPATH = %s/\n" % tmp_path)
+    r = subprocess.run([BIN, str(f)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    import re
+
+    table = dict(re.findall(r'kernel "([^"]+)" executed (\d+) times', r.stdout))
+    sweeps = 1 + 2 * 2   # NAV + (ITER - 1) NITER
+    assert int(table["heatbath half-step"]) == 8 * sweeps, table
+    assert int(table["overrelaxation half-step"]) == 8 * sweeps
+    assert int(table["plaquette (lattice_measurement_plq)"]) == 3
+    assert int(table["Polyakov loop (lattice_polyakov)"]) == 3
+    assert int(table["final sums (reduce)"]) == 3
+    times = [float(t) for t in re.findall(r"execution time: ([0-9.]+) \(\+/-", r.stdout)]
+    assert len(times) == len(table) and all(t > 0 for t in times)

@@ -22,16 +22,23 @@ def _alone(dims, beta, seed, prng, prec, cycles, n_or):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("prng,prec", [("RANLUX", "fp32"), ("XOR128", "fp64")])
-@pytest.mark.parametrize("tune", [False, True])
+@pytest.mark.parametrize("tune", ["default", "autotune", "s1p1", "s3p2", "s1p2"])
 def test_batch_equals_chains_alone(prng, prec, tune):
+    """Every kernel shape and unit placement (by hand, or as the autotuner
+    picks them) gives each chain exactly the chain run alone."""
     print("[case] batch %s %s tune=%s" % (prng, prec, tune), flush=True)
     cycles, n_or = 2, 2
     with Q.Batch(CHAINS, precision=prec, prng=prng, max_measurements=16) as b:
         assert b.info() == {"chains": 4, "buckets": 2}
         b.init("hot")
-        if tune:
-            shapes = b.autotune(repeats=2)
-            assert all(s in (1, 3) for pair in shapes for s in pair)
+        if tune == "autotune":
+            t = b.autotune(repeats=2)
+            assert all(s in (1, 3) for d in t for s in d["shape"])
+            assert all(p in (0, 1, 2) for d in t for p in d["placement"])
+        elif tune != "default":
+            shape, place = int(tune[1]), int(tune[3])
+            for k in range(2):
+                b.set_tuning(k, shape=(shape, shape), placement=(place, 2 - place if k else place))
         b.capture_cycle(n_hb=1, n_or=n_or)
         b.launch_cycle(cycles)
         b.sync()
@@ -60,13 +67,20 @@ def test_c3_batched_ensemble():
     with Q.Batch(chains, precision="fp32", prng="RANLUX", max_measurements=320) as b:
         assert b.info()["chains"] == 96
         b.init("hot")
-        shapes = b.autotune(repeats=4)
+        tuning = b.autotune(repeats=4)
         b.capture_cycle(n_hb=1, n_or=4)
         ms = b.time_cycle(300)
         b.sync()
         V = sum(int(np.prod(c[0])) for c in chains)
         rate = 300 * 5 * 4 * V / (ms * 1e-3)
-        print("  shapes per bucket %s; %.1f ms for 300 cycles, %.3e link updates/s" % (shapes, ms, rate))
+        # algorithmic bytes per cycle: 4V link updates per sweep at 9 x 72 B
+        # (+ 2 x 112 B of RANLUX state for the heatbath), plaquette and
+        # Polyakov 5V x 72 B
+        alg = 4 * V * ((9 * 72 + 2 * 112) + 4 * 9 * 72) + 5 * V * 72
+        gbs = alg * 300 / (ms * 1e-3) / 1e9
+        print("  tuning %s" % tuning)
+        print("  %.1f ms for 300 cycles, %.3e link updates/s, %.0f GB/s algorithmic = %.1f %% of 8 TB/s"
+              % (ms, rate, gbs, gbs / 80.0))
         plaq, chi = [], []
         for k, (dims, beta, _) in enumerate(chains):
             m = b.chain(k).measurements()
