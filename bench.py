@@ -133,10 +133,17 @@ def copy_calibration(mib=256, repeats=20):
     n = mib * (1 << 20) // 4
     src = torch.ones(n, dtype=torch.float32, device="cuda")
     dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0.record()
     for _ in range(repeats):
         dst.copy_(src)
+    t1.record()
     torch.cuda.synchronize()
-    print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "launches": repeats}))
+    ms = t0.elapsed_time(t1) / repeats
+    print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "launches": repeats + 1,
+                      "copy_GBs": 2.0 * n * 4 / (ms * 1e-3) / 1e9}))
 
 
 def c4(args):
@@ -228,6 +235,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--xcd-map", type=int, default=None, choices=(0, 1, 2, 3),
                     help="dispatch order of the update kernels (default: the library's)")
+    ap.add_argument("--blocks-per-cu", type=int, default=None,
+                    help="cap on resident update workgroups per CU (default: the library's, none)")
+    ap.add_argument("--update-shape", type=int, default=None, choices=(1, 3),
+                    help="wavefronts per 64-link unit of the update kernels (default: the library's, 3)")
     ap.add_argument("--profile-steps", type=int, default=50)
     ap.add_argument(
         "--copy-calibration",
@@ -275,8 +286,7 @@ def main():
         randseries=54,
         max_measurements=args.steps + args.warmup + args.profile_steps + 8,
     )
-    if args.xcd_map is not None:
-        lat.tune(xcd_map=args.xcd_map)
+    lat.tune(xcd_map=args.xcd_map, blocks_per_cu=args.blocks_per_cu, update_shape=args.update_shape)
     lat.init("cold")
     lat.prng_init()
     lat.capture_cycle(n_hb=1, n_or=N_OR, plaquette=True, polyakov=True)
