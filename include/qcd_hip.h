@@ -144,9 +144,44 @@ int  qcd_lattice_sync(qcd_lattice *lat);
                           LDS), 0 = no cap (default).
    QCD_TUNE_UPDATE_SHAPE  3: three wavefronts share 64 links, one staple
                           direction each (default); 1: one wavefront per 64
-                          links, workgroups of four.                         */
-enum { QCD_TUNE_XCD_MAP = 1, QCD_TUNE_BLOCKS_PER_CU = 2, QCD_TUNE_UPDATE_SHAPE = 3 };
+                          links, workgroups of four.
+   QCD_TUNE_SHADOW        race check (new; the reference has none): 1 makes
+                          every update half-step gather its staples from a
+                          copy of the table taken just before it (one more
+                          table-sized device buffer and one device copy per
+                          half-step) and write into the table, 0 (default)
+                          updates in place.  A half-step reads only links it
+                          does not write, so the two give identical bits; a
+                          difference means a kernel read a link that the same
+                          launch writes.  Single lattices only.              */
+enum { QCD_TUNE_XCD_MAP = 1, QCD_TUNE_BLOCKS_PER_CU = 2, QCD_TUNE_UPDATE_SHAPE = 3, QCD_TUNE_SHADOW = 4 };
 int  qcd_lattice_tune(qcd_lattice *lat, int knob, int value);
+/* ---- kernels by name (GPU::kernel_init / kernel_run, clinterface.cpp:1132-1278,
+        with the names lattice_make_programs creates, suncl.cpp:3265-3648,
+        and the PRNG's, random.cpp:487-493,668,886), for a host written
+        against the reference's registry: update_{odd,even}_{X,Y,Z,T} (a
+        heatbath half-step, its PRN production included),
+        overrelax_{odd,even}_{X,Y,Z,T} (new), lattice_init_cold / _gid,
+        lattice_init_hot_X (the reference's whole hot start; _Y, _Z, _T run
+        nothing), lattice_GramSchmidt, lattice_measurement_plq,
+        lattice_measurement (with F_munu), lattice_polyakov,
+        lattice_measurement_wilson, lattice_polyakov_diff_x and
+        lattice_action_diff_x (the slices of all three directions; _y, _z
+        nothing); the reduce_* kernels, clear_measurement and the PRNG
+        kernels xor128 / pm / ranlux are accepted and run nothing (their
+        work is part of the launches above).  Ids count from 1.  _run
+        enqueues and waits (as kernel_run); _enqueue only enqueues.         */
+int  qcd_kernel_create(qcd_lattice *lat, const char *name, int *id);
+int  qcd_kernel_run(qcd_lattice *lat, int id);
+int  qcd_kernel_enqueue(qcd_lattice *lat, int id);
+/* Name and number of runs (kernel_number_of_starts) of kernel `id`.        */
+int  qcd_kernel_info(const qcd_lattice *lat, int id, char *name, int name_len, unsigned *starts);
+/* Records the kernels enqueued between _begin and _end as the lattice's
+   graph (replacing a captured cycle), replayed by qcd_cycle_launch /
+   qcd_cycle_time.  With fused XOR128 / PM streams the recorded heatbath
+   half-steps must be even in number.                                       */
+int  qcd_graph_begin(qcd_lattice *lat);
+int  qcd_graph_end(qcd_lattice *lat);
 /* ---- kernel timing (GPU::print_time_detailed, clinterface.cpp:1626-1666,
         shown when GPU_debug.profiling is set, suncl.cpp:265).  With it on,
         every kernel family the lattice launches eagerly (update half-steps,

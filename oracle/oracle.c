@@ -793,6 +793,33 @@ int orc_halfstep(const orc_lattice *L, double *links, int parity, int mu, int mo
     return rejected;
 }
 
+/* The same half-step with the staples read from `src` (a copy of the
+   links taken before it) and the updated links written into `links`: the
+   double-buffered form the GPU's shadow mode runs.  For parity = 2 the
+   half-step takes the sites of both parities in one pass (a planted read
+   of links the same pass writes, for the tests of the race check).        */
+int orc_halfstep_src(const orc_lattice *L, double *links, const double *src, int parity, int mu, int mode,
+                     orc_rng *rng)
+{
+    unsigned V = orc_sites(L);
+    double beta = L->beta / L->sun;
+    int N = L->sun, rejected = 0;
+    double A[18];
+    const int np = parity == 2 ? 2 : 1;
+    for (int k = 0; k < np; ++k) {
+        const int par = parity == 2 ? 1 - k : parity;
+        for (unsigned i = 0; i < V / 2; ++i) {
+            unsigned s = orc_parity_site(L, i, par);
+            orc_staple(L, src ? src : links, s, mu, A);
+            kp_ctx c = {rng, i, 0, s, V};
+            double *U = LINK(L, links, s, mu);
+            if (N == 3) rejected += update_su3(U, A, beta, L->nhit, mode, &c);
+            else rejected += update_su2(U, A, beta, L->nhit, mode, &c);
+        }
+    }
+    return rejected;
+}
+
 int orc_sweep(const orc_lattice *L, double *links, int mode, orc_rng *rng)
 {
     int rejected = 0;

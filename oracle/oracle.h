@@ -106,6 +106,8 @@ void     orc_staple(const orc_lattice *L, const double *links, unsigned site, in
 /* One (parity, mu) half-step.  For ORC_RNG_QUADS the caller provides the
    quads of this half-step. Returns the number of rejected-all subgroups. */
 int      orc_halfstep(const orc_lattice *L, double *links, int parity, int mu, int mode, orc_rng *rng);
+int      orc_halfstep_src(const orc_lattice *L, double *links, const double *src, int parity, int mu, int mode,
+                          orc_rng *rng);
 /* One sweep in the reference order odd X..T, even X..T. For QUADS the
    caller-provided callback is not available, so only GID/SEQ are allowed. */
 int      orc_sweep(const orc_lattice *L, double *links, int mode, orc_rng *rng);

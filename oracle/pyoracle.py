@@ -295,6 +295,13 @@ def halfstep(L, links, parity, mu, mode, rng):
     return lib().orc_halfstep(ctypes.byref(L), _p(links, _F64P), parity, mu, mode, ctypes.byref(rng.rng))
 
 
+def halfstep_src(L, links, src, parity, mu, mode, rng):
+    """orc_halfstep with the staples read from `src` (parity 2: both
+    parities in one pass)."""
+    return lib().orc_halfstep_src(ctypes.byref(L), _p(links, _F64P), _p(src, _F64P), parity, mu, mode,
+                                  ctypes.byref(rng.rng))
+
+
 def sweep(L, links, mode, rng):
     return lib().orc_sweep(ctypes.byref(L), _p(links, _F64P), mode, ctypes.byref(rng.rng))
 

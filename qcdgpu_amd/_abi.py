@@ -20,7 +20,7 @@ STREAM_REFERENCE, STREAM_FAST = 0, 1
 HEATBATH, OVERRELAX = 0, 1
 MEAS_PLAQUETTE, MEAS_POLYAKOV, MEAS_WILSON, MEAS_POLYAKOV_SLICES = 1, 2, 4, 8
 MEAS_FMUNU, MEAS_F0MU, MEAS_ACTION_SLICES = 16, 32, 64
-TUNE_XCD_MAP, TUNE_BLOCKS_PER_CU, TUNE_UPDATE_SHAPE = 1, 2, 3
+TUNE_XCD_MAP, TUNE_BLOCKS_PER_CU, TUNE_UPDATE_SHAPE, TUNE_SHADOW = 1, 2, 3, 4
 
 
 class Params(ctypes.Structure):
@@ -85,6 +85,12 @@ _SIGNATURES = {
     "qcd_lattice_sync": [_P],
     "qcd_lattice_table_bytes": [_P, ctypes.POINTER(ctypes.c_size_t)],
     "qcd_lattice_tune": [_P, ctypes.c_int, ctypes.c_int],
+    "qcd_kernel_create": [_P, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int)],
+    "qcd_kernel_run": [_P, ctypes.c_int],
+    "qcd_kernel_enqueue": [_P, ctypes.c_int],
+    "qcd_kernel_info": [_P, ctypes.c_int, ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_uint)],
+    "qcd_graph_begin": [_P],
+    "qcd_graph_end": [_P],
     "qcd_lattice_profile": [_P, ctypes.c_int],
     "qcd_profile_count": [_P, ctypes.POINTER(ctypes.c_int)],
     "qcd_profile_entry": [_P, ctypes.c_int, ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_uint), _D, _D],

@@ -56,6 +56,10 @@ enum RngKind { RK_NONE = 0, RK_GID = 1, RK_XOR128 = 2, RK_PM = 3, RK_BUF = 4 };
 
 struct UpdArgs {
     void* table;
+    // Table the staple gathers read: the table itself, or (shadow mode, a
+    // race check) a copy of it taken before the half-step.  A half-step
+    // reads only links it does not write, so both give the same bits.
+    const void* src;
     Geom g;
     int parity;
     int mu;
@@ -570,6 +574,7 @@ __device__ __forceinline__ void update3(const UpdArgs& a, uint32_t blk, uint32_t
     const uint32_t i = (a.unit0 + blk) * kLinks + lane;
     const bool active = i < a.g.vh;
     Table<R, N, NREAL> Tb{(typename Vec2<R>::T*)a.table, a.g.vh_pad};
+    const Table<R, N, NREAL> Ts{(typename Vec2<R>::T*)a.src, a.g.vh_pad};
     const int p = a.parity, mu = (MU >= 0) ? MU : a.mu;
     const int nu = w + (w >= mu ? 1 : 0);
     const Site s = site_of(a.g, active ? i : 0u, p);
@@ -602,7 +607,7 @@ __device__ __forceinline__ void update3(const UpdArgs& a, uint32_t blk, uint32_t
         }
     }
     M<R, N> S;
-    if (active) S = staple_nu<R, N, NREAL, TW>(Tb, a.g, s, i, p, mu, nu, a.tw);
+    if (active) S = staple_nu<R, N, NREAL, TW>(Ts, a.g, s, i, p, mu, nu, a.tw);
     else m_zero(S);
     // Wavefront 0 issues the load of the link it updates before the
     // barrier, so its latency overlaps the wait for the other shares rather
@@ -696,6 +701,7 @@ __device__ __forceinline__ void update1(const UpdArgs& a, uint32_t unit, uint32_
     const uint32_t i = (a.unit0 + unit) * kLinks + lane;   // < number of streams
     const bool active = i < a.g.vh;
     Table<R, N, NREAL> Tb{(typename Vec2<R>::T*)a.table, a.g.vh_pad};
+    const Table<R, N, NREAL> Ts{(typename Vec2<R>::T*)a.src, a.g.vh_pad};
     const int p = a.parity, mu = a.mu;
     const Site s = site_of(a.g, active ? i : 0u, p);
 
@@ -710,7 +716,7 @@ __device__ __forceinline__ void update1(const UpdArgs& a, uint32_t unit, uint32_
     if (active) {
         for (int k = 0; k < 3; ++k) {
             const int nu = k + (k >= mu ? 1 : 0);
-            const M<R, N> S = staple_nu<R, N, NREAL>(Tb, a.g, s, i, p, mu, nu);
+            const M<R, N> S = staple_nu<R, N, NREAL>(Ts, a.g, s, i, p, mu, nu);
             m_acc(A, S);
             if constexpr (MEAS) {
                 if (nu < mu) m_acc(Alow, S);
@@ -1768,6 +1774,7 @@ struct qcd_lattice {
     int tune_xcd_map = 2;       // QCD_TUNE_XCD_MAP
     int tune_blocks_per_cu = 0; // QCD_TUNE_BLOCKS_PER_CU (0 = no cap)
     int tune_update_shape = 3;  // QCD_TUNE_UPDATE_SHAPE: wavefronts per 64 links (3 or 1)
+    void* shadow = nullptr;     // QCD_TUNE_SHADOW: the copy the update gathers read (race check)
     // A chain of a qcd_batch: the batch's device descriptors hold this
     // lattice's buffers and stream sizing, so neither may change (no
     // re-sizing start, no destroy except through the batch).
@@ -1788,6 +1795,12 @@ struct qcd_lattice {
     bool profiling = false;
     std::vector<ProfStat> prof;
     hipEvent_t prof_ev[2] = {nullptr, nullptr};
+    // Kernels created by name (GPU::kernel_init): registry index -> entry of
+    // the name table, and how often each ran (GPU_kernels[i].kernel_number_of_starts).
+    std::vector<std::pair<int, unsigned>> kernels;
+    bool graph_open = false;        // qcd_graph_begin .. qcd_graph_end
+    uint32_t graph_counter0 = 0;
+    int graph_rng0 = 0;
 };
 
 namespace {
@@ -1992,6 +2005,11 @@ int launch_update_rn(qcd_lattice* L, int parity, int mu, int kind, int start_gid
 {
     UpdArgs a;
     a.table = L->table;
+    a.src = L->table;
+    if (L->shadow) {   // race check: the gathers read a copy taken before the half-step
+        HIPCHK(hipMemcpyAsync(L->shadow, L->table, L->table_bytes, hipMemcpyDeviceToDevice, L->stream));
+        a.src = L->shadow;
+    }
     a.g = L->g;
     a.parity = parity;
     a.mu = mu;
@@ -2481,6 +2499,7 @@ int qcd_lattice_destroy(qcd_lattice* L)
     for (auto& e : L->prof_ev)
         if (e) (void)hipEventDestroy(e);
     (void)hipFree(L->table);
+    (void)hipFree(L->shadow);
     (void)hipFree(L->rng_buf[0]);
     (void)hipFree(L->rng_buf[1]);
     (void)hipFree(L->jump_cols);
@@ -2506,6 +2525,142 @@ int qcd_lattice_sync(qcd_lattice* L)
 {
     if (!L) return fail(QCD_E_ARG, "null lattice");
     HIPCHK(hipStreamSynchronize(L->stream));
+    return QCD_OK;
+}
+
+// Kernel names of the reference's lattice_make_programs (suncl.cpp:3265-3648)
+// and PRNG::initialize (random.cpp:487-493,668,886), each bound to what this
+// library runs for it.  Names whose work is done inside another launch here
+// (the PRN production inside every heatbath update, the reductions inside
+// the measurement's final sums, the four directions of the reference's hot
+// start in one call) are accepted and run nothing.
+enum KAct { KA_NONE, KA_HB, KA_OR, KA_COLD, KA_GID, KA_HOT, KA_GS, KA_MEAS };
+struct KName {
+    const char* name;
+    KAct act;
+    int parity, mu, mask;
+};
+static const KName kKernelNames[] = {
+    {"update_odd_X", KA_HB, 1, 0, 0},   {"update_odd_Y", KA_HB, 1, 1, 0},   {"update_odd_Z", KA_HB, 1, 2, 0},
+    {"update_odd_T", KA_HB, 1, 3, 0},   {"update_even_X", KA_HB, 0, 0, 0},  {"update_even_Y", KA_HB, 0, 1, 0},
+    {"update_even_Z", KA_HB, 0, 2, 0},  {"update_even_T", KA_HB, 0, 3, 0},
+    // overrelaxation half-steps: no reference kernel, named after the updates
+    {"overrelax_odd_X", KA_OR, 1, 0, 0}, {"overrelax_odd_Y", KA_OR, 1, 1, 0}, {"overrelax_odd_Z", KA_OR, 1, 2, 0},
+    {"overrelax_odd_T", KA_OR, 1, 3, 0}, {"overrelax_even_X", KA_OR, 0, 0, 0}, {"overrelax_even_Y", KA_OR, 0, 1, 0},
+    {"overrelax_even_Z", KA_OR, 0, 2, 0}, {"overrelax_even_T", KA_OR, 0, 3, 0},
+    {"lattice_init_cold", KA_COLD, 0, 0, 0}, {"lattice_init_gid", KA_GID, 0, 0, 0},
+    {"lattice_init_hot_X", KA_HOT, 0, 0, 0}, {"lattice_init_hot_Y", KA_NONE, 0, 0, 0},
+    {"lattice_init_hot_Z", KA_NONE, 0, 0, 0}, {"lattice_init_hot_T", KA_NONE, 0, 0, 0},
+    {"lattice_GramSchmidt", KA_GS, 0, 0, 0},
+    {"lattice_measurement_plq", KA_MEAS, 0, 0, QCD_MEAS_PLAQUETTE},
+    {"lattice_measurement", KA_MEAS, 0, 0, QCD_MEAS_PLAQUETTE | QCD_MEAS_FMUNU},
+    {"lattice_polyakov", KA_MEAS, 0, 0, QCD_MEAS_POLYAKOV},
+    {"lattice_measurement_wilson", KA_MEAS, 0, 0, QCD_MEAS_WILSON},
+    {"lattice_polyakov_diff_x", KA_MEAS, 0, 0, QCD_MEAS_POLYAKOV_SLICES},
+    {"lattice_polyakov_diff_y", KA_NONE, 0, 0, 0}, {"lattice_polyakov_diff_z", KA_NONE, 0, 0, 0},
+    {"lattice_action_diff_x", KA_MEAS, 0, 0, QCD_MEAS_ACTION_SLICES},
+    {"lattice_action_diff_y", KA_NONE, 0, 0, 0}, {"lattice_action_diff_z", KA_NONE, 0, 0, 0},
+    {"reduce_measurement_plq_double2", KA_NONE, 0, 0, 0}, {"reduce_measurement_double2", KA_NONE, 0, 0, 0},
+    {"reduce_polyakov_double2", KA_NONE, 0, 0, 0}, {"reduce_wilson_double2", KA_NONE, 0, 0, 0},
+    {"reduce_polyakov_diff_x_double2", KA_NONE, 0, 0, 0}, {"reduce_polyakov_diff_y_double2", KA_NONE, 0, 0, 0},
+    {"reduce_polyakov_diff_z_double2", KA_NONE, 0, 0, 0}, {"reduce_action_diff_x_double2", KA_NONE, 0, 0, 0},
+    {"reduce_action_diff_y_double2", KA_NONE, 0, 0, 0}, {"reduce_action_diff_z_double2", KA_NONE, 0, 0, 0},
+    {"clear_measurement", KA_NONE, 0, 0, 0},
+    {"xor128", KA_NONE, 0, 0, 0}, {"pm", KA_NONE, 0, 0, 0}, {"ranlux", KA_NONE, 0, 0, 0},
+};
+
+int qcd_kernel_create(qcd_lattice* L, const char* name, int* id)
+{
+    if (!L || !name || !id) return fail(QCD_E_ARG, "null argument");
+    for (size_t k = 0; k < sizeof(kKernelNames) / sizeof(kKernelNames[0]); ++k)
+        if (strcmp(kKernelNames[k].name, name) == 0) {
+            L->kernels.push_back({(int)k, 0u});
+            *id = (int)L->kernels.size();   // ids count from 1, as GPU::kernel_init's
+            return QCD_OK;
+        }
+    return fail(QCD_E_ARG, "no kernel \"%s\" on this path", name);
+}
+
+int qcd_kernel_enqueue(qcd_lattice* L, int id)
+{
+    if (!L) return fail(QCD_E_ARG, "null lattice");
+    if (id < 1 || id > (int)L->kernels.size()) return fail(QCD_E_ARG, "kernel id %d was not created", id);
+    const KName& k = kKernelNames[L->kernels[id - 1].first];
+    int rc = QCD_OK;
+    switch (k.act) {
+        case KA_HB: rc = qcd_halfstep(L, k.parity, k.mu, QCD_HEATBATH); break;
+        case KA_OR: rc = qcd_halfstep(L, k.parity, k.mu, QCD_OVERRELAX); break;
+        case KA_COLD: rc = qcd_lattice_init(L, QCD_START_COLD); break;
+        case KA_GID: rc = qcd_lattice_init(L, QCD_START_GID); break;
+        case KA_HOT: rc = qcd_lattice_init(L, QCD_START_RANDOM); break;
+        case KA_GS: rc = qcd_reunitarize(L); break;
+        case KA_MEAS: rc = qcd_measure(L, k.mask); break;
+        default: break;
+    }
+    if (rc == QCD_OK) ++L->kernels[id - 1].second;
+    return rc;
+}
+
+int qcd_kernel_run(qcd_lattice* L, int id)
+{
+    const int rc = qcd_kernel_enqueue(L, id);
+    if (rc) return rc;
+    if (L->graph_open) return QCD_OK;   // under capture nothing runs yet
+    HIPCHK(hipStreamSynchronize(L->stream));
+    return QCD_OK;
+}
+
+int qcd_kernel_info(const qcd_lattice* L, int id, char* name, int name_len, unsigned* starts)
+{
+    if (!L) return fail(QCD_E_ARG, "null lattice");
+    if (id < 1 || id > (int)L->kernels.size()) return fail(QCD_E_ARG, "kernel id %d was not created", id);
+    if (name && name_len > 0) snprintf(name, (size_t)name_len, "%s", kKernelNames[L->kernels[id - 1].first].name);
+    if (starts) *starts = L->kernels[id - 1].second;
+    return QCD_OK;
+}
+
+int qcd_graph_begin(qcd_lattice* L)
+{
+    if (!L) return fail(QCD_E_ARG, "null lattice");
+    if (L->graph_open) return fail(QCD_E_STATE, "a graph is already being recorded");
+    if (L->profiling) return fail(QCD_E_STATE, "kernel timing waits for every kernel: no graph while it is on");
+    if (L->exec) {
+        (void)hipGraphExecDestroy(L->exec);
+        L->exec = nullptr;
+    }
+    if (L->graph) {
+        (void)hipGraphDestroy(L->graph);
+        L->graph = nullptr;
+    }
+    HIPCHK(hipStreamSynchronize(L->stream));
+    HIPCHK(hipStreamBeginCapture(L->stream, hipStreamCaptureModeThreadLocal));
+    L->graph_open = true;
+    L->graph_counter0 = L->prng_counter;
+    L->graph_rng0 = L->rng_cur;
+    L->cyc_hb = L->cyc_or = L->cyc_mask = 0;
+    return QCD_OK;
+}
+
+int qcd_graph_end(qcd_lattice* L)
+{
+    if (!L) return fail(QCD_E_ARG, "null lattice");
+    if (!L->graph_open) return fail(QCD_E_STATE, "no graph is being recorded");
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(L->stream, &graph);
+    L->graph_open = false;
+    // productions count when the graph is launched, not when recorded
+    L->cyc_prn = L->prng_counter - L->graph_counter0;
+    L->prng_counter = L->graph_counter0;
+    if (e != hipSuccess) return fail(QCD_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    // fused XOR128 / PM heatbath half-steps alternate the state buffers: a
+    // replayed graph must end on the buffer it starts from
+    if (L->rng_cur != L->graph_rng0) {
+        (void)hipGraphDestroy(graph);
+        L->rng_cur = L->graph_rng0;
+        return fail(QCD_E_STATE, "a graph needs an even number of heatbath half-steps with fused streams");
+    }
+    L->graph = graph;
+    HIPCHK(hipGraphInstantiate(&L->exec, graph, nullptr, nullptr, 0));
     return QCD_OK;
 }
 
@@ -2610,6 +2765,20 @@ int qcd_lattice_tune(qcd_lattice* L, int knob, int value)
             if (value != 1 && value != 3) return fail(QCD_E_ARG, "update shape must be 1 or 3");
             L->tune_update_shape = value;
             break;
+        case QCD_TUNE_SHADOW:
+            if (value != 0 && value != 1) return fail(QCD_E_ARG, "shadow mode must be 0 or 1");
+            if (L->batch_owned) return fail(QCD_E_STATE, "shadow mode applies to single lattices, not batched chains");
+            if (value && !L->shadow) {
+                if (hipMalloc(&L->shadow, L->table_bytes) != hipSuccess) {
+                    L->shadow = nullptr;
+                    return fail(QCD_E_NOMEM, "shadow table (%zu bytes)", L->table_bytes);
+                }
+            } else if (!value && L->shadow) {
+                HIPCHK(hipStreamSynchronize(L->stream));
+                HIPCHK(hipFree(L->shadow));
+                L->shadow = nullptr;
+            }
+            break;
         default:
             return fail(QCD_E_ARG, "unknown tuning knob %d", knob);
     }
@@ -2632,6 +2801,7 @@ int qcd_lattice_get_tune(const qcd_lattice* L, int knob, int* value, int* detail
         }
         case QCD_TUNE_BLOCKS_PER_CU: *value = L->tune_blocks_per_cu; break;
         case QCD_TUNE_UPDATE_SHAPE: *value = L->tw.on ? 3 : L->tune_update_shape; break;
+        case QCD_TUNE_SHADOW: *value = L->shadow ? 1 : 0; break;
         default: return fail(QCD_E_ARG, "unknown tuning knob %d", knob);
     }
     return QCD_OK;
@@ -3731,6 +3901,7 @@ UpdArgs chain_base_args(const qcd_lattice* L)
 {
     UpdArgs a{};
     a.table = L->table;
+    a.src = L->table;
     a.g = L->g;
     a.beta = L->p.beta / L->p.sun;
     a.nhit = L->p.nhit;

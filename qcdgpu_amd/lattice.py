@@ -321,16 +321,42 @@ class Lattice:
         )
         return {"hb_us": hb.value, "or_us": orr.value, "meas_us": meas.value, "total_ms": tot.value}
 
-    def tune(self, xcd_map=None, blocks_per_cu=None, update_shape=None):
+    def tune(self, xcd_map=None, blocks_per_cu=None, update_shape=None, shadow=None):
         """Launch-shape knobs of the update kernels (scheduling only; results
         are identical): xcd_map 0/1/2/3, blocks_per_cu cap (0 = none),
-        update_shape 3 (three wavefronts per 64 links) or 1."""
+        update_shape 3 (three wavefronts per 64 links) or 1; shadow = True
+        makes each half-step gather from a copy of the table taken before
+        it (a race check: the links must come out the same)."""
         if xcd_map is not None:
             check(lib().qcd_lattice_tune(self._h, _abi.TUNE_XCD_MAP, int(xcd_map)))
         if blocks_per_cu is not None:
             check(lib().qcd_lattice_tune(self._h, _abi.TUNE_BLOCKS_PER_CU, int(blocks_per_cu)))
         if update_shape is not None:
             check(lib().qcd_lattice_tune(self._h, _abi.TUNE_UPDATE_SHAPE, int(update_shape)))
+        if shadow is not None:
+            check(lib().qcd_lattice_tune(self._h, _abi.TUNE_SHADOW, int(bool(shadow))))
+
+    def kernel(self, name):
+        """Id of the reference kernel `name` bound to this lattice
+        (GPU::kernel_init's registry)."""
+        i = ctypes.c_int()
+        check(lib().qcd_kernel_create(self._h, name.encode(), ctypes.byref(i)))
+        return i.value
+
+    def run_kernel(self, kid, wait=True):
+        check((lib().qcd_kernel_run if wait else lib().qcd_kernel_enqueue)(self._h, int(kid)))
+
+    def kernel_info(self, kid):
+        name = ctypes.create_string_buffer(64)
+        n = ctypes.c_uint()
+        check(lib().qcd_kernel_info(self._h, int(kid), name, 64, ctypes.byref(n)))
+        return name.value.decode(), n.value
+
+    def graph_begin(self):
+        check(lib().qcd_graph_begin(self._h))
+
+    def graph_end(self):
+        check(lib().qcd_graph_end(self._h))
 
     def set_slab(self, on=True):
         """x-slab mode: x = 0 and x = N1 - 1 are halo slices that half-steps
@@ -357,7 +383,8 @@ class Lattice:
         det = (ctypes.c_int * 2)()
         check(lib().qcd_lattice_get_tune(self._h, _abi.TUNE_XCD_MAP, ctypes.byref(v), det))
         out = {"xcd_map": v.value, "xcd_block": (det[0], det[1])}
-        for name, knob in (("blocks_per_cu", _abi.TUNE_BLOCKS_PER_CU), ("update_shape", _abi.TUNE_UPDATE_SHAPE)):
+        for name, knob in (("blocks_per_cu", _abi.TUNE_BLOCKS_PER_CU), ("update_shape", _abi.TUNE_UPDATE_SHAPE),
+                           ("shadow", _abi.TUNE_SHADOW)):
             check(lib().qcd_lattice_get_tune(self._h, knob, ctypes.byref(v), None))
             out[name] = v.value
         return out
