@@ -59,7 +59,35 @@ def cmd_gen(a):
     print(json.dumps({"gen_configs": a.n, "gen_seconds": round(time.perf_counter() - t0, 2)}), flush=True)
 
 
+def cmd_train_device(a):
+    """Layer-local forward KL with the device's training kernels."""
+    from qcdgpu_amd import flow as QF
+
+    dims = tuple(a.dims)
+    data = np.load(a.data, mmap_mode="r")
+    X = np.array(data[: a.n])
+    rng = np.random.default_rng(a.seed)
+    with QF.Flow(dims=dims, layers=a.layers, bins=a.bins, max_batch=a.batch) as f:
+        f.set_params(np.zeros(f.param_count(), dtype=np.float32))
+        t0 = time.perf_counter()
+        hist = []
+        for it in range(a.steps):
+            idx = rng.choice(len(X), size=a.batch, replace=False)
+            hist.append(f.train_step(X[idx], a.lr))
+            if it % max(a.steps // 10, 1) == 0 or it == a.steps - 1:
+                print("  step %5d  layer-local -log q %.3f" % (it, hist[-1]), flush=True)
+        secs = time.perf_counter() - t0
+        np.save(a.out, f.get_params())
+    print(json.dumps({"train_configs": int(len(X)), "train_steps": a.steps, "batch": a.batch, "trainer": "device",
+                      "train_seconds": round(secs, 1), "loss_first": round(hist[0], 3),
+                      "loss_last": round(float(np.mean(hist[-10:])), 3)}), flush=True)
+
+
 def cmd_train(a):
+    if a.lr is None:
+        a.lr = 1e-4 if a.torch else 1e-3
+    if not a.torch:
+        return cmd_train_device(a)
     import torch
 
     from qcdgpu_amd import flow as QF
@@ -113,7 +141,7 @@ def run_arm(arm, a, flow):
             t0 = time.perf_counter()
             for step in range(a.steps):
                 lat.sweep("hb")
-                flow.imh_sweep(lat, seed * 7919 + step, a.beta)
+                flow.imh_sweep(lat, (seed << 32) + step, a.beta)
                 lat.measure(wilson=True)
             lat.sync()
             secs = time.perf_counter() - t0
@@ -192,7 +220,9 @@ def main(argv=None):
     t.add_argument("--batch", type=int, default=16)
     # each output of the second convolution sums 2592 inputs: Adam steps of
     # 1e-3 per weight move the spline parameters by O(1)
-    t.add_argument("--lr", type=float, default=1e-4)
+    t.add_argument("--lr", type=float, default=None, help="Adam step (default 1e-3 device, 1e-4 PyTorch)")
+    t.add_argument("--torch", action="store_true",
+                   help="train the whole-flow forward KL with the PyTorch twin instead of the device kernels")
     c = sub.add_parser("chain")
     c.add_argument("--params", default=None)
     c.add_argument("--steps", type=int, default=4000)

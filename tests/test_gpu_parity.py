@@ -321,19 +321,28 @@ def test_fused_plaquette_equals_standalone(O, sun, prec):
 
 def test_fast_stream_mode_statistics():
     """Fast stream mode (no skip to S) is a different but equally valid
-    stream: plaquette after thermalisation agrees with reference mode within
-    statistical errors, and links stay unitary."""
+    stream: the mean plaquette after thermalisation (100 sweeps of 1 HB +
+    1 OR from a cold start, then 1000 measured ones) agrees with reference
+    mode within 2 binned standard errors (20 bins), and links stay unitary.
+    SU(3) 8^3 x 4 at beta = 5.9, deconfined and away from the N_t = 4
+    transition.  (This test ran at beta = 5.7 with a 4-sigma bar whose error
+    assumed 5 independent values among 20 measurements; at 2 sigma over 400
+    sweeps there it gave z = 2.34 with errors of 0.0009 and 0.0005 for the
+    two modes: beta = 5.7 is at the N_t = 4 deconfinement transition, where
+    the plaquette moves between two phases far slower than 40-sweep bins.)"""
+    from qcdgpu_amd import analysis as A
+
     _case("fast stream")
     dims = (8, 8, 8, 4)
     res = {}
     for mode in ("reference", "fast"):
-        with _lat(3, dims, "fp32", beta=5.7, stream=mode, seed=9) as lat:
+        with _lat(3, dims, "fp32", beta=5.9, stream=mode, seed=9, max_measurements=1004) as lat:
             lat.init("cold")
             lat.prng_init()
-            for _ in range(40):
+            for _ in range(100):
                 lat.sweep("hb")
                 lat.sweep("or")
-            for _ in range(20):
+            for _ in range(1000):
                 lat.sweep("hb")
                 lat.sweep("or")
                 lat.measure(polyakov=False)
@@ -343,9 +352,12 @@ def test_fast_stream_mode_statistics():
             Uc = U[..., 0] + 1j * U[..., 1]
             dev = np.abs(np.einsum("...ji,...jk->...ik", Uc.conj(), Uc) - np.eye(3)).max()
             assert dev < 1e-5
-    a, b = res["reference"], res["fast"]
-    err = np.hypot(a.std(), b.std()) / np.sqrt(5)
-    assert abs(a.mean() - b.mean()) < 4 * err + 1e-4
+    (m1, e1), (m2, e2) = A.binned_error(res["reference"], nbins=20), A.binned_error(res["fast"], nbins=20)
+    z = (m2 - m1) / np.hypot(e1, e2)
+    print("  plaquette reference %.6f(%.6f) fast %.6f(%.6f) z %.2f, tau_int %.1f / %.1f"
+          % (m1, e1, m2, e2, z, A.tau_int(res["reference"])[0], A.tau_int(res["fast"])[0]))
+    assert len(res["fast"]) == 1000
+    assert abs(z) < 2.0
 
 
 def test_c1_size_invariants():
@@ -451,7 +463,7 @@ def test_c0_su2_ensemble_vs_oracle(O):
     plaquette every sweep.  Against an independent chain of the oracle with
     the reference CPU path's semantics (cold start, sequential RANLUX3,
     heatbath only): the mean plaquette over sweeps 50..199 agrees within
-    3 combined binned standard errors."""
+    2 combined binned standard errors."""
     _case("C0 SU(2) 8^4 beta 2.3 ensemble")
     dims, beta = (8, 8, 8, 8), 2.3
     with _lat(2, dims, "fp64", beta=beta, prng="RANLUX3", seed=54, max_measurements=256) as lat:
@@ -478,7 +490,7 @@ def test_c0_su2_ensemble_vs_oracle(O):
     z = (g_mean - o_mean) / np.hypot(g_err, o_err)
     print("  <ReTr U_p>: GPU %.5f +- %.5f, oracle %.5f +- %.5f, z = %.2f (per colour: %.5f)"
           % (g_mean, g_err, o_mean, o_err, z, g_mean / 2))
-    assert abs(z) < 3.0
+    assert abs(z) < 2.0
     # Polyakov loop of a confined SU(2) ensemble at this beta averages to ~0
     assert abs(np.mean(m["polyakov_re"][50:])) < 0.1
 
