@@ -197,6 +197,7 @@ def c3(args):
     with Q.Batch(chains, precision="fp32", prng="RANLUX", max_measurements=args.steps + args.warmup + 4) as b:
         b.init("hot")
         tuning = b.autotune(repeats=4)
+        b.set_merge(not args.c3_per_bucket)
         b.capture_cycle(n_hb=1, n_or=N_OR)
         b.launch_cycle(args.warmup)
         b.sync()
@@ -219,7 +220,9 @@ def c3(args):
         "data": "synthetic",
         "config": {"workload": "96 SU(3) chains N_s in {12,16,20,24}, N_t in {4,6,8}, beta 5.5-6.2, RANLUX3; "
                                "1 CM heatbath + 4 overrelaxation sweeps + plaquette + Polyakov per step",
-                   "sites": V, "tuning": tuning},
+                   "sites": V, "tuning": tuning,
+                   "launches": "one per size bucket and half-step" if args.c3_per_bucket
+                               else "one per half-step and kernel shape for all chains"},
         "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
                      "algorithmic_bytes_per_step": step_bytes},
     }), flush=True)
@@ -240,6 +243,8 @@ def main():
     ap.add_argument("--update-shape", type=int, default=None, choices=(1, 3),
                     help="wavefronts per 64-link unit of the update kernels (default: the library's, 3)")
     ap.add_argument("--profile-steps", type=int, default=50)
+    ap.add_argument("--c3-per-bucket", action="store_true",
+                    help="with --c3: one launch per size bucket and half-step instead of one for all chains")
     ap.add_argument(
         "--copy-calibration",
         action="store_true",

@@ -182,6 +182,28 @@ int  qcd_kernel_info(const qcd_lattice *lat, int id, char *name, int name_len, u
    half-steps must be even in number.                                       */
 int  qcd_graph_begin(qcd_lattice *lat);
 int  qcd_graph_end(qcd_lattice *lat);
+/* ---- device buffers by id (GPU::buffer_init, buffer_write, buffer_map,
+        buffer_kill, buffer_size_align, clinterface.cpp:1296-1440,1558-1590),
+        for a host that keeps its own buffers in the reference's registry
+        style.  Types as GPU_buffer_types (clinterface.h:73-82): INPUT, IO and
+        CONSTANT copy `count * elem_size` bytes from host_ptr at creation,
+        GLOBAL, OUTPUT and UAV start zeroed, LDS records a size and holds no
+        memory.  _write uploads the whole host copy again (blocking);
+        _map downloads the buffer into a host mirror the library owns until
+        _kill and returns it (blocking, as buffer_map with CL_TRUE); _unmap
+        writes the mirror back.  Ids count from 1.  _size_align rounds `size`
+        up to a multiple of `step` (0: the device's largest workgroup,
+        1024).  The lattice's own data in the reference's layouts goes
+        through qcd_lattice_export / qcd_read_measurements / qcd_qcg_*.      */
+enum { QCD_BUFFER_GLOBAL = 1, QCD_BUFFER_INPUT = 2, QCD_BUFFER_IO = 3, QCD_BUFFER_OUTPUT = 4,
+       QCD_BUFFER_CONSTANT = 5, QCD_BUFFER_LDS = 6, QCD_BUFFER_UAV = 7 };
+int  qcd_buffer_create(int type, size_t count, size_t elem_size, void *host_ptr, int *id);
+int  qcd_buffer_write(int id);
+int  qcd_buffer_map(int id, void **host);
+int  qcd_buffer_unmap(int id);
+int  qcd_buffer_info(int id, size_t *bytes, void **device_ptr);
+int  qcd_buffer_kill(int id);
+unsigned qcd_buffer_size_align(unsigned size, unsigned step);
 /* ---- kernel timing (GPU::print_time_detailed, clinterface.cpp:1626-1666,
         shown when GPU_debug.profiling is set, suncl.cpp:265).  With it on,
         every kernel family the lattice launches eagerly (update half-steps,
@@ -487,6 +509,12 @@ int  qcd_batch_autotune(qcd_batch *batch, int repeats, int *shapes_out);
 int  qcd_batch_get_tuning(qcd_batch *batch, int bucket, int *out);
 /* Sets bucket `bucket`'s choices by hand (shapes 1 or 3, placements 0..2). */
 int  qcd_batch_set_tuning(qcd_batch *batch, int bucket, int shape_hb, int shape_or, int place_hb, int place_or);
+/* Launch grouping of the batched half-steps (new): 1 (default) updates every
+   chain with one launch per (parity, mu) and kernel shape, over the
+   buckets' chosen work maps concatenated; 0 launches once per bucket.  The
+   chains' results are the same either way.  A captured cycle keeps the
+   grouping it was captured with. */
+int  qcd_batch_set_merge(qcd_batch *batch, int on);
 /* Chain `index` as a lattice handle (export, records, measure); the batch's
    work is complete when this returns.  The batch owns it: destroying it,
    re-sizing its streams (a reference hot start) or resuming a hot run on it

@@ -21,6 +21,7 @@ HEATBATH, OVERRELAX = 0, 1
 MEAS_PLAQUETTE, MEAS_POLYAKOV, MEAS_WILSON, MEAS_POLYAKOV_SLICES = 1, 2, 4, 8
 MEAS_FMUNU, MEAS_F0MU, MEAS_ACTION_SLICES = 16, 32, 64
 TUNE_XCD_MAP, TUNE_BLOCKS_PER_CU, TUNE_UPDATE_SHAPE, TUNE_SHADOW = 1, 2, 3, 4
+BUFFER_GLOBAL, BUFFER_INPUT, BUFFER_IO, BUFFER_OUTPUT, BUFFER_CONSTANT, BUFFER_LDS, BUFFER_UAV = 1, 2, 3, 4, 5, 6, 7
 
 
 class Params(ctypes.Structure):
@@ -91,6 +92,13 @@ _SIGNATURES = {
     "qcd_kernel_info": [_P, ctypes.c_int, ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_uint)],
     "qcd_graph_begin": [_P],
     "qcd_graph_end": [_P],
+    "qcd_buffer_create": [ctypes.c_int, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)],
+    "qcd_buffer_write": [ctypes.c_int],
+    "qcd_buffer_map": [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)],
+    "qcd_buffer_unmap": [ctypes.c_int],
+    "qcd_buffer_info": [ctypes.c_int, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_void_p)],
+    "qcd_buffer_kill": [ctypes.c_int],
+    "qcd_buffer_size_align": [ctypes.c_uint, ctypes.c_uint],
     "qcd_lattice_profile": [_P, ctypes.c_int],
     "qcd_profile_count": [_P, ctypes.POINTER(ctypes.c_int)],
     "qcd_profile_entry": [_P, ctypes.c_int, ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_uint), _D, _D],
@@ -135,6 +143,7 @@ _SIGNATURES = {
     "qcd_batch_autotune": [_P, ctypes.c_int, _I],
     "qcd_batch_get_tuning": [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_int)],
     "qcd_batch_set_tuning": [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int],
+    "qcd_batch_set_merge": [_P, ctypes.c_int],
     "qcd_batch_chain": [_P, ctypes.c_int, ctypes.POINTER(_P)],
     "qcd_lattice_get_params": [_P, ctypes.POINTER(Params)],
     "qcd_prng_counter": [_P, ctypes.POINTER(ctypes.c_uint)],
@@ -191,6 +200,7 @@ def lib():
         fn.restype = ctypes.c_int
     L.qcd_last_error.restype = ctypes.c_char_p
     L.qcd_last_error.argtypes = []
+    L.qcd_buffer_size_align.restype = ctypes.c_uint
     _lib = L
     return L
 

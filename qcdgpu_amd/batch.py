@@ -112,6 +112,11 @@ class Batch:
         check(lib().qcd_batch_set_tuning(self._h, int(bucket), int(shape[0]), int(shape[1]), int(placement[0]),
                                          int(placement[1])))
 
+    def set_merge(self, on=True):
+        """One launch per (parity, mu) and kernel shape for all buckets (on,
+        the default) or one per bucket; the chains' results are the same."""
+        check(lib().qcd_batch_set_merge(self._h, 1 if on else 0))
+
     def tuning(self):
         """Per bucket: {"n_s", "chains", "shape": (heatbath, overrelaxation),
         "placement": (heatbath, overrelaxation)}."""

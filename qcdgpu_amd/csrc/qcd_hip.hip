@@ -2619,6 +2619,111 @@ int qcd_kernel_info(const qcd_lattice* L, int id, char* name, int name_len, unsi
     return QCD_OK;
 }
 
+// Device buffers by integer id, as GPU::buffer_init / buffer_write /
+// buffer_map / buffer_kill keep them (clinterface.cpp:1296-1440,1558-1568):
+// a host written against that registry keeps its auxiliary buffers here.
+// Types and their numbers follow GPU_buffer_types (clinterface.h:73-82).
+namespace {
+struct DevBuffer {
+    int type = 0;
+    size_t bytes = 0;
+    void* dev = nullptr;
+    void* host = nullptr;               // the caller's host copy (buffer_write uploads it)
+    std::vector<unsigned char> mirror;  // what qcd_buffer_map returns
+    bool live = false;
+};
+std::vector<DevBuffer>& buffers()
+{
+    static std::vector<DevBuffer> b(1);   // ids count from 1
+    return b;
+}
+DevBuffer* buffer_of(int id)
+{
+    auto& b = buffers();
+    if (id <= 0 || id >= (int)b.size() || !b[id].live) return nullptr;
+    return &b[id];
+}
+}  // namespace
+
+int qcd_buffer_create(int type, size_t count, size_t elem_size, void* host_ptr, int* id)
+{
+    if (!id) return fail(QCD_E_ARG, "null id");
+    if (type < QCD_BUFFER_GLOBAL || type > QCD_BUFFER_UAV) return fail(QCD_E_ARG, "buffer type %d", type);
+    if (count && elem_size > SIZE_MAX / count) return fail(QCD_E_ARG, "buffer size overflows");
+    const bool copy = type == QCD_BUFFER_INPUT || type == QCD_BUFFER_IO || type == QCD_BUFFER_CONSTANT;
+    if (copy && !host_ptr && count) return fail(QCD_E_ARG, "buffer type %d copies the host data: null host pointer", type);
+    DevBuffer d;
+    d.type = type;
+    d.bytes = count * elem_size;
+    d.host = host_ptr;
+    d.live = true;
+    if (type != QCD_BUFFER_LDS && d.bytes) {   // LDS: a size record only (a __local argument)
+        if (hipMalloc(&d.dev, d.bytes) != hipSuccess) return fail(QCD_E_NOMEM, "buffer of %zu bytes", d.bytes);
+        const hipError_t e = copy ? hipMemcpy(d.dev, host_ptr, d.bytes, hipMemcpyHostToDevice) : hipMemset(d.dev, 0, d.bytes);
+        if (e != hipSuccess) {
+            (void)hipFree(d.dev);
+            return fail(QCD_E_HIP, "buffer set-up: %s", hipGetErrorString(e));
+        }
+    }
+    buffers().push_back(std::move(d));
+    *id = (int)buffers().size() - 1;
+    return QCD_OK;
+}
+
+int qcd_buffer_write(int id)
+{
+    DevBuffer* d = buffer_of(id);
+    if (!d) return fail(QCD_E_ARG, "no buffer %d", id);
+    if (!d->dev) return d->bytes ? fail(QCD_E_STATE, "buffer %d has no device memory (LDS)", id) : QCD_OK;
+    if (!d->host) return fail(QCD_E_STATE, "buffer %d was created without a host copy", id);
+    HIPCHK(hipMemcpy(d->dev, d->host, d->bytes, hipMemcpyHostToDevice));
+    return QCD_OK;
+}
+
+int qcd_buffer_map(int id, void** host)
+{
+    DevBuffer* d = buffer_of(id);
+    if (!d || !host) return fail(QCD_E_ARG, "no buffer %d", id);
+    if (!d->dev && d->bytes) return fail(QCD_E_STATE, "buffer %d has no device memory (LDS)", id);
+    d->mirror.resize(d->bytes);
+    if (d->bytes) HIPCHK(hipMemcpy(d->mirror.data(), d->dev, d->bytes, hipMemcpyDeviceToHost));
+    *host = d->mirror.data();
+    return QCD_OK;
+}
+
+int qcd_buffer_unmap(int id)
+{
+    DevBuffer* d = buffer_of(id);
+    if (!d) return fail(QCD_E_ARG, "no buffer %d", id);
+    if (d->mirror.size() != d->bytes) return fail(QCD_E_STATE, "buffer %d is not mapped", id);
+    if (d->bytes) HIPCHK(hipMemcpy(d->dev, d->mirror.data(), d->bytes, hipMemcpyHostToDevice));
+    return QCD_OK;
+}
+
+int qcd_buffer_info(int id, size_t* bytes, void** device_ptr)
+{
+    DevBuffer* d = buffer_of(id);
+    if (!d) return fail(QCD_E_ARG, "no buffer %d", id);
+    if (bytes) *bytes = d->bytes;
+    if (device_ptr) *device_ptr = d->dev;
+    return QCD_OK;
+}
+
+int qcd_buffer_kill(int id)
+{
+    DevBuffer* d = buffer_of(id);
+    if (!d) return fail(QCD_E_ARG, "no buffer %d", id);
+    if (d->dev) HIPCHK(hipFree(d->dev));
+    *d = DevBuffer();
+    return QCD_OK;
+}
+
+unsigned qcd_buffer_size_align(unsigned size, unsigned step)
+{
+    if (step == 0) step = 1024;   // the device's largest workgroup
+    return size ? (1u + (size - 1u) / step) * step : 0u;
+}
+
 int qcd_graph_begin(qcd_lattice* L)
 {
     if (!L) return fail(QCD_E_ARG, "null lattice");
@@ -3888,6 +3993,16 @@ struct qcd_batch {
         int place[2] = {0, 0};
     };
     std::vector<Bucket> buckets;
+    std::vector<uint2> h_map;              // host copy of d_map
+    // Merged maps (merge on): per update kind [heatbath, overrelaxation] and
+    // kernel shape [3, 1], the chosen maps of every bucket with that shape,
+    // concatenated, so one launch per (parity, mu) and shape updates all
+    // chains.  Every bucket map holds a multiple of 8 workgroups, so a
+    // workgroup keeps its XCD (b mod 8) in the concatenation.
+    bool merge = true;
+    uint2* d_mmap = nullptr;
+    std::vector<uint2*> mmap_old;          // earlier merged maps: a captured cycle may still launch them
+    uint32_t moff[2][2] = {}, mlen[2][2] = {};
     uint32_t off_units = 0, len_units = 0; // every (chain, unit): RANLUX produce
     uint32_t off_poly = 0, len_poly = 0;   // every (chain, Polyakov workgroup)
     int rng_cur = 0;
@@ -4015,13 +4130,44 @@ void build_maps(qcd_batch* B, std::vector<uint2>& map)
     B->len_poly = (uint32_t)map.size() - B->off_poly;
 }
 
-template <typename R, int N, int NREAL, bool OVER, int RK>
-void launch_batch_update(qcd_batch* B, const qcd_batch::Bucket& bk, int parity, int mu, int meas_mode, const LaunchOpt& o)
+// Rebuilds the merged maps from the buckets' current shapes and placements
+// (host calls only: never inside a capture).
+int build_merged(qcd_batch* B)
 {
-    const int shape = bk.shape[OVER ? 1 : 0], pl = bk.place[OVER ? 1 : 0];
+    std::vector<uint2> m;
+    for (int kind = 0; kind < 2; ++kind)
+        for (int si = 0; si < 2; ++si) {
+            const int shape = si == 0 ? 3 : 1;
+            B->moff[kind][si] = (uint32_t)m.size();
+            for (const auto& bk : B->buckets) {
+                if (bk.shape[kind] != shape) continue;
+                const int pl = bk.place[kind];
+                const uint32_t off = shape == 3 ? bk.off3[pl] : bk.off1[pl], len = shape == 3 ? bk.len3[pl] : bk.len1[pl];
+                m.insert(m.end(), B->h_map.begin() + off, B->h_map.begin() + off + len);
+            }
+            B->mlen[kind][si] = (uint32_t)m.size() - B->moff[kind][si];
+        }
+    if (B->d_mmap) {   // kept until the batch is destroyed: a captured cycle may hold it
+        B->mmap_old.push_back(B->d_mmap);
+        B->d_mmap = nullptr;
+    }
+    if (m.empty()) return QCD_OK;
+    if (hipMalloc((void**)&B->d_mmap, sizeof(uint2) * m.size()) != hipSuccess) {
+        B->d_mmap = nullptr;
+        return fail(QCD_E_NOMEM, "merged batch maps");
+    }
+    HIPCHK(hipMemcpy(B->d_mmap, m.data(), sizeof(uint2) * m.size(), hipMemcpyHostToDevice));
+    return QCD_OK;
+}
+
+// One launch over `len` map entries of the given shape (entries per
+// workgroup: 1 for shape 3, kWaves1 for shape 1).
+template <typename R, int N, int NREAL, bool OVER, int RK>
+void launch_batch_map(qcd_batch* B, int shape, const uint2* map, uint32_t len, int parity, int mu, int meas_mode,
+                      const LaunchOpt& o)
+{
     if (shape == 1) {
-        const dim3 grid(bk.len1[pl] / kWaves1), block(kWaves1 * kLinks);
-        const uint2* map = B->d_map + bk.off1[pl];
+        const dim3 grid(len / kWaves1), block(kWaves1 * kLinks);
         if (meas_mode)
             launch(k_update_batch1<R, N, NREAL, OVER, RK, true>, grid, block, B->stream, o, (const ChainDesc*)B->d_desc,
                    map, parity, mu, meas_mode, B->rng_cur);
@@ -4029,8 +4175,7 @@ void launch_batch_update(qcd_batch* B, const qcd_batch::Bucket& bk, int parity, 
             launch(k_update_batch1<R, N, NREAL, OVER, RK, false>, grid, block, B->stream, o, (const ChainDesc*)B->d_desc,
                    map, parity, mu, meas_mode, B->rng_cur);
     } else {
-        const dim3 grid(bk.len3[pl]), block(kUpdThreads);
-        const uint2* map = B->d_map + bk.off3[pl];
+        const dim3 grid(len), block(kUpdThreads);
         if (meas_mode)
             launch(k_update_batch3<R, N, NREAL, OVER, RK, true>, grid, block, B->stream, o, (const ChainDesc*)B->d_desc,
                    map, parity, mu, meas_mode, B->rng_cur);
@@ -4038,6 +4183,26 @@ void launch_batch_update(qcd_batch* B, const qcd_batch::Bucket& bk, int parity, 
             launch(k_update_batch3<R, N, NREAL, OVER, RK, false>, grid, block, B->stream, o, (const ChainDesc*)B->d_desc,
                    map, parity, mu, meas_mode, B->rng_cur);
     }
+}
+
+template <typename R, int N, int NREAL, bool OVER, int RK>
+void launch_batch_merged(qcd_batch* B, int parity, int mu, int meas_mode, const LaunchOpt& o)
+{
+    const int kind = OVER ? 1 : 0;
+    for (int si = 0; si < 2; ++si)
+        if (B->mlen[kind][si])
+            launch_batch_map<R, N, NREAL, OVER, RK>(B, si == 0 ? 3 : 1, B->d_mmap + B->moff[kind][si],
+                                                    B->mlen[kind][si], parity, mu, meas_mode, o);
+}
+
+template <typename R, int N, int NREAL, bool OVER, int RK>
+void launch_batch_update(qcd_batch* B, const qcd_batch::Bucket& bk, int parity, int mu, int meas_mode, const LaunchOpt& o)
+{
+    const int shape = bk.shape[OVER ? 1 : 0], pl = bk.place[OVER ? 1 : 0];
+    if (shape == 1)
+        launch_batch_map<R, N, NREAL, OVER, RK>(B, 1, B->d_map + bk.off1[pl], bk.len1[pl], parity, mu, meas_mode, o);
+    else
+        launch_batch_map<R, N, NREAL, OVER, RK>(B, 3, B->d_map + bk.off3[pl], bk.len3[pl], parity, mu, meas_mode, o);
 }
 
 // One (parity, mu) half-step of every chain; `only` >= 0 restricts it to one
@@ -4058,13 +4223,20 @@ int batch_halfstep_rn(qcd_batch* B, int parity, int mu, int kind, int meas_mode,
             default: hipLaunchKernelGGL(k_ranlux_produce_batch<4>, grid, block, 0, B->stream, B->d_desc, map, S); break;
         }
     }
-    for (size_t k = 0; k < B->buckets.size(); ++k) {
-        if (only >= 0 && (int)k != only) continue;
-        const auto& bk = B->buckets[k];
-        if (kind == QCD_OVERRELAX) launch_batch_update<R, N, NREAL, true, RK_NONE>(B, bk, parity, mu, meas_mode, o);
-        else if (p.prng == QCD_PRNG_RANLUX) launch_batch_update<R, N, NREAL, false, RK_BUF>(B, bk, parity, mu, meas_mode, o);
-        else if (p.prng == QCD_PRNG_XOR128) launch_batch_update<R, N, NREAL, false, RK_XOR128>(B, bk, parity, mu, meas_mode, o);
-        else launch_batch_update<R, N, NREAL, false, RK_PM>(B, bk, parity, mu, meas_mode, o);
+    if (only < 0 && B->merge) {
+        if (kind == QCD_OVERRELAX) launch_batch_merged<R, N, NREAL, true, RK_NONE>(B, parity, mu, meas_mode, o);
+        else if (p.prng == QCD_PRNG_RANLUX) launch_batch_merged<R, N, NREAL, false, RK_BUF>(B, parity, mu, meas_mode, o);
+        else if (p.prng == QCD_PRNG_XOR128) launch_batch_merged<R, N, NREAL, false, RK_XOR128>(B, parity, mu, meas_mode, o);
+        else launch_batch_merged<R, N, NREAL, false, RK_PM>(B, parity, mu, meas_mode, o);
+    } else {
+        for (size_t k = 0; k < B->buckets.size(); ++k) {
+            if (only >= 0 && (int)k != only) continue;
+            const auto& bk = B->buckets[k];
+            if (kind == QCD_OVERRELAX) launch_batch_update<R, N, NREAL, true, RK_NONE>(B, bk, parity, mu, meas_mode, o);
+            else if (p.prng == QCD_PRNG_RANLUX) launch_batch_update<R, N, NREAL, false, RK_BUF>(B, bk, parity, mu, meas_mode, o);
+            else if (p.prng == QCD_PRNG_XOR128) launch_batch_update<R, N, NREAL, false, RK_XOR128>(B, bk, parity, mu, meas_mode, o);
+            else launch_batch_update<R, N, NREAL, false, RK_PM>(B, bk, parity, mu, meas_mode, o);
+        }
     }
     HIPCHK(hipGetLastError());
     if (kind == QCD_HEATBATH && p.prng != QCD_PRNG_RANLUX) {
@@ -4199,6 +4371,11 @@ int qcd_batch_create(const qcd_params* common, int nchains, const qcd_chain_desc
         qcd_batch_destroy(B);
         return fail(QCD_E_HIP, "batch set-up: %s", hipGetErrorString(e));
     }
+    B->h_map = std::move(map);
+    if (int rc = build_merged(B)) {
+        qcd_batch_destroy(B);
+        return rc;
+    }
     *out = B;
     return QCD_OK;
 }
@@ -4215,6 +4392,8 @@ int qcd_batch_destroy(qcd_batch* B)
     }
     (void)hipFree(B->d_desc);
     (void)hipFree(B->d_map);
+    (void)hipFree(B->d_mmap);
+    for (uint2* m : B->mmap_old) (void)hipFree(m);
     if (B->stream) (void)hipStreamDestroy(B->stream);
     delete B;
     return QCD_OK;
@@ -4403,7 +4582,7 @@ int qcd_batch_autotune(qcd_batch* B, int repeats, int* shapes_out)
             shapes_out[2 * k + 1] = B->buckets[k].shape[1];
         }
     // a captured cycle keeps the shapes it was captured with
-    return QCD_OK;
+    return build_merged(B);
 }
 
 int qcd_batch_set_tuning(qcd_batch* B, int bucket, int shape_hb, int shape_or, int place_hb, int place_or)
@@ -4419,7 +4598,14 @@ int qcd_batch_set_tuning(qcd_batch* B, int bucket, int shape_hb, int shape_or, i
     bk.shape[1] = shape_or;
     bk.place[0] = place_hb;
     bk.place[1] = place_or;
-    return QCD_OK;   // a captured cycle keeps the choices it was captured with
+    return build_merged(B);   // a captured cycle keeps the choices it was captured with
+}
+
+int qcd_batch_set_merge(qcd_batch* B, int on)
+{
+    if (!B) return fail(QCD_E_ARG, "null batch");
+    B->merge = on != 0;
+    return QCD_OK;   // a captured cycle keeps the launches it was captured with
 }
 
 int qcd_batch_get_tuning(qcd_batch* B, int bucket, int* out)

@@ -22,10 +22,13 @@ def _alone(dims, beta, seed, prng, prec, cycles, n_or):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("prng,prec", [("RANLUX", "fp32"), ("XOR128", "fp64")])
-@pytest.mark.parametrize("tune", ["default", "autotune", "s1p1", "s3p2", "s1p2"])
+@pytest.mark.parametrize("tune", ["default", "autotune", "s1p1", "s3p2", "s1p2", "mixed", "per-bucket"])
 def test_batch_equals_chains_alone(prng, prec, tune):
     """Every kernel shape and unit placement (by hand, or as the autotuner
-    picks them) gives each chain exactly the chain run alone."""
+    picks them) gives each chain exactly the chain run alone, with the
+    buckets' half-steps merged into one launch per shape (the default; in
+    "mixed" the two buckets use different shapes, so each half-step is two
+    launches) or launched per bucket ("per-bucket")."""
     print("[case] batch %s %s tune=%s" % (prng, prec, tune), flush=True)
     cycles, n_or = 2, 2
     with Q.Batch(CHAINS, precision=prec, prng=prng, max_measurements=16) as b:
@@ -35,6 +38,11 @@ def test_batch_equals_chains_alone(prng, prec, tune):
             t = b.autotune(repeats=2)
             assert all(s in (1, 3) for d in t for s in d["shape"])
             assert all(p in (0, 1, 2) for d in t for p in d["placement"])
+        elif tune == "mixed":
+            b.set_tuning(0, shape=(3, 1), placement=(1, 2))
+            b.set_tuning(1, shape=(1, 3), placement=(2, 0))
+        elif tune == "per-bucket":
+            b.set_merge(False)
         elif tune != "default":
             shape, place = int(tune[1]), int(tune[3])
             for k in range(2):
