@@ -748,36 +748,43 @@ __device__ __forceinline__ float4 pm_quad(uint4& s)
 constexpr float kRlTwom24 = 5.9604644775390625e-08f;
 constexpr float kRlTwom12 = 2.44140625e-04f;
 
+// The ring holds the 24-bit integers k of the values k 2^-24 (every RANLUX
+// number is one, the seeds included), so a subtract-with-borrow step is one
+// 32-bit subtraction with borrow in and out and a 24-bit mask: for
+// s[J], s[I] < 2^24 and carry c in {0, 1}, s[J] - s[I] - c < 0 exactly when
+// the 32-bit subtraction borrows, and then the masked result is
+// s[J] - s[I] - c + 2^24 -- the float form's "uni += 1, carry = 2^-24"
+// step for step, two VALU instructions (v_subb_co_u32 with the borrow in
+// VCC, v_and_b32) against six for the float compare-and-select form.
+// Outputs are converted to float (exact) and get the float form's
+// small-value fix with the same float arithmetic.
 template <int I, int J>
-__host__ __device__ __forceinline__ float rl_swb(float (&s)[24], float& carry)
+__host__ __device__ __forceinline__ uint32_t rl_swb(uint32_t (&s)[24], uint32_t& carry)
 {
-    float uni = s[J] - s[I] - carry;
-    if (uni < 0.0f) {
-        uni += 1.0f;
-        carry = kRlTwom24;
-    } else {
-        carry = 0.0f;
-    }
-    s[I] = uni;
-    return uni;
+    uint32_t borrow;
+    const uint32_t d = __builtin_subc(s[J], s[I], carry, &borrow) & 0xFFFFFFu;
+    carry = borrow;
+    s[I] = d;
+    return d;
 }
 
 // One output: SWB at (I, J) then the small-value fix with the value at the
 // decremented j pointer (prngcl_ranlux.cl:178-179).
 template <int I>
-__host__ __device__ __forceinline__ float rl_out(float (&s)[24], float& carry)
+__host__ __device__ __forceinline__ float rl_out(uint32_t (&s)[24], uint32_t& carry)
 {
     constexpr int J = (I + 10) % 24;
-    float uni = rl_swb<I, J>(s, carry);
+    float uni = (float)rl_swb<I, J>(s, carry) * kRlTwom24;
     if (uni < kRlTwom12) {
-        uni = uni + kRlTwom24 * s[(J + 23) % 24];
+        const float lo = (float)s[(J + 23) % 24] * kRlTwom24;
+        uni = uni + kRlTwom24 * lo;
         if (uni == 0.0f) uni = kRlTwom24 * kRlTwom24;
     }
     return uni;
 }
 
 template <int K>
-__host__ __device__ __forceinline__ float4 rl_quad(float (&s)[24], float& carry)
+__host__ __device__ __forceinline__ float4 rl_quad(uint32_t (&s)[24], uint32_t& carry)
 {
     float4 q;
     q.x = rl_out<23 - 4 * K>(s, carry);
@@ -788,7 +795,7 @@ __host__ __device__ __forceinline__ float4 rl_quad(float (&s)[24], float& carry)
 }
 
 template <int T>
-__host__ __device__ __forceinline__ void rl_steps(float (&s)[24], float& carry)
+__host__ __device__ __forceinline__ void rl_steps(uint32_t (&s)[24], uint32_t& carry)
 {
     if constexpr (T > 0) {
         constexpr int I = (23 - (T - 1) + 24) % 24;   // step T-1 of a cycle
@@ -799,12 +806,12 @@ __host__ __device__ __forceinline__ void rl_steps(float (&s)[24], float& carry)
 
 // Skip P = 24 * FULL + REM numbers from the start of a block, then rename.
 template <int FULL, int REM>
-__host__ __device__ __forceinline__ void rl_skip(float (&s)[24], float& carry)
+__host__ __device__ __forceinline__ void rl_skip(uint32_t (&s)[24], uint32_t& carry)
 {
     for (int f = 0; f < FULL; ++f) rl_steps<24>(s, carry);
     rl_steps<REM>(s, carry);
     if constexpr (REM != 0) {
-        float t[24];
+        uint32_t t[24];
 #pragma unroll
         for (int i = 0; i < 24; ++i) t[(i + REM) % 24] = s[i];
 #pragma unroll

@@ -449,7 +449,7 @@ constexpr int kUpdThreads = 3 * kLinks;
 // written to rng_out.
 template <typename R, int N, int NREAL, bool OVER, int RK, bool MEAS, bool TW = false>
 __device__ __forceinline__ double update_link(const UpdArgs& a, const Table<R, N, NREAL>& Tb, uint32_t i, const Site& s,
-                                              const M<R, N>& A, const M<R, N>& Alow, uint4 rs, M<R, N> U)
+                                              const M<R, N>& A, const M<R, N>& Alow, uint4 rs, M<R, N>& U)
 {
     using T = R;
     const int p = a.parity, mu = a.mu;
@@ -566,7 +566,9 @@ __device__ __forceinline__ void update3(const UpdArgs& a, uint32_t blk, uint32_t
 {
     using T = R;   // compute type of the SU(2) subgroup arithmetic
     constexpr int NC = 2 * N * N;   // reals per matrix
-    __shared__ R lds[2][NC][kLinks];
+    // shares of wavefronts 1 and 2; with MEAS also wavefront 0's own, so
+    // that no staple share stays in registers through the link update
+    __shared__ R lds[MEAS ? 3 : 2][NC][kLinks];
     // Wavefront index, made visibly uniform so that nu and every plane base
     // derived from it live in scalar registers.
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
@@ -650,20 +652,42 @@ __device__ __forceinline__ void update3(const UpdArgs& a, uint32_t blk, uint32_t
             S1.e[r][c] = {lds[0][2 * (r * N + c)][lane], lds[0][2 * (r * N + c) + 1][lane]};
             S2.e[r][c] = {lds[1][2 * (r * N + c)][lane], lds[1][2 * (r * N + c) + 1][lane]};
         }
-    // Directions below mu are shares 0 .. mu-1 (nu = w for w < mu).
-    M<R, N> Alow;
     if constexpr (MEAS) {
-        m_zero(Alow);
-        if (mu > 0) m_acc(Alow, S);
-        if (mu > 1) m_acc(Alow, S1);
-        if (mu > 2) m_acc(Alow, S2);
+#pragma unroll
+        for (int r = 0; r < N; ++r)
+#pragma unroll
+            for (int c = 0; c < N; ++c) {
+                lds[2][2 * (r * N + c)][lane] = S.e[r][c].re;
+                lds[2][2 * (r * N + c) + 1][lane] = S.e[r][c].im;
+            }
     }
     M<R, N> A = S;
     m_acc(A, S1);
     m_acc(A, S2);
 
     double trace = 0.0;
-    if (active) trace = update_link<R, N, NREAL, OVER, RK, MEAS, TW>(a, Tb, i, s, A, Alow, rs, U);
+    if (active) {
+        M<R, N> Alow;   // unused by the update itself
+        (void)update_link<R, N, NREAL, OVER, RK, false, TW>(a, Tb, i, s, A, Alow, rs, U);
+        if constexpr (MEAS) {
+            // ReTr(U' sum_{nu<mu} S_nu): directions below mu are shares
+            // 0 .. mu-1 (nu = w for w < mu), summed in that order; the
+            // shares come back from LDS (this lane's own slots)
+            auto share = [&](int k) {
+                M<R, N> X;
+#pragma unroll
+                for (int r = 0; r < N; ++r)
+#pragma unroll
+                    for (int c = 0; c < N; ++c) X.e[r][c] = {lds[k][2 * (r * N + c)][lane], lds[k][2 * (r * N + c) + 1][lane]};
+                return X;
+            };
+            m_zero(Alow);
+            if (mu > 0) m_acc(Alow, share(2));
+            if (mu > 1) m_acc(Alow, share(0));
+            if (mu > 2) m_acc(Alow, share(1));
+            trace = m_retr_mul(U, Alow);
+        }
+    }
     if constexpr (MEAS) {
         trace = wave_sum(trace);
         // slot = 64-link unit, so the final sum runs in lattice order
@@ -724,7 +748,10 @@ __device__ __forceinline__ void update1(const UpdArgs& a, uint32_t unit, uint32_
         }
     }
     double trace = 0.0;
-    if (active) trace = update_link<R, N, NREAL, OVER, RK, MEAS>(a, Tb, i, s, A, Alow, rs, Tb.load_u(p, mu, i));
+    if (active) {
+        M<R, N> U = Tb.load_u(p, mu, i);
+        trace = update_link<R, N, NREAL, OVER, RK, MEAS>(a, Tb, i, s, A, Alow, rs, U);
+    }
     if constexpr (kStream) {
         // Reference stream layout: every stream advances by exactly s_quads
         // quads from its start state.
@@ -823,18 +850,19 @@ __global__ __launch_bounds__(kBlock) void k_pm_produce(uint4* st, uint32_t n, ui
 template <int LUX>
 __device__ __forceinline__ void ranlux_body(float4* state, uint32_t n, uint32_t samples, float4* quads, uint32_t g)
 {
-    float s[24];
+    // the ring as 24-bit integers (the stored floats are k 2^-24 exactly)
+    uint32_t s[24];
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
         const float4 v = state[(size_t)q * n + g];
-        s[4 * q] = v.x;
-        s[4 * q + 1] = v.y;
-        s[4 * q + 2] = v.z;
-        s[4 * q + 3] = v.w;
+        s[4 * q] = (uint32_t)(v.x * 16777216.0f);
+        s[4 * q + 1] = (uint32_t)(v.y * 16777216.0f);
+        s[4 * q + 2] = (uint32_t)(v.z * 16777216.0f);
+        s[4 * q + 3] = (uint32_t)(v.w * 16777216.0f);
     }
     const float4 c6 = state[(size_t)6 * n + g];
     int k = __builtin_amdgcn_readfirstlane((int)c6.z);
-    float carry = c6.w;
+    uint32_t carry = c6.w > 0.0f ? 1u : 0u;
     for (uint32_t m = 0; m < samples; ++m) {
         if (k >= 6) {
             rl_skip<RlSkip<LUX>::full, RlSkip<LUX>::rem>(s, carry);
@@ -853,8 +881,10 @@ __device__ __forceinline__ void ranlux_body(float4* state, uint32_t n, uint32_t 
         ++k;
     }
 #pragma unroll
-    for (int q = 0; q < 6; ++q) state[(size_t)q * n + g] = make_float4(s[4 * q], s[4 * q + 1], s[4 * q + 2], s[4 * q + 3]);
-    state[(size_t)6 * n + g] = make_float4(23.0f, 9.0f, (float)k, carry);
+    for (int q = 0; q < 6; ++q)
+        state[(size_t)q * n + g] = make_float4((float)s[4 * q] * kRlTwom24, (float)s[4 * q + 1] * kRlTwom24,
+                                               (float)s[4 * q + 2] * kRlTwom24, (float)s[4 * q + 3] * kRlTwom24);
+    state[(size_t)6 * n + g] = make_float4(23.0f, 9.0f, (float)k, carry ? kRlTwom24 : 0.0f);
 }
 
 template <int LUX>
@@ -1168,30 +1198,62 @@ __device__ __forceinline__ double block_sum_1(double v)
     return x;
 }
 
+// block_sum_1 of K values at once: the same wavefront sums and the same
+// order over wavefronts, one pair of barriers for all of them.
+template <int K>
+__device__ __forceinline__ void block_sum_k(double (&v)[K])
+{
+    __shared__ double sm[K][kBlock / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] = wave_sum(v[k]);
+    __syncthreads();
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < K; ++k) sm[k][wave] = v[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        double x = sm[k][0];
+        for (int w = 1; w < kBlock / 64; ++w) x += sm[k][w];
+        v[k] = x;
+    }
+}
+
 __device__ __forceinline__ void finalize_body(const FinArgs& f)
 {
+    // Every thread's partial sums of the plaquette, Polyakov and Wilson
+    // slots first (their loads all go out before any reduction), then one
+    // block reduction of the seven values.
     const uint32_t t = threadIdx.x;
-    if (f.fused || f.plq) {
-        double sp = 0.0, tm = 0.0;
-        if (f.fused) {
+    double v[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};   // sp, tm, poly[4], wilson
+    if (f.fused) {
 #pragma unroll 8
-            for (uint32_t j = t; j < f.n_fused; j += kBlock) {
-                sp += f.fused[j];
-                sp += f.fused[f.n_fused + j];
-                tm += f.fused[2 * f.n_fused + j];
-            }
-        } else {
-#pragma unroll 8
-            for (uint32_t j = t; j < f.n_plq; j += kBlock) {
-                sp += f.plq[2 * j];
-                tm += f.plq[2 * j + 1];
-            }
+        for (uint32_t j = t; j < f.n_fused; j += kBlock) {
+            v[0] += f.fused[j];
+            v[0] += f.fused[f.n_fused + j];
+            v[1] += f.fused[2 * f.n_fused + j];
         }
-        // every thread reads the record index before the sums' barriers;
-        // thread 0 advances it after them
-        const int rec = f.rec_index[0];
-        sp = block_sum_1(sp);
-        tm = block_sum_1(tm);
+    } else if (f.plq) {
+#pragma unroll 8
+        for (uint32_t j = t; j < f.n_plq; j += kBlock) {
+            v[0] += f.plq[2 * j];
+            v[1] += f.plq[2 * j + 1];
+        }
+    }
+    if (f.poly) {
+#pragma unroll 8
+        for (uint32_t j = t; j < f.n_poly; j += kBlock)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) v[2 + k] += f.poly[4 * j + k];
+    }
+    if (f.wil)
+        for (uint32_t j = t; j < f.n_wil; j += kBlock) v[6] += f.wil[j];
+    // every thread reads the plaquette record index before the barriers;
+    // thread 0 advances the indices after them
+    const int rec = f.rec_index[0];
+    block_sum_k<7>(v);
+    if (f.fused || f.plq) {
         double fm[12];
         if (f.fmunu) {
             for (int k = 0; k < 12; ++k) {
@@ -1202,8 +1264,8 @@ __device__ __forceinline__ void finalize_body(const FinArgs& f)
         }
         if (rec < f.capacity) {
             if (t == 0) {
-                f.plq_rec[2 * rec] = sp;
-                f.plq_rec[2 * rec + 1] = tm;
+                f.plq_rec[2 * rec] = v[0];
+                f.plq_rec[2 * rec + 1] = v[1];
                 for (int k = 0; k < 12; ++k) f.f_rec[12 * rec + k] = f.fmunu ? fm[k] : 0.0;
             }
             if (f.a_rec && !f.aslices)   // a plaquette record without action slices
@@ -1211,31 +1273,17 @@ __device__ __forceinline__ void finalize_body(const FinArgs& f)
         }
         if (t == 0) f.rec_index[0] = rec + 1;
     }
-    if (f.poly) {
-        double v[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll 8
-        for (uint32_t j = t; j < f.n_poly; j += kBlock)
+    if (f.poly && t == 0) {
+        const int r = f.rec_index[1];
+        if (r < f.capacity)
 #pragma unroll
-            for (int k = 0; k < 4; ++k) v[k] += f.poly[4 * j + k];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) v[k] = block_sum_1(v[k]);
-        if (t == 0) {
-            const int rec = f.rec_index[1];
-            if (rec < f.capacity)
-#pragma unroll
-                for (int k = 0; k < 4; ++k) f.poly_rec[4 * rec + k] = v[k];
-            f.rec_index[1] = rec + 1;
-        }
+            for (int k = 0; k < 4; ++k) f.poly_rec[4 * r + k] = v[2 + k];
+        f.rec_index[1] = r + 1;
     }
-    if (f.wil) {
-        double w = 0.0;
-        for (uint32_t j = t; j < f.n_wil; j += kBlock) w += f.wil[j];
-        w = block_sum_1(w);
-        if (t == 0) {
-            const int rec = f.rec_index[2];
-            if (rec < f.capacity) f.wil_rec[rec] = w;
-            f.rec_index[2] = rec + 1;
-        }
+    if (f.wil && t == 0) {
+        const int r = f.rec_index[2];
+        if (r < f.capacity) f.wil_rec[r] = v[6];
+        f.rec_index[2] = r + 1;
     }
     if (f.slices && t == 0) f.rec_index[3] += 1;
 }
@@ -2623,6 +2671,8 @@ int qcd_kernel_info(const qcd_lattice* L, int id, char* name, int name_len, unsi
 // buffer_map / buffer_kill keep them (clinterface.cpp:1296-1440,1558-1568):
 // a host written against that registry keeps its auxiliary buffers here.
 // Types and their numbers follow GPU_buffer_types (clinterface.h:73-82).
+}  // extern "C"
+
 namespace {
 struct DevBuffer {
     int type = 0;
@@ -2644,6 +2694,8 @@ DevBuffer* buffer_of(int id)
     return &b[id];
 }
 }  // namespace
+
+extern "C" {
 
 int qcd_buffer_create(int type, size_t count, size_t elem_size, void* host_ptr, int* id)
 {
