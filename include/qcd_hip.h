@@ -561,6 +561,14 @@ int  qcd_flow_evaluate(qcd_flow *flow, const float *links, int batch, double bet
    layer's conditioners see frozen links only and no plaquette holds two
    active links.  Eight layers visit every link once.  Asynchronous.        */
 int  qcd_flow_imh_sweep(qcd_flow *flow, qcd_lattice *lat, unsigned long long seed, double beta);
+/* The sweep's seed lives on the device: qcd_flow_imh_sweep sets it;
+   qcd_flow_imh_sweep_next runs the sweep with the previous seed + 1, so a
+   graph recorded around it (qcd_graph_begin / _end on the lattice) replays
+   sweeps with successive seeds, exactly the sweeps qcd_flow_imh_sweep would
+   run with those seeds; qcd_flow_imh_seed sets the seed without a sweep
+   (synchronous). */
+int  qcd_flow_imh_sweep_next(qcd_flow *flow, qcd_lattice *lat, double beta);
+int  qcd_flow_imh_seed(qcd_flow *flow, unsigned long long seed);
 /* Training by hand-written kernels (BF16 MFMA, FP32 accumulation): the
    layer-local forward KL of `batch` heatbath configurations (host links in
    the sample / evaluate layout) -- for every coupling layer, the sum over

@@ -162,6 +162,8 @@ _SIGNATURES = {
     "qcd_flow_sample": [_P, ctypes.c_ulonglong, ctypes.c_int, ctypes.c_double, ctypes.POINTER(ctypes.c_float), _D, _D],
     "qcd_flow_evaluate": [_P, ctypes.POINTER(ctypes.c_float), ctypes.c_int, ctypes.c_double, _D, _D],
     "qcd_flow_imh_sweep": [_P, _P, ctypes.c_ulonglong, ctypes.c_double],
+    "qcd_flow_imh_sweep_next": [_P, _P, ctypes.c_double],
+    "qcd_flow_imh_seed": [_P, ctypes.c_ulonglong],
     "qcd_flow_train_grad": [_P, ctypes.POINTER(ctypes.c_float), ctypes.c_int, _D, ctypes.POINTER(ctypes.c_float)],
     "qcd_flow_train_step": [_P, ctypes.POINTER(ctypes.c_float), ctypes.c_int, ctypes.c_float, _D],
     "qcd_flow_get_params": [_P, ctypes.POINTER(ctypes.c_float), ctypes.c_size_t],

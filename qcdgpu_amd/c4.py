@@ -117,7 +117,7 @@ def _stats(series, sweeps_per_s):
     return m, e, tau, sweeps_per_s / (2.0 * tau)
 
 
-def run_arm(arm, a, flow):
+def run_arm(arm, a, flow, seed=None):
     """One chain: "plain" (heatbath), "learned" (heatbath + learned
     Metropolis sweep) or "global" (heatbath + whole-lattice independence
     Metropolis).  Returns the arm's result dict."""
@@ -125,7 +125,8 @@ def run_arm(arm, a, flow):
     from qcdgpu_amd import flow as QF
 
     dims = tuple(a.dims)
-    seed = a.seed + {"plain": 0, "learned": 1000, "global": 2000}[arm]
+    if seed is None:
+        seed = a.seed + {"plain": 0, "learned": 1000, "global": 2000}[arm]
     with Q.Lattice(sun=2, dims=dims, beta=a.beta, precision="fp32", prng="XOR128", randseries=seed,
                    max_measurements=a.steps + 2) as lat:
         lat.set_wilson(2, 2)
@@ -135,18 +136,28 @@ def run_arm(arm, a, flow):
             lat.sweep("hb")
         lat.reset_measurements()
         lw = {}
-        if arm == "learned":
-            flow.imh_counts(reset=True)
+        if arm in ("plain", "learned"):
+            # one step (1 heatbath sweep [+ 1 learned sweep] + measurements)
+            # recorded once as a graph and replayed; the learned sweeps take
+            # successive seeds (seed << 32) + 1, + 2, ... from the device
+            if arm == "learned":
+                flow.imh_seed(seed << 32)
+                flow.imh_counts(reset=True)
+            lat.graph_begin()
+            lat.sweep("hb")
+            if arm == "learned":
+                flow.imh_sweep_next(lat, a.beta)
+            lat.measure(wilson=True)
+            lat.graph_end()
             lat.sync()
             t0 = time.perf_counter()
-            for step in range(a.steps):
-                lat.sweep("hb")
-                flow.imh_sweep(lat, (seed << 32) + step, a.beta)
-                lat.measure(wilson=True)
+            lat.launch_cycle(a.steps)
             lat.sync()
             secs = time.perf_counter() - t0
-            acc_n, prop_n = flow.imh_counts()
-            acc = acc_n / max(prop_n, 1)
+            acc = 0.0
+            if arm == "learned":
+                acc_n, prop_n = flow.imh_counts()
+                acc = acc_n / max(prop_n, 1)
         else:
             acc, secs = QF.imh_chain(lat, flow if arm == "global" else None, a.beta, a.steps, hb_per_step=1,
                                      seed=seed + 17, wilson=True, log_weights=lw)
@@ -178,7 +189,26 @@ def cmd_chain(a):
         flow = QF.Flow(dims=dims, layers=a.layers, bins=a.bins, max_batch=a.batch)
         flow.set_params(np.load(a.params).astype(np.float32))
     arms = ["plain"] + (["learned"] + (["global"] if a.global_imh else []) if flow is not None else [])
-    for arm in arms:
+    # The judge of the observables is the plain heatbath ensemble: chains
+    # with random series 1 .. plain_chains, pooled (one chain can sit 3
+    # sigma off; the pool's error is what a comparison at 2 sigma needs).
+    # Its rate and independent configurations per second are the chains'
+    # means.
+    plain = []
+    for k in range(1, a.plain_chains + 1):
+        plain.append(run_arm("plain", a, None, seed=k))
+        print("  plain %d: %s" % (k, json.dumps(plain[-1])), flush=True)
+    pooled = {}
+    for name in ("plaquette", "polyakov", "wilson_2x2"):
+        ms = np.array([c[name]["mean"] for c in plain])
+        es = np.array([c[name]["err"] for c in plain])
+        pooled[name] = {"mean": float(ms.mean()), "err": float(np.sqrt((es ** 2).sum()) / len(es)),
+                        "tau_int": round(float(np.mean([c[name]["tau_int"] for c in plain])), 2)}
+    res["plain"] = dict(pooled, chains=len(plain),
+                        steps_per_s=round(float(np.mean([c["steps_per_s"] for c in plain])), 1),
+                        independent_configs_per_s=round(float(np.mean([c["independent_configs_per_s"]
+                                                                        for c in plain])), 2))
+    for arm in arms[1:]:
         res[arm] = run_arm(arm, a, flow)
         print("  %s: %s" % (arm, json.dumps(res[arm])), flush=True)
     z = {}
@@ -230,6 +260,8 @@ def main(argv=None):
     c.add_argument("--batch", type=int, default=64)
     c.add_argument("--global", dest="global_imh", action="store_true",
                    help="also run the whole-lattice independence-Metropolis arm")
+    c.add_argument("--plain-chains", type=int, default=8,
+                   help="plain heatbath chains (random series 1..n) pooled as the judge")
     for q in (g, t, c):
         q.add_argument("--dims", type=int, nargs=4, default=list(DIMS))
         q.add_argument("--beta", type=float, default=BETA)

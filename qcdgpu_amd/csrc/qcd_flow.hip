@@ -223,10 +223,13 @@ __global__ void k_flow_features(const float4* L, uint16_t* feat, FGeom g, uint32
 // One wavefront: a tile of 16 sites x 16 output channels.  in[(row) * cin +
 // channel] (BF16), wt[n * (81 cin) + k] (BF16, k = st * cin + channel),
 // out: tanh -> BF16 [row * cout + n] (act) or FP32 [row * cout + n].
+template <int CIN>
 __global__ __launch_bounds__(256) void k_flow_conv(const uint16_t* in, int cin, const uint16_t* wt, const float* bias,
                                                    int cout, uint16_t* out_bf, float* out_f, FGeom g, uint32_t B,
                                                    int act)
 {
+    static_assert(CIN % 4 == 0, "a lane's 4 values stay inside one stencil point");
+    (void)cin;   // == CIN (the host dispatches on it)
     const uint32_t wave = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
     const int lane = threadIdx.x & 63;
     const uint32_t ntile_m = B * g.V / 16, ntile_n = (uint32_t)cout / 16;
@@ -239,22 +242,33 @@ __global__ __launch_bounds__(256) void k_flow_conv(const uint16_t* in, int cin, 
     const int n[4] = {g.n1, g.n2, g.n3, g.n4};
     const int kq = 4 * (lane >> 4);                // k offset of this lane's 4 values
     const uint32_t ncol = tn * 16 + (lane & 15);   // B column of this lane
-    const int K = kStencil * cin;
+    constexpr int K = kStencil * CIN;
     const uint16_t* wrow = wt + (size_t)ncol * K;
+    const uint16_t* inb = in + (size_t)b * g.V * CIN;
     f4 acc = {0.f, 0.f, 0.f, 0.f};
-    for (int k0 = 0; k0 < K; k0 += 16) {
-        const int k = k0 + kq;   // K is a multiple of 4: a lane's 4 values are all inside or all past K
-        s4 av = {0, 0, 0, 0}, bv = {0, 0, 0, 0};
-        if (k < K) {
-            const int st = k / cin, ch = k % cin;
-            const int o[4] = {st / 27 - 1, (st / 9) % 3 - 1, (st / 3) % 3 - 1, st % 3 - 1};
-            int e[4];
-            for (int d = 0; d < 4; ++d) e[d] = wrap(c[d] + o[d], n[d]);
-            const uint32_t nb = site(g, e);
-            av = *reinterpret_cast<const s4*>(in + ((size_t)b * g.V + nb) * cin + ch);
-            bv = *reinterpret_cast<const s4*>(wrow + k);
+    // The operands of U consecutive k-steps are loaded before their MFMAs,
+    // so U gathers are in flight at once; the MFMAs run in k order, as one
+    // step at a time would.
+    constexpr int U = 8;
+    for (int k0 = 0; k0 < K; k0 += 16 * U) {
+        s4 av[U], bv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int k = k0 + 16 * u + kq;   // K is a multiple of 4: a lane's 4 values are all inside or all past K
+            av[u] = s4{0, 0, 0, 0};
+            bv[u] = s4{0, 0, 0, 0};
+            if (k < K) {
+                const int st = k / CIN, ch = k % CIN;
+                const int o[4] = {st / 27 - 1, (st / 9) % 3 - 1, (st / 3) % 3 - 1, st % 3 - 1};
+                int e[4];
+#pragma unroll
+                for (int d = 0; d < 4; ++d) e[d] = wrap(c[d] + o[d], n[d]);
+                av[u] = *reinterpret_cast<const s4*>(inb + (size_t)site(g, e) * CIN + ch);
+                bv[u] = *reinterpret_cast<const s4*>(wrow + k);
+            }
         }
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
+#pragma unroll
+        for (int u = 0; u < U; ++u) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[u], bv[u], acc, 0, 0, 0);
     }
     // lane holds D[4 * (lane / 16) + i][lane % 16]
     for (int i = 0; i < 4; ++i) {
@@ -389,10 +403,23 @@ __device__ inline float log_jac(float dlog, float th_in, float th_out)
 // min(1, exp(beta k ((U'V)_0 - (UV)_0)) q_a(U) / q_a(U')): detailed balance
 // for every link, so the Wilson-action distribution stays exact.  Counts
 // accepted links in *accepted (one atomic per wavefront).
+// The sweep's seed: mode 0 sets it, 1 advances it by one and counts a
+// sweep (a recorded sweep), 2 sets it and counts a sweep.
+__global__ void k_flow_imh_key(unsigned long long* imh, unsigned long long seed, int mode)
+{
+    if (threadIdx.x == 0) {
+        imh[2] = mode == 1 ? imh[2] + 1ull : seed;
+        if (mode) imh[1] += 1ull;
+    }
+}
+
 __global__ __launch_bounds__(256) void k_flow_imh_couple(float4* L, const float* prm, FGeom g, int mu, int par, int K,
-                                                         float beta, uint64_t seed, unsigned long long* accepted)
+                                                         float beta, uint64_t lmix, unsigned long long* imh)
 {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    // the layer's draw key: (sweep seed, layer) hashed
+    const uint64_t seed = mix64(mix64((uint64_t)imh[2]) ^ lmix);
+    unsigned long long* accepted = imh;
     bool acc = false;
     if (s < g.V) {
         int c[4];
@@ -716,8 +743,10 @@ struct qcd_flow {
     float* logj = nullptr;     // [max_batch * V]
     double* act = nullptr;     // [max_batch * V]
     double* sums = nullptr;    // [2][max_batch]
-    unsigned long long* imh = nullptr;   // [0] links accepted by qcd_flow_imh_sweep
-    unsigned long long imh_proposed = 0;
+    // learned sweeps, on the device so that a recorded graph replays them
+    // with successive keys: [0] links accepted, [1] sweeps run, [2] the
+    // current sweep's seed
+    unsigned long long* imh = nullptr;
     // training (allocated by the first qcd_flow_train_step)
     float* tp = nullptr;        // FP32 parameters, the flat layout of qcd_flow_set_params
     float* tgrad = nullptr;     // their gradient
@@ -758,9 +787,9 @@ int run_layers(qcd_flow* F, uint32_t B, bool inverse)
         const int mu = l % 4, par = (l / 4) % 2;
         hipLaunchKernelGGL(k_flow_features, dim3(nb), dim3(256), 0, F->stream, F->links, F->feat, g, B, mu, par);
         const uint32_t w1n = (n / 16) * (kHidden / 16), w2n = (n / 16) * (kOut / 16);
-        hipLaunchKernelGGL(k_flow_conv, dim3((w1n + 3) / 4), dim3(256), 0, F->stream, F->feat, kCin0, F->w1 + l * kW1,
+        hipLaunchKernelGGL(k_flow_conv<kCin0>, dim3((w1n + 3) / 4), dim3(256), 0, F->stream, F->feat, kCin0, F->w1 + l * kW1,
                            F->b1 + l * kHidden, kHidden, F->hid, (float*)nullptr, g, B, 1);
-        hipLaunchKernelGGL(k_flow_conv, dim3((w2n + 3) / 4), dim3(256), 0, F->stream, F->hid, kHidden, F->w2 + l * kW2,
+        hipLaunchKernelGGL(k_flow_conv<kHidden>, dim3((w2n + 3) / 4), dim3(256), 0, F->stream, F->hid, kHidden, F->w2 + l * kW2,
                            F->b2 + l * kOut, kOut, (uint16_t*)nullptr, F->prm, g, B, 0);
         hipLaunchKernelGGL(k_flow_couple, dim3(nb), dim3(256), 0, F->stream, F->links, F->prm, F->logj, g, B, mu, par,
                            F->d.bins, inverse ? 1 : 0);
@@ -837,8 +866,8 @@ int qcd_flow_create(const qcd_flow_desc* desc, qcd_flow** out)
     ok = ok && hipMalloc((void**)&F->logj, sizeof(float) * n) == hipSuccess;
     ok = ok && hipMalloc((void**)&F->act, sizeof(double) * n) == hipSuccess;
     ok = ok && hipMalloc((void**)&F->sums, sizeof(double) * 2 * d.max_batch) == hipSuccess;
-    ok = ok && hipMalloc((void**)&F->imh, sizeof(unsigned long long)) == hipSuccess;
-    ok = ok && hipMemset(F->imh, 0, sizeof(unsigned long long)) == hipSuccess;
+    ok = ok && hipMalloc((void**)&F->imh, 3 * sizeof(unsigned long long)) == hipSuccess;
+    ok = ok && hipMemset(F->imh, 0, 3 * sizeof(unsigned long long)) == hipSuccess;
     if (!ok) {
         qcd_flow_destroy(F);
         return err(QCD_E_NOMEM, "flow buffers");
@@ -936,7 +965,8 @@ int qcd_flow_evaluate(qcd_flow* F, const float* links, int batch, double beta, d
     return finish(F, (uint32_t)batch, beta, logq, action);
 }
 
-int qcd_flow_imh_sweep(qcd_flow* F, qcd_lattice* lat, unsigned long long seed, double beta)
+namespace {
+int imh_sweep(qcd_flow* F, qcd_lattice* lat, unsigned long long seed, int mode, double beta)
 {
     if (!F || !lat) return err(QCD_E_ARG, "null argument");
     if (F->nparam == 0 || F->host_params.empty()) return err(QCD_E_STATE, "flow parameters not set");
@@ -947,24 +977,42 @@ int qcd_flow_imh_sweep(qcd_flow* F, qcd_lattice* lat, unsigned long long seed, d
         if (n[k] != F->d.n[k]) return err(QCD_E_ARG, "the lattice and the flow have different extents");
     hipStream_t st = nullptr;
     if ((rc = qcdhip::su2_quats(lat, F->links, true, &st))) return rc;   // the lattice's stream from here on
+    hipLaunchKernelGGL(k_flow_imh_key, dim3(1), dim3(64), 0, st, F->imh, seed, mode);
     const FGeom g = F->g;
     const uint32_t nb = (g.V + 255) / 256;
     const uint32_t w1n = (g.V / 16) * (kHidden / 16), w2n = (g.V / 16) * (kOut / 16);
     for (int l = 0; l < F->d.layers; ++l) {
         const int mu = l % 4, par = (l / 4) % 2;
         hipLaunchKernelGGL(k_flow_features, dim3(nb), dim3(256), 0, st, F->links, F->feat, g, 1u, mu, par);
-        hipLaunchKernelGGL(k_flow_conv, dim3((w1n + 3) / 4), dim3(256), 0, st, F->feat, kCin0, F->w1 + l * kW1,
+        hipLaunchKernelGGL(k_flow_conv<kCin0>, dim3((w1n + 3) / 4), dim3(256), 0, st, F->feat, kCin0, F->w1 + l * kW1,
                            F->b1 + l * kHidden, kHidden, F->hid, (float*)nullptr, g, 1u, 1);
-        hipLaunchKernelGGL(k_flow_conv, dim3((w2n + 3) / 4), dim3(256), 0, st, F->hid, kHidden, F->w2 + l * kW2,
+        hipLaunchKernelGGL(k_flow_conv<kHidden>, dim3((w2n + 3) / 4), dim3(256), 0, st, F->hid, kHidden, F->w2 + l * kW2,
                            F->b2 + l * kOut, kOut, (uint16_t*)nullptr, F->prm, g, 1u, 0);
-        // the layer's draw key: (seed, layer) hashed
-        const uint64_t key = mix64(mix64((uint64_t)seed) ^ ((uint64_t)l + 1) * 0xD1B54A32D192ED03ull);
         hipLaunchKernelGGL(k_flow_imh_couple, dim3(nb), dim3(256), 0, st, F->links, F->prm, g, mu, par, F->d.bins,
-                           (float)beta, key, F->imh);
-        F->imh_proposed += g.V / 2;
+                           (float)beta, ((uint64_t)l + 1) * 0xD1B54A32D192ED03ull, F->imh);
     }
     FCHK(hipGetLastError());
     return qcdhip::su2_quats(lat, F->links, false, nullptr);
+}
+}  // namespace
+
+int qcd_flow_imh_sweep(qcd_flow* F, qcd_lattice* lat, unsigned long long seed, double beta)
+{
+    return imh_sweep(F, lat, seed, 2, beta);
+}
+
+int qcd_flow_imh_sweep_next(qcd_flow* F, qcd_lattice* lat, double beta)
+{
+    return imh_sweep(F, lat, 0ull, 1, beta);
+}
+
+int qcd_flow_imh_seed(qcd_flow* F, unsigned long long seed)
+{
+    if (!F) return err(QCD_E_ARG, "null flow");
+    if (!F->imh) return err(QCD_E_STATE, "flow not set up");
+    hipLaunchKernelGGL(k_flow_imh_key, dim3(1), dim3(64), 0, F->stream, F->imh, seed, 0);
+    FCHK(hipStreamSynchronize(F->stream));
+    return QCD_OK;
 }
 
 }  // extern "C"
@@ -1015,9 +1063,9 @@ int train_grad(qcd_flow* F, const float* links, int batch, double* loss)
         float* gl = F->tgrad + (size_t)l * kLayerParams;   // W1, b1, W2, b2 of layer l
         // forward of the conditioner on the data configurations
         hipLaunchKernelGGL(k_flow_features, dim3(nb), dim3(256), 0, F->stream, F->links, F->feat, g, B, mu, par);
-        hipLaunchKernelGGL(k_flow_conv, dim3((w1n + 3) / 4), dim3(256), 0, F->stream, F->feat, kCin0, F->w1 + l * kW1,
+        hipLaunchKernelGGL(k_flow_conv<kCin0>, dim3((w1n + 3) / 4), dim3(256), 0, F->stream, F->feat, kCin0, F->w1 + l * kW1,
                            F->b1 + l * kHidden, kHidden, F->hid, (float*)nullptr, g, B, 1);
-        hipLaunchKernelGGL(k_flow_conv, dim3((w2n + 3) / 4), dim3(256), 0, F->stream, F->hid, kHidden, F->w2 + l * kW2,
+        hipLaunchKernelGGL(k_flow_conv<kHidden>, dim3((w2n + 3) / 4), dim3(256), 0, F->stream, F->hid, kHidden, F->w2 + l * kW2,
                            F->b2 + l * kOut, kOut, (uint16_t*)nullptr, F->prm, g, B, 0);
         // -log q_a and its gradient with respect to the spline parameters
         hipLaunchKernelGGL(k_flow_nll_grad, dim3(nb), dim3(256), 0, F->stream, (const float4*)F->links, F->prm, F->tg,
@@ -1038,7 +1086,7 @@ int train_grad(qcd_flow* F, const float* links, int batch, double* loss)
             hipLaunchKernelGGL(k_flow_w2_transpose, dim3((nt + 255) / 256), dim3(256), 0, F->stream,
                                (const uint16_t*)(F->w2 + l * kW2), F->tw2t);
             const uint32_t wtn = (R / 16) * (kHidden / 16);
-            hipLaunchKernelGGL(k_flow_conv, dim3((wtn + 3) / 4), dim3(256), 0, F->stream, (const uint16_t*)F->tg, kOut,
+            hipLaunchKernelGGL(k_flow_conv<kOut>, dim3((wtn + 3) / 4), dim3(256), 0, F->stream, (const uint16_t*)F->tg, kOut,
                                (const uint16_t*)F->tw2t, (const float*)F->tzero, kHidden, (uint16_t*)nullptr, F->tdhid,
                                g, B, 0);
             hipLaunchKernelGGL(k_flow_tanh_back, dim3((R * kHidden + 255) / 256), dim3(256), 0, F->stream,
@@ -1119,14 +1167,11 @@ int qcd_flow_imh_counts(qcd_flow* F, unsigned long long* accepted, unsigned long
 {
     if (!F) return err(QCD_E_ARG, "null flow");
     FCHK(hipDeviceSynchronize());
-    unsigned long long a = 0;
-    FCHK(hipMemcpy(&a, F->imh, sizeof(a), hipMemcpyDeviceToHost));
-    if (accepted) *accepted = a;
-    if (proposed) *proposed = F->imh_proposed;
-    if (reset) {
-        FCHK(hipMemset(F->imh, 0, sizeof(unsigned long long)));
-        F->imh_proposed = 0;
-    }
+    unsigned long long a[2] = {0, 0};
+    FCHK(hipMemcpy(a, F->imh, sizeof(a), hipMemcpyDeviceToHost));
+    if (accepted) *accepted = a[0];
+    if (proposed) *proposed = a[1] * (unsigned long long)F->d.layers * (F->g.V / 2);   // every layer proposes V/2 links
+    if (reset) FCHK(hipMemset(F->imh, 0, 2 * sizeof(unsigned long long)));   // the seed stays
     return QCD_OK;
 }
 

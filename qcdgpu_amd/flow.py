@@ -127,6 +127,16 @@ class Flow:
         exact Wilson-action Metropolis probability."""
         check(lib().qcd_flow_imh_sweep(self._h, lat._h, int(seed), float(beta)))
 
+    def imh_sweep_next(self, lat, beta):
+        """imh_sweep with the previous sweep's seed + 1 (the seed lives on
+        the device, so a graph recorded around this call replays sweeps
+        with successive seeds)."""
+        check(lib().qcd_flow_imh_sweep_next(self._h, lat._h, float(beta)))
+
+    def imh_seed(self, seed):
+        """Sets the learned sweeps' seed without a sweep."""
+        check(lib().qcd_flow_imh_seed(self._h, int(seed)))
+
     def imh_counts(self, reset=False):
         """(accepted, proposed) links of imh_sweep so far."""
         a, p = ctypes.c_ulonglong(), ctypes.c_ulonglong()

@@ -166,3 +166,45 @@ def test_c4_chains_match_plain_heatbath():
         print("  %s plain %.6f(%.6f) [chain scatter %.6f] learned %.6f(%.6f) z %.2f"
               % (name, m1, e1, pm.std(ddof=1) / np.sqrt(len(pm)), m2, e2, z))
         assert abs(z) < 2.0
+
+
+def test_learned_sweep_graph_equals_eager():
+    """A step recorded as a graph -- 1 heatbath sweep, 1 learned sweep with
+    the device's next seed, measurements -- and replayed equals the same
+    steps run one call at a time with explicit seeds, bit for bit (links,
+    generator states, records), with the same accepted and proposed
+    counts."""
+    _case("learned sweep: graph replay vs eager")
+    steps = 30
+    V = int(np.prod(DIMS))
+    with _flow(DIMS, batch=1) as f:
+        def run(graph):
+            with Q.Lattice(sun=2, dims=DIMS, beta=BETA, precision="fp32", prng="XOR128", randseries=5,
+                           max_measurements=steps + 2) as lat:
+                lat.prng_init()
+                lat.init("hot")
+                for _ in range(20):
+                    lat.sweep("hb")
+                lat.reset_measurements()
+                f.imh_counts(reset=True)
+                if graph:
+                    f.imh_seed(1000)
+                    lat.graph_begin()
+                    lat.sweep("hb")
+                    f.imh_sweep_next(lat, BETA)
+                    lat.measure()
+                    lat.graph_end()
+                    lat.launch_cycle(steps)
+                else:
+                    for i in range(steps):
+                        lat.sweep("hb")
+                        f.imh_sweep(lat, 1000 + i + 1, BETA)
+                        lat.measure()
+                lat.sync()
+                return lat.save_state(), f.imh_counts()
+
+        a, ca = run(False)
+        b, cb = run(True)
+    print("  accepted %d of %d" % ca)
+    assert ca == cb and ca[1] == steps * 4 * V
+    assert a.size == b.size and int(np.count_nonzero(a != b)) == 0
