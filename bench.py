@@ -240,6 +240,9 @@ def main():
                     help="dispatch order of the update kernels (default: the library's)")
     ap.add_argument("--blocks-per-cu", type=int, default=None,
                     help="cap on resident update workgroups per CU (default: the library's, none)")
+    ap.add_argument("--cycles-per-graph", type=int, default=None,
+                    help="cycles recorded per graph, measurements overlapped with the next cycle (default: the "
+                         "library's, 1 = one cycle per graph)")
     ap.add_argument("--update-shape", type=int, default=None, choices=(1, 3),
                     help="wavefronts per 64-link unit of the update kernels (default: the library's, 3)")
     ap.add_argument("--profile-steps", type=int, default=50)
@@ -291,13 +294,26 @@ def main():
         randseries=54,
         max_measurements=args.steps + args.warmup + args.profile_steps + 8,
     )
-    lat.tune(xcd_map=args.xcd_map, blocks_per_cu=args.blocks_per_cu, update_shape=args.update_shape)
+    k_cycles = args.cycles_per_graph or lat.tuning()["cycles_per_graph"]
+    lat.tune(xcd_map=args.xcd_map, blocks_per_cu=args.blocks_per_cu, update_shape=args.update_shape,
+             cycles_per_graph=1)
     lat.init("cold")
     lat.prng_init()
+    # Single-cycle graph first: its step time scales the per-kernel event
+    # durations below (no measurement overlapped with the next cycle there).
     lat.capture_cycle(n_hb=1, n_or=N_OR, plaquette=True, polyakov=True)
+    lat.launch_cycle(min(args.warmup, 20))
+    lat.sync()
+    n_single = max(1, min(args.steps, 150))
+    ms_single = lat.time_cycle(n_single)
+    # The timed workload: K cycles per graph (each cycle's measurement
+    # beside the next cycle's first half-steps).
+    lat.tune(cycles_per_graph=k_cycles)
+    lat.capture_cycle(n_hb=1, n_or=N_OR, plaquette=True, polyakov=True)
+    k_captured = lat.tuning()["captured_cycles_per_graph"]
     lat.launch_cycle(args.warmup)
     lat.sync()
-    # Timed region: K replays of the captured sweep, HIP events on the
+    # Timed region: `steps` replays of the captured cycle, HIP events on the
     # lattice stream around the whole region (synchronised on both sides).
     ms = lat.time_cycle(args.steps)
     updates = 5.0 * 4.0 * V * args.steps
@@ -311,13 +327,13 @@ def main():
     hb_bytes = bytes_per_launch + 2.0 * 16 * (V // 2)
     # Kernel durations: HIP events around each kernel of an eagerly launched
     # cycle exclude the gaps between the graph's kernels; scaled so that one
-    # cycle's kernels sum to the graph-timed step, they are the per-kernel
-    # share of the timed region (what a kernel trace of the graph replays
-    # reports, which counts each gap into the kernel before it).
+    # cycle's kernels sum to the step of the single-cycle graph, they are the
+    # per-kernel share of its timed region (what a kernel trace of those
+    # replays reports, which counts each gap into the kernel before it).
     ms_step = ms / args.steps
     n_hb_launch, n_or_launch = 8, 8 * N_OR - 3   # three OR launches of the last sweep carry the plaquette
     event_sum_us = n_hb_launch * prof["hb_us"] + n_or_launch * prof["or_us"] + prof["meas_us"]
-    gap_scale = 1000.0 * ms_step / event_sum_us
+    gap_scale = 1000.0 * (ms_single / n_single) / event_sum_us
     or_us, hb_us = prof["or_us"] * gap_scale, prof["hb_us"] * gap_scale
     achieved = bytes_per_launch / (or_us * 1e-6) / 1e9
     hb_achieved = hb_bytes / (hb_us * 1e-6) / 1e9
@@ -361,6 +377,8 @@ def main():
                           "measure_per_step": prof["meas_us"] * gap_scale},
             "kernel_us_events": {"heatbath_halfstep": prof["hb_us"], "overrelax_halfstep": prof["or_us"],
                                  "measure_per_step": prof["meas_us"], "gap_scale": gap_scale},
+            "cycles_per_graph": k_captured,
+            "ms_per_step_single_cycle_graph": ms_single / n_single,
             "sweeps_per_s": args.steps / (ms * 1e-3),
             "heatbath_roofline": {
                 "achieved": hb_achieved,

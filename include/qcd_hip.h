@@ -153,8 +153,25 @@ int  qcd_lattice_sync(qcd_lattice *lat);
                           updates in place.  A half-step reads only links it
                           does not write, so the two give identical bits; a
                           difference means a kernel read a link that the same
-                          launch writes.  Single lattices only.              */
-enum { QCD_TUNE_XCD_MAP = 1, QCD_TUNE_BLOCKS_PER_CU = 2, QCD_TUNE_UPDATE_SHAPE = 3, QCD_TUNE_SHADOW = 4 };
+                          launch writes.  Single lattices only.
+   QCD_TUNE_CYCLES_PER_GRAPH  K (1..64, default 1): qcd_cycle_capture also
+                          records K consecutive cycles as one graph in which
+                          each cycle's measurement (plaquette fused into the
+                          last sweep, Polyakov loop, final sums) runs on a
+                          second stream beside the next cycle's odd X, Y, Z
+                          half-steps, which touch nothing it reads or
+                          writes; the odd T half-step waits for it.
+                          qcd_cycle_launch / _time replay whole K-blocks,
+                          then single cycles.  Only for masks of the
+                          plaquette and the Polyakov loop with the plaquette
+                          fused (otherwise single cycles).  Results are the
+                          same bits; qcd_lattice_get_tune's detail[0] gives
+                          the K of the captured graph (0: none).  Measured
+                          on MI355X at 32^3 x 8: K = 8 is 1.7 % slower than
+                          single cycles (the overlapped launches slow the
+                          half-steps more than the measurement costs).     */
+enum { QCD_TUNE_XCD_MAP = 1, QCD_TUNE_BLOCKS_PER_CU = 2, QCD_TUNE_UPDATE_SHAPE = 3, QCD_TUNE_SHADOW = 4,
+       QCD_TUNE_CYCLES_PER_GRAPH = 5 };
 int  qcd_lattice_tune(qcd_lattice *lat, int knob, int value);
 /* ---- kernels by name (GPU::kernel_init / kernel_run, clinterface.cpp:1132-1278,
         with the names lattice_make_programs creates, suncl.cpp:3265-3648,

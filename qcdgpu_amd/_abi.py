@@ -20,7 +20,7 @@ STREAM_REFERENCE, STREAM_FAST = 0, 1
 HEATBATH, OVERRELAX = 0, 1
 MEAS_PLAQUETTE, MEAS_POLYAKOV, MEAS_WILSON, MEAS_POLYAKOV_SLICES = 1, 2, 4, 8
 MEAS_FMUNU, MEAS_F0MU, MEAS_ACTION_SLICES = 16, 32, 64
-TUNE_XCD_MAP, TUNE_BLOCKS_PER_CU, TUNE_UPDATE_SHAPE, TUNE_SHADOW = 1, 2, 3, 4
+TUNE_XCD_MAP, TUNE_BLOCKS_PER_CU, TUNE_UPDATE_SHAPE, TUNE_SHADOW, TUNE_CYCLES_PER_GRAPH = 1, 2, 3, 4, 5
 BUFFER_GLOBAL, BUFFER_INPUT, BUFFER_IO, BUFFER_OUTPUT, BUFFER_CONSTANT, BUFFER_LDS, BUFFER_UAV = 1, 2, 3, 4, 5, 6, 7
 
 

@@ -1810,6 +1810,14 @@ struct qcd_lattice {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int cyc_hb = 0, cyc_or = 0, cyc_mask = 0;
+    // cyc_k consecutive cycles recorded as one graph, each cycle's
+    // measurement on a second stream beside the next cycle's first three
+    // half-steps (QCD_TUNE_CYCLES_PER_GRAPH)
+    hipGraph_t graph_k = nullptr;
+    hipGraphExec_t exec_k = nullptr;
+    int cyc_k = 0;
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     bool rng_ready = false;
     bool hot_run = false;       // produce() sized as in the reference's hot runs (QCD_START_RANDOM)
     bool gid_updates = false;   // INTS = 2: GID known-answer numbers in the heatbath
@@ -1823,6 +1831,7 @@ struct qcd_lattice {
     int tune_blocks_per_cu = 0; // QCD_TUNE_BLOCKS_PER_CU (0 = no cap)
     int tune_update_shape = 3;  // QCD_TUNE_UPDATE_SHAPE: wavefronts per 64 links (3 or 1)
     void* shadow = nullptr;     // QCD_TUNE_SHADOW: the copy the update gathers read (race check)
+    int tune_cycles = 1;        // QCD_TUNE_CYCLES_PER_GRAPH (1: single-cycle graphs, the faster on MI355X)
     // A chain of a qcd_batch: the batch's device descriptors hold this
     // lattice's buffers and stream sizing, so neither may change (no
     // re-sizing start, no destroy except through the batch).
@@ -1850,6 +1859,22 @@ struct qcd_lattice {
     uint32_t graph_counter0 = 0;
     int graph_rng0 = 0;
 };
+
+namespace {
+// Drops the captured cycle graphs (the single cycle and the multi-cycle one).
+void drop_cycle_graphs(qcd_lattice* L)
+{
+    if (L->exec) (void)hipGraphExecDestroy(L->exec);
+    if (L->graph) (void)hipGraphDestroy(L->graph);
+    if (L->exec_k) (void)hipGraphExecDestroy(L->exec_k);
+    if (L->graph_k) (void)hipGraphDestroy(L->graph_k);
+    L->exec = nullptr;
+    L->graph = nullptr;
+    L->exec_k = nullptr;
+    L->graph_k = nullptr;
+    L->cyc_k = 0;
+}
+}  // namespace
 
 namespace {
 
@@ -2404,14 +2429,7 @@ static int set_sizing(qcd_lattice* L, bool hot_run, bool* changed)
     *changed = true;
     L->hot_run = hot_run;
     L->s_quads = S;
-    if (L->exec) {
-        (void)hipGraphExecDestroy(L->exec);
-        L->exec = nullptr;
-    }
-    if (L->graph) {
-        (void)hipGraphDestroy(L->graph);
-        L->graph = nullptr;
-    }
+    drop_cycle_graphs(L);
     if (L->quad_buffer && L->prn_quads < S) {
         (void)hipFree(L->prn_buf);
         L->prn_buf = nullptr;
@@ -2542,8 +2560,10 @@ int qcd_lattice_destroy(qcd_lattice* L)
     if (!L) return QCD_OK;
     if (L->batch_owned) return fail(QCD_E_STATE, "a batched chain is destroyed with its batch (qcd_batch_destroy)");
     if (L->stream) (void)hipStreamSynchronize(L->stream);
-    if (L->exec) (void)hipGraphExecDestroy(L->exec);
-    if (L->graph) (void)hipGraphDestroy(L->graph);
+    drop_cycle_graphs(L);
+    if (L->side) (void)hipStreamDestroy(L->side);
+    if (L->ev_fork) (void)hipEventDestroy(L->ev_fork);
+    if (L->ev_join) (void)hipEventDestroy(L->ev_join);
     for (auto& e : L->prof_ev)
         if (e) (void)hipEventDestroy(e);
     (void)hipFree(L->table);
@@ -2781,14 +2801,7 @@ int qcd_graph_begin(qcd_lattice* L)
     if (!L) return fail(QCD_E_ARG, "null lattice");
     if (L->graph_open) return fail(QCD_E_STATE, "a graph is already being recorded");
     if (L->profiling) return fail(QCD_E_STATE, "kernel timing waits for every kernel: no graph while it is on");
-    if (L->exec) {
-        (void)hipGraphExecDestroy(L->exec);
-        L->exec = nullptr;
-    }
-    if (L->graph) {
-        (void)hipGraphDestroy(L->graph);
-        L->graph = nullptr;
-    }
+    drop_cycle_graphs(L);
     HIPCHK(hipStreamSynchronize(L->stream));
     HIPCHK(hipStreamBeginCapture(L->stream, hipStreamCaptureModeThreadLocal));
     L->graph_open = true;
@@ -2835,12 +2848,7 @@ int qcd_lattice_set_slab(qcd_lattice* L, int on)
         if (L->tw.on) return fail(QCD_E_UNSUPPORTED, "no twisted boundary on an x-slab");
     }
     HIPCHK(hipStreamSynchronize(L->stream));
-    if (L->slab != (on != 0)) {   // a captured cycle holds the other launch shape
-        if (L->exec) (void)hipGraphExecDestroy(L->exec);
-        if (L->graph) (void)hipGraphDestroy(L->graph);
-        L->exec = nullptr;
-        L->graph = nullptr;
-    }
+    if (L->slab != (on != 0)) drop_cycle_graphs(L);   // a captured cycle holds the other launch shape
     L->slab = on != 0;
     return QCD_OK;
 }
@@ -2922,6 +2930,10 @@ int qcd_lattice_tune(qcd_lattice* L, int knob, int value)
             if (value != 1 && value != 3) return fail(QCD_E_ARG, "update shape must be 1 or 3");
             L->tune_update_shape = value;
             break;
+        case QCD_TUNE_CYCLES_PER_GRAPH:
+            if (value < 1 || value > 64) return fail(QCD_E_ARG, "cycles per graph must be 1..64");
+            L->tune_cycles = value;   // from the next qcd_cycle_capture
+            break;
         case QCD_TUNE_SHADOW:
             if (value != 0 && value != 1) return fail(QCD_E_ARG, "shadow mode must be 0 or 1");
             if (L->batch_owned) return fail(QCD_E_STATE, "shadow mode applies to single lattices, not batched chains");
@@ -2959,6 +2971,10 @@ int qcd_lattice_get_tune(const qcd_lattice* L, int knob, int* value, int* detail
         case QCD_TUNE_BLOCKS_PER_CU: *value = L->tune_blocks_per_cu; break;
         case QCD_TUNE_UPDATE_SHAPE: *value = L->tw.on ? 3 : L->tune_update_shape; break;
         case QCD_TUNE_SHADOW: *value = L->shadow ? 1 : 0; break;
+        case QCD_TUNE_CYCLES_PER_GRAPH:
+            *value = L->tune_cycles;
+            if (detail) detail[0] = L->cyc_k;   // cycles in the captured multi-cycle graph (0: none)
+            break;
         default: return fail(QCD_E_ARG, "unknown tuning knob %d", knob);
     }
     return QCD_OK;
@@ -3224,10 +3240,14 @@ int qcd_halfstep(qcd_lattice* L, int parity, int mu, int kind)
 
 // One sweep; with fuse_plaquette the even Y, Z and T half-steps also produce
 // the plaquette record of the configuration the sweep leaves behind.
-static int sweep_impl(qcd_lattice* L, int kind, bool fuse_plaquette, std::vector<hipEvent_t>* ev, int* ev_used)
+static int sweep_impl(qcd_lattice* L, int kind, bool fuse_plaquette, std::vector<hipEvent_t>* ev, int* ev_used,
+                      hipEvent_t join_before_odd_t = nullptr)
 {
     for (int parity = 1; parity >= 0; --parity)
         for (int mu = 0; mu < 4; ++mu) {
+            // the odd T half-step is the first to write links a measurement
+            // of the previous cycle (plaquette sums, Polyakov loop) reads
+            if (join_before_odd_t && parity == 1 && mu == 3) HIPCHK(hipStreamWaitEvent(L->stream, join_before_odd_t, 0));
             LaunchOpt o;
             if (fuse_plaquette && parity == 0 && mu >= 1) o.meas_mode = mu;   // 1, 2, 3
             if (ev) {
@@ -3491,15 +3511,37 @@ static bool cycle_fuses(const qcd_lattice* L, int mask, int total)
 // A measurement cycle: n_hb heatbath sweeps, n_or overrelaxation sweeps,
 // then the measurements.  The plaquette rides on the last sweep's even
 // Y/Z/T half-steps when there is a sweep; the Polyakov loop is its own kernel.
-static int enqueue_cycle(qcd_lattice* L, int n_hb, int n_or, int mask, std::vector<hipEvent_t>* ev, int* ev_used)
+// Whether a cycle's measurement can run beside the next cycle's first three
+// half-steps (odd X, Y, Z): it may read only T links and its own partial
+// sums -- the plaquette fused into the last sweep, the Polyakov loop -- so
+// neither it nor those half-steps touch what the other writes.
+static bool cycle_overlaps(const qcd_lattice* L, int mask, int total)
+{
+    return total > 0 && cycle_fuses(L, mask, total) && !(mask & ~(QCD_MEAS_PLAQUETTE | QCD_MEAS_POLYAKOV));
+}
+
+static int enqueue_cycle(qcd_lattice* L, int n_hb, int n_or, int mask, std::vector<hipEvent_t>* ev, int* ev_used,
+                         bool join_prev = false, bool fork_meas = false)
 {
     const int total = n_hb + n_or;
     const bool fuse = cycle_fuses(L, mask, total);
     for (int k = 0; k < total; ++k) {
-        int rc = sweep_impl(L, k < n_hb ? QCD_HEATBATH : QCD_OVERRELAX, fuse && k == total - 1, ev, ev_used);
+        int rc = sweep_impl(L, k < n_hb ? QCD_HEATBATH : QCD_OVERRELAX, fuse && k == total - 1, ev, ev_used,
+                            join_prev && k == 0 ? L->ev_join : nullptr);
         if (rc) return rc;
     }
     if (!mask) return QCD_OK;
+    if (fork_meas) {   // on the second stream, joined by the next cycle's odd T half-step
+        HIPCHK(hipEventRecord(L->ev_fork, L->stream));
+        HIPCHK(hipStreamWaitEvent(L->side, L->ev_fork, 0));
+        hipStream_t main_stream = L->stream;
+        L->stream = L->side;
+        const int rc = measure_impl(L, mask, fuse, LaunchOpt{}, LaunchOpt{}, LaunchOpt{});
+        L->stream = main_stream;
+        if (rc) return rc;
+        HIPCHK(hipEventRecord(L->ev_join, L->side));
+        return QCD_OK;
+    }
     LaunchOpt op, oq, of;
     if (ev) {
         LaunchOpt* slots[3] = {&op, &oq, &of};
@@ -3519,15 +3561,32 @@ int qcd_cycle_capture(qcd_lattice* L, int n_hb, int n_or, int mask)
     if (!L) return fail(QCD_E_ARG, "null lattice");
     if (L->profiling) return fail(QCD_E_STATE, "kernel timing waits for every kernel: run cycles eagerly while it is on");
     if (n_hb < 0 || n_or < 0) return fail(QCD_E_ARG, "negative sweep count");
-    if (L->exec) {
-        (void)hipGraphExecDestroy(L->exec);
-        L->exec = nullptr;
-    }
-    if (L->graph) {
-        (void)hipGraphDestroy(L->graph);
-        L->graph = nullptr;
-    }
+    drop_cycle_graphs(L);
     HIPCHK(hipStreamSynchronize(L->stream));
+    // K = QCD_TUNE_CYCLES_PER_GRAPH consecutive cycles as one graph, each
+    // cycle's measurement beside the next cycle's first half-steps (replays
+    // of whole K-blocks); the single cycle below covers the rest
+    const int K = cycle_overlaps(L, mask, n_hb + n_or) ? L->tune_cycles : 1;
+    if (K > 1) {
+        if (!L->side) HIPCHK(hipStreamCreateWithFlags(&L->side, hipStreamNonBlocking));
+        if (!L->ev_fork) HIPCHK(hipEventCreateWithFlags(&L->ev_fork, hipEventDisableTiming));
+        if (!L->ev_join) HIPCHK(hipEventCreateWithFlags(&L->ev_join, hipEventDisableTiming));
+        HIPCHK(hipStreamBeginCapture(L->stream, hipStreamCaptureModeThreadLocal));
+        const uint32_t c0 = L->prng_counter;
+        int rk = QCD_OK;
+        for (int j = 0; j < K && !rk; ++j) rk = enqueue_cycle(L, n_hb, n_or, mask, nullptr, nullptr, j > 0, j < K - 1);
+        L->prng_counter = c0;
+        hipGraph_t gk = nullptr;
+        const hipError_t ek = hipStreamEndCapture(L->stream, &gk);
+        if (rk) {
+            if (gk) (void)hipGraphDestroy(gk);
+            return rk;
+        }
+        if (ek != hipSuccess) return fail(QCD_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(ek));
+        L->graph_k = gk;
+        HIPCHK(hipGraphInstantiate(&L->exec_k, gk, nullptr, nullptr, 0));
+        L->cyc_k = K;
+    }
     HIPCHK(hipStreamBeginCapture(L->stream, hipStreamCaptureModeThreadLocal));
     const uint32_t counter0 = L->prng_counter;
     int rc = enqueue_cycle(L, n_hb, n_or, mask, nullptr, nullptr);
@@ -3556,14 +3615,26 @@ int qcd_cycle_run(qcd_lattice* L, int n_hb, int n_or, int mask)
     return enqueue_cycle(L, n_hb, n_or, mask, nullptr, nullptr);
 }
 
-int qcd_cycle_launch(qcd_lattice* L, int repeats)
+// `repeats` cycles: whole blocks of the multi-cycle graph, then single ones.
+static int launch_cycles(qcd_lattice* L, int repeats)
 {
-    if (!L || !L->exec) return fail(QCD_E_STATE, "no captured cycle");
-    for (int r = 0; r < repeats; ++r) {
+    int r = repeats;
+    if (L->exec_k && L->cyc_k > 1)
+        for (; r >= L->cyc_k; r -= L->cyc_k) {
+            HIPCHK(hipGraphLaunch(L->exec_k, L->stream));
+            L->prng_counter += L->cyc_prn * (uint32_t)L->cyc_k;
+        }
+    for (; r > 0; --r) {
         HIPCHK(hipGraphLaunch(L->exec, L->stream));
         L->prng_counter += L->cyc_prn;
     }
     return QCD_OK;
+}
+
+int qcd_cycle_launch(qcd_lattice* L, int repeats)
+{
+    if (!L || !L->exec) return fail(QCD_E_STATE, "no captured cycle");
+    return launch_cycles(L, repeats);
 }
 
 int qcd_cycle_time(qcd_lattice* L, int repeats, double* ms)
@@ -3574,10 +3645,7 @@ int qcd_cycle_time(qcd_lattice* L, int repeats, double* ms)
     HIPCHK(hipEventCreate(&b));
     HIPCHK(hipStreamSynchronize(L->stream));
     HIPCHK(hipEventRecord(a, L->stream));
-    for (int r = 0; r < repeats; ++r) {
-        HIPCHK(hipGraphLaunch(L->exec, L->stream));
-        L->prng_counter += L->cyc_prn;
-    }
+    if (int rc = launch_cycles(L, repeats)) return rc;
     HIPCHK(hipEventRecord(b, L->stream));
     HIPCHK(hipEventSynchronize(b));
     float f = 0.f;

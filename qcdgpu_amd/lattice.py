@@ -295,8 +295,9 @@ class Lattice:
         return out
 
     # -- graphs ---------------------------------------------------------------
-    def capture_cycle(self, n_hb=1, n_or=0, plaquette=True, polyakov=True):
-        mask = (_abi.MEAS_PLAQUETTE if plaquette else 0) | (_abi.MEAS_POLYAKOV if polyakov else 0)
+    def capture_cycle(self, n_hb=1, n_or=0, plaquette=True, polyakov=True, wilson=False):
+        mask = ((_abi.MEAS_PLAQUETTE if plaquette else 0) | (_abi.MEAS_POLYAKOV if polyakov else 0)
+                | (_abi.MEAS_WILSON if wilson else 0))
         check(lib().qcd_cycle_capture(self._h, n_hb, n_or, mask))
 
     def run_cycle(self, n_hb=1, n_or=0, plaquette=True, polyakov=True):
@@ -321,12 +322,14 @@ class Lattice:
         )
         return {"hb_us": hb.value, "or_us": orr.value, "meas_us": meas.value, "total_ms": tot.value}
 
-    def tune(self, xcd_map=None, blocks_per_cu=None, update_shape=None, shadow=None):
+    def tune(self, xcd_map=None, blocks_per_cu=None, update_shape=None, shadow=None, cycles_per_graph=None):
         """Launch-shape knobs of the update kernels (scheduling only; results
         are identical): xcd_map 0/1/2/3, blocks_per_cu cap (0 = none),
         update_shape 3 (three wavefronts per 64 links) or 1; shadow = True
         makes each half-step gather from a copy of the table taken before
-        it (a race check: the links must come out the same)."""
+        it (a race check: the links must come out the same);
+        cycles_per_graph K: capture_cycle also records K cycles as one graph
+        with each measurement beside the next cycle's first half-steps."""
         if xcd_map is not None:
             check(lib().qcd_lattice_tune(self._h, _abi.TUNE_XCD_MAP, int(xcd_map)))
         if blocks_per_cu is not None:
@@ -335,6 +338,8 @@ class Lattice:
             check(lib().qcd_lattice_tune(self._h, _abi.TUNE_UPDATE_SHAPE, int(update_shape)))
         if shadow is not None:
             check(lib().qcd_lattice_tune(self._h, _abi.TUNE_SHADOW, int(bool(shadow))))
+        if cycles_per_graph is not None:
+            check(lib().qcd_lattice_tune(self._h, _abi.TUNE_CYCLES_PER_GRAPH, int(cycles_per_graph)))
 
     def kernel(self, name):
         """Id of the reference kernel `name` bound to this lattice
@@ -387,6 +392,9 @@ class Lattice:
                            ("shadow", _abi.TUNE_SHADOW)):
             check(lib().qcd_lattice_get_tune(self._h, knob, ctypes.byref(v), None))
             out[name] = v.value
+        check(lib().qcd_lattice_get_tune(self._h, _abi.TUNE_CYCLES_PER_GRAPH, ctypes.byref(v), det))
+        out["cycles_per_graph"] = v.value
+        out["captured_cycles_per_graph"] = det[0]
         return out
 
     def table_bytes(self):
