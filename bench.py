@@ -19,6 +19,10 @@ step.  `config.step_roofline` is the whole step's algorithmic bytes over the
 timed step.  `cpu_baseline` times the reference CPU path (suncpp, built from
 the reference sources into oracle/_ref/) on the host's cores for a bounded
 sample of heatbath sweeps of the same lattice.
+
+Other lines: --c3 (the 96-chain batched ensemble), --c4 (the learned
+proposal), --table (heatbath-only CPU and GPU rows of SU(2) 8^4, SU(3)
+32^3x8, SU(3) 64^4 and the 96-chain ensemble), --copy-calibration.
 """
 import argparse
 import json
@@ -124,6 +128,160 @@ def cpu_baseline(sweeps=4, o3_sweeps=2, agg_sweeps=2):
                             "overrelaxation: 1 sweep of the oracle (FP64 C)" % (o3_sweeps, agg_sweeps))
 
 
+def _ref_bench(sun, dims, beta, seed, sweeps, warm=0, progress=False):
+    """One ref_harness bench process timing `sweeps` heatbath sweeps after
+    `warm` untimed ones (with `progress`, a stderr line per half-step)."""
+    exe = os.path.join(REPO, "oracle", "_ref", "ref_harness")
+    return subprocess.Popen([exe, "bench", str(sun)] + [str(d) for d in dims]
+                            + [repr(float(beta)), "10", "RANLUX3", str(seed), str(sweeps), str(warm),
+                               "1" if progress else "0"],
+                            stdout=subprocess.PIPE, text=True)
+
+
+def _ref_wait(procs, timeout=900):
+    """(seconds, link updates/s) of each process, in order."""
+    out = []
+    for pr in procs:
+        text, _ = pr.communicate(timeout=timeout)
+        if pr.returncode != 0:
+            raise RuntimeError("ref_harness failed (exit %d)" % pr.returncode)
+        f = text.split()
+        out.append((float(f[1]), float(f[2])))
+    return out
+
+
+def _c3_chains():
+    import numpy as np
+
+    rng = np.random.default_rng(54)
+    ns = rng.choice([12, 16, 20, 24], size=96)
+    nt = rng.choice([4, 6, 8], size=96)
+    return [((int(ns[i]),) * 3 + (int(nt[i]),), 5.5 + 0.7 * i / 95, 1000 + i) for i in range(96)]
+
+
+def baseline_table(args):
+    """The four configurations of the CPU-baseline plan (SU(2) 8^4 beta 2.3;
+    SU(3) 32^3x8 beta 5.9; SU(3) 64^4 beta 6.0; the 96-chain SU(3)
+    ensemble), each as heatbath-only sweeps with a plaquette record: the
+    reference's CPU path (suncpp -O2, FP64, RANLUX3) on one core and as the
+    aggregate of one process per core, and this library on the GPU at FP32
+    and FP64.  Prints one JSON line per configuration and GPU precision.
+    Heatbath-only on both sides because suncpp has no overrelaxation; the
+    headline line covers the 1 heatbath + 4 overrelaxation workload."""
+    import numpy as np
+
+    import qcdgpu_amd as Q
+
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    cores = max(1, min(cores, 16))
+    host = {"cpu_model": _cpu_model(), "nproc": os.cpu_count(), "processes_all_core": cores}
+    singles = [
+        ("C0", 2, (8, 8, 8, 8), 2.3, 2000, 200, 200, 20, cores),
+        ("C1", 3, DIMS, BETA, 400, 40, 20, 2, cores),
+        # 64^4 FP64 is 9.7 GB of links per suncpp process: the aggregate uses
+        # at most 8 processes so the host memory stays bounded.
+        ("C2", 3, (64, 64, 64, 64), 6.0, 40, 4, 1, 0, min(cores, 8)),
+    ]
+    have_ref = os.path.exists(os.path.join(REPO, "oracle", "_ref", "ref_harness"))
+    if not have_ref:
+        sys.exit("bench.py --table: oracle/_ref/ref_harness is missing (oracle/build_ref.sh builds it where the "
+                 "reference sources are)")
+
+    def hb_bytes(V, s, state):
+        # a heatbath sweep updates 4V links at 9 s (+ the stream state read
+        # and written) and the plaquette reads the 4V links once
+        return 4.0 * V * (9 * s + 2 * state) + 4.0 * V * s
+
+    for name, sun, dims, beta, steps, warm, cpu_sweeps, cpu_warm, nproc in singles:
+        V = int(np.prod(dims))
+        for prec in ("fp32", "fp64"):
+            print("bench.py --table: %s GPU %s" % (name, prec), file=sys.stderr, flush=True)
+            with Q.Lattice(sun=sun, dims=dims, beta=beta, precision=prec, prng="XOR128", randseries=54,
+                           max_measurements=steps + warm + 8) as lat:
+                lat.init("cold")
+                lat.prng_init()
+                lat.capture_cycle(n_hb=1, n_or=0, plaquette=True, polyakov=False)
+                lat.launch_cycle(warm)
+                lat.sync()
+                ms = lat.time_cycle(steps)
+                plq = lat.measurements()["plq_total"][-1] / sun
+                s = (lat.params.storage_reals) * (4 if prec == "fp32" else 8)
+            sweep_s = steps / (ms * 1e-3)
+            gbs = hb_bytes(V, s, 16) * sweep_s / 1e9
+            row = {"config": name, "group": "SU(%d)" % sun, "lattice": list(dims), "beta": beta,
+                   "gpu": {"precision": prec, "prng": "XOR128", "sweeps_per_s": sweep_s,
+                           "link_updates_per_s": 4.0 * V * sweep_s, "hbm_frac": gbs / HBM_PEAK_GBS,
+                           "achieved_GBs": gbs, "algorithmic_bytes_per_sweep": hb_bytes(V, s, 16),
+                           "sweeps_timed": steps, "plaquette_last": float(plq)}}
+            if prec == "fp64":
+                print("bench.py --table: %s CPU 1 core, %d sweeps" % (name, cpu_sweeps), file=sys.stderr, flush=True)
+                big = V > 1 << 22   # a 64^4 sweep runs for minutes: progress lines
+                (sec, rate), = _ref_wait([_ref_bench(sun, dims, beta, 54, cpu_sweeps, cpu_warm, progress=big)])
+                print("bench.py --table: %s CPU %d processes" % (name, nproc), file=sys.stderr, flush=True)
+                t0 = time.perf_counter()
+                agg = _ref_wait([_ref_bench(sun, dims, beta, 54 + k, cpu_sweeps, cpu_warm, progress=big and k == 0)
+                                 for k in range(nproc)])
+                wall = time.perf_counter() - t0
+                row["cpu"] = dict(host, kind="reference", build="suncpp -g -O2, FP64, RANLUX3",
+                                  sweeps_timed=cpu_sweeps, warmup=cpu_warm,
+                                  one_core_sweeps_per_s=cpu_sweeps / sec, one_core_link_updates_per_s=rate,
+                                  all_core_processes=nproc, all_core_link_updates_per_s=sum(r for _, r in agg),
+                                  all_core_wall_s=wall)
+            print(json.dumps(row), flush=True)
+
+    chains = _c3_chains()
+    Vs = [int(np.prod(c[0])) for c in chains]
+    V = sum(Vs)
+    for prec in ("fp32", "fp64"):
+        print("bench.py --table: C3 GPU %s" % prec, file=sys.stderr, flush=True)
+        with Q.Batch(chains, precision=prec, prng="RANLUX", max_measurements=300) as b:
+            b.init("hot")
+            b.autotune(repeats=4)
+            b.set_merge(True)
+            b.capture_cycle(n_hb=1, n_or=0, plaquette=True, polyakov=False)
+            b.launch_cycle(20)
+            b.sync()
+            ms = b.time_cycle(200)
+            b.sync()
+        s = 72 if prec == "fp32" else 144
+        sweep_s = 200 / (ms * 1e-3)
+        gbs = hb_bytes(V, s, 112) * sweep_s / 1e9
+        row = {"config": "C3", "group": "SU(3)", "chains": 96, "sites": V, "beta": [5.5, 6.2],
+               "gpu": {"precision": prec, "prng": "RANLUX3", "ensemble_sweeps_per_s": sweep_s,
+                       "link_updates_per_s": 4.0 * V * sweep_s, "hbm_frac": gbs / HBM_PEAK_GBS,
+                       "achieved_GBs": gbs, "algorithmic_bytes_per_sweep": hb_bytes(V, s, 112),
+                       "sweeps_timed": 200}}
+        if prec == "fp64":
+            # 10 sweeps of every chain, one process per chain, `cores` at a time
+            print("bench.py --table: C3 CPU, 96 chains x 10 sweeps on %d processes" % cores, file=sys.stderr,
+                  flush=True)
+            from concurrent.futures import ThreadPoolExecutor
+
+            def one(chain):
+                dims, beta, seed = chain
+                (sec, _), = _ref_wait([_ref_bench(3, dims, beta, seed, 10)])
+                return sec
+
+            t0 = time.perf_counter()
+            with ThreadPoolExecutor(max_workers=cores) as pool:
+                secs = list(pool.map(one, chains))
+            wall = time.perf_counter() - t0
+            cpu_sec = sum(secs)
+            row["cpu"] = dict(host, kind="reference", build="suncpp -g -O2, FP64, RANLUX3", sweeps_timed=10,
+                              one_core_ensemble_sweeps_per_s=10.0 / cpu_sec,
+                              one_core_link_updates_per_s=40.0 * V / cpu_sec,
+                              one_core_note="sum of the single-threaded processes' own timed seconds, "
+                                            "measured while %d ran at once" % cores,
+                              all_core_processes=cores, all_core_link_updates_per_s=40.0 * V / wall,
+                              all_core_note="wall time of the whole pool, process starts and cold starts "
+                                            "included",
+                              all_core_wall_s=wall)
+        print(json.dumps(row), flush=True)
+
+
 def copy_calibration(mib=256, repeats=20):
     """A plain copy of known size: each launch reads and writes exactly
     `mib` MiB, so its FETCH_SIZE / WRITE_SIZE per dispatch give the factors
@@ -189,10 +347,7 @@ def c3(args):
 
     import qcdgpu_amd as Q
 
-    rng = np.random.default_rng(54)
-    ns = rng.choice([12, 16, 20, 24], size=96)
-    nt = rng.choice([4, 6, 8], size=96)
-    chains = [((int(ns[i]),) * 3 + (int(nt[i]),), 5.5 + 0.7 * i / 95, 1000 + i) for i in range(96)]
+    chains = _c3_chains()
     V = sum(int(np.prod(c[0])) for c in chains)
     with Q.Batch(chains, precision="fp32", prng="RANLUX", max_measurements=args.steps + args.warmup + 4) as b:
         b.init("hot")
@@ -260,6 +415,8 @@ def main():
     ap.add_argument("--c4-train-steps", type=int, default=1000)
     ap.add_argument("--c4-steps", type=int, default=4000, help="steps of each chain")
     ap.add_argument("--c3", action="store_true", help="run the batched-ensemble workload (96 chains) instead")
+    ap.add_argument("--table", action="store_true",
+                    help="heatbath-only CPU (suncpp) and GPU rows of the four plan configurations, one JSON line each")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
@@ -269,6 +426,9 @@ def main():
         return
     if args.c3:
         c3(args)
+        return
+    if args.table:
+        baseline_table(args)
         return
     if args.gpus != 1:
         print(

@@ -23,9 +23,11 @@
 //       float64, [site][dir][row][col][re,im]) after each sweep k (k=0 is the
 //       start) and <out_prefix>_plq.txt (sweep, spat, temp, total).
 //   ref_harness bench <sun> <n1> <n2> <n3> <n4> <beta> <nhit> <GEN>
-//                     <randseries> <nsweeps>
-//       Cold start, then times <nsweeps> heatbath sweeps of suncpp on one
-//       thread; prints "sweeps seconds link_updates_per_second plaquette".
+//                     <randseries> <nsweeps> [<warmup> [<progress>]]
+//       Cold start and <warmup> untimed sweeps (default 0), then times
+//       <nsweeps> heatbath sweeps of suncpp on one thread; prints
+//       "sweeps seconds link_updates_per_second plaquette".  With
+//       <progress> = 1 a line goes to stderr after every timed half-step.
 
 #include "suncpp/Update/sun_update.h"
 #include "suncpp/Measurements/Plq.h"
@@ -99,7 +101,7 @@ static int run_sweeps(int argc, char** argv) {
 }
 
 template <typename su_n>
-static int run_bench(char** argv) {
+static int run_bench(int argc, char** argv) {
     modelCPU<su_n> m;
     m.lattice_ndCPU = 4;
     m.lattice_size = (int*)calloc(4, sizeof(int));
@@ -118,11 +120,23 @@ static int run_bench(char** argv) {
     int nsweeps = atoi(argv[11]);
     m.create_latticeCPU();
     m.lattice_initializeCPU();
+    const int nwarm = argc >= 13 ? atoi(argv[12]) : 0;
+    for (int k = 0; k < nwarm; ++k) {
+        for (int dir = 0; dir < 4; ++dir) lattice_update_odd(&m, dir, &prng);
+        for (int dir = 0; dir < 4; ++dir) lattice_update_even(&m, dir, &prng);
+    }
+    const bool progress = argc >= 14 && atoi(argv[13]) == 1;
     timespec t0, t1;
     clock_gettime(CLOCK_MONOTONIC, &t0);
     for (int k = 0; k < nsweeps; ++k) {
-        for (int dir = 0; dir < 4; ++dir) lattice_update_odd(&m, dir, &prng);
-        for (int dir = 0; dir < 4; ++dir) lattice_update_even(&m, dir, &prng);
+        for (int dir = 0; dir < 4; ++dir) {
+            lattice_update_odd(&m, dir, &prng);
+            if (progress) fprintf(stderr, "ref_harness: sweep %d odd direction %d done\n", k, dir);
+        }
+        for (int dir = 0; dir < 4; ++dir) {
+            lattice_update_even(&m, dir, &prng);
+            if (progress) fprintf(stderr, "ref_harness: sweep %d even direction %d done\n", k, dir);
+        }
     }
     clock_gettime(CLOCK_MONOTONIC, &t1);
     double sec = (t1.tv_sec - t0.tv_sec) + 1e-9 * (t1.tv_nsec - t0.tv_nsec);
@@ -137,8 +151,8 @@ static int run_bench(char** argv) {
 int main(int argc, char** argv) {
     if (argc >= 12 && !strcmp(argv[1], "bench")) {
         int sun = atoi(argv[2]);
-        if (sun == 2) return run_bench<su_2>(argv);
-        if (sun == 3) return run_bench<su_3>(argv);
+        if (sun == 2) return run_bench<su_2>(argc, argv);
+        if (sun == 3) return run_bench<su_3>(argc, argv);
     }
     if (argc >= 6 && !strcmp(argv[1], "prng")) {
         PRNG_CL::PRNG prng;
