@@ -160,9 +160,9 @@ def _c3_chains():
 
 
 def baseline_table(args):
-    """The four configurations of the CPU-baseline plan (SU(2) 8^4 beta 2.3;
-    SU(3) 32^3x8 beta 5.9; SU(3) 64^4 beta 6.0; the 96-chain SU(3)
-    ensemble), each as heatbath-only sweeps with a plaquette record: the
+    """Four configurations (SU(2) 8^4 beta 2.3; SU(3) 32^3x8 beta 5.9;
+    SU(3) 64^4 beta 6.0; the 96-chain SU(3) ensemble of --c3), each as
+    heatbath-only sweeps with a plaquette record: the
     reference's CPU path (suncpp -O2, FP64, RANLUX3) on one core and as the
     aggregate of one process per core, and this library on the GPU at FP32
     and FP64.  Prints one JSON line per configuration and GPU precision.
@@ -416,7 +416,8 @@ def main():
     ap.add_argument("--c4-steps", type=int, default=4000, help="steps of each chain")
     ap.add_argument("--c3", action="store_true", help="run the batched-ensemble workload (96 chains) instead")
     ap.add_argument("--table", action="store_true",
-                    help="heatbath-only CPU (suncpp) and GPU rows of the four plan configurations, one JSON line each")
+                    help="heatbath-only CPU (suncpp) and GPU rows of SU(2) 8^4, SU(3) 32^3x8, 64^4 and the 96-chain "
+                         "ensemble, one JSON line each")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()

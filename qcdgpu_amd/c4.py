@@ -189,7 +189,7 @@ def cmd_chain(a):
         flow = QF.Flow(dims=dims, layers=a.layers, bins=a.bins, max_batch=a.batch)
         flow.set_params(np.load(a.params).astype(np.float32))
     arms = ["plain"] + (["learned"] + (["global"] if a.global_imh else []) if flow is not None else [])
-    # The judge of the observables is the plain heatbath ensemble: chains
+    # The observables are checked against the plain heatbath ensemble: chains
     # with random series 1 .. plain_chains, pooled (one chain can sit 3
     # sigma off; the pool's error is what a comparison at 2 sigma needs).
     # Its rate and independent configurations per second are the chains'
@@ -261,7 +261,7 @@ def main(argv=None):
     c.add_argument("--global", dest="global_imh", action="store_true",
                    help="also run the whole-lattice independence-Metropolis arm")
     c.add_argument("--plain-chains", type=int, default=8,
-                   help="plain heatbath chains (random series 1..n) pooled as the judge")
+                   help="plain heatbath chains (random series 1..n) pooled as the reference ensemble")
     for q in (g, t, c):
         q.add_argument("--dims", type=int, nargs=4, default=list(DIMS))
         q.add_argument("--beta", type=float, default=BETA)

@@ -13,10 +13,10 @@ that lattice, stored as float16).
   errors), acceptance strictly between 0 and 1 (an error in q or in the
   local action shows at many sigma: neighbouring sweeps that shared random
   numbers gave 7 sigma);
-- the C4 chains: 1 heatbath sweep per step (8 chains, the judge) against 1
+- the C4 chains: 1 heatbath sweep per step (8 chains, the reference ensemble) against 1
   heatbath sweep plus one learned sweep per step (4 chains), plaquette,
   Polyakov loop and 2x2 Wilson loop at 2 sigma, on fixed seeds, 20000 steps
-  per chain; the judge's plaquette against the oracle's FP64 chains
+  per chain; the reference ensemble's plaquette against the oracle's FP64 chains
   (tests/golden/c4_oracle_plaquette.json) at 2 sigma."""
 import json
 import os
@@ -126,17 +126,17 @@ def _pool(chains, name):
 
 
 def test_c4_chains_match_plain_heatbath():
-    """The judge is the plain heatbath ensemble: 8 chains (random series
+    """The reference ensemble is the plain heatbath ensemble: 8 chains (random series
     1..8) of 20000 steps, pooled, its plaquette checked against the oracle's
     FP64 chains; the learned chain is 4 chains (random series 2001..2004) of
     20000 steps of 1 heatbath + 1 learned sweep, pooled; plaquette, Polyakov
     loop and 2x2 Wilson loop agree at 2 sigma.
 
     How the test came to this form: with one plain chain (random series 54)
-    as the judge it failed at 4000 and at 20000 steps (plaquette z = -2.38,
+    as the reference it failed at 4000 and at 20000 steps (plaquette z = -2.38,
     -3.56); that chain's first 20000 steps are a 3.4-sigma excursion (21
     further plain chains: 0.569368, scatter 0.000112 = the binned errors;
-    oracle: 0.569363(42)).  With the 8-chain judge and 2 learned chains
+    oracle: 0.569363(42)).  With the 8-chain reference ensemble and 2 learned chains
     whose learned-sweep keys overlapped (seed * 7919 + step) the plaquette
     came out at z = -2.01.  Eight learned chains of 40000 steps (random
     series 2001..2008, disjoint keys) then gave 0.569388(24) against the
@@ -151,7 +151,7 @@ def test_c4_chains_match_plain_heatbath():
     print("  acceptance %.3f" % (acc / prop))
     assert prop == 4 * steps * 4 * int(np.prod(DIMS))   # 8 layers x V/2 active links per learned sweep
     assert 0.05 < acc / prop < 0.95
-    # the judge itself against the oracle's FP64 chains (reference CPU path
+    # the reference ensemble itself against the oracle's FP64 chains (reference CPU path
     # semantics, tests/golden/make_c4_oracle_plaquette.py)
     with open(os.path.join(HERE, "golden", "c4_oracle_plaquette.json")) as fh:
         ref = json.load(fh)

@@ -1,6 +1,6 @@
 """The oracle's C4 plaquette fixture (tests/golden/c4_oracle_plaquette.json,
 written by tests/golden/make_c4_oracle_plaquette.py): eight independent FP64
-chains whose pooled mean is what the GPU's plain heatbath ensemble is judged
+chains whose pooled mean is what the GPU's plain heatbath ensemble is checked
 against.  Checked here: the pooled figures follow from the chains, the
 chains scatter as their own binned errors say (chi-square of the chain
 means about the pooled mean), and the value lies between the first-order
