@@ -65,3 +65,37 @@ def test_wilson_mask_keeps_single_cycles():
     b, kb = _run(case, 1, mask_wilson=True)
     assert (ka, kb) == (0, 0)
     assert int(np.count_nonzero(a != b)) == 0
+
+
+@pytest.mark.parametrize("sun,dims,prec", [(3, (32, 32, 32, 8), "fp32"), (2, (8, 8, 8, 8), "fp64")],
+                         ids=["su3-32x32x32x8-fp32", "su2-8x8x8x8-fp64"])
+def test_heatbath_only_graph_equals_eager(sun, dims, prec):
+    """The heatbath-only cycle with a plaquette record (the workload of
+    bench.py --table) against the same sweeps and measurements launched one
+    call at a time: links and generator states bit for bit.  The cycle
+    sums the plaquette inside the sweep's even Y/Z/T half-steps, each trace
+    formed in the links' precision, and the eager measurement with the
+    standalone kernel, which forms the traces in FP64: the records agree to
+    1e-8 relative for FP32 links (measured: 1.2e-10) and 1e-12 for FP64."""
+    _case("heatbath-only cycle graph vs eager: SU(%d) %s %s" % (sun, dims, prec))
+    out = []
+    for graph in (True, False):
+        with Q.Lattice(sun=sun, dims=dims, beta=5.9 if sun == 3 else 2.3, precision=prec, prng="XOR128",
+                       randseries=54, max_measurements=12) as lat:
+            lat.init("cold")
+            lat.prng_init()
+            if graph:
+                lat.capture_cycle(n_hb=1, n_or=0, plaquette=True, polyakov=False)
+                lat.launch_cycle(6)
+            else:
+                for _ in range(6):
+                    lat.sweep("hb")
+                    lat.measure(plaquette=True, polyakov=False)
+            lat.sync()
+            plq, poly = lat.raw_measurements()
+            assert len(plq) == 6 and len(poly) == 0
+            out.append((lat.export_links(), lat.prng_counter(), plq))
+    (la, ca, pa), (lb, cb, pb) = out
+    assert ca == cb
+    assert la.shape == lb.shape and np.array_equal(la.view(np.uint8), lb.view(np.uint8))
+    assert np.allclose(pa, pb, rtol=1e-8 if prec == "fp32" else 1e-12, atol=0.0)
